@@ -1,0 +1,838 @@
+// Sampled-Brandes engine for MI355X (gfx950): device state, the plain
+// one-root-per-round kernels and the C ABI of include/axbc.h.
+//
+// Data layout in HBM (one context = one resident graph):
+//   col[N+1], row[ned]      uint32 CSC, adjacency of u = row[col[u]..col[u+1])
+//   reach[N]                uint32 pendant counts (-H 1), rlist = {v: reach>0}
+//   lvl[N] int32, sigma[N] uint32, delta[N] f64, bc[N] f64
+//   queue[N]                uint32, the frontiers of the round back to back;
+//                           qoff[L] = start of level L (qoff[0]=0, qoff[1]=1)
+//   hubs[], hub_part[]      deferred high-degree vertices of one level and
+//                           their per-workgroup partial sums
+//   ctl                     device counters (queue tail, done flag, ...)
+//
+// One round = reset of the previous round's visited vertices, root init,
+// per level a load-balanced push expansion (wave = 64 frontier vertices,
+// their edges dealt out 64 per step) plus a grid-wide pass over the
+// deferred hubs, then per depth an atomic-free FP64 pull over the level's
+// vertices (segmented wave reduction) plus the hub pass, then the bc update
+// over the visited vertices and the reach>0 list.  Levels are enqueued
+// speculatively; kernels past the last level exit on the device done flag,
+// so the host waits once per round.
+//
+// Reference semantics restated: src/cuda_kernels.cu:588-658 (forward),
+// 719-772 (backward), 269-302 (bc update), 339-373 (reach sum),
+// 1521-1554 (reset), 862-876 (root), src/axbc2d.c:1514-1659 (driver).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../../include/axbc.h"
+#include "wave.h"
+
+using namespace axbc;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kExpandGrid = 1024;   // workgroups of the level kernels
+constexpr int kHubGrid = 512;       // workgroups of the hub passes
+constexpr uint32_t kHubDeg = 2048;  // degree above which a vertex is a hub
+
+struct DevCtl {
+  uint32_t tail;       // queue length of the current round
+  uint32_t done;       // forward finished
+  uint32_t empty_lev;  // first empty level (valid when done)
+  uint32_t nhub;       // hubs deferred in the current level
+  uint32_t wg_done;    // last-workgroup counter
+  uint32_t all;        // reach sum of visited non-root vertices
+  uint32_t stop_any;   // stop test: a check vertex below threshold
+  uint32_t pad;
+  unsigned long long e_r, e_dag;
+};
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, const char *a = "", int line = 0) {
+  snprintf(g_err, sizeof(g_err), fmt, a, line);
+  return code;
+}
+
+#define HIP_TRY(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(AXBC_EHIP, "HIP error %s (line %d)", hipGetErrorString(e_), __LINE__); \
+  } while (0)
+
+__device__ __forceinline__ uint32_t ld_acq(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Round set-up
+// ---------------------------------------------------------------------------
+
+__global__ void k_init_state(uint32_t n, const uint32_t *__restrict__ reach, int *lvl,
+                             uint32_t *sigma, double *delta, double *bc) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    lvl[v] = -1;
+    sigma[v] = 0;
+    delta[v] = (double)reach[v];
+    bc[v] = 0.0;
+  }
+}
+
+// Undo the previous round on its visited vertices only: every other vertex
+// still holds lvl -1, sigma 0, delta reach.
+__global__ void k_reset_visited(const uint32_t *__restrict__ queue, const DevCtl *ctl,
+                                const uint32_t *__restrict__ reach, int *lvl, uint32_t *sigma,
+                                double *delta) {
+  const uint32_t n = ctl->tail;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = queue[i];
+    lvl[v] = -1;
+    sigma[v] = 0;
+    delta[v] = (double)reach[v];
+  }
+}
+
+__global__ void k_set_root(uint32_t v0, int *lvl, uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
+                           DevCtl *ctl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  lvl[v0] = 0;
+  sigma[v0] = 1;
+  queue[0] = v0;
+  qoff[0] = 0;
+  qoff[1] = 1;
+  ctl->tail = 1;
+  ctl->done = 0;
+  ctl->empty_lev = 0;
+  ctl->nhub = 0;
+  ctl->wg_done = 0;
+  ctl->all = 0;
+  ctl->e_r = 0;
+  ctl->e_dag = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Forward level L: the frontier queue[qoff[L-1] .. qoff[L]) discovers level L.
+// For every edge u->w: an unvisited w is claimed with a CAS on lvl[w] and
+// appended; when lvl[w] == L the path count sigma[u] is added to sigma[w]
+// (uint32, wrapping, order-free).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool visit_edge(uint32_t w, uint32_t su, int L, int *lvl, uint32_t *sigma,
+                                           unsigned long long &dag) {
+  int l = lvl[w];
+  bool fresh = false;
+  if (l == -1) {
+    const int old = atomicCAS(&lvl[w], -1, L);
+    fresh = old == -1;
+    l = fresh ? L : old;
+  }
+  if (l == L) {
+    atomicAdd(&sigma[w], su);
+    ++dag;
+  }
+  return fresh;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ row, int *lvl,
+                                                       uint32_t *sigma, uint32_t *queue,
+                                                       const uint32_t *__restrict__ qoff,
+                                                       DevCtl *ctl, uint32_t *hubs, int L) {
+  if (ld_acq(&ctl->done)) return;
+  const uint32_t fb = qoff[L - 1], fe = qoff[L];
+  const int lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
+  unsigned long long er = 0, dag = 0;
+  for (uint32_t base = fb + wid * kWave; base < fe; base += nw * kWave) {
+    const uint32_t i = base + lane;
+    uint32_t u = 0, st = 0, d = 0, su = 0;
+    if (i < fe) {
+      u = queue[i];
+      st = col[u];
+      d = col[u + 1] - st;
+      su = sigma[u];
+    }
+    er += d;
+    if (d > kHubDeg) {
+      hubs[atomicAdd(&ctl->nhub, 1u)] = u;
+      d = 0;
+    }
+    const uint32_t incl = wave_incl_scan(d);
+    const uint32_t total = __shfl(incl, kWave - 1);
+    for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+      const uint32_t e = e0 + lane;
+      const int own = wave_owner(incl, e) & (kWave - 1);
+      const uint32_t ost = __shfl(st, own);
+      const uint32_t osu = __shfl(su, own);
+      const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+      const uint32_t oex = own ? pin : 0u;
+      bool fresh = false;
+      uint32_t w = 0;
+      if (e < total) {
+        w = row[ost + (e - oex)];
+        fresh = visit_edge(w, osu, L, lvl, sigma, dag);
+      }
+      wave_append(fresh, w, queue, &ctl->tail);
+    }
+  }
+  er = wave_sum_u64(er);
+  dag = wave_sum_u64(dag);
+  if (lane == 0 && (er | dag)) {
+    atomicAdd(&ctl->e_r, er);
+    atomicAdd(&ctl->e_dag, dag);
+  }
+}
+
+// Last workgroup of a level's second kernel closes the level.
+__device__ __forceinline__ bool last_workgroup(DevCtl *ctl) {
+  __shared__ bool is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(&ctl->wg_done, 1u);
+    is_last = prev == gridDim.x - 1;
+    if (is_last) __threadfence();
+  }
+  __syncthreads();
+  return is_last;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ row, int *lvl,
+                                                     uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
+                                                     DevCtl *ctl, const uint32_t *__restrict__ hubs,
+                                                     int L) {
+  if (ld_acq(&ctl->done)) return;
+  const uint32_t nh = ctl->nhub;
+  const int lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
+  unsigned long long dag = 0;
+  for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t u = hubs[h];
+    const uint32_t st = col[u], d = col[u + 1] - st, su = sigma[u];
+    for (uint32_t k0 = wid * kWave; k0 < d; k0 += nw * kWave) {
+      const uint32_t k = k0 + lane;
+      bool fresh = false;
+      uint32_t w = 0;
+      if (k < d) {
+        w = row[st + k];
+        fresh = visit_edge(w, su, L, lvl, sigma, dag);
+      }
+      wave_append(fresh, w, queue, &ctl->tail);
+    }
+  }
+  dag = wave_sum_u64(dag);
+  if (lane == 0 && dag) atomicAdd(&ctl->e_dag, dag);
+  if (last_workgroup(ctl) && threadIdx.x == 0) {
+    const uint32_t tail = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    qoff[L + 1] = tail;
+    if (tail == qoff[L]) {
+      ctl->empty_lev = (uint32_t)L;
+      __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    ctl->nhub = 0;
+    ctl->wg_done = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward depth d: every v in queue[qoff[d] .. qoff[d+1]) pulls
+//   delta[v] = reach[v] + sum_{w in adj(v), lvl[w] = d+1} (delta[w]+1)/sigma[w]*sigma[v]
+// with no atomics (each vertex is written by exactly one wave or, for hubs,
+// by the last workgroup of the hub pass).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double dep_term(uint32_t w, int dn, double sv, const int *__restrict__ lvl,
+                                           const uint32_t *__restrict__ sigma,
+                                           const double *__restrict__ delta) {
+  return lvl[w] == dn ? ((delta[w] + 1.0) / (double)sigma[w]) * sv : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwd_pull(const uint32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ row,
+                                                     const int *__restrict__ lvl,
+                                                     const uint32_t *__restrict__ sigma, double *delta,
+                                                     const uint32_t *__restrict__ reach,
+                                                     const uint32_t *__restrict__ queue,
+                                                     const uint32_t *__restrict__ qoff, DevCtl *ctl,
+                                                     uint32_t *hubs, int depth) {
+  const uint32_t fb = qoff[depth], fe = qoff[depth + 1];
+  const int dn = depth + 1;
+  const int lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t base = fb + wid * kWave; base < fe; base += nw * kWave) {
+    const uint32_t i = base + lane;
+    uint32_t v = 0, st = 0, d = 0;
+    double sv = 0.0;
+    if (i < fe) {
+      v = queue[i];
+      st = col[v];
+      d = col[v + 1] - st;
+      sv = (double)sigma[v];
+    }
+    bool hub = d > kHubDeg;
+    if (hub) {
+      hubs[atomicAdd(&ctl->nhub, 1u)] = v;
+      d = 0;
+    }
+    const uint32_t incl = wave_incl_scan(d);
+    const uint32_t excl = incl - d;
+    const uint32_t total = __shfl(incl, kWave - 1);
+    double acc = 0.0;
+    for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+      const uint32_t e = e0 + lane;
+      const int own = wave_owner(incl, e) & (kWave - 1);
+      const uint32_t ost = __shfl(st, own);
+      const double osv = __shfl(sv, own);
+      const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+      const uint32_t oex = own ? pin : 0u;
+      double t = 0.0;
+      int seg = kWave + lane;  // inactive lanes form singleton segments
+      if (e < total) {
+        seg = own;
+        t = dep_term(row[ost + (e - oex)], dn, osv, lvl, sigma, delta);
+      }
+      // segmented inclusive scan over contiguous owner segments
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const double y = __shfl_up(t, off);
+        const int sy = __shfl_up(seg, off);
+        if (lane >= off && sy == seg) t += y;
+      }
+      // every vertex with edges in this step pulls its segment's total from
+      // the lane holding its last edge of the step
+      const bool has = d > 0 && excl < e0 + kWave && incl > e0;
+      const uint32_t last = (incl < e0 + kWave ? incl : e0 + kWave) - 1u - e0;
+      const double got = __shfl(t, has ? (int)last : lane);
+      if (has) acc += got;
+    }
+    if (i < fe && !hub) delta[v] = (double)reach[v] + acc;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwd_hubs(const uint32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ row,
+                                                     const int *__restrict__ lvl,
+                                                     const uint32_t *__restrict__ sigma, double *delta,
+                                                     const uint32_t *__restrict__ reach, DevCtl *ctl,
+                                                     const uint32_t *__restrict__ hubs, double *part,
+                                                     uint32_t *hubcnt, int depth) {
+  __shared__ double red[kBlock / kWave];
+  __shared__ bool last_for_hub;
+  const uint32_t nh = ctl->nhub;
+  const int dn = depth + 1;
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, ng = gridDim.x * blockDim.x;
+  for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t v = hubs[h];
+    const uint32_t st = col[v], d = col[v + 1] - st;
+    const double sv = (double)sigma[v];
+    double s = 0.0;
+    for (uint32_t k = gt; k < d; k += ng) s += dep_term(row[st + k], dn, sv, lvl, sigma, delta);
+    // deterministic block reduction: wave tree, then waves in order
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b = 0.0;
+      for (int j = 0; j < kBlock / kWave; ++j) b += red[j];
+      part[(size_t)h * gridDim.x + blockIdx.x] = b;
+      __threadfence();
+      last_for_hub = atomicAdd(&hubcnt[h], 1u) == gridDim.x - 1;
+      if (last_for_hub) {
+        __threadfence();
+        double tot = 0.0;
+        for (uint32_t j = 0; j < gridDim.x; ++j)
+          tot += __hip_atomic_load(&part[(size_t)h * gridDim.x + j], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        delta[v] = (double)reach[v] + tot;
+        hubcnt[h] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  if (last_workgroup(ctl) && threadIdx.x == 0) {
+    ctl->nhub = 0;
+    ctl->wg_done = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bc update (update_bc, cu:269-286).  Only vertices whose delta is not the
+// reset value 0 can change bc: the visited ones, and with -H 1 the
+// unvisited ones with reach > 0 (delta = reach).  bc[v] += delta*(r0+1)/p.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_update_visited(const uint32_t *__restrict__ queue,
+                                                           const DevCtl *ctl, uint32_t v0, double r1,
+                                                           double p, const double *__restrict__ delta,
+                                                           double *bc, const int *__restrict__ lvl,
+                                                           const uint32_t *__restrict__ reach,
+                                                           int sum_reach, uint32_t *all) {
+  const uint32_t n = ctl->tail;
+  uint32_t s = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = queue[i];
+    if (v != v0) bc[v] += delta[v] * r1 / p;
+    if (sum_reach && lvl[v] > 0) s += reach[v];
+  }
+  if (sum_reach) {
+    s = (uint32_t)wave_sum_u64(s);
+    if (lane_id() == 0 && s) atomicAdd(all, s);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_reach_list(const uint32_t *__restrict__ rlist,
+                                                              uint32_t nr, double r1, double p,
+                                                              const int *__restrict__ lvl,
+                                                              const double *__restrict__ delta,
+                                                              double *bc) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
+    const uint32_t v = rlist[i];
+    if (lvl[v] == -1) bc[v] += delta[v] * r1 / p;
+  }
+}
+
+__global__ void k_update_root(uint32_t v0, uint32_t r0, const DevCtl *ctl, double *bc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t all = ctl->all + r0;
+  const unsigned long long nvt = (unsigned long long)ctl->tail + all;
+  if (r0 > 0) bc[v0] += (double)((unsigned long long)r0 * (nvt - 2ull));
+}
+
+__global__ void k_stop_test(const uint32_t *__restrict__ check, uint32_t x, uint32_t n,
+                            const double *__restrict__ bc, double cn, DevCtl *ctl) {
+  uint32_t below = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < x; i += gridDim.x * blockDim.x) {
+    const uint32_t a = check[i];
+    const double b = a < n ? bc[a] : 0.0;
+    if (b / 2.0 < cn) below = 1;
+  }
+  if (__ballot(below) && lane_id() == 0) atomicOr(&ctl->stop_any, 1u);
+}
+
+__global__ void k_degrees(const uint32_t *__restrict__ col, uint32_t n, uint32_t *deg) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    deg[v] = col[v + 1] - col[v];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+struct axbc_ctx {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t ned = 0;
+  int heuristic = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t bytes = 0;
+  // graph
+  uint32_t *col = nullptr, *row = nullptr, *reach = nullptr, *rlist = nullptr;
+  uint32_t nr = 0;
+  std::vector<uint32_t> h_reach;
+  // state
+  int *lvl = nullptr;
+  uint32_t *sigma = nullptr;
+  double *delta = nullptr, *bc = nullptr;
+  uint32_t *queue = nullptr, *qoff = nullptr, *hubs = nullptr, *hubcnt = nullptr;
+  double *hub_part = nullptr;
+  uint32_t nhub_cap = 0;
+  DevCtl *ctl = nullptr;
+  DevCtl *h_ctl = nullptr;  // pinned mirror
+  uint32_t *check = nullptr;
+  uint32_t ncheck = 0;
+  // round bookkeeping
+  int lev_guess = 8;
+  bool have_round = false;
+  uint32_t last_v0 = 0;
+  int last_nlev = 0;
+  float t_fwd = 0, t_bwd = 0, t_upd = 0;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(axbc_ctx *c, T **p, size_t count) {
+  size_t b = sizeof(T) * (count ? count : 1);
+  hipError_t e = hipMalloc((void **)p, b);
+  if (e != hipSuccess) return fail(AXBC_ENOMEM, "hipMalloc of %s bytes failed (line %d)", "device", __LINE__);
+  c->bytes += b;
+  return AXBC_OK;
+}
+
+int grid_for(uint64_t n, int cap = 2048) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > (uint64_t)cap) g = cap;
+  return (int)g;
+}
+
+int pull_ctl(axbc_ctx *c) {
+  HIP_TRY(hipMemcpyAsync(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int enqueue_forward(axbc_ctx *c, int L) {
+  hipLaunchKernelGGL(k_fwd_expand, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
+                     c->lvl, c->sigma, c->queue, c->qoff, c->ctl, c->hubs, L);
+  hipLaunchKernelGGL(k_fwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
+                     c->sigma, c->queue, c->qoff, c->ctl, c->hubs, L);
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+int enqueue_backward(axbc_ctx *c, int depth) {
+  hipLaunchKernelGGL(k_bwd_pull, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
+                     c->lvl, c->sigma, c->delta, c->reach, c->queue, c->qoff, c->ctl, c->hubs, depth);
+  hipLaunchKernelGGL(k_bwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
+                     c->sigma, c->delta, c->reach, c->ctl, c->hubs, c->hub_part, c->hubcnt, depth);
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+int enqueue_reset_and_root(axbc_ctx *c, uint32_t v0) {
+  if (c->have_round)
+    hipLaunchKernelGGL(k_reset_visited, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->queue,
+                       c->ctl, c->reach, c->lvl, c->sigma, c->delta);
+  hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, c->stream, v0, c->lvl, c->sigma, c->queue,
+                     c->qoff, c->ctl);
+  HIP_TRY(hipGetLastError());
+  c->have_round = true;
+  return AXBC_OK;
+}
+
+int enqueue_update(axbc_ctx *c, uint32_t v0, double p) {
+  const uint32_t r0 = c->h_reach.empty() ? 0u : c->h_reach[v0];
+  const double r1 = (double)(uint32_t)(r0 + 1u);
+  const int sum_reach = c->heuristic == 1 || c->heuristic == 3;
+  hipLaunchKernelGGL(k_update_visited, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->queue,
+                     c->ctl, v0, r1, p, c->delta, c->bc, c->lvl, c->reach, sum_reach, &c->ctl->all);
+  if (c->nr)
+    hipLaunchKernelGGL(k_update_reach_list, dim3(grid_for(c->nr)), dim3(kBlock), 0, c->stream,
+                       c->rlist, c->nr, r1, p, c->lvl, c->delta, c->bc);
+  hipLaunchKernelGGL(k_update_root, dim3(1), dim3(64), 0, c->stream, v0, r0, c->ctl, c->bc);
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *axbc_last_error(void) { return g_err; }
+
+int axbc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int axbc_ctx_create(int device, uint64_t N, uint64_t ned, const uint32_t *col, const uint32_t *row,
+                    const uint32_t *reach, int heuristic, axbc_ctx **out, size_t *dev_bytes) {
+  if (!out || !col || (ned && !row)) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  if (N == 0 || N > 0xFFFFFFF0ull || ned > 0xFFFFFFFFull)
+    return fail(AXBC_EINVAL, "graph size out of range%s (line %d)", "", __LINE__);
+  if (col[0] != 0 || col[N] != ned) return fail(AXBC_EINVAL, "col[] does not span row[]%s (line %d)", "", __LINE__);
+  HIP_TRY(hipSetDevice(device));
+  axbc_ctx *c = new axbc_ctx();
+  c->device = device;
+  c->n = (uint32_t)N;
+  c->ned = ned;
+  c->heuristic = heuristic;
+  int rc;
+#define TRY_ALLOC(p, cnt)                  \
+  if ((rc = dalloc(c, &(p), (cnt))) != 0) { \
+    axbc_ctx_destroy(c);                   \
+    return rc;                             \
+  }
+  // hubs: vertices whose degree exceeds the hub threshold
+  uint32_t nhub = 0;
+  for (uint64_t v = 0; v < N; ++v) nhub += (col[v + 1] - col[v]) > kHubDeg;
+  c->nhub_cap = nhub ? nhub : 1;
+  c->h_reach.assign(N, 0u);
+  std::vector<uint32_t> rl;
+  if (reach) {
+    memcpy(c->h_reach.data(), reach, sizeof(uint32_t) * N);
+    for (uint64_t v = 0; v < N; ++v)
+      if (reach[v]) rl.push_back((uint32_t)v);
+  }
+  c->nr = (uint32_t)rl.size();
+  TRY_ALLOC(c->col, N + 1);
+  TRY_ALLOC(c->row, ned);
+  TRY_ALLOC(c->reach, N);
+  TRY_ALLOC(c->rlist, rl.size());
+  TRY_ALLOC(c->lvl, N);
+  TRY_ALLOC(c->sigma, N);
+  TRY_ALLOC(c->delta, N);
+  TRY_ALLOC(c->bc, N);
+  TRY_ALLOC(c->queue, N);
+  TRY_ALLOC(c->qoff, N + 2);
+  TRY_ALLOC(c->hubs, c->nhub_cap);
+  TRY_ALLOC(c->hubcnt, c->nhub_cap);
+  TRY_ALLOC(c->hub_part, (size_t)c->nhub_cap * kHubGrid);
+  TRY_ALLOC(c->ctl, 1);
+#undef TRY_ALLOC
+  if (hipHostMalloc((void **)&c->h_ctl, sizeof(DevCtl), hipHostMallocDefault) != hipSuccess) {
+    axbc_ctx_destroy(c);
+    return fail(AXBC_ENOMEM, "pinned allocation failed%s (line %d)", "", __LINE__);
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    axbc_ctx_destroy(c);
+    return fail(AXBC_EHIP, "stream creation failed%s (line %d)", "", __LINE__);
+  }
+  for (auto &e : c->ev) (void)hipEventCreate(&e);
+  hipError_t e1 = hipMemcpyAsync(c->col, col, sizeof(uint32_t) * (N + 1), hipMemcpyHostToDevice, c->stream);
+  hipError_t e2 = ned ? hipMemcpyAsync(c->row, row, sizeof(uint32_t) * ned, hipMemcpyHostToDevice, c->stream)
+                      : hipSuccess;
+  hipError_t e3 = hipMemcpyAsync(c->reach, c->h_reach.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice,
+                                 c->stream);
+  hipError_t e4 = rl.empty() ? hipSuccess
+                             : hipMemcpyAsync(c->rlist, rl.data(), sizeof(uint32_t) * rl.size(),
+                                              hipMemcpyHostToDevice, c->stream);
+  (void)hipMemsetAsync(c->ctl, 0, sizeof(DevCtl), c->stream);
+  (void)hipMemsetAsync(c->hubcnt, 0, sizeof(uint32_t) * c->nhub_cap, c->stream);
+  hipLaunchKernelGGL(k_init_state, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, (uint32_t)N, c->reach,
+                     c->lvl, c->sigma, c->delta, c->bc);
+  hipError_t e5 = hipStreamSynchronize(c->stream);
+  if (e1 || e2 || e3 || e4 || e5) {
+    axbc_ctx_destroy(c);
+    return fail(AXBC_EHIP, "upload failed: %s (line %d)", hipGetErrorString(e5 ? e5 : e1), __LINE__);
+  }
+  *out = c;
+  if (dev_bytes) *dev_bytes = c->bytes;
+  return AXBC_OK;
+}
+
+int axbc_ctx_destroy(axbc_ctx *c) {
+  if (!c) return AXBC_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->col,   c->row,  c->reach, c->rlist,  c->lvl,      c->sigma, c->delta, c->bc,
+                  c->queue, c->qoff, c->hubs,  c->hubcnt, c->hub_part, c->ctl,   c->check};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AXBC_OK;
+}
+
+int axbc_pivot(axbc_ctx *c, uint32_t v0, double p, axbc_pivot_stats *stats) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  if (v0 >= c->n) return fail(AXBC_EINVAL, "root out of range%s (line %d)", "", __LINE__);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc;
+  (void)hipEventRecord(c->ev[0], c->stream);
+  if ((rc = enqueue_reset_and_root(c, v0))) return rc;
+  // Forward levels, enqueued in chunks; the device done flag makes the
+  // kernels past the last level exit immediately.
+  int L = 1, chunk = c->lev_guess;
+  for (;;) {
+    for (int j = 0; j < chunk; ++j)
+      if ((rc = enqueue_forward(c, L + j))) return rc;
+    L += chunk;
+    if ((rc = pull_ctl(c))) return rc;
+    if (c->h_ctl->done) break;
+    if ((uint64_t)L > (uint64_t)c->n + 1) return fail(AXBC_ESTATE, "forward did not terminate%s (line %d)", "", __LINE__);
+    chunk = chunk * 2;
+  }
+  (void)hipEventRecord(c->ev[1], c->stream);
+  const int nlev = (int)c->h_ctl->empty_lev - 1;
+  c->last_nlev = nlev;
+  c->lev_guess = nlev + 2;
+  // Backward depths with bc_func_mono's do-while: nlev-1 .. 1, or 0 alone
+  // when the deepest level is 1; nothing when only the root was reached.
+  if (nlev >= 1) {
+    int depth = nlev - 1;
+    do {
+      if ((rc = enqueue_backward(c, depth))) return rc;
+      --depth;
+    } while (depth > 0);
+  }
+  (void)hipEventRecord(c->ev[2], c->stream);
+  if ((rc = enqueue_update(c, v0, p))) return rc;
+  (void)hipEventRecord(c->ev[3], c->stream);
+  c->last_v0 = v0;
+  if (stats) return axbc_last_stats(c, stats);
+  return AXBC_OK;
+}
+
+int axbc_last_stats(axbc_ctx *c, axbc_pivot_stats *st) {
+  if (!c || !st) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  int rc;
+  if ((rc = pull_ctl(c))) return rc;
+  const DevCtl *h = c->h_ctl;
+  const uint32_t r0 = c->h_reach.empty() ? 0u : c->h_reach[c->last_v0];
+  st->nvisited = h->tail;
+  st->nlevels = (uint64_t)(h->empty_lev ? h->empty_lev - 1 : 0);
+  st->e_r = h->e_r;
+  st->e_dag = h->e_dag;
+  st->all = (uint32_t)(h->all + r0);
+  return AXBC_OK;
+}
+
+int axbc_last_timing(axbc_ctx *c, float *f, float *b, float *u) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  HIP_TRY(hipEventSynchronize(c->ev[3]));
+  float x = 0, y = 0, z = 0;
+  HIP_TRY(hipEventElapsedTime(&x, c->ev[0], c->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&y, c->ev[1], c->ev[2]));
+  HIP_TRY(hipEventElapsedTime(&z, c->ev[2], c->ev[3]));
+  if (f) *f = x;
+  if (b) *b = y;
+  if (u) *u = z;
+  return AXBC_OK;
+}
+
+int axbc_sync(axbc_ctx *c) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_reset_bc(axbc_ctx *c) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemsetAsync(c->bc, 0, sizeof(double) * c->n, c->stream));
+  return AXBC_OK;
+}
+
+static int copy_out(axbc_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || !dst) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_get_bc(axbc_ctx *c, double *bc) { return copy_out(c, bc, c ? c->bc : nullptr, c ? sizeof(double) * c->n : 0); }
+int axbc_get_delta(axbc_ctx *c, double *d) { return copy_out(c, d, c ? c->delta : nullptr, c ? sizeof(double) * c->n : 0); }
+int axbc_get_lvl(axbc_ctx *c, int32_t *l) { return copy_out(c, l, c ? c->lvl : nullptr, c ? sizeof(int) * c->n : 0); }
+int axbc_get_sigma(axbc_ctx *c, uint32_t *s) { return copy_out(c, s, c ? c->sigma : nullptr, c ? sizeof(uint32_t) * c->n : 0); }
+
+int axbc_get_deg(axbc_ctx *c, uint32_t *deg) {
+  if (!c || !deg) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  uint32_t *d = nullptr;
+  HIP_TRY(hipMalloc((void **)&d, sizeof(uint32_t) * c->n));
+  hipLaunchKernelGGL(k_degrees, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->col, c->n, d);
+  HIP_TRY(hipMemcpyAsync(deg, d, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(d));
+  return AXBC_OK;
+}
+
+int axbc_set_bc(axbc_ctx *c, const double *bc) {
+  if (!c || !bc) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(c->bc, bc, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_set_checks(axbc_ctx *c, const uint32_t *check, uint32_t x) {
+  if (!c || (x && !check)) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  if (c->check) {
+    (void)hipFree(c->check);
+    c->check = nullptr;
+  }
+  c->ncheck = x;
+  if (!x) return AXBC_OK;
+  HIP_TRY(hipMalloc((void **)&c->check, sizeof(uint32_t) * x));
+  HIP_TRY(hipMemcpyAsync(c->check, check, sizeof(uint32_t) * x, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_stop_test(axbc_ctx *c, uint32_t cc, int *stop) {
+  if (!c || !stop) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemsetAsync(&c->ctl->stop_any, 0, sizeof(uint32_t), c->stream));
+  if (c->ncheck)
+    hipLaunchKernelGGL(k_stop_test, dim3(grid_for(c->ncheck, 64)), dim3(kBlock), 0, c->stream, c->check,
+                       c->ncheck, c->n, c->bc, (double)((uint64_t)cc * (uint64_t)c->n), c->ctl);
+  int rc;
+  if ((rc = pull_ctl(c))) return rc;
+  *stop = c->h_ctl->stop_any ? 0 : 1;
+  return AXBC_OK;
+}
+
+// ---- fine-grained entry points (one host synchronisation per call) -------
+
+int axbc_begin_round(axbc_ctx *c) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  if (c->have_round)
+    hipLaunchKernelGGL(k_reset_visited, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->queue,
+                       c->ctl, c->reach, c->lvl, c->sigma, c->delta);
+  HIP_TRY(hipMemsetAsync(c->ctl, 0, sizeof(DevCtl), c->stream));
+  c->have_round = false;
+  return axbc_sync(c);
+}
+
+int axbc_set_root(axbc_ctx *c, uint32_t v0) {
+  if (!c || v0 >= c->n) return fail(AXBC_EINVAL, "bad root%s (line %d)", "", __LINE__);
+  hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, c->stream, v0, c->lvl, c->sigma, c->queue, c->qoff,
+                     c->ctl);
+  c->have_round = true;
+  c->last_v0 = v0;
+  return axbc_sync(c);
+}
+
+int axbc_forward_level(axbc_ctx *c, int level, uint32_t *nfrt) {
+  if (!c || level < 1) return fail(AXBC_EINVAL, "bad level%s (line %d)", "", __LINE__);
+  int rc;
+  if ((rc = enqueue_forward(c, level))) return rc;
+  if ((rc = pull_ctl(c))) return rc;
+  uint32_t q[2] = {0, 0};
+  HIP_TRY(hipMemcpy(q, c->qoff + level, sizeof(q), hipMemcpyDeviceToHost));
+  if (nfrt) *nfrt = c->h_ctl->done && (int)c->h_ctl->empty_lev <= level ? 0u : q[1] - q[0];
+  return AXBC_OK;
+}
+
+int axbc_backward_depth(axbc_ctx *c, int depth) {
+  if (!c || depth < 0) return AXBC_OK;  // the reference tolerates depth -1
+  int rc;
+  if ((rc = enqueue_backward(c, depth))) return rc;
+  return axbc_sync(c);
+}
+
+int axbc_reach_sum(axbc_ctx *c, uint32_t *all) {
+  if (!c || !all) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  // The reach sum is produced by the visited-vertex pass of the update; a
+  // separate pass is only needed for this fine-grained entry point.
+  int rc;
+  if ((rc = pull_ctl(c))) return rc;
+  std::vector<uint32_t> q(c->h_ctl->tail);
+  std::vector<int> l(c->n);
+  HIP_TRY(hipMemcpy(q.data(), c->queue, sizeof(uint32_t) * q.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(l.data(), c->lvl, sizeof(int) * c->n, hipMemcpyDeviceToHost));
+  uint32_t s = 0;
+  for (uint32_t v : q)
+    if (l[v] > 0) s += c->h_reach[v];
+  *all = s;
+  return AXBC_OK;
+}
+
+int axbc_update_bc(axbc_ctx *c, uint32_t v0, double p, uint64_t nvisited) {
+  if (!c || v0 >= c->n) return fail(AXBC_EINVAL, "bad root%s (line %d)", "", __LINE__);
+  (void)nvisited;  // recomputed on the device from the queue and reach sum
+  int rc;
+  if ((rc = enqueue_update(c, v0, p))) return rc;
+  return axbc_sync(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+/*
+ * axbc.h -- C ABI of the MI355X (gfx950) sampled-Brandes engine,
+ * libaxbc_hip.so.
+ *
+ * The engine replaces the device side of AxBC's 1x1 "mono" path, i.e. the
+ * entry points src/bc2d.h:136-190 declares and src/cuda_kernels.cu defines.
+ * Plain C types only: pointers and sizes, no HIP or torch types.
+ *
+ * Design differences from the reference boundary, all deliberate:
+ *  - Device state lives in a context (one resident graph per context, any
+ *    number of contexts per process) instead of file-static globals
+ *    (src/cuda_kernels.cu:41-80).  The reference names are provided on top
+ *    of a default context by liblegacy (see axbc_legacy.h).
+ *  - delta and bc are FP64 (the reference keeps float).
+ *  - Functions return a status instead of exiting (MY_CUDA_CHECK,
+ *    src/cudamacro.h:1-16); axbc_last_error() gives the message.
+ *  - axbc_pivot() runs a whole Brandes round on the device with no host
+ *    synchronisation per BFS level.
+ *
+ * Threading: a context is used by one host thread at a time; all work of a
+ * context is ordered on the context's own stream.
+ */
+#ifndef AXBC_H
+#define AXBC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXBC_OK 0
+#define AXBC_EINVAL (-1) /* bad argument                      */
+#define AXBC_ENOMEM (-2) /* device or host allocation failed  */
+#define AXBC_EHIP (-3)   /* HIP runtime error                 */
+#define AXBC_ESTATE (-4) /* call out of order                 */
+#define AXBC_EBUDGET (-5)/* request exceeds the memory budget */
+
+typedef struct axbc_ctx axbc_ctx;
+
+/* Counters of the last pivot. */
+typedef struct {
+  uint64_t nvisited; /* vertices reached, root included                */
+  uint64_t nlevels;  /* deepest non-empty BFS level                    */
+  uint64_t e_r;      /* adjacency entries of the reached vertices      */
+  uint64_t e_dag;    /* shortest-path DAG edges (sigma contributions)  */
+  uint64_t all;      /* reach sum used by the bc update (-H 1)         */
+} axbc_pivot_stats;
+
+/* Message of the last failing call on this thread. */
+const char *axbc_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int axbc_device_count(void);
+
+/*
+ * Upload a CSC graph and allocate the per-vertex state.
+ * Replaces initcuda() (src/cuda_kernels.cu:1558-1656) + set_degree +
+ * init_bc_1degree_device() (cu:1515-1519).
+ *   N            vertices; col[N+1], row[ned] uint32 CSC (column = the
+ *                second endpoint, rows sorted, as build_csc makes it,
+ *                src/axbc2d.c:883-907)
+ *   reach        N pendant counts from the 1-degree reduction, or NULL
+ *   heuristic    -H value (0 or 1): 1 adds the reach sum of the visited
+ *                vertices to nvisited in the bc update
+ * *dev_bytes (may be NULL) receives the device bytes allocated.
+ */
+int axbc_ctx_create(int device, uint64_t N, uint64_t ned, const uint32_t *col,
+                    const uint32_t *row, const uint32_t *reach, int heuristic,
+                    axbc_ctx **out, size_t *dev_bytes);
+/* Release everything (fincuda(), cu:1659-1693). */
+int axbc_ctx_destroy(axbc_ctx *ctx);
+
+/*
+ * One sampled-Brandes round from root v0 with sampling probability p:
+ * reset of the previous round's visited state (setcuda, cu:1532-1554),
+ * root (set_mlp_cuda, cu:862-876), the forward levels
+ * (scan_col_csc_cuda_mono, cu:1088-1177), the backward depths
+ * (scan_frt_csc_cuda_mono, cu:1013-1080), the reach sum
+ * (pre_update_bc_cuda, cu:358-373) and the bc update (update_bc_cuda,
+ * cu:289-302), as bc_func_mono (src/axbc2d.c:1514-1659) sequences them.
+ * stats may be NULL; when given the call waits for the round to finish.
+ */
+int axbc_pivot(axbc_ctx *ctx, uint32_t v0, double p, axbc_pivot_stats *stats);
+
+/* Wait for all queued work of the context. */
+int axbc_sync(axbc_ctx *ctx);
+
+/* bc := 0. */
+int axbc_reset_bc(axbc_ctx *ctx);
+
+/* Device -> host copies of the state after the last round.
+ * get_bc / get_delta of the reference (cu:1466-1473) at FP64, get_deg
+ * (cu:1458), and the per-vertex levels / path counts for parity checks. */
+int axbc_get_bc(axbc_ctx *ctx, double *bc);
+int axbc_get_delta(axbc_ctx *ctx, double *delta);
+int axbc_get_deg(axbc_ctx *ctx, uint32_t *deg);
+int axbc_get_lvl(axbc_ctx *ctx, int32_t *lvl);
+int axbc_get_sigma(axbc_ctx *ctx, uint32_t *sigma);
+int axbc_set_bc(axbc_ctx *ctx, const double *bc);
+
+/*
+ * Adaptive stop test of the pivot loop (src/axbc2d.c:2846-2870) on the
+ * device: *stop = 1 when bc[a]/2 >= c*N for every check vertex a (ids >= N
+ * read as bc 0).  The check list is uploaded once with axbc_set_checks.
+ */
+int axbc_set_checks(axbc_ctx *ctx, const uint32_t *check, uint32_t x);
+int axbc_stop_test(axbc_ctx *ctx, uint32_t c, int *stop);
+
+/* Fine-grained per-level entry points with the reference's call structure
+ * (each call synchronises, as the reference's wrappers do).  They drive
+ * the same kernels as axbc_pivot; used by the legacy names.
+ *   axbc_begin_round   setcuda()          cu:1532
+ *   axbc_set_root      set_mlp_cuda()     cu:862
+ *   axbc_forward_level scan_col_csc_cuda_mono(ncol, level) -> nfrt
+ *   axbc_backward_depth scan_frt_csc_cuda_mono(offset, ncol, depth)
+ *   axbc_reach_sum     pre_update_bc_cuda() -> all (lvl > 0 only)
+ *   axbc_update_bc     update_bc_cuda(v0, p, ncol, nvisited)           */
+int axbc_begin_round(axbc_ctx *ctx);
+int axbc_set_root(axbc_ctx *ctx, uint32_t v0);
+int axbc_forward_level(axbc_ctx *ctx, int level, uint32_t *nfrt);
+int axbc_backward_depth(axbc_ctx *ctx, int depth);
+int axbc_reach_sum(axbc_ctx *ctx, uint32_t *all);
+int axbc_update_bc(axbc_ctx *ctx, uint32_t v0, double p, uint64_t nvisited);
+
+/* Last round's counters (waits for the context). */
+int axbc_last_stats(axbc_ctx *ctx, axbc_pivot_stats *stats);
+
+/* Time (ms) of the last axbc_pivot measured with HIP events on the context
+ * stream, split into forward, backward and update phases. */
+int axbc_last_timing(axbc_ctx *ctx, float *fwd_ms, float *bwd_ms, float *upd_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXBC_H */
