@@ -1,0 +1,110 @@
+"""In-tree build of the engine's native libraries (gfx950).
+
+    python -m axbc_amd.build
+
+produces
+    axbc_amd/lib/libaxbc_hip.so    HIP kernels + C ABI (include/axbc.h)
+    axbc_amd/lib/libaxbc_host.so   graph tooling (C, OpenMP)
+    axbc_amd/bin/axbc2d            command-line driver with AxBC's flags
+and the checker builds under oracle/ (the CPU oracle always; the reference
+generator under oracle/_ref only when the reference sources are present).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "axbc_amd")
+LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+HIP_SRC = os.path.join(PKG, "csrc", "hip")
+HOST_SRC = os.path.join(PKG, "csrc", "host")
+INCLUDE = os.path.join(ROOT, "include")
+REFERENCE = "/root/reference"
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+HIP_SOURCES = ["engine.hip", "batch.hip", "sampler.hip"]
+HOST_SOURCES = ["kronecker.c", "graph.c"]
+
+
+def _run(cmd: list[str], cwd: str = ROOT) -> None:
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+
+
+def _hip_objects() -> list[str]:
+    return [os.path.join(HIP_SRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(HIP_SRC, s))]
+
+
+def build_hip() -> str:
+    out = os.path.join(LIB, "libaxbc_hip.so")
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    objs = []
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        futs = []
+        for src in _hip_objects():
+            obj = os.path.join(LIB, os.path.basename(src) + ".o")
+            objs.append(obj)
+            futs.append(ex.submit(_run, [hipcc, *HIP_FLAGS, "-I", INCLUDE, "-c", src, "-o", obj]))
+        for f in futs:
+            f.result()
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_host() -> str:
+    out = os.path.join(LIB, "libaxbc_host.so")
+    srcs = [os.path.join(HOST_SRC, s) for s in HOST_SOURCES]
+    _run(["gcc", "-O3", "-std=gnu11", "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wextra",
+          "-Wno-unused-parameter", *srcs, "-o", out + ".tmp", "-lm", "-lpthread"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_driver() -> str | None:
+    src = os.path.join(HOST_SRC, "axbc2d.c")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(BIN, "axbc2d")
+    _run(["gcc", "-O3", "-std=gnu11", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter",
+          "-I", INCLUDE, "-I", HOST_SRC, src, "-o", out + ".tmp",
+          "-L", LIB, "-laxbc_host", "-laxbc_hip", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    if os.path.isdir(os.path.join(REFERENCE, "generator")):
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REFERENCE}"])
+
+
+def build_all() -> None:
+    os.makedirs(LIB, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        fh = ex.submit(build_hip)
+        fo = ex.submit(build_host)
+        fr = ex.submit(build_oracle)
+        fh.result()
+        fo.result()
+        fr.result()
+    build_driver()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB, BIN)

@@ -1,0 +1,1185 @@
+// Bit-packed multi-source sampled-Brandes rounds (B = 64 or 128 roots per
+// batch) for MI355X.
+//
+// One lane of a wave owns one root (two for B = 128), so every per-root row
+// of a vertex (sigma u32, delta f64, x = (delta+1)/sigma f64) is read and
+// written as one coalesced row, and the adjacency of a vertex is read once
+// for the whole batch.  Per vertex the batch keeps bit words (NW = B/64
+// uint64 each): VIS (roots that reached it so far), NX (roots reaching it
+// at the current level) and a double-buffered dense frontier FR.  The
+// frontier of level L is a list of (vertex, word) entries; all levels stay
+// in one list (q, qw, qoff) for the backward sweep.
+//
+// Per level L:
+//   push    frontier entries (u, f) of L-1 mark every w in adj(u) with
+//           f & ~VIS[w] (atomicOr on NX; the first marker appends w)
+//   sigma   every new w pulls sigma_s[w] = sum of sigma_s[u] over u in
+//           radj(w) with s in FR[u] & NX[w]   (uint32, wrapping)
+//   close   FR/VIS/list words for level L, per-root counters, x rows
+// Per depth d (deepest first): VIS loses the level d+1 bits, then every
+// entry (v, f) of level d pulls, per root s in f,
+//   delta_s[v] = reach[v] + sum_{w in adj(v), s not in VIS[w]} x_s[w] * sigma_s[v]
+// in adjacency order (the same left fold as the sequential oracle), no
+// atomics.  Vertices with more edges than one chunk are split into chunks
+// processed by separate waves and combined in chunk order.
+// The bc update applies the batch's roots to every vertex in root order
+// (bc = ((bc + t_0) + t_1) + ...), i.e. exactly the sequence of one-root
+// rounds, and stops at the first root after which the adaptive stop test
+// (src/axbc2d.c:2846-2870) holds, evaluated exactly per root.
+//
+// Reference semantics restated: src/cuda_kernels.cu:588-658, 719-772,
+// 269-302, 339-373; src/axbc2d.c:1514-1659 (per root) and 2834-3156 (loop).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../../include/axbc.h"
+#include "engine_internal.h"
+#include "wave.h"
+
+using namespace axbc;
+
+namespace axbc {
+
+constexpr uint32_t kChunk = 256;  // edges per wave work item for split vertices
+constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
+
+struct BCtl {
+  uint32_t tail;       // list length
+  uint32_t done;
+  uint32_t empty_lev;
+  uint32_t overflow;
+  uint32_t nsplit;     // split vertices in the current pass
+  uint32_t wg_done;
+  uint32_t stop_round; // first round after which the stop test holds (nb if none)
+  uint32_t pad;
+  unsigned long long split_claim;  // (entry index << 40) | chunk base
+};
+
+struct BatchState {
+  int nw = 1;        // 64-bit words per vertex
+  int B = 64;        // roots per batch (64 * nw)
+  uint32_t n = 0;
+  uint64_t *vis = nullptr, *nx = nullptr, *fr[2] = {nullptr, nullptr};
+  uint32_t *sig = nullptr;
+  double *dlt = nullptr, *xr = nullptr;
+  uint32_t *mark = nullptr;  // B = 128: level stamp of the appender
+  uint32_t *seen = nullptr;  // batch stamp of visited vertices
+  uint32_t *q = nullptr;
+  uint64_t *qw = nullptr;
+  uint32_t qcap = 0;
+  uint32_t *qoff = nullptr;
+  uint64_t *act = nullptr;   // per level: OR of the level's words
+  // split vertices: entry list index, vertex, chunk base per split entry
+  uint32_t *sp_v = nullptr, *sp_e = nullptr, *sp_cnt = nullptr;
+  uint64_t *sp_base = nullptr;
+  uint32_t sp_cap = 0;
+  double *part = nullptr;
+  uint64_t part_rows = 0;
+  unsigned long long *cnt = nullptr;  // per root: nvisited, all, e_r, e_dag
+  uint32_t *src = nullptr;
+  double *prob = nullptr;
+  uint32_t *r0 = nullptr;            // reach of each root
+  uint32_t *check = nullptr;
+  BCtl *ctl = nullptr, *h_ctl = nullptr;
+  unsigned long long *h_cnt = nullptr;
+  uint64_t *h_act = nullptr;
+  uint32_t stamp = 0;
+  uint32_t epoch = 0;  // level stamps for `mark`, unique across batches
+  bool have = false;
+  int lev_guess = 12;
+  size_t bytes = 0;
+};
+
+}  // namespace axbc
+
+namespace {
+
+__device__ __forceinline__ uint32_t ldg_acq(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool bit(uint64_t w, int i) { return (w >> i) & 1ull; }
+
+template <int NW>
+struct Words {
+  uint64_t w[NW];
+  __device__ __forceinline__ bool any() const {
+    uint64_t a = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) a |= w[j];
+    return a != 0;
+  }
+};
+
+template <int NW>
+__device__ __forceinline__ Words<NW> ldw(const uint64_t *p, uint32_t v) {
+  Words<NW> r;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) r.w[j] = p[(size_t)v * NW + j];
+  return r;
+}
+
+template <int NW>
+__device__ __forceinline__ Words<NW> shflw(const Words<NW> &x, int src) {
+  Words<NW> r;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) r.w[j] = __shfl(x.w[j], src);
+  return r;
+}
+
+// Last workgroup of a kernel (counter reset by the caller of the result).
+__device__ __forceinline__ bool last_wg(uint32_t *counter) {
+  __shared__ bool is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    if (is_last) __threadfence();
+  }
+  __syncthreads();
+  return is_last;
+}
+
+// ---------------------------------------------------------------------------
+// Batch set-up
+// ---------------------------------------------------------------------------
+
+template <int NW>
+__global__ void k_bfill(uint32_t n, const uint32_t *__restrict__ reach, uint64_t *vis, uint64_t *nx,
+                        uint64_t *fr0, uint64_t *fr1, uint32_t *sig, double *dlt, uint32_t *mark,
+                        uint32_t *seen) {
+  constexpr int B = 64 * NW;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    for (int j = 0; j < NW; ++j) {
+      vis[(size_t)v * NW + j] = 0;
+      nx[(size_t)v * NW + j] = 0;
+      fr0[(size_t)v * NW + j] = 0;
+      fr1[(size_t)v * NW + j] = 0;
+    }
+    for (int s = 0; s < B; ++s) {
+      sig[(size_t)v * B + s] = 0;
+      dlt[(size_t)v * B + s] = (double)reach[v];
+    }
+    if (mark) mark[v] = 0;
+    seen[v] = 0;
+  }
+}
+
+// Undo the previous batch on the vertices it listed (a vertex may be
+// listed at several levels; the writes are idempotent).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_breset(const uint32_t *__restrict__ q, const axbc::BCtl *ctl,
+                                                   const uint32_t *__restrict__ reach, uint64_t *vis,
+                                                   uint64_t *fr0, uint64_t *fr1, uint32_t *sig, double *dlt,
+                                                   uint32_t qcap) {
+  constexpr int B = 64 * NW;
+  const uint32_t n = min(ctl->tail, qcap);
+  const int lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nw = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t i = wid; i < n; i += nw) {
+    const uint32_t v = q[i];
+    const double r = (double)reach[v];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      sig[(size_t)v * B + lane + 64 * j] = 0;
+      dlt[(size_t)v * B + lane + 64 * j] = r;
+    }
+    if (lane < NW) {
+      vis[(size_t)v * NW + lane] = 0;
+      fr0[(size_t)v * NW + lane] = 0;
+      fr1[(size_t)v * NW + lane] = 0;
+    }
+  }
+}
+
+// Level 0: the roots.  One thread per root.
+template <int NW>
+__global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis, uint64_t *fr0, uint32_t *sig,
+                         uint32_t *q, uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
+                         unsigned long long *cnt, uint32_t stamp, uint32_t *seen) {
+  constexpr int B = 64 * NW;
+  const int s = threadIdx.x + blockIdx.x * blockDim.x;
+  if (s < nb) {
+    const uint32_t v = src[s];
+    atomicOr((unsigned long long *)&vis[(size_t)v * NW + s / 64], 1ull << (s % 64));
+    atomicOr((unsigned long long *)&fr0[(size_t)v * NW + s / 64], 1ull << (s % 64));
+    sig[(size_t)v * B + s] = 1;
+    q[s] = v;
+    for (int j = 0; j < NW; ++j) qw[(size_t)s * NW + j] = j == s / 64 ? 1ull << (s % 64) : 0ull;
+    seen[v] = stamp;
+    cnt[4 * s + 0] = 1;  // nvisited (root)
+    cnt[4 * s + 1] = 0;  // reach sum of levels > 0
+    cnt[4 * s + 2] = 0;  // e_r (root's degree added by the first push)
+    cnt[4 * s + 3] = 0;  // e_dag
+  }
+  if (s == 0) {
+    qoff[0] = 0;
+    qoff[1] = (uint32_t)nb;
+    for (int j = 0; j < NW; ++j) {
+      uint64_t a = 0;
+      for (int t = 0; t < nb; ++t)
+        if (t / 64 == j) a |= 1ull << (t % 64);
+      act[j] = a;
+    }
+    ctl->tail = (uint32_t)nb;
+    ctl->done = 0;
+    ctl->empty_lev = 0;
+    ctl->overflow = 0;
+    ctl->nsplit = 0;
+    ctl->wg_done = 0;
+    ctl->split_claim = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward level L
+// ---------------------------------------------------------------------------
+
+// Mark w from a frontier word f; returns true for the one caller that must
+// append w to the level's list.
+template <int NW>
+__device__ __forceinline__ bool push_edge(uint32_t w, const Words<NW> &f, const uint64_t *__restrict__ vis,
+                                          uint64_t *nx, uint32_t *mark, uint32_t L) {
+  bool first = false;
+  if (NW == 1) {
+    const uint64_t nw = f.w[0] & ~vis[w];
+    if (nw) {
+      const uint64_t old = atomicOr((unsigned long long *)&nx[w], (unsigned long long)nw);
+      first = old == 0ull;
+    }
+  } else {
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      const uint64_t nw = f.w[j] & ~vis[(size_t)w * NW + j];
+      if (nw) {
+        atomicOr((unsigned long long *)&nx[(size_t)w * NW + j], (unsigned long long)nw);
+        any = true;
+      }
+    }
+    if (any && mark[w] != L) first = atomicExch(&mark[w], L) != L;
+  }
+  return first;
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                  const uint64_t *__restrict__ vis, uint64_t *nx, uint32_t *mark,
+                                                  uint64_t *fr_clear, uint32_t *q, const uint64_t *__restrict__ qw,
+                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                  uint32_t *sp_v, uint32_t sp_cap, uint32_t qcap, int L,
+                                                  uint32_t mark_id) {
+  if (ldg_acq(&ctl->done)) return;
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  // clear the dense frontier buffer of level L-2 (it receives level L)
+  if (L >= 2) {
+    const uint32_t cb = qoff[L - 2], ce = qoff[L - 1];
+    for (uint32_t i = cb + blockIdx.x * blockDim.x + threadIdx.x; i < ce; i += gridDim.x * blockDim.x) {
+      const uint32_t v = q[i];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) fr_clear[(size_t)v * NW + j] = 0;
+    }
+  }
+  const uint32_t fb = qoff[L - 1], fe = qoff[L];
+  for (uint32_t base = fb + gw * kWave; base < fe; base += nwv * kWave) {
+    const uint32_t i = base + lane;
+    uint32_t u = 0, st = 0, d = 0;
+    Words<NW> f;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) f.w[j] = 0;
+    if (i < fe) {
+      u = q[i];
+      st = col[u];
+      d = col[u + 1] - st;
+      f = ldw<NW>(qw, i);
+    }
+    if (d > kChunk) {  // split: handled by k_bpush_split, one chunk per wave
+      const uint32_t k = atomicAdd(&ctl->nsplit, 1u);
+      if (k < sp_cap)
+        sp_v[k] = i;
+      else
+        atomicOr(&ctl->overflow, 1u);
+      d = 0;
+    }
+    const uint32_t incl = wave_incl_scan(d);
+    const uint32_t total = __shfl(incl, kWave - 1);
+    for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+      const uint32_t e = e0 + lane;
+      const int own = wave_owner(incl, e) & (kWave - 1);
+      const uint32_t ost = __shfl(st, own);
+      const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+      const uint32_t oex = own ? pin : 0u;
+      const Words<NW> of = shflw<NW>(f, own);
+      bool first = false;
+      uint32_t w = 0;
+      if (e < total) {
+        w = row[ost + (e - oex)];
+        first = push_edge<NW>(w, of, vis, nx, mark, mark_id);
+      }
+      // append to the list (bounded)
+      const unsigned long long m = __ballot(first);
+      if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
+        b = __shfl(b, leader);
+        const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (first) {
+          if (slot < qcap)
+            q[slot] = w;
+          else
+            atomicOr(&ctl->overflow, 1u);
+        }
+      }
+    }
+  }
+}
+
+// Frontier vertices with more than kChunk edges: every wave takes one chunk
+// of kChunk edges of one of them (entries in sp_v, chunk counts derived from
+// the degrees, scanned implicitly by a linear walk that every wave repeats
+// only over the split entries).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restrict__ col,
+                                                        const uint32_t *__restrict__ row,
+                                                        const uint64_t *__restrict__ vis, uint64_t *nx,
+                                                        uint32_t *mark, uint32_t *q, const uint64_t *__restrict__ qw,
+                                                        axbc::BCtl *ctl, const uint32_t *__restrict__ sp_v,
+                                                        uint32_t sp_cap, uint32_t qcap, uint32_t mark_id) {
+  if (ldg_acq(&ctl->done)) return;
+  const uint32_t ns = min(ctl->nsplit, sp_cap);
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  uint32_t wave_item = gw;  // global chunk index this wave works on next
+  uint32_t acc_chunks = 0;
+  for (uint32_t h = 0; h < ns; ++h) {
+    const uint32_t i = sp_v[h];
+    const uint32_t u = q[i];
+    const uint32_t st = col[u], d = col[u + 1] - st;
+    const uint32_t nch = (d + kChunk - 1) / kChunk;
+    const Words<NW> f = ldw<NW>(qw, i);
+    while (wave_item < acc_chunks + nch) {
+      const uint32_t c = wave_item - acc_chunks;
+      const uint32_t cb = c * kChunk, ce = min(d, cb + kChunk);
+      for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
+        const uint32_t k = k0 + lane;
+        bool first = false;
+        uint32_t w = 0;
+        if (k < ce) {
+          w = row[st + k];
+          first = push_edge<NW>(w, f, vis, nx, mark, mark_id);
+        }
+        const unsigned long long m = __ballot(first);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          uint32_t b = 0;
+          if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
+          b = __shfl(b, leader);
+          const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+          if (first) {
+            if (slot < qcap)
+              q[slot] = w;
+            else
+              atomicOr(&ctl->overflow, 1u);
+          }
+        }
+      }
+      wave_item += nwv;
+    }
+    acc_chunks += nch;
+  }
+}
+
+// sigma pull for the vertices new at level L: q[qoff[L] .. tail).  A wave
+// per vertex; lanes = roots.  Vertices with more than kChunk pull edges are
+// split into kChunk work items that add their partial counts atomically.
+template <int NW>
+__device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t ce, const uint32_t *__restrict__ rrow,
+                                            const uint64_t *__restrict__ frp, const Words<NW> &nw,
+                                            const uint32_t *__restrict__ sig, uint32_t acc[NW], uint32_t dag[NW]) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
+    const uint32_t k = k0 + lane;
+    uint32_t u = 0;
+    Words<NW> m;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) m.w[j] = 0;
+    if (k < ce) {
+      u = rrow[st + k];
+      const Words<NW> f = ldw<NW>(frp, u);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) m.w[j] = f.w[j] & nw.w[j];
+    }
+    unsigned long long todo = __ballot(m.any());
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t uu = __shfl(u, i);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const uint64_t mm = __shfl(m.w[j], i);
+        if (bit(mm, lane)) {
+          acc[j] += sig[(size_t)uu * B + lane + 64 * j];
+          ++dag[j];
+        }
+      }
+    }
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
+                                                   const uint64_t *__restrict__ nx, const uint64_t *__restrict__ frp,
+                                                   uint32_t *sig, const uint32_t *__restrict__ q,
+                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                   unsigned long long *cnt, uint32_t *sp_e, uint32_t sp_cap, int L,
+                                                   uint32_t qcap) {
+  constexpr int B = 64 * NW;
+  if (ldg_acq(&ctl->done)) return;
+  const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  uint32_t dag_tot[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
+  for (uint32_t i = nb + gw; i < ne; i += nwv) {
+    const uint32_t w = q[i];
+    const uint32_t st = rcol[w], d = rcol[w + 1] - st;
+    if (d > kChunk) {
+      if (lane == 0) {
+        const uint32_t k = atomicAdd(&ctl->nsplit, 1u);
+        if (k < sp_cap)
+          sp_e[k] = i;
+        else
+          atomicOr(&ctl->overflow, 1u);
+      }
+      continue;
+    }
+    const Words<NW> nw = ldw<NW>(nx, w);
+    uint32_t acc[NW], dag[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
+    sigma_chunk<NW>(st, 0, d, rrow, frp, nw, sig, acc, dag);
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      if (bit(nw.w[j], lane)) sig[(size_t)w * B + lane + 64 * j] = acc[j];
+      dag_tot[j] += dag[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restrict__ rcol,
+                                                         const uint32_t *__restrict__ rrow,
+                                                         const uint64_t *__restrict__ nx,
+                                                         const uint64_t *__restrict__ frp, uint32_t *sig,
+                                                         const uint32_t *__restrict__ q, axbc::BCtl *ctl,
+                                                         unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
+                                                         uint32_t sp_cap) {
+  constexpr int B = 64 * NW;
+  if (ldg_acq(&ctl->done)) return;
+  const uint32_t ns = min(ctl->nsplit, sp_cap);
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  uint32_t wave_item = gw, acc_chunks = 0;
+  uint32_t dag_tot[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
+  for (uint32_t h = 0; h < ns; ++h) {
+    const uint32_t w = q[sp_e[h]];
+    const uint32_t st = rcol[w], d = rcol[w + 1] - st;
+    const uint32_t nch = (d + kChunk - 1) / kChunk;
+    const Words<NW> nw = ldw<NW>(nx, w);
+    while (wave_item < acc_chunks + nch) {
+      const uint32_t c = wave_item - acc_chunks;
+      uint32_t acc[NW], dag[NW];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
+      sigma_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), rrow, frp, nw, sig, acc, dag);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        if (acc[j]) atomicAdd(&sig[(size_t)w * B + lane + 64 * j], acc[j]);
+        dag_tot[j] += dag[j];
+      }
+      wave_item += nwv;
+    }
+    acc_chunks += nch;
+  }
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+}
+
+// Close level L: frontier words, VIS, list words, counters, x rows.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
+                                                   uint64_t *nx, uint64_t *vis, uint64_t *frc, const uint32_t *sig,
+                                                   double *xr, const uint32_t *__restrict__ q, uint64_t *qw,
+                                                   uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
+                                                   unsigned long long *cnt, uint32_t stamp, uint32_t *seen, int L,
+                                                   uint32_t qcap) {
+  constexpr int B = 64 * NW;
+  if (ldg_acq(&ctl->done)) return;
+  const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  unsigned long long c_vis[NW], c_all[NW], c_er[NW];
+  uint64_t a_or[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) c_vis[j] = c_all[j] = c_er[j] = a_or[j] = 0;
+  for (uint32_t i = nb + gw; i < ne; i += nwv) {
+    const uint32_t w = q[i];
+    const Words<NW> nw = ldw<NW>(nx, w);
+    const uint32_t rw = reach[w];
+    const uint32_t dw = col[w + 1] - col[w];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      if (bit(nw.w[j], lane)) {
+        const uint32_t s = sig[(size_t)w * B + lane + 64 * j];
+        xr[(size_t)w * B + lane + 64 * j] = ((double)rw + 1.0) / (double)s;
+        c_vis[j] += 1;
+        c_all[j] += rw;
+        c_er[j] += dw;
+      }
+      a_or[j] |= nw.w[j];
+    }
+    if (lane < NW) {
+      const uint64_t word = nw.w[lane < NW ? lane : 0];
+      frc[(size_t)w * NW + lane] = word;
+      vis[(size_t)w * NW + lane] |= word;
+      qw[(size_t)i * NW + lane] = word;
+      nx[(size_t)w * NW + lane] = 0;
+    }
+    if (lane == 0) seen[w] = stamp;
+  }
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    const int s = lane + 64 * j;
+    if (c_vis[j]) {
+      atomicAdd(&cnt[4 * s + 0], c_vis[j]);
+      atomicAdd(&cnt[4 * s + 1], c_all[j]);
+      atomicAdd(&cnt[4 * s + 2], c_er[j]);
+    }
+    if (lane == 0 && a_or[j]) atomicOr((unsigned long long *)&act[(size_t)L * NW + j], (unsigned long long)a_or[j]);
+  }
+  if (last_wg(&ctl->wg_done) && threadIdx.x == 0) {
+    uint32_t tail = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tail > qcap) {  // list overflow: stop the sweep, the host retries larger
+      tail = qcap;
+      ctl->overflow = 1;
+    }
+    qoff[L + 1] = tail;
+    if (tail == qoff[L] || ctl->overflow) {
+      ctl->empty_lev = (uint32_t)L;
+      __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    ctl->nsplit = 0;
+    ctl->wg_done = 0;
+  }
+}
+
+// e_r of the roots (degree of the root itself).
+__global__ void k_broot_deg(const uint32_t *__restrict__ col, const uint32_t *__restrict__ src, int nb,
+                            unsigned long long *cnt) {
+  const int s = threadIdx.x + blockIdx.x * blockDim.x;
+  if (s < nb) atomicAdd(&cnt[4 * s + 2], (unsigned long long)(col[src[s] + 1] - col[src[s]]));
+}
+
+// ---------------------------------------------------------------------------
+// Backward depth d
+// ---------------------------------------------------------------------------
+
+template <int NW>
+__global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                         const uint32_t *__restrict__ qoff, int lev) {
+  const uint32_t b = qoff[lev], e = qoff[lev + 1];
+  for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+    const uint32_t v = q[i];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) vis[(size_t)v * NW + j] &= ~qw[(size_t)i * NW + j];
+  }
+}
+
+// Dependencies of one vertex over adj[cb, ce), per root lane, left fold in
+// adjacency order into acc.
+template <int NW>
+__device__ __forceinline__ void dep_chunk(uint32_t st, uint32_t cb, uint32_t ce, const uint32_t *__restrict__ row,
+                                          const uint64_t *__restrict__ vis, const Words<NW> &f,
+                                          const double *__restrict__ xr, const double sv[NW], double acc[NW]) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
+    const uint32_t k = k0 + lane;
+    uint32_t w = 0;
+    Words<NW> m;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) m.w[j] = 0;
+    if (k < ce) {
+      w = row[st + k];
+      const Words<NW> vw = ldw<NW>(vis, w);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) m.w[j] = f.w[j] & ~vw.w[j];
+    }
+    unsigned long long todo = __ballot(m.any());
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t ww = __shfl(w, i);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const uint64_t mm = __shfl(m.w[j], i);
+        if (bit(mm, lane)) acc[j] += xr[(size_t)ww * B + lane + 64 * j] * sv[j];
+      }
+    }
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                 const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
+                                                 double *dlt, double *xr, const uint32_t *__restrict__ reach,
+                                                 const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                                                 const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                 uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
+                                                 int depth, uint64_t root_mask0, uint64_t root_mask1) {
+  constexpr int B = 64 * NW;
+  const uint32_t b = qoff[depth], e = qoff[depth + 1];
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t i = b + gw; i < e; i += nwv) {
+    const uint32_t v = q[i];
+    Words<NW> f = ldw<NW>(qw, i);
+    if (depth == 0) {
+      f.w[0] &= root_mask0;
+      if (NW > 1) f.w[NW - 1] &= root_mask1;
+    }
+    if (!f.any()) continue;
+    const uint32_t st = col[v], d = col[v + 1] - st;
+    if (d > kChunk) {
+      if (lane == 0) {
+        const unsigned long long nch = (d + kChunk - 1) / kChunk;
+        const unsigned long long old = atomicAdd(&ctl->split_claim, (1ull << 40) + nch);
+        const uint32_t k = (uint32_t)(old >> 40);
+        const uint64_t base = old & ((1ull << 40) - 1);
+        if (k < sp_cap && base + nch <= part_rows) {
+          sp_e[k] = i;
+          sp_base[k] = base;
+        } else {
+          atomicOr(&ctl->overflow, 1u);
+        }
+      }
+      continue;
+    }
+    double sv[NW], acc[NW];
+    const double r = (double)reach[v];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
+      acc[j] = r;
+    }
+    dep_chunk<NW>(st, 0, d, row, vis, f, xr, sv, acc);
+#pragma unroll
+    for (int j = 0; j < NW; ++j)
+      if (bit(f.w[j], lane)) {
+        dlt[(size_t)v * B + lane + 64 * j] = acc[j];
+        xr[(size_t)v * B + lane + 64 * j] = (acc[j] + 1.0) / sv[j];
+      }
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                       const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
+                                                       double *dlt, double *xr, const uint32_t *__restrict__ reach,
+                                                       const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                                                       axbc::BCtl *ctl, const uint32_t *__restrict__ sp_e,
+                                                       const uint64_t *__restrict__ sp_base, uint32_t *sp_cnt,
+                                                       uint32_t sp_cap, double *part, int depth, uint64_t root_mask0,
+                                                       uint64_t root_mask1) {
+  constexpr int B = 64 * NW;
+  const unsigned long long claim = ctl->split_claim;
+  const uint32_t ns = min((uint32_t)(claim >> 40), sp_cap);
+  const uint64_t total = claim & ((1ull << 40) - 1);
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint64_t item = gw; item < total; item += nwv) {
+    // entry whose chunk range holds `item` (bases are increasing in k)
+    uint32_t lo = 0, hi = ns;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (sp_base[mid] <= item)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t k = lo;
+    const uint32_t i = sp_e[k];
+    const uint32_t v = q[i];
+    Words<NW> f = ldw<NW>(qw, i);
+    if (depth == 0) {
+      f.w[0] &= root_mask0;
+      if (NW > 1) f.w[NW - 1] &= root_mask1;
+    }
+    const uint32_t st = col[v], d = col[v + 1] - st;
+    const uint32_t nch = (d + kChunk - 1) / kChunk;
+    const uint32_t c = (uint32_t)(item - sp_base[k]);
+    double sv[NW], acc[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
+      acc[j] = 0.0;
+    }
+    dep_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), row, vis, f, xr, sv, acc);
+#pragma unroll
+    for (int j = 0; j < NW; ++j) part[(size_t)item * B + lane + 64 * j] = acc[j];
+    __threadfence();
+    uint32_t prev = 0;
+    if (lane == 0) prev = atomicAdd(&sp_cnt[k], 1u);
+    prev = __shfl(prev, 0);
+    if (prev == nch - 1) {  // last chunk of this vertex: combine in chunk order
+      __threadfence();
+      const double r = (double)reach[v];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        double t = r;
+        for (uint32_t cc = 0; cc < nch; ++cc)
+          t += __hip_atomic_load(&part[(size_t)(sp_base[k] + cc) * B + lane + 64 * j], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        if (bit(f.w[j], lane)) {
+          dlt[(size_t)v * B + lane + 64 * j] = t;
+          xr[(size_t)v * B + lane + 64 * j] = (t + 1.0) / sv[j];
+        }
+      }
+      if (lane == 0) sp_cnt[k] = 0;
+    }
+  }
+}
+
+__global__ void k_bsplit_reset(axbc::BCtl *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->split_claim = 0;
+    ctl->nsplit = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stop test and bc update
+// ---------------------------------------------------------------------------
+
+// Term of root s for vertex v (the sequence of one-root updates).
+template <int NW>
+__device__ __forceinline__ double bc_term(uint32_t v, int s, const uint32_t *__restrict__ src,
+                                          const double *__restrict__ prob, const uint32_t *__restrict__ r0,
+                                          const double *__restrict__ dlt, const unsigned long long *__restrict__ cnt,
+                                          bool *has) {
+  constexpr int B = 64 * NW;
+  *has = true;
+  if (v == src[s]) {
+    const uint32_t rr = r0[s];
+    if (rr == 0) {
+      *has = false;
+      return 0.0;
+    }
+    const uint32_t all = (uint32_t)cnt[4 * s + 1] + rr;
+    const unsigned long long nvt = cnt[4 * s + 0] + (unsigned long long)all;
+    return (double)((unsigned long long)rr * (nvt - 2ull));
+  }
+  const double r1 = (double)(uint32_t)(r0[s] + 1u);
+  return dlt[(size_t)v * B + s] * r1 / prob[s];
+}
+
+template <int NW>
+__global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t n, const double *__restrict__ bc,
+                        double cn, int nb, const uint32_t *__restrict__ src, const double *__restrict__ prob,
+                        const uint32_t *__restrict__ r0, const double *__restrict__ dlt,
+                        const unsigned long long *__restrict__ cnt, unsigned long long *ok_bits) {
+  // ok_bits[j] bit s: every check vertex is satisfied after round s
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < x; i += gridDim.x * blockDim.x) {
+    const uint32_t a = check[i];
+    double b = a < n ? bc[a] : 0.0;
+    uint64_t sat[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) sat[j] = 0;
+    for (int s = 0; s < nb; ++s) {
+      if (a < n) {
+        bool has;
+        const double t = bc_term<NW>(a, s, src, prob, r0, dlt, cnt, &has);
+        if (has) b += t;
+      }
+      if (!(b / 2.0 < cn)) sat[s / 64] |= 1ull << (s % 64);
+    }
+#pragma unroll
+    for (int j = 0; j < NW; ++j) atomicAnd(&ok_bits[j], (unsigned long long)sat[j]);
+  }
+}
+
+// bc += the batch's terms in root order, roots [0, nr).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bupdate(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
+                                                    const uint32_t *__restrict__ reach, int nr,
+                                                    const uint32_t *__restrict__ src, const double *__restrict__ prob,
+                                                    const uint32_t *__restrict__ r0, const double *__restrict__ dlt,
+                                                    const unsigned long long *__restrict__ cnt, double *bc) {
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t v0 = gw * kWave; v0 < n; v0 += nwv * kWave) {
+    const uint32_t v = v0 + lane;
+    const bool want = v < n && (seen[v] == stamp || reach[v] != 0);
+    unsigned long long todo = __ballot(want);
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t vv = v0 + (uint32_t)i;
+      double t[NW];
+      bool h[NW];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const int s = lane + 64 * j;
+        t[j] = 0.0;
+        h[j] = false;
+        if (s < nr) t[j] = bc_term<NW>(vv, s, src, prob, r0, dlt, cnt, &h[j]);
+      }
+      // sequential in root order
+      double acc = 0.0;
+      if (lane == 0) acc = bc[vv];
+#pragma unroll
+      for (int j = 0; j < NW; ++j)
+        for (int s = 0; s < 64; ++s) {
+          const double ts = __shfl(t[j], s);
+          const int hs = __shfl((int)h[j], s);
+          if (lane == 0 && hs) acc += ts;
+        }
+      if (lane == 0) bc[vv] = acc;
+    }
+  }
+}
+
+template <int NW>
+__global__ void k_bcopy_delta(uint32_t n, int s, const double *__restrict__ dlt, double *out) {
+  constexpr int B = 64 * NW;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    out[v] = dlt[(size_t)v * B + s];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace axbc {
+
+void batch_free(axbc_ctx *c) {
+  BatchState *b = c->batch;
+  if (!b) return;
+  void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
+               b->act, b->sp_v, b->sp_e, b->sp_cnt, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
+               b->check, b->ctl};
+  for (void *p : d)
+    if (p) (void)hipFree(p);
+  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  if (b->h_cnt) (void)hipHostFree(b->h_cnt);
+  if (b->h_act) (void)hipHostFree(b->h_act);
+  delete b;
+  c->batch = nullptr;
+}
+
+}  // namespace axbc
+
+namespace {
+
+template <typename T>
+int balloc(BatchState *b, T **p, size_t count) {
+  const size_t bytes = sizeof(T) * (count ? count : 1);
+  if (hipMalloc((void **)p, bytes) != hipSuccess)
+    return fail(AXBC_ENOMEM, "batch allocation of %s failed (line %d)", "device memory", __LINE__);
+  b->bytes += bytes;
+  return AXBC_OK;
+}
+
+// Pull direction: radj(w) = {u : w in adj(u)}.  Symmetric graphs reuse the
+// CSC; otherwise the transpose is built on the host once.
+int ensure_transpose(axbc_ctx *c) {
+  if (c->rcol) return AXBC_OK;
+  std::vector<uint32_t> col(c->n + 1), row(c->ned);
+  HIP_TRY(hipMemcpy(col.data(), c->col, sizeof(uint32_t) * (c->n + 1), hipMemcpyDeviceToHost));
+  if (c->ned) HIP_TRY(hipMemcpy(row.data(), c->row, sizeof(uint32_t) * c->ned, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> rcol(c->n + 1, 0), rrow(c->ned);
+  for (uint64_t k = 0; k < c->ned; ++k) rcol[row[k] + 1]++;
+  for (uint32_t v = 0; v < c->n; ++v) rcol[v + 1] += rcol[v];
+  std::vector<uint32_t> fill(rcol.begin(), rcol.end() - 1);
+  for (uint32_t u = 0; u < c->n; ++u)
+    for (uint32_t k = col[u]; k < col[u + 1]; ++k) rrow[fill[row[k]]++] = u;
+  if (rcol == col && rrow == row) {
+    c->symmetric = true;
+    c->rcol = c->col;
+    c->rrow = c->row;
+    return AXBC_OK;
+  }
+  c->symmetric = false;
+  HIP_TRY(hipMalloc((void **)&c->rcol, sizeof(uint32_t) * (c->n + 1)));
+  HIP_TRY(hipMalloc((void **)&c->rrow, sizeof(uint32_t) * (c->ned ? c->ned : 1)));
+  HIP_TRY(hipMemcpy(c->rcol, rcol.data(), sizeof(uint32_t) * (c->n + 1), hipMemcpyHostToDevice));
+  if (c->ned) HIP_TRY(hipMemcpy(c->rrow, rrow.data(), sizeof(uint32_t) * c->ned, hipMemcpyHostToDevice));
+  c->bytes += sizeof(uint32_t) * (c->n + 1 + c->ned);
+  return AXBC_OK;
+}
+
+int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
+  int rc;
+  if ((rc = ensure_transpose(c))) return rc;
+  BatchState *b = new BatchState();
+  c->batch = b;
+  b->nw = width / 64;
+  b->B = width;
+  b->n = c->n;
+  const size_t n = c->n, nw = b->nw, B = b->B;
+  // split entries: vertices with more than kChunk edges, per level at most
+  // all of them; chunk rows: at most ned / kChunk + number of them
+  uint32_t nsplit = 0;
+  for (size_t v = 0; v < n; ++v) nsplit += (c->h_col[v + 1] - c->h_col[v]) > kChunk;
+  b->sp_cap = nsplit + 1;
+  b->part_rows = c->ned / kChunk + nsplit + 1;
+  b->qcap = qcap;
+#define BA(p, cnt)                         \
+  if ((rc = balloc(b, &(p), (cnt))) != 0) { \
+    batch_free(c);                         \
+    return rc;                             \
+  }
+  BA(b->vis, n * nw);
+  BA(b->nx, n * nw);
+  BA(b->fr[0], n * nw);
+  BA(b->fr[1], n * nw);
+  BA(b->sig, n * B);
+  BA(b->dlt, n * B);
+  BA(b->xr, n * B);
+  if (nw > 1) BA(b->mark, n);
+  BA(b->seen, n);
+  BA(b->q, qcap);
+  BA(b->qw, (size_t)qcap * nw);
+  BA(b->qoff, n + 2);
+  BA(b->act, (n + 2) * nw);
+  BA(b->sp_v, b->sp_cap);
+  BA(b->sp_e, b->sp_cap);
+  BA(b->sp_cnt, b->sp_cap);
+  BA(b->sp_base, b->sp_cap);
+  BA(b->part, b->part_rows * B);
+  BA(b->cnt, 4 * B);
+  BA(b->src, B);
+  BA(b->prob, B);
+  BA(b->r0, B);
+  BA(b->ctl, 1);
+#undef BA
+  HIP_TRY(hipHostMalloc((void **)&b->h_ctl, sizeof(BCtl), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void **)&b->h_cnt, sizeof(unsigned long long) * 4 * B, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void **)&b->h_act, sizeof(uint64_t) * (n + 2) * nw, hipHostMallocDefault));
+  HIP_TRY(hipMemsetAsync(b->ctl, 0, sizeof(BCtl), c->stream));
+  HIP_TRY(hipMemsetAsync(b->sp_cnt, 0, sizeof(uint32_t) * b->sp_cap, c->stream));
+  HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (n + 2) * nw, c->stream));
+  if (nw == 1)
+    hipLaunchKernelGGL(k_bfill<1>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, (uint32_t)n, c->reach, b->vis,
+                       b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->mark, b->seen);
+  else
+    hipLaunchKernelGGL(k_bfill<2>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, (uint32_t)n, c->reach, b->vis,
+                       b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->mark, b->seen);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->bytes += b->bytes;
+  return AXBC_OK;
+}
+
+int pull_bctl(axbc_ctx *c) {
+  HIP_TRY(hipMemcpyAsync(c->batch->h_ctl, c->batch->ctl, sizeof(BCtl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+template <int NW>
+int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
+              axbc_pivot_stats *stats) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  std::vector<uint32_t> r0(nb);
+  for (int k = 0; k < nb; ++k) r0[k] = c->h_reach[v0s[k]];
+  HIP_TRY(hipMemcpyAsync(b->src, v0s, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->prob, ps, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->r0, r0.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  // reset the previous batch, then level 0
+  if (b->have)
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, b->q, b->ctl, c->reach, b->vis, b->fr[0],
+                       b->fr[1], b->sig, b->dlt, b->qcap);
+  b->have = true;
+  b->stamp += 1;
+  HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)(b->lev_guess + 2) * NW, s));
+  hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
+                     b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen);
+  hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
+  HIP_TRY(hipGetLastError());
+  // forward levels, speculatively in chunks
+  int L = 1, chunk = b->lev_guess;
+  int zeroed = b->lev_guess + 2;
+  for (;;) {
+    if (L + chunk + 2 > zeroed) {
+      const int upto = L + chunk + 2;
+      HIP_TRY(hipMemsetAsync(b->act + (size_t)zeroed * NW, 0, sizeof(uint64_t) * (size_t)(upto - zeroed) * NW, s));
+      zeroed = upto;
+    }
+    for (int j = 0; j < chunk; ++j) {
+      const int lv = L + j;
+      uint64_t *frp = b->fr[(lv - 1) & 1], *frc = b->fr[lv & 1];
+      const uint32_t mark_id = b->epoch + (uint32_t)lv;
+      hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, frc,
+                         b->q, b->qw, b->qoff, b->ctl, b->sp_v, b->sp_cap, b->qcap, lv, mark_id);
+      hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
+                         b->mark, b->q, b->qw, b->ctl, b->sp_v, b->sp_cap, b->qcap, mark_id);
+      hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+      hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp, b->sig, b->q,
+                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_cap, lv, b->qcap);
+      hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
+                         b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_cap);
+      hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, frc,
+                         b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap);
+    }
+    HIP_TRY(hipGetLastError());
+    L += chunk;
+    int rc;
+    if ((rc = pull_bctl(c))) return rc;
+    if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
+    if (b->h_ctl->done) break;
+    if ((uint64_t)L > (uint64_t)n + 1) return fail(AXBC_ESTATE, "batch forward did not terminate%s (line %d)", "", __LINE__);
+    chunk *= 2;
+  }
+  const int empty = (int)b->h_ctl->empty_lev;
+  const int dmax = empty - 1;  // deepest non-empty level over the batch
+  b->lev_guess = dmax + 2;
+  b->epoch += (uint32_t)(L + 1);
+  // per-root deepest level from the active words: roots whose deepest
+  // level is 1 also accumulate their own dependency (bc_func_mono's do-while)
+  HIP_TRY(hipMemcpyAsync(b->h_act, b->act, sizeof(uint64_t) * (size_t)(empty + 1) * NW, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  uint64_t rootmask[2] = {0, 0};
+  std::vector<int> nlev(nb, 0);
+  for (int k = 0; k < nb; ++k) {
+    for (int lv = 1; lv <= dmax; ++lv)
+      if ((b->h_act[(size_t)lv * NW + k / 64] >> (k % 64)) & 1ull) nlev[k] = lv;
+    if (nlev[k] == 1) rootmask[k / 64] |= 1ull << (k % 64);
+  }
+  // backward: depth d = dmax-1 .. 0; before depth d the level d+1 bits
+  // leave VIS.  Depth 0 only for the roots with a single level.
+  for (int d = dmax - 1; d >= 0; --d) {
+    hipLaunchKernelGGL(k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, b->vis, b->q, b->qw, b->qoff, d + 1);
+    if (d == 0 && !(rootmask[0] | rootmask[1])) break;
+    hipLaunchKernelGGL(k_bdep<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
+                       c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, d,
+                       rootmask[0], NW > 1 ? rootmask[1] : 0ull);
+    hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt,
+                       b->xr, c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cnt, b->sp_cap, b->part, d,
+                       rootmask[0], NW > 1 ? rootmask[1] : 0ull);
+    hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+  }
+  HIP_TRY(hipGetLastError());
+  // exact stop test per root, then the bc update for the applied roots
+  int nr = nb;
+  if (stop_c > 0 && c->ncheck > 0) {
+    unsigned long long *okb = (unsigned long long *)b->part;  // scratch, rows are free now
+    std::vector<unsigned long long> ones(NW, ~0ull);
+    HIP_TRY(hipMemcpyAsync(okb, ones.data(), sizeof(unsigned long long) * NW, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_bstop<NW>, dim3(grid_for(c->ncheck, 64)), dim3(kBlock), 0, s, c->check, c->ncheck, n, c->bc,
+                       (double)((uint64_t)stop_c * (uint64_t)n), nb, b->src, b->prob, b->r0, b->dlt, b->cnt, okb);
+    unsigned long long okh[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(okh, okb, sizeof(unsigned long long) * NW, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < nb; ++k)
+      if ((okh[k / 64] >> (k % 64)) & 1ull) {
+        nr = k + 1;
+        break;
+      }
+  }
+  hipLaunchKernelGGL(k_bupdate<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, nr, b->src,
+                     b->prob, b->r0, b->dlt, b->cnt, c->bc);
+  HIP_TRY(hipGetLastError());
+  if (applied) *applied = nr;
+  if (stats) {
+    HIP_TRY(hipMemcpyAsync(b->h_cnt, b->cnt, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < nb; ++k) {
+      stats[k].nvisited = b->h_cnt[4 * k + 0];
+      stats[k].nlevels = (uint64_t)nlev[k];
+      stats[k].e_r = b->h_cnt[4 * k + 2];
+      stats[k].e_dag = b->h_cnt[4 * k + 3];
+      stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * k + 1] * (c->heuristic == 1 || c->heuristic == 3) + r0[k]);
+    }
+  }
+  return AXBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int axbc_set_batch_width(axbc_ctx *c, int width) {
+  if (!c || (width != 64 && width != 128)) return fail(AXBC_EINVAL, "batch width must be 64 or 128%s (line %d)", "", __LINE__);
+  if (c->batch && c->batch->B != width) batch_free(c);
+  c->batch_width = width;
+  return AXBC_OK;
+}
+
+int axbc_batch_width(axbc_ctx *c) { return c ? c->batch_width : 0; }
+
+int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
+               axbc_pivot_stats *stats) {
+  if (!c || !v0s || !ps) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  if (nb <= 0 || nb > c->batch_width)
+    return fail(AXBC_EINVAL, "batch size must be 1..width%s (line %d)", "", __LINE__);
+  for (int k = 0; k < nb; ++k)
+    if (v0s[k] >= c->n) return fail(AXBC_EINVAL, "root out of range%s (line %d)", "", __LINE__);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc;
+  if (!c->batch) {
+    // list capacity: every vertex once per distinct level among the roots;
+    // start at 4N and grow on overflow
+    uint64_t cap = (uint64_t)c->n * 4 + 1024;
+    if (cap > 0xFFFFFFF0ull) cap = 0xFFFFFFF0ull;
+    if ((rc = batch_create(c, c->batch_width, (uint32_t)cap))) return rc;
+  }
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    rc = c->batch->nw == 1 ? run_batch<1>(c, v0s, ps, nb, stop_c, applied, stats)
+                           : run_batch<2>(c, v0s, ps, nb, stop_c, applied, stats);
+    if (rc != AXBC_ENOMEM || !c->batch || !c->batch->h_ctl->overflow) return rc;
+    // grow the level list and retry the batch from scratch (bc untouched:
+    // the update runs only after a complete sweep)
+    const uint64_t cap = (uint64_t)c->batch->qcap * 2;
+    const int w = c->batch->B;
+    batch_free(c);
+    if ((rc = batch_create(c, w, (uint32_t)(cap > 0xFFFFFFF0ull ? 0xFFFFFFF0ull : cap)))) return rc;
+  }
+  return fail(AXBC_ENOMEM, "batch list capacity%s (line %d)", "", __LINE__);
+}
+
+int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
+  if (!c || !c->batch || !delta || k < 0 || k >= c->batch->B)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  double *tmp = nullptr;
+  HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * c->n));
+  if (c->batch->nw == 1)
+    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, k, c->batch->dlt, tmp);
+  else
+    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, k, c->batch->dlt, tmp);
+  HIP_TRY(hipMemcpyAsync(delta, tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(tmp));
+  return AXBC_OK;
+}
+
+}  // extern "C"

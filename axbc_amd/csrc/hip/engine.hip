@@ -33,41 +33,28 @@
 #include <vector>
 
 #include "../../../include/axbc.h"
+#include "engine_internal.h"
 #include "wave.h"
 
 using namespace axbc;
 
-namespace {
-
-constexpr int kBlock = 256;
-constexpr int kExpandGrid = 1024;   // workgroups of the level kernels
-constexpr int kHubGrid = 512;       // workgroups of the hub passes
-constexpr uint32_t kHubDeg = 2048;  // degree above which a vertex is a hub
-
-struct DevCtl {
-  uint32_t tail;       // queue length of the current round
-  uint32_t done;       // forward finished
-  uint32_t empty_lev;  // first empty level (valid when done)
-  uint32_t nhub;       // hubs deferred in the current level
-  uint32_t wg_done;    // last-workgroup counter
-  uint32_t all;        // reach sum of visited non-root vertices
-  uint32_t stop_any;   // stop test: a check vertex below threshold
-  uint32_t pad;
-  unsigned long long e_r, e_dag;
-};
-
+namespace axbc {
 thread_local char g_err[512] = "";
 
-int fail(int code, const char *fmt, const char *a = "", int line = 0) {
+int fail(int code, const char *fmt, const char *a, int line) {
   snprintf(g_err, sizeof(g_err), fmt, a, line);
   return code;
 }
 
-#define HIP_TRY(call)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess) return fail(AXBC_EHIP, "HIP error %s (line %d)", hipGetErrorString(e_), __LINE__); \
-  } while (0)
+int grid_for(uint64_t n, int cap) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > (uint64_t)cap) g = cap;
+  return (int)g;
+}
+}  // namespace axbc
+
+namespace {
 
 __device__ __forceinline__ uint32_t ld_acq(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -430,41 +417,6 @@ __global__ void k_degrees(const uint32_t *__restrict__ col, uint32_t n, uint32_t
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// Context
-// ---------------------------------------------------------------------------
-
-struct axbc_ctx {
-  int device = 0;
-  uint32_t n = 0;
-  uint64_t ned = 0;
-  int heuristic = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  size_t bytes = 0;
-  // graph
-  uint32_t *col = nullptr, *row = nullptr, *reach = nullptr, *rlist = nullptr;
-  uint32_t nr = 0;
-  std::vector<uint32_t> h_reach;
-  // state
-  int *lvl = nullptr;
-  uint32_t *sigma = nullptr;
-  double *delta = nullptr, *bc = nullptr;
-  uint32_t *queue = nullptr, *qoff = nullptr, *hubs = nullptr, *hubcnt = nullptr;
-  double *hub_part = nullptr;
-  uint32_t nhub_cap = 0;
-  DevCtl *ctl = nullptr;
-  DevCtl *h_ctl = nullptr;  // pinned mirror
-  uint32_t *check = nullptr;
-  uint32_t ncheck = 0;
-  // round bookkeeping
-  int lev_guess = 8;
-  bool have_round = false;
-  uint32_t last_v0 = 0;
-  int last_nlev = 0;
-  float t_fwd = 0, t_bwd = 0, t_upd = 0;
-};
-
 namespace {
 
 template <typename T>
@@ -474,13 +426,6 @@ int dalloc(axbc_ctx *c, T **p, size_t count) {
   if (e != hipSuccess) return fail(AXBC_ENOMEM, "hipMalloc of %s bytes failed (line %d)", "device", __LINE__);
   c->bytes += b;
   return AXBC_OK;
-}
-
-int grid_for(uint64_t n, int cap = 2048) {
-  uint64_t g = (n + kBlock - 1) / kBlock;
-  if (g < 1) g = 1;
-  if (g > (uint64_t)cap) g = cap;
-  return (int)g;
 }
 
 int pull_ctl(axbc_ctx *c) {
@@ -574,6 +519,7 @@ int axbc_ctx_create(int device, uint64_t N, uint64_t ned, const uint32_t *col, c
       if (reach[v]) rl.push_back((uint32_t)v);
   }
   c->nr = (uint32_t)rl.size();
+  c->h_col.assign(col, col + N + 1);
   TRY_ALLOC(c->col, N + 1);
   TRY_ALLOC(c->row, ned);
   TRY_ALLOC(c->reach, N);
@@ -624,6 +570,12 @@ int axbc_ctx_destroy(axbc_ctx *c) {
   if (!c) return AXBC_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  sampler_free(c);
+  batch_free(c);
+  if (!c->symmetric) {
+    if (c->rcol) (void)hipFree(c->rcol);
+    if (c->rrow) (void)hipFree(c->rrow);
+  }
   void *ptrs[] = {c->col,   c->row,  c->reach, c->rlist,  c->lvl,      c->sigma, c->delta, c->bc,
                   c->queue, c->qoff, c->hubs,  c->hubcnt, c->hub_part, c->ctl,   c->check};
   for (void *p : ptrs)

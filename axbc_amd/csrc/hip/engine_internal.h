@@ -1,0 +1,103 @@
+// Internal definitions shared by the engine's HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <vector>
+
+#include "../../../include/axbc.h"
+
+namespace axbc {
+
+constexpr int kBlock = 256;
+constexpr int kExpandGrid = 1024;   // workgroups of the level kernels
+constexpr int kHubGrid = 512;       // workgroups of the hub passes
+constexpr uint32_t kHubDeg = 2048;  // degree above which a vertex is a hub
+
+struct DevCtl {
+  uint32_t tail;       // queue length of the current round
+  uint32_t done;       // forward finished
+  uint32_t empty_lev;  // first empty level (valid when done)
+  uint32_t nhub;       // hubs deferred in the current level
+  uint32_t wg_done;    // last-workgroup counter
+  uint32_t all;        // reach sum of visited non-root vertices
+  uint32_t stop_any;   // stop test: a check vertex below threshold
+  uint32_t overflow;   // a bounded list ran out of room
+  unsigned long long e_r, e_dag;
+};
+
+// Device buffers of the samplers (allocated on first use).
+struct SamplerBufs {
+  void *tmp = nullptr;    // N x 8 bytes: deg^2 (u32) or fixed-point weights (u64)
+  void *u32a = nullptr;   // N x 4: wrapped deg^2 prefix
+  void *u32b = nullptr;   // N x 4: its running maximum
+  void *u32c = nullptr;   // N x 4: neighbour-degree sums
+  void *u64a = nullptr;   // N x 8: weight prefix
+  void *tiles = nullptr;  // scan tile totals
+  unsigned long long *wmax = nullptr;
+  uint32_t *pick = nullptr;
+  uint32_t *pick_host = nullptr;  // pinned
+  uint64_t *host64 = nullptr;     // pinned, 2 values
+  uint64_t total = 0;
+  int hdn_ready = 0, weights_ready = 0;
+};
+
+struct BatchState;  // batch.hip
+
+thread_local extern char g_err[512];
+int fail(int code, const char *fmt, const char *a = "", int line = 0);
+
+#define HIP_TRY(call)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return ::axbc::fail(AXBC_EHIP, "HIP error %s (line %d)", hipGetErrorString(e_), __LINE__); \
+  } while (0)
+
+}  // namespace axbc
+
+struct axbc_ctx {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t ned = 0;
+  int heuristic = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t bytes = 0;
+  // graph
+  uint32_t *col = nullptr, *row = nullptr, *reach = nullptr, *rlist = nullptr;
+  uint32_t *rcol = nullptr, *rrow = nullptr;  // transpose (pull direction); alias when symmetric
+  bool symmetric = true;
+  uint32_t nr = 0;
+  std::vector<uint32_t> h_reach;
+  std::vector<uint32_t> h_col;
+  // state of the plain rounds
+  int *lvl = nullptr;
+  uint32_t *sigma = nullptr;
+  double *delta = nullptr, *bc = nullptr;
+  uint32_t *queue = nullptr, *qoff = nullptr, *hubs = nullptr, *hubcnt = nullptr;
+  double *hub_part = nullptr;
+  uint32_t nhub_cap = 0;
+  axbc::DevCtl *ctl = nullptr;
+  axbc::DevCtl *h_ctl = nullptr;  // pinned mirror
+  uint32_t *check = nullptr;
+  uint32_t ncheck = 0;
+  // round bookkeeping
+  int lev_guess = 8;
+  bool have_round = false;
+  uint32_t last_v0 = 0;
+  int last_nlev = 0;
+  // samplers and batches
+  axbc::SamplerBufs *sampler = nullptr;
+  axbc::BatchState *batch = nullptr;
+  int batch_width = 64;
+  size_t budget = 0;  // 0: no limit
+};
+
+namespace axbc {
+int sampler_bufs(axbc_ctx *c, SamplerBufs **out);
+void sampler_free(axbc_ctx *c);
+void batch_free(axbc_ctx *c);
+int grid_for(uint64_t n, int cap = 2048);
+}  // namespace axbc

@@ -1,0 +1,323 @@
+// Pivot samplers on the device: distributions built with device prefix
+// scans, draws resolved by a search over the scanned distribution.
+//
+//  - high-degree-neighbourhood tables (-z 1, src/axbc2d.c:1758-1845,
+//    2758-2786, 2895-2928): pref[v] = wrapping uint32 prefix of
+//    (uint32)deg(v)^2, its running maximum pmax (so that the reference's
+//    "first index whose prefix >= r" over a non-monotone wrapped prefix
+//    becomes a binary search over a monotone array), and sdn[v] = wrapping
+//    sum of the neighbours' degrees.
+//  - dynamic distributions (-z 3 bc, -z 4 delta of the last round, -z 5
+//    1/(1+bc); src/axbc2d.c:2929-3060) as 64-bit fixed-point weights
+//    W[v] = floor(w[v] * 2^s), one shift s = 61 - e for the whole array
+//    where wmax * N = m * 2^e, so that the prefix is exact in any order.
+//    A draw r in [0, total] picks the first v with prefix(v) >= r.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../../include/axbc.h"
+#include "engine_internal.h"
+#include "wave.h"
+
+using namespace axbc;
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+struct AddU64 {
+  __device__ static uint64_t op(uint64_t a, uint64_t b) { return a + b; }
+  __device__ static uint64_t id() { return 0; }
+};
+struct AddU32 {
+  __device__ static uint32_t op(uint32_t a, uint32_t b) { return a + b; }
+  __device__ static uint32_t id() { return 0; }
+};
+struct MaxU32 {
+  __device__ static uint32_t op(uint32_t a, uint32_t b) { return a > b ? a : b; }
+  __device__ static uint32_t id() { return 0; }
+};
+
+// Block-wide inclusive scan of one value per thread (Hillis-Steele in LDS).
+template <typename T, typename Op>
+__device__ T block_incl_scan(T x, T *sh) {
+  const int t = threadIdx.x;
+  sh[t] = x;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    T y = t >= off ? sh[t - off] : Op::id();
+    __syncthreads();
+    if (t >= off) sh[t] = Op::op(y, sh[t]);
+    __syncthreads();
+  }
+  T r = sh[t];
+  __syncthreads();
+  return r;
+}
+
+// Pass 1: per-tile reduction.
+template <typename T, typename Op>
+__global__ __launch_bounds__(kScanBlock) void k_tile_reduce(const T *__restrict__ in, uint64_t n, T *tiles) {
+  __shared__ T sh[kScanBlock];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  T acc = Op::id();
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) acc = Op::op(acc, in[base + k]);
+  T inc = block_incl_scan<T, Op>(acc, sh);
+  if (threadIdx.x == kScanBlock - 1) tiles[blockIdx.x] = inc;
+}
+
+// Pass 2: exclusive scan of the tile totals in one block (any count).
+template <typename T, typename Op>
+__global__ __launch_bounds__(kScanBlock) void k_tiles_scan(T *tiles, uint32_t ntiles) {
+  __shared__ T sh[kScanBlock];
+  __shared__ T prev[kScanBlock];
+  T carry = Op::id();
+  for (uint32_t b = 0; b < ntiles; b += kScanBlock) {
+    const uint32_t i = b + threadIdx.x;
+    const T x = i < ntiles ? tiles[i] : Op::id();
+    const T inc = block_incl_scan<T, Op>(x, sh);
+    prev[threadIdx.x] = inc;
+    __syncthreads();
+    const T before = threadIdx.x ? prev[threadIdx.x - 1] : Op::id();
+    const T total = prev[kScanBlock - 1];
+    if (i < ntiles) tiles[i] = Op::op(carry, before);
+    __syncthreads();
+    carry = Op::op(carry, total);
+  }
+}
+
+// Pass 3: inclusive scan inside each tile seeded with the tile's prefix.
+template <typename T, typename Op>
+__global__ __launch_bounds__(kScanBlock) void k_tile_scan(const T *__restrict__ in, uint64_t n,
+                                                          const T *__restrict__ tiles, T *out) {
+  __shared__ T sh[kScanBlock];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  T v[kScanItems];
+  T acc = Op::id();
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : Op::id();
+    acc = Op::op(acc, v[k]);
+  }
+  T inc = block_incl_scan<T, Op>(acc, sh);
+  __shared__ T prev[kScanBlock];
+  prev[threadIdx.x] = inc;
+  __syncthreads();
+  T run = Op::op(tiles[blockIdx.x], threadIdx.x ? prev[threadIdx.x - 1] : Op::id());
+  for (int k = 0; k < kScanItems; ++k) {
+    run = Op::op(run, v[k]);
+    if (base + k < n) out[base + k] = run;
+  }
+}
+
+template <typename T, typename Op>
+int device_scan(const T *in, T *out, uint64_t n, T *tiles, hipStream_t s) {
+  const uint32_t ntiles = (uint32_t)((n + kScanTile - 1) / kScanTile);
+  if (ntiles == 0) return 0;
+  hipLaunchKernelGGL((k_tile_reduce<T, Op>), dim3(ntiles), dim3(kScanBlock), 0, s, in, n, tiles);
+  hipLaunchKernelGGL((k_tiles_scan<T, Op>), dim3(1), dim3(kScanBlock), 0, s, tiles, ntiles);
+  hipLaunchKernelGGL((k_tile_scan<T, Op>), dim3(ntiles), dim3(kScanBlock), 0, s, in, n, tiles, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---- high-degree-neighbourhood tables ------------------------------------
+
+__global__ void k_hdn_terms(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row, uint32_t n,
+                            uint32_t *deg2, uint32_t *sdn) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t b = col[v], e = col[v + 1];
+    const uint64_t d = e - b;
+    deg2[v] = (uint32_t)(d * d);  // (LOCINT)pow(deg, 2): exact square, low 32 bits
+    uint32_t s = 0;
+    for (uint32_t k = b; k < e; ++k) {
+      const uint32_t w = row[k];
+      s += col[w + 1] - col[w];
+    }
+    sdn[v] = s;
+  }
+}
+
+// ---- dynamic fixed-point weights -----------------------------------------
+
+__device__ __forceinline__ double weight_of(int kind, uint32_t v, const double *__restrict__ bc,
+                                            const double *__restrict__ delta) {
+  double w = kind == 3 ? bc[v] : kind == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
+  return (isfinite(w) && w > 0.0) ? w : 0.0;
+}
+
+__global__ void k_weight_max(int kind, const double *__restrict__ bc, const double *__restrict__ delta,
+                             uint32_t n, unsigned long long *wmax_bits) {
+  double m = 0.0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    m = fmax(m, weight_of(kind, v, bc, delta));
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  // non-negative doubles order like their bit patterns
+  if (lane_id() == 0) atomicMax(wmax_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_weight_fixed(int kind, const double *__restrict__ bc, const double *__restrict__ delta,
+                               uint32_t n, const unsigned long long *__restrict__ wmax_bits, uint64_t *W) {
+  const double wmax = __longlong_as_double((long long)*wmax_bits);
+  int e = 0;
+  frexp(wmax * (double)n, &e);
+  const int s = 61 - e;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const double w = wmax > 0.0 ? weight_of(kind, v, bc, delta) : 0.0;
+    W[v] = w > 0.0 ? (uint64_t)ldexp(w, s) : 0ull;
+  }
+}
+
+// First index with P[i] >= r over a non-decreasing array; n-1 when none.
+template <typename T>
+__global__ void k_first_geq(const T *__restrict__ P, uint32_t n, T r, uint32_t *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t lo = 0, hi = n;  // answer in [lo, hi]
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (P[mid] >= r)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  *out = lo < n ? lo : n - 1;
+}
+
+int grid_n(uint64_t n) {
+  uint64_t g = (n + 255) / 256;
+  return (int)(g < 1 ? 1 : g > 4096 ? 4096 : g);
+}
+
+}  // namespace
+
+namespace axbc {
+
+int sampler_bufs(axbc_ctx *c, SamplerBufs **out) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  if (c->sampler) {
+    *out = c->sampler;
+    return AXBC_OK;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  SamplerBufs *sb = new SamplerBufs();
+  c->sampler = sb;
+  const size_t n = c->n;
+  const size_t ntiles = (n + kScanTile - 1) / kScanTile + 1;
+  HIP_TRY(hipMalloc(&sb->tmp, 8 * n));
+  HIP_TRY(hipMalloc(&sb->u32a, 4 * n));
+  HIP_TRY(hipMalloc(&sb->u32b, 4 * n));
+  HIP_TRY(hipMalloc(&sb->u32c, 4 * n));
+  HIP_TRY(hipMalloc(&sb->u64a, 8 * n));
+  HIP_TRY(hipMalloc(&sb->tiles, 8 * ntiles));
+  HIP_TRY(hipMalloc((void **)&sb->wmax, sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc((void **)&sb->pick, sizeof(uint32_t)));
+  HIP_TRY(hipHostMalloc((void **)&sb->pick_host, sizeof(uint32_t), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void **)&sb->host64, 2 * sizeof(uint64_t), hipHostMallocDefault));
+  c->bytes += 28 * n + 8 * ntiles;
+  *out = sb;
+  return AXBC_OK;
+}
+
+void sampler_free(axbc_ctx *c) {
+  SamplerBufs *sb = c->sampler;
+  if (!sb) return;
+  void *d[] = {sb->tmp, sb->u32a, sb->u32b, sb->u32c, sb->u64a, sb->tiles, sb->wmax, sb->pick};
+  for (void *p : d)
+    if (p) (void)hipFree(p);
+  if (sb->pick_host) (void)hipHostFree(sb->pick_host);
+  if (sb->host64) (void)hipHostFree(sb->host64);
+  delete sb;
+  c->sampler = nullptr;
+}
+
+}  // namespace axbc
+
+extern "C" {
+
+int axbc_hdn_build(axbc_ctx *c, uint32_t *pmax_host, uint32_t *sdn_host, uint32_t *sum) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  if (!sum) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  const uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  uint32_t *deg2 = (uint32_t *)sb->tmp, *pref = (uint32_t *)sb->u32a, *pmax = (uint32_t *)sb->u32b;
+  uint32_t *sdn = (uint32_t *)sb->u32c;
+  hipLaunchKernelGGL(k_hdn_terms, dim3(grid_n(n)), dim3(256), 0, s, c->col, c->row, n, deg2, sdn);
+  if (device_scan<uint32_t, AddU32>(deg2, pref, n, (uint32_t *)sb->tiles, s))
+    return fail(AXBC_EHIP, "scan launch failed%s (line %d)", "", __LINE__);
+  if (device_scan<uint32_t, MaxU32>(pref, pmax, n, (uint32_t *)sb->tiles, s))
+    return fail(AXBC_EHIP, "scan launch failed%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(&sb->host64[0], pref + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (pmax_host) HIP_TRY(hipMemcpyAsync(pmax_host, pmax, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  if (sdn_host) HIP_TRY(hipMemcpyAsync(sdn_host, sdn, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *sum = (uint32_t)(sb->host64[0] & 0xFFFFFFFFull);
+  sb->hdn_ready = 1;
+  return AXBC_OK;
+}
+
+int axbc_hdn_pick(axbc_ctx *c, uint32_t r, uint32_t *u, uint32_t *sdn_u) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  if (!sb->hdn_ready || !u) return fail(AXBC_ESTATE, "tables not built%s (line %d)", "", __LINE__);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL((k_first_geq<uint32_t>), dim3(1), dim3(64), 0, s, (const uint32_t *)sb->u32b, c->n, r,
+                     sb->pick);
+  HIP_TRY(hipMemcpyAsync(sb->pick_host, sb->pick, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *u = *sb->pick_host;
+  if (sdn_u) {
+    HIP_TRY(hipMemcpyAsync(sb->pick_host, (const uint32_t *)sb->u32c + *u, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *sdn_u = *sb->pick_host;
+  }
+  return AXBC_OK;
+}
+
+int axbc_weights(axbc_ctx *c, int kind, uint64_t *total) {
+  if (kind != 3 && kind != 4 && kind != 5) return fail(AXBC_EINVAL, "weight kind%s (line %d)", "", __LINE__);
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  const uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  uint64_t *W = (uint64_t *)sb->tmp, *P = (uint64_t *)sb->u64a;
+  HIP_TRY(hipMemsetAsync(sb->wmax, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_weight_max, dim3(grid_n(n)), dim3(256), 0, s, kind, c->bc, c->delta, n, sb->wmax);
+  hipLaunchKernelGGL(k_weight_fixed, dim3(grid_n(n)), dim3(256), 0, s, kind, c->bc, c->delta, n, sb->wmax, W);
+  if (device_scan<uint64_t, AddU64>(W, P, n, (uint64_t *)sb->tiles, s))
+    return fail(AXBC_EHIP, "scan launch failed%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(&sb->host64[0], P + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  sb->total = sb->host64[0];
+  if (total) *total = sb->total;
+  sb->weights_ready = 1;
+  return AXBC_OK;
+}
+
+int axbc_weights_pick(axbc_ctx *c, uint64_t r, uint32_t *v, double *p) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  if (!sb->weights_ready || !v || !p) return fail(AXBC_ESTATE, "weights not built%s (line %d)", "", __LINE__);
+  hipStream_t s = c->stream;
+  const uint64_t *W = (const uint64_t *)sb->tmp, *P = (const uint64_t *)sb->u64a;
+  hipLaunchKernelGGL((k_first_geq<uint64_t>), dim3(1), dim3(64), 0, s, P, c->n, r, sb->pick);
+  HIP_TRY(hipMemcpyAsync(sb->pick_host, sb->pick, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *v = *sb->pick_host;
+  HIP_TRY(hipMemcpyAsync(&sb->host64[1], W + *v, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *p = (double)sb->host64[1] / (double)sb->total;
+  return AXBC_OK;
+}
+
+}  // extern "C"

@@ -239,6 +239,9 @@ int64_t axbc_btrd_binomial(int64_t n_in, double p, axbc_mrg *st) {
     double v = axbc_mrg_next_double(st);
     if (v <= urvr) {
       double u = v / vr - .43;
+      /* The paper's step 1 divides by (0.5 - |u|); the reference generator
+       * divides by (0.5 + |u|), which narrows the variates (variance about
+       * 0.8 n p q).  Kept, since the graphs must equal the reference's. */
       return (int)floor((2 * a / (.5 + fabs(u)) + b) * u + c);
     }
     /* step 2 */

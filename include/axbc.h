@@ -124,6 +124,48 @@ int axbc_backward_depth(axbc_ctx *ctx, int depth);
 int axbc_reach_sum(axbc_ctx *ctx, uint32_t *all);
 int axbc_update_bc(axbc_ctx *ctx, uint32_t v0, double p, uint64_t nvisited);
 
+/*
+ * Pivot samplers on the device.
+ *
+ * High-degree-neighbourhood tables (-z 1; src/axbc2d.c:1758-1845,
+ * 2758-2786): builds on the device the wrapping uint32 prefix of
+ * (uint32)deg^2, its running maximum and the neighbour-degree sums; *sum is
+ * the wrapped total (sum_deg).  pmax_host / sdn_host (N entries each, may
+ * be NULL) receive copies.  axbc_hdn_pick returns the first vertex whose
+ * wrapped prefix reaches r, as findsample_LOCINT (axbc2d.c:2187-2198) with
+ * its "N-1 when none" fallback, and optionally its neighbour-degree sum.
+ */
+int axbc_hdn_build(axbc_ctx *ctx, uint32_t *pmax_host, uint32_t *sdn_host, uint32_t *sum);
+int axbc_hdn_pick(axbc_ctx *ctx, uint32_t r, uint32_t *u, uint32_t *sdn_u);
+
+/*
+ * Dynamic distributions (-z 3 bc, 4 delta of the last round, 5 1/(1+bc);
+ * src/axbc2d.c:2929-3060) as 64-bit fixed-point weights
+ * W[v] = floor(w[v] * 2^s) with one shift s for the array, scanned on the
+ * device; *total = sum W.  axbc_weights_pick returns the first v whose
+ * prefix reaches r and p = W[v] / total.
+ */
+int axbc_weights(axbc_ctx *ctx, int kind, uint64_t *total);
+int axbc_weights_pick(axbc_ctx *ctx, uint64_t r, uint32_t *v, double *p);
+
+/*
+ * Bit-packed multi-source rounds: nb (1..width) roots in one sweep, width
+ * 64 or 128 (axbc_set_batch_width; default 64).  Results are those of nb
+ * consecutive axbc_pivot calls in the given order: the bc update applies
+ * the roots' contributions vertex by vertex in root order.  With stop_c > 0
+ * and a check list (axbc_set_checks), the adaptive stop test
+ * (src/axbc2d.c:2846-2870) is evaluated after every root of the batch and
+ * only the roots up to the first one after which it holds are applied;
+ * *applied receives their count.  stats (nb entries) may be NULL.
+ * axbc_batch_delta copies root k's delta of the last batch.
+ * No reference counterpart: the reference runs one root per round.
+ */
+int axbc_set_batch_width(axbc_ctx *ctx, int width);
+int axbc_batch_width(axbc_ctx *ctx);
+int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
+               int *applied, axbc_pivot_stats *stats);
+int axbc_batch_delta(axbc_ctx *ctx, int k, double *delta);
+
 /* Last round's counters (waits for the context). */
 int axbc_last_stats(axbc_ctx *ctx, axbc_pivot_stats *stats);
 

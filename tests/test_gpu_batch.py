@@ -1,0 +1,112 @@
+"""Bit-packed multi-source rounds against the CPU oracle: a batch of roots
+must give the same per-root delta and the same accumulated bc as the
+sequence of one-root oracle rounds (delta/bc within 1e-9 relative), and the
+same per-root counters."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from axbc_amd import Engine, Graph, _ffi
+from oracle import oracle
+from tests.test_gpu_plain import close
+
+pytestmark = pytest.mark.gpu
+
+
+def run_batch(e, roots, probs, stop_c=0):
+    L = _ffi.hip()
+    v = np.ascontiguousarray(roots, np.uint32)
+    p = np.ascontiguousarray(probs, np.float64)
+    st = (_ffi.PivotStats * len(v))()
+    applied = C.c_int(0)
+    rc = L.axbc_batch(e.ctx, v, p, len(v), stop_c, C.byref(applied), C.cast(st, C.c_void_p))
+    assert rc == 0, L.axbc_last_error()
+    return applied.value, [s.as_dict() for s in st]
+
+
+def delta_of(e, k):
+    out = np.empty(e.N, np.float64)
+    assert _ffi.hip().axbc_batch_delta(e.ctx, k, out) == 0
+    return out
+
+
+def check_batch(g, roots, width, label, batches=1):
+    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)}", flush=True)
+    bc_ref = np.zeros(g.N)
+    probs = np.full(len(roots), 1.0 / g.N)
+    with Engine(g) as e:
+        assert _ffi.hip().axbc_set_batch_width(e.ctx, width) == 0
+        step = (len(roots) + batches - 1) // batches
+        for b0 in range(0, len(roots), step):
+            rb = roots[b0:b0 + step]
+            applied, stats = run_batch(e, rb, probs[b0:b0 + step])
+            assert applied == len(rb)
+            for k, v0 in enumerate(rb):
+                lvl, sig, dl, bc_ref, ost = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
+                ok, err = close(delta_of(e, k), dl)
+                assert ok, f"{label} root #{b0 + k} ({v0}): delta rel err {err}"
+                for key in ("nvisited", "nlevels", "e_r", "e_dag", "all"):
+                    assert stats[k][key] == ost[key], f"{label} root {v0}: {key} {stats[k][key]} != {ost[key]}"
+            ok, err = close(e.bc(), bc_ref)
+            assert ok, f"{label} after batch at {b0}: bc rel err {err}"
+
+
+def test_require(require_gpu):
+    pass
+
+
+@pytest.mark.parametrize("width", [64, 128])
+@pytest.mark.parametrize("scale,seed,undirected,heur", [
+    (10, (1, 2), True, 0),
+    (11, (23, 24), True, 1),
+    (10, (3, 4), False, 0),
+])
+def test_kronecker_batches(require_gpu, width, scale, seed, undirected, heur):
+    g = Graph.kronecker(scale, 16, seed=seed, undirected=undirected, heuristic=heur)
+    deg = g.degrees()
+    rng = np.random.default_rng(scale + width)
+    cand = np.flatnonzero(deg > 0)
+    roots = rng.choice(cand, size=min(len(cand), width + width // 2), replace=False)
+    check_batch(g, roots, width, f"kron s{scale} H{heur} U{int(not undirected)}", batches=2)
+
+
+def test_split_vertices_and_long_paths(require_gpu):
+    # a star centre with 5000 leaves plus a long path: split chunks in push,
+    # sigma and dependency passes, and ~2000 levels
+    n = 7000
+    e = [(0, i) for i in range(1, 5000)] + [(i, i + 1) for i in range(4999, n - 1)]
+    g = Graph.from_edges(n, np.array(e))
+    roots = np.array([0, 1, 2, 4999, 6999, 5500, 3], np.uint32)
+    check_batch(g, roots, 64, "star+path")
+
+
+def test_partial_batch_and_single_level_roots(require_gpu):
+    n = 40  # star: the centre's round has a single level
+    g = Graph.from_edges(n, np.array([(0, i) for i in range(1, n)]))
+    check_batch(g, np.array([0, 5, 7], np.uint32), 64, "star")
+
+
+def test_stop_inside_batch(require_gpu):
+    g = Graph.kronecker(10, 16, seed=(1, 2))
+    deg = g.degrees()
+    rng = np.random.default_rng(3)
+    roots = rng.choice(np.flatnonzero(deg > 0), size=64, replace=False)
+    probs = np.full(64, 1.0 / g.N)
+    checks = np.array([int(np.argmax(deg))], np.uint32)  # a hub: satisfied early
+    c = 1
+    # the oracle sequence: stop after the first root k with bc[a]/2 >= c*N
+    bc_ref = np.zeros(g.N)
+    stop_k = None
+    for k, v0 in enumerate(roots):
+        _, _, _, bc_ref, _ = oracle.pivot(g, int(v0), probs[k], bc_ref)
+        if bc_ref[checks[0]] / 2.0 >= c * g.N:
+            stop_k = k
+            break
+    assert stop_k is not None and stop_k < 63
+    with Engine(g) as e:
+        e.set_checks(checks)
+        applied, _ = run_batch(e, roots, probs, stop_c=c)
+        assert applied == stop_k + 1
+        ok, err = close(e.bc(), bc_ref)
+        assert ok, err

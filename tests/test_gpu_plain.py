@@ -1,0 +1,73 @@
+"""Plain one-root-per-round HIP path against the CPU oracle.
+
+Per round: lvl and sigma bit-exact (int32 / uint32 with wrap-around), delta
+and the accumulated bc within 1e-9 relative (FP64 on both sides; the device
+sums dependencies in a different order).
+"""
+import numpy as np
+import pytest
+
+from axbc_amd import Engine, Graph
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-9
+
+
+def close(a, b, rtol=RTOL):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(a), np.abs(b))
+    ok = (a == b) | (np.abs(a - b) <= rtol * scale)
+    return bool(ok.all()), (np.abs(a - b) / np.where(scale > 0, scale, 1)).max() if a.size else 0.0
+
+
+def check_rounds(g, roots, label):
+    print(f"[{label}] N={g.N} ned={g.ned} roots={len(roots)}", flush=True)
+    bc_ref = np.zeros(g.N)
+    with Engine(g) as e:
+        for k, v0 in enumerate(roots):
+            p = 1.0 / g.N
+            st = e.pivot(int(v0), p, stats=True)
+            lvl, sig, dl, bc_ref, ost = oracle.pivot(g, int(v0), p, bc_ref)
+            assert np.array_equal(e.lvl(), lvl), f"{label} round {k} root {v0}: lvl differs"
+            assert np.array_equal(e.sigma(), sig), f"{label} round {k} root {v0}: sigma differs"
+            ok, err = close(e.delta(), dl)
+            assert ok, f"{label} round {k} root {v0}: delta rel err {err}"
+            ok, err = close(e.bc(), bc_ref)
+            assert ok, f"{label} round {k} root {v0}: bc rel err {err}"
+            for key in ("nvisited", "nlevels", "e_r", "e_dag", "all"):
+                assert st[key] == ost[key], f"{label} round {k}: {key} {st[key]} != {ost[key]}"
+
+
+def test_require(require_gpu):
+    pass
+
+
+@pytest.mark.parametrize("scale,ef,seed,undirected,heur", [
+    (10, 16, (1, 2), True, 0),
+    (12, 16, (23, 24), True, 1),
+    (12, 8, (3, 4), False, 0),
+])
+def test_kronecker_rounds(require_gpu, scale, ef, seed, undirected, heur):
+    g = Graph.kronecker(scale, ef, seed=seed, undirected=undirected, heuristic=heur)
+    deg = g.degrees()
+    rng = np.random.default_rng(scale * 7 + heur)
+    cand = np.flatnonzero(deg > 0)
+    roots = list(rng.choice(cand, size=8, replace=False)) + [int(np.argmax(deg))]
+    check_rounds(g, roots, f"kron s{scale} ef{ef} H{heur} U{int(not undirected)}")
+
+
+def test_hub_paths(require_gpu):
+    # star + clique core: the centre exceeds the hub degree threshold, so
+    # both forward and backward hub passes run
+    n = 6000
+    edges = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]
+    g = Graph.from_edges(n, np.array(edges), undirected=True)
+    check_rounds(g, [0, 1, 2999, n - 1], "star-path")
+
+
+def test_path_depths(require_gpu):
+    n = 3000
+    g = Graph.from_edges(n, np.array([(i, i + 1) for i in range(n - 1)]), undirected=True)
+    check_rounds(g, [0, n // 2, n - 1], "long path")

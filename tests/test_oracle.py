@@ -1,0 +1,241 @@
+"""Pins of the CPU oracle (oracle/brandes_oracle.c) by brute force, closed
+forms and invariants, so that the oracle can judge the HIP path."""
+import math
+
+import numpy as np
+import pytest
+
+from axbc_amd import Graph
+from oracle import oracle
+from tests import bruteforce as bf
+
+
+def rand_graph(rng, n, m, undirected=True):
+    e = rng.integers(0, n, size=(m, 2))
+    return Graph.from_edges(n, e, undirected=undirected)
+
+
+@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("undirected", [True, False])
+def test_bruteforce_small(seed, undirected):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(3, 12))
+    g = rand_graph(rng, n, int(rng.integers(n, 3 * n)), undirected)
+    adj = bf.adjacency(g.N, g.col, g.row)
+    bc_expect = np.zeros(n)
+    bc = np.zeros(n)
+    for s in range(n):
+        if g.col[s + 1] == g.col[s]:
+            continue
+        p = 1.0 / n
+        lvl, sig, dl, bc, st = oracle.pivot(g, s, p, bc)
+        dist, cnt = bf.bfs(adj, s)
+        assert list(lvl) == dist
+        assert [int(x) for x in sig] == [c % (1 << 32) for c in cnt]
+        dep = bf.dependencies(adj, s)
+        nlev = max(dist)
+        for v in range(n):
+            if v == s:
+                # the root's dependency is only accumulated when the BFS
+                # has a single level (bc_func_mono's do-while); then every
+                # level-1 vertex contributes (0 + 1) / 1 * 1
+                exp = float(sum(1 for t in range(n) if dist[t] == 1)) if nlev == 1 else 0.0
+                assert dl[v] == exp, (s, v)
+            else:
+                assert dl[v] == pytest.approx(dep[v], rel=1e-12, abs=1e-12), (s, v)
+                bc_expect[v] += dep[v] / p
+        assert st["nvisited"] == sum(1 for d in dist if d >= 0)
+        assert st["nlevels"] == nlev
+    np.testing.assert_allclose(bc, bc_expect, rtol=1e-12, atol=1e-9)
+
+
+def path(n):
+    return Graph.from_edges(n, np.array([(i, i + 1) for i in range(n - 1)]))
+
+
+def test_path_closed_form():
+    n = 50
+    g = path(n)
+    lvl, sig, dl, _, st = oracle.pivot(g, 0, 1.0)
+    assert list(lvl) == list(range(n))
+    assert (sig == 1).all()
+    assert list(dl[1:]) == [float(n - 1 - i) for i in range(1, n)]
+    assert dl[0] == 0.0  # more than one level: the root keeps reach = 0
+    assert st["e_dag"] == n - 1 and st["e_r"] == 2 * (n - 1)
+
+
+def test_star_closed_form():
+    n = 40
+    g = Graph.from_edges(n, np.array([(0, i) for i in range(1, n)]))
+    # from the centre: one level, the root's delta is the do-while case
+    lvl, sig, dl, _, st = oracle.pivot(g, 0, 1.0)
+    assert st["nlevels"] == 1 and dl[0] == n - 1 and (dl[1:] == 0).all()
+    # from a leaf: the centre lies on the paths to the other n-2 leaves
+    lvl, sig, dl, _, _ = oracle.pivot(g, 5, 1.0)
+    assert dl[0] == n - 2
+
+
+def test_cycle_antipode_two_paths():
+    k = 9
+    n = 2 * k
+    g = Graph.from_edges(n, np.array([(i, (i + 1) % n) for i in range(n)]))
+    lvl, sig, dl, _, _ = oracle.pivot(g, 0, 1.0)
+    assert lvl[k] == k and sig[k] == 2
+    assert (sig >= 1).all()
+    # every non-antipodal vertex at distance j carries the half of the
+    # antipode's paths plus the k-1-j vertices behind it
+    for j in range(1, k):
+        assert dl[j] == pytest.approx((k - 1 - j) + 0.5)
+
+
+def test_complete_graph():
+    n = 12
+    e = [(i, j) for i in range(n) for j in range(i + 1, n)]
+    g = Graph.from_edges(n, np.array(e))
+    lvl, sig, dl, _, st = oracle.pivot(g, 3, 1.0)
+    assert (sig == 1).all() and st["nlevels"] == 1
+    assert (dl[np.arange(n) != 3] == 0).all()
+
+
+def test_ladder_binomial_counts():
+    k = 20  # 2 x k grid: paths from a corner to (r, c) = binom(c + r, r)
+    idx = lambda r, c: r * k + c  # noqa: E731
+    e = [(idx(r, c), idx(r, c + 1)) for r in range(2) for c in range(k - 1)]
+    e += [(idx(0, c), idx(1, c)) for c in range(k)]
+    g = Graph.from_edges(2 * k, np.array(e))
+    lvl, sig, _, _, _ = oracle.pivot(g, idx(0, 0), 1.0)
+    for c in range(k):
+        assert sig[idx(0, c)] == 1
+        assert sig[idx(1, c)] == c + 1
+
+
+def test_sigma_wraps_mod_2_32():
+    # layers of a vertices, consecutive layers completely connected: sigma of
+    # layer j is a^j, which passes 2^32 and must wrap like the uint32 atomics
+    a, layers = 3, 24
+    e = []
+    for j in range(layers - 1):
+        for x in range(a):
+            for y in range(a):
+                e.append((1 + j * a + x, 1 + (j + 1) * a + y))
+    e += [(0, 1 + x) for x in range(a)]
+    n = 1 + layers * a
+    g = Graph.from_edges(n, np.array(e))
+    lvl, sig, dl, _, _ = oracle.pivot(g, 0, 1.0)
+    for j in range(layers):
+        for x in range(a):
+            assert int(sig[1 + j * a + x]) == pow(a, j, 1 << 32)
+    assert a ** (layers - 1) > (1 << 32)
+
+
+@pytest.mark.parametrize("scale,seed", [(9, (1, 2)), (10, (23, 24))])
+def test_dependency_sum_invariant(scale, seed):
+    # sum_v delta_s(v) = sum_t (d(s,t) - 1) over reached t != s
+    g = Graph.kronecker(scale, 16, seed=seed)
+    deg = g.degrees()
+    for s in np.flatnonzero(deg > 0)[:6]:
+        lvl, sig, dl, _, st = oracle.pivot(g, int(s), 1.0)
+        reached = lvl > 0
+        expect = float((lvl[reached] - 1).sum())
+        got = float(dl[np.arange(g.N) != s].sum())
+        assert got == pytest.approx(expect, rel=1e-9)
+
+
+def test_lim_x_sample_wraps():
+    L = oracle.lib()
+    assert L.oracle_lim_x_sample(1 << 12) == 1 << 24
+    assert L.oracle_lim_x_sample(1 << 16) == 0
+    assert L.oracle_lim_x_sample((1 << 24) - 13) == ((((1 << 24) - 13) ** 2) % (1 << 32))
+
+
+def test_fixed_point_weights_exact_and_monotone():
+    rng = np.random.default_rng(5)
+    w = rng.random(1000) * 1e3
+    w[::7] = 0.0
+    W = np.zeros(1000, np.uint64)
+    total = oracle.lib().oracle_fixed_weights(1000, w, W)
+    assert total == int(W.astype(object).sum())
+    assert total < (1 << 63)
+    # the shift is one power of two for the whole array
+    scale = 2.0 ** round(math.log2(float(W.max()) / w.max()))
+    frac = w * scale - W.astype(np.float64)
+    assert ((frac > -1e-3) & (frac < 1.0 + 1e-3)).all()  # W = floor(w * 2^s)
+
+
+def test_weighted_sample_first_reaching_prefix():
+    W = np.array([0, 5, 0, 10, 5], np.uint64)
+    p = oracle.C.c_double()
+    L = oracle.lib()
+    total = 20
+    # draw 0 -> r = 0 -> first prefix >= 0 is index 0 (weight 0)
+    assert L.oracle_weighted_sample(5, W, total, 0, oracle.C.byref(p)) == 0
+    # draw just below half -> r ~ 10 -> prefix [0,5,5,15,20] reaches at 3
+    d = int(2147483647 * 0.5) - 1
+    assert L.oracle_weighted_sample(5, W, total, d, oracle.C.byref(p)) == 3
+    assert p.value == 10 / 20
+
+
+def test_hdn_sample_probability():
+    # path 0-1-2: deg = 1,2,1; deg^2 prefix = 1,5,6
+    g = path(3)
+    p = oracle.C.c_double()
+    v = oracle.lib().oracle_hdn_sample(3, g.col, g.row, 0, oracle.C.byref(p))
+    # r = 0 -> u = 0 (degree 1) -> its only neighbour 1
+    assert v == 1 and p.value == pytest.approx(2 / 6)
+
+
+def test_run_uniform_draw_order():
+    # Pivot loop with c = 0 (no stop test): x check draws, one discarded
+    # initial draw, then one lrand48() % (N+1) per attempt; a repeated root
+    # is redrawn inside the same iteration (LIMxSAMPLE = 2^20 at N = 2^10),
+    # a degree-0 root consumes the iteration.  The draws come from libc's
+    # lrand48, itself pinned against the POSIX recurrence in
+    # test_lrand48_posix_recurrence.
+    import ctypes
+    g = Graph.kronecker(10, 16, seed=(1, 2))
+    nbfs, x = 60, 3
+    pv, pr, bc, rs = oracle.run(g, nbfs, c=0, x=x, z=0)
+    libc = ctypes.CDLL(None)
+    libc.lrand48.restype = ctypes.c_long
+    libc.srand48(ctypes.c_long(4343543))
+    N = g.N
+    for _ in range(x + 1):
+        libc.lrand48()
+    deg = g.degrees()
+    seen, expect, skipped = set(), [], 0
+    for _ in range(nbfs):
+        v = libc.lrand48() % (N + 1)
+        while v < N and v in seen:
+            v = libc.lrand48() % (N + 1)
+        if v >= N or deg[v] == 0:
+            skipped += 1
+            if v < N:
+                seen.add(v)
+            continue
+        seen.add(v)
+        expect.append(v)
+    assert list(pv) == expect
+    assert rs["skipped"] == skipped and rs["nrounds"] == len(expect)
+    assert (pr == 1.0 / N).all()
+
+
+def test_lrand48_posix_recurrence():
+    # X_{n+1} = (0x5DEECE66D X_n + 0xB) mod 2^48, srand48: X = seed << 16 |
+    # 0x330E, lrand48 = X >> 17
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.lrand48.restype = ctypes.c_long
+    libc.srand48(ctypes.c_long(4343543))
+    X = (4343543 << 16) | 0x330E
+    for _ in range(1000):
+        X = (0x5DEECE66D * X + 0xB) % (1 << 48)
+        assert libc.lrand48() == X >> 17
+
+
+def test_stop_test_stops():
+    g = Graph.kronecker(8, 8, seed=(1, 2))
+    pv, pr, bc, rs = oracle.run(g, g.N, c=1, x=1, z=0)
+    # with c = 1 and one check vertex the loop may stop early; whenever it
+    # stopped the criterion holds
+    if rs["stopped"]:
+        assert math.isfinite(bc.sum())
