@@ -79,11 +79,21 @@ def hip() -> C.CDLL:
         "axbc_last_timing": (C.c_int, [ctxp, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                        C.POINTER(C.c_float)]),
     }
-    optional = {
-        "axbc_batch": (C.c_int, [ctxp, u32p, f64p, C.c_int, C.c_void_p]),
+    sig.update({
+        "axbc_batch": (C.c_int, [ctxp, u32p, f64p, C.c_int, C.c_uint32, C.POINTER(C.c_int), C.c_void_p]),
         "axbc_batch_width": (C.c_int, [ctxp]),
         "axbc_set_batch_width": (C.c_int, [ctxp, C.c_int]),
-    }
+        "axbc_batch_delta": (C.c_int, [ctxp, C.c_int, f64p]),
+        "axbc_hdn_build": (C.c_int, [ctxp, u32p, u32p, C.POINTER(C.c_uint32)]),
+        "axbc_hdn_pick": (C.c_int, [ctxp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "axbc_weights": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_uint64)]),
+        "axbc_weights_pick": (C.c_int, [ctxp, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+        "axbc_kernel_timing": (C.c_int, [ctxp, C.c_int]),
+        "axbc_kernel_stats": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]),
+        "axbc_copy_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
+    })
+    optional = {}
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype, f.argtypes = res, args

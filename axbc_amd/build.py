@@ -86,6 +86,32 @@ def build_driver() -> str | None:
     return out
 
 
+def build_legacy() -> str:
+    """libaxbc_legacy.so: the reference's device entry points; its globals
+    N and heuristic resolve against the host program that loads it."""
+    out = os.path.join(LIB, "libaxbc_legacy.so")
+    _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-I", INCLUDE,
+          os.path.join(HOST_SRC, "legacy.c"), "-o", out + ".tmp", "-L", LIB, "-laxbc_hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_legacy_test_host() -> str:
+    """Test host of the legacy layer (tests/legacy), built in-tree so the GPU
+    tests only run it."""
+    src = os.path.join(ROOT, "tests", "legacy", "legacy_host.c")
+    if not os.path.exists(src):
+        return ""
+    bindir = os.path.join(ROOT, "tests", "legacy", "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "legacy_host")
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", INCLUDE, src, "-o", out + ".tmp", "-L", LIB,
+          "-laxbc_legacy", "-laxbc_hip", "-Wl,-rpath," + os.path.relpath(LIB, bindir).join(["$ORIGIN/", ""])])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir(os.path.join(REFERENCE, "generator")):
@@ -103,6 +129,8 @@ def build_all() -> None:
         fo.result()
         fr.result()
     build_driver()
+    build_legacy()
+    build_legacy_test_host()
 
 
 if __name__ == "__main__":

@@ -273,9 +273,10 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
                                                   uint64_t *fr_clear, uint32_t *q, const uint64_t *__restrict__ qw,
                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                   uint32_t *sp_v, uint32_t sp_cap, uint32_t qcap, int L,
-                                                  uint32_t mark_id) {
+                                                  uint32_t mark_id, unsigned long long *kb) {
   if (ldg_acq(&ctl->done)) return;
   const int lane = lane_id();
+  unsigned long long bytes = 0;
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
   // clear the dense frontier buffer of level L-2 (it receives level L)
   if (L >= 2) {
@@ -309,6 +310,9 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
     }
     const uint32_t incl = wave_incl_scan(d);
     const uint32_t total = __shfl(incl, kWave - 1);
+    // entries: id 4 + offsets 8 + word 8NW; per edge: id 4 + VIS 8NW
+    bytes += (unsigned long long)__popcll(__ballot(i < fe)) * (12 + 8 * NW) +
+             (unsigned long long)total * (4 + 8 * NW);
     for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
       const uint32_t e = e0 + lane;
       const int own = wave_owner(incl, e) & (kWave - 1);
@@ -336,9 +340,11 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
           else
             atomicOr(&ctl->overflow, 1u);
         }
+        bytes += 4ull * (unsigned long long)__popcll(m);  // appended ids
       }
     }
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
 // Frontier vertices with more than kChunk edges: every wave takes one chunk
@@ -402,7 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restri
 template <int NW>
 __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t ce, const uint32_t *__restrict__ rrow,
                                             const uint64_t *__restrict__ frp, const Words<NW> &nw,
-                                            const uint32_t *__restrict__ sig, uint32_t acc[NW], uint32_t dag[NW]) {
+                                            const uint32_t *__restrict__ sig, uint32_t acc[NW], uint32_t dag[NW],
+                                            uint32_t *contrib) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
   for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
@@ -425,6 +432,7 @@ __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t c
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         const uint64_t mm = __shfl(m.w[j], i);
+        *contrib += (uint32_t)__popcll(mm);
         if (bit(mm, lane)) {
           acc[j] += sig[(size_t)uu * B + lane + 64 * j];
           ++dag[j];
@@ -440,9 +448,10 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
                                                    uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t *sp_e, uint32_t sp_cap, int L,
-                                                   uint32_t qcap) {
+                                                   uint32_t qcap, unsigned long long *kb) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
+  unsigned long long bytes = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -466,13 +475,20 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
     uint32_t acc[NW], dag[NW];
 #pragma unroll
     for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
-    sigma_chunk<NW>(st, 0, d, rrow, frp, nw, sig, acc, dag);
+    uint32_t contrib = 0;
+    sigma_chunk<NW>(st, 0, d, rrow, frp, nw, sig, acc, dag, &contrib);
+    uint32_t roots = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
+      roots += (uint32_t)__popcll(nw.w[j]);
       if (bit(nw.w[j], lane)) sig[(size_t)w * B + lane + 64 * j] = acc[j];
       dag_tot[j] += dag[j];
     }
+    // vertex: id 4 + offsets 8 + word 8NW; per edge: id 4 + frontier word
+    // 8NW; per term: sigma 4; per root reached: sigma write 4
+    bytes += 12 + 8 * NW + (unsigned long long)d * (4 + 8 * NW) + 4ull * contrib + 4ull * roots;
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 #pragma unroll
   for (int j = 0; j < NW; ++j)
     if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
@@ -505,7 +521,8 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
       uint32_t acc[NW], dag[NW];
 #pragma unroll
       for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
-      sigma_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), rrow, frp, nw, sig, acc, dag);
+      uint32_t contrib = 0;
+      sigma_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), rrow, frp, nw, sig, acc, dag, &contrib);
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         if (acc[j]) atomicAdd(&sig[(size_t)w * B + lane + 64 * j], acc[j]);
@@ -613,11 +630,12 @@ __global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const ui
 // Dependencies of one vertex over adj[cb, ce), per root lane, left fold in
 // adjacency order into acc.
 template <int NW>
-__device__ __forceinline__ void dep_chunk(uint32_t st, uint32_t cb, uint32_t ce, const uint32_t *__restrict__ row,
-                                          const uint64_t *__restrict__ vis, const Words<NW> &f,
-                                          const double *__restrict__ xr, const double sv[NW], double acc[NW]) {
+__device__ __forceinline__ uint32_t dep_chunk(uint32_t st, uint32_t cb, uint32_t ce, const uint32_t *__restrict__ row,
+                                              const uint64_t *__restrict__ vis, const Words<NW> &f,
+                                              const double *__restrict__ xr, const double sv[NW], double acc[NW]) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
+  uint32_t contrib = 0;  // (edge, root) terms, wave-uniform
   for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
     const uint32_t k = k0 + lane;
     uint32_t w = 0;
@@ -638,10 +656,12 @@ __device__ __forceinline__ void dep_chunk(uint32_t st, uint32_t cb, uint32_t ce,
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         const uint64_t mm = __shfl(m.w[j], i);
+        contrib += (uint32_t)__popcll(mm);
         if (bit(mm, lane)) acc[j] += xr[(size_t)ww * B + lane + 64 * j] * sv[j];
       }
     }
   }
+  return contrib;
 }
 
 template <int NW>
@@ -651,9 +671,11 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
                                                  const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
-                                                 int depth, uint64_t root_mask0, uint64_t root_mask1) {
+                                                 int depth, uint64_t root_mask0, uint64_t root_mask1,
+                                                 unsigned long long *kb) {
   constexpr int B = 64 * NW;
   const uint32_t b = qoff[depth], e = qoff[depth + 1];
+  unsigned long long bytes = 0;
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
   for (uint32_t i = b + gw; i < e; i += nwv) {
@@ -687,14 +709,21 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
       sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
       acc[j] = r;
     }
-    dep_chunk<NW>(st, 0, d, row, vis, f, xr, sv, acc);
+    const uint32_t contrib = dep_chunk<NW>(st, 0, d, row, vis, f, xr, sv, acc);
+    uint32_t roots = 0;
 #pragma unroll
-    for (int j = 0; j < NW; ++j)
+    for (int j = 0; j < NW; ++j) {
+      roots += (uint32_t)__popcll(f.w[j]);
       if (bit(f.w[j], lane)) {
         dlt[(size_t)v * B + lane + 64 * j] = acc[j];
         xr[(size_t)v * B + lane + 64 * j] = (acc[j] + 1.0) / sv[j];
       }
+    }
+    // entry: id 4 + word 8NW + offsets 8 + reach 4; per edge: id 4 + VIS 8NW;
+    // per term: x 8; per root of the entry: sigma 4 + delta 8 + x 8
+    bytes += 16 + 8 * NW + (unsigned long long)d * (4 + 8 * NW) + 8ull * contrib + 20ull * roots;
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
 template <int NW>
@@ -739,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restric
       sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
       acc[j] = 0.0;
     }
-    dep_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), row, vis, f, xr, sv, acc);
+    (void)dep_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), row, vis, f, xr, sv, acc);
 #pragma unroll
     for (int j = 0; j < NW; ++j) part[(size_t)item * B + lane + 64 * j] = acc[j];
     __threadfence();
@@ -1039,17 +1068,23 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       const int lv = L + j;
       uint64_t *frp = b->fr[(lv - 1) & 1], *frc = b->fr[lv & 1];
       const uint32_t mark_id = b->epoch + (uint32_t)lv;
+      kt_begin(c, KK_PUSH);
       hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, frc,
-                         b->q, b->qw, b->qoff, b->ctl, b->sp_v, b->sp_cap, b->qcap, lv, mark_id);
+                         b->q, b->qw, b->qoff, b->ctl, b->sp_v, b->sp_cap, b->qcap, lv, mark_id, kt_bytes(c, KK_PUSH));
+      kt_end(c, KK_PUSH);
       hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
                          b->mark, b->q, b->qw, b->ctl, b->sp_v, b->sp_cap, b->qcap, mark_id);
       hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+      kt_begin(c, KK_SIGMA);
       hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp, b->sig, b->q,
-                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_cap, lv, b->qcap);
+                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_cap, lv, b->qcap, kt_bytes(c, KK_SIGMA));
+      kt_end(c, KK_SIGMA);
       hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
                          b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_cap);
+      kt_begin(c, KK_CLOSE);
       hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, frc,
                          b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap);
+      kt_end(c, KK_CLOSE);
     }
     HIP_TRY(hipGetLastError());
     L += chunk;
@@ -1080,9 +1115,11 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   for (int d = dmax - 1; d >= 0; --d) {
     hipLaunchKernelGGL(k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, b->vis, b->q, b->qw, b->qoff, d + 1);
     if (d == 0 && !(rootmask[0] | rootmask[1])) break;
+    kt_begin(c, KK_DEP);
     hipLaunchKernelGGL(k_bdep<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
                        c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, d,
-                       rootmask[0], NW > 1 ? rootmask[1] : 0ull);
+                       rootmask[0], NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP));
+    kt_end(c, KK_DEP);
     hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt,
                        b->xr, c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cnt, b->sp_cap, b->part, d,
                        rootmask[0], NW > 1 ? rootmask[1] : 0ull);
@@ -1106,8 +1143,10 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
         break;
       }
   }
+  kt_begin(c, KK_UPDATE);
   hipLaunchKernelGGL(k_bupdate<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, nr, b->src,
                      b->prob, b->r0, b->dlt, b->cnt, c->bc);
+  kt_end(c, KK_UPDATE);
   HIP_TRY(hipGetLastError());
   if (applied) *applied = nr;
   if (stats) {

@@ -46,6 +46,54 @@ int fail(int code, const char *fmt, const char *a, int line) {
   return code;
 }
 
+static hipEvent_t kt_event(axbc_ctx *c) {
+  KernelTimer &t = c->ktimer;
+  if (!t.pool.empty()) {
+    hipEvent_t e = t.pool.back();
+    t.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void kt_begin(axbc_ctx *c, int kind) {
+  if (!c->ktimer.on) return;
+  hipEvent_t a = kt_event(c), b = kt_event(c);
+  (void)hipEventRecord(a, c->stream);
+  c->ktimer.pending.push_back({kind, {a, b}});
+}
+
+void kt_end(axbc_ctx *c, int kind) {
+  if (!c->ktimer.on || c->ktimer.pending.empty()) return;
+  auto &pr = c->ktimer.pending.back();
+  if (pr.first != kind) return;
+  (void)hipEventRecord(pr.second.second, c->stream);
+}
+
+unsigned long long *kt_bytes(axbc_ctx *c, int kind) {
+  return c->ktimer.on && c->ktimer.dbytes ? c->ktimer.dbytes + kind : nullptr;
+}
+
+// Fold finished event pairs into the per-kind totals.
+static int kt_collect(axbc_ctx *c) {
+  KernelTimer &t = c->ktimer;
+  if (t.pending.empty()) return AXBC_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto &pr : t.pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
+      t.ms[pr.first] += ms;
+      t.launches[pr.first] += 1;
+    }
+    t.pool.push_back(pr.second.first);
+    t.pool.push_back(pr.second.second);
+  }
+  t.pending.clear();
+  return AXBC_OK;
+}
+
 int grid_for(uint64_t n, int cap) {
   uint64_t g = (n + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
@@ -410,6 +458,11 @@ __global__ void k_stop_test(const uint32_t *__restrict__ check, uint32_t x, uint
   if (__ballot(below) && lane_id() == 0) atomicOr(&ctl->stop_any, 1u);
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy(const uint4 *__restrict__ src, uint4 *dst, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_degrees(const uint32_t *__restrict__ col, uint32_t n, uint32_t *deg) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     deg[v] = col[v + 1] - col[v];
@@ -435,8 +488,10 @@ int pull_ctl(axbc_ctx *c) {
 }
 
 int enqueue_forward(axbc_ctx *c, int L) {
+  kt_begin(c, KK_FWD);
   hipLaunchKernelGGL(k_fwd_expand, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
                      c->lvl, c->sigma, c->queue, c->qoff, c->ctl, c->hubs, L);
+  kt_end(c, KK_FWD);
   hipLaunchKernelGGL(k_fwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
                      c->sigma, c->queue, c->qoff, c->ctl, c->hubs, L);
   HIP_TRY(hipGetLastError());
@@ -444,8 +499,10 @@ int enqueue_forward(axbc_ctx *c, int L) {
 }
 
 int enqueue_backward(axbc_ctx *c, int depth) {
+  kt_begin(c, KK_BWD);
   hipLaunchKernelGGL(k_bwd_pull, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
                      c->lvl, c->sigma, c->delta, c->reach, c->queue, c->qoff, c->ctl, c->hubs, depth);
+  kt_end(c, KK_BWD);
   hipLaunchKernelGGL(k_bwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
                      c->sigma, c->delta, c->reach, c->ctl, c->hubs, c->hub_part, c->hubcnt, depth);
   HIP_TRY(hipGetLastError());
@@ -572,6 +629,12 @@ int axbc_ctx_destroy(axbc_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   sampler_free(c);
   batch_free(c);
+  for (auto &pr : c->ktimer.pending) {
+    (void)hipEventDestroy(pr.second.first);
+    (void)hipEventDestroy(pr.second.second);
+  }
+  for (hipEvent_t e : c->ktimer.pool) (void)hipEventDestroy(e);
+  if (c->ktimer.dbytes) (void)hipFree(c->ktimer.dbytes);
   if (!c->symmetric) {
     if (c->rcol) (void)hipFree(c->rcol);
     if (c->rrow) (void)hipFree(c->rrow);
@@ -655,6 +718,64 @@ int axbc_last_timing(axbc_ctx *c, float *f, float *b, float *u) {
   return AXBC_OK;
 }
 
+int axbc_kernel_timing(axbc_ctx *c, int enable) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  KernelTimer &t = c->ktimer;
+  if (enable) {
+    if (!t.dbytes) HIP_TRY(hipMalloc((void **)&t.dbytes, sizeof(unsigned long long) * KK_COUNT));
+    HIP_TRY(hipMemsetAsync(t.dbytes, 0, sizeof(unsigned long long) * KK_COUNT, c->stream));
+    int rc = kt_collect(c);
+    if (rc) return rc;
+    for (int k = 0; k < KK_COUNT; ++k) {
+      t.ms[k] = 0;
+      t.launches[k] = 0;
+    }
+  } else {
+    int rc = kt_collect(c);
+    if (rc) return rc;
+  }
+  t.on = enable != 0;
+  return AXBC_OK;
+}
+
+int axbc_kernel_stats(axbc_ctx *c, int kind, double *ms, uint64_t *launches, uint64_t *bytes) {
+  if (!c || kind < 0 || kind >= KK_COUNT) return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  int rc = kt_collect(c);
+  if (rc) return rc;
+  unsigned long long b = 0;
+  if (c->ktimer.dbytes)
+    HIP_TRY(hipMemcpy(&b, c->ktimer.dbytes + kind, sizeof(b), hipMemcpyDeviceToHost));
+  if (ms) *ms = c->ktimer.ms[kind];
+  if (launches) *launches = c->ktimer.launches[kind];
+  if (bytes) *bytes = b;
+  return AXBC_OK;
+}
+
+int axbc_copy_kernel(axbc_ctx *c, uint64_t bytes, int repeats) {
+  if (!c || bytes < 16) return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  void *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, bytes));
+  HIP_TRY(hipMalloc(&b, bytes));
+  HIP_TRY(hipMemsetAsync(a, 1, bytes, c->stream));
+  for (int r = 0; r < repeats; ++r) {
+    kt_begin(c, KK_COPY);
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(kBlock), 0, c->stream, (const uint4 *)a, (uint4 *)b, bytes / 16);
+    kt_end(c, KK_COPY);
+  }
+  if (unsigned long long *kb = kt_bytes(c, KK_COPY)) {
+    const unsigned long long add = 2ull * (bytes / 16) * 16 * (unsigned long long)repeats;
+    unsigned long long cur = 0;
+    HIP_TRY(hipMemcpyAsync(&cur, kb, sizeof(cur), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    cur += add;
+    HIP_TRY(hipMemcpyAsync(kb, &cur, sizeof(cur), hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(a));
+  HIP_TRY(hipFree(b));
+  return AXBC_OK;
+}
+
 int axbc_sync(axbc_ctx *c) {
   if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -695,6 +816,42 @@ int axbc_set_bc(axbc_ctx *c, const double *bc) {
   HIP_TRY(hipMemcpyAsync(c->bc, bc, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AXBC_OK;
+}
+
+int axbc_set_reach(axbc_ctx *c, const uint32_t *reach) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  std::vector<uint32_t> rl;
+  if (reach) {
+    memcpy(c->h_reach.data(), reach, sizeof(uint32_t) * c->n);
+    for (uint32_t v = 0; v < c->n; ++v)
+      if (reach[v]) rl.push_back(v);
+  } else {
+    std::fill(c->h_reach.begin(), c->h_reach.end(), 0u);
+  }
+  if (c->rlist) (void)hipFree(c->rlist);
+  c->rlist = nullptr;
+  c->nr = (uint32_t)rl.size();
+  HIP_TRY(hipMalloc((void **)&c->rlist, sizeof(uint32_t) * (rl.empty() ? 1 : rl.size())));
+  if (!rl.empty())
+    HIP_TRY(hipMemcpyAsync(c->rlist, rl.data(), sizeof(uint32_t) * rl.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->reach, c->h_reach.data(), sizeof(uint32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+  // the reset state holds delta = reach
+  hipLaunchKernelGGL(k_init_state, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->reach, c->lvl,
+                     c->sigma, c->delta, c->bc);
+  c->have_round = false;
+  batch_free(c);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+void *axbc_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void axbc_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
 }
 
 int axbc_set_checks(axbc_ctx *c, const uint32_t *check, uint32_t x) {

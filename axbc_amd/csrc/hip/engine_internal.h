@@ -45,6 +45,30 @@ struct SamplerBufs {
 
 struct BatchState;  // batch.hip
 
+// Per-kernel instrumentation for the roofline figures: HIP events around
+// every launch of the selected kernel kinds, and algorithmic byte counts
+// accumulated by the kernels themselves (counted only while enabled).
+enum KernelKind {
+  KK_PUSH = 0,      // batched forward marking (k_bpush)
+  KK_SIGMA = 1,     // batched path-count pull (k_bsigma)
+  KK_CLOSE = 2,     // batched level close (k_bclose)
+  KK_DEP = 3,       // batched dependency pull (k_bdep)
+  KK_UPDATE = 4,    // batched bc update (k_bupdate)
+  KK_FWD = 5,       // plain forward expansion (k_fwd_expand)
+  KK_BWD = 6,       // plain dependency pull (k_bwd_pull)
+  KK_COPY = 7,      // calibration copy kernel
+  KK_COUNT = 8
+};
+
+struct KernelTimer {
+  bool on = false;
+  unsigned long long *dbytes = nullptr;  // device: KK_COUNT counters
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[KK_COUNT] = {0};
+  uint64_t launches[KK_COUNT] = {0};
+};
+
 thread_local extern char g_err[512];
 int fail(int code, const char *fmt, const char *a = "", int line = 0);
 
@@ -93,6 +117,7 @@ struct axbc_ctx {
   axbc::BatchState *batch = nullptr;
   int batch_width = 64;
   size_t budget = 0;  // 0: no limit
+  axbc::KernelTimer ktimer;
 };
 
 namespace axbc {
@@ -100,4 +125,9 @@ int sampler_bufs(axbc_ctx *c, SamplerBufs **out);
 void sampler_free(axbc_ctx *c);
 void batch_free(axbc_ctx *c);
 int grid_for(uint64_t n, int cap = 2048);
+// Event pair around one launch of kind k when instrumentation is on.
+void kt_begin(axbc_ctx *c, int kind);
+void kt_end(axbc_ctx *c, int kind);
+// Device counter for kind k, or nullptr when instrumentation is off.
+unsigned long long *kt_bytes(axbc_ctx *c, int kind);
 }  // namespace axbc

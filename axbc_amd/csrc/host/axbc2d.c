@@ -1,0 +1,461 @@
+/*
+ * axbc2d -- command-line driver of the MI355X sampled-Brandes engine.
+ *
+ * Same flags and pivot-loop semantics as AxBC's main() at -p 1x1
+ * (src/axbc2d.c:2214-3223): graph from the Kronecker generator (-S/-E) or a
+ * text edge list (-f/-n), optional 1-degree reduction (-H 1), pivots drawn
+ * with srand48/lrand48 in the reference's order by the chosen heuristic
+ * (-z), adaptive stop test (-c/-x), -N attempts, per-pivot log line and
+ * result table, -o result file.  The per-pivot work runs on the GPU
+ * through libaxbc_hip.so (include/axbc.h).
+ *
+ * Flags of this engine only:
+ *   -B W   roots per batch: 0 one root per round (default), 64 or 128
+ *   -s S   generator seed pair (S, S+1) (default: the reference's 23, 24)
+ *   -T T   host threads for graph construction
+ *   -q     no per-pivot lines
+ *   -G f   write the generated graph as a text edge list to f and exit
+ */
+#define _GNU_SOURCE
+#include <getopt.h>
+#include <inttypes.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include <time.h>
+#include <unistd.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "axbc.h"
+#include "axbc_host.h"
+
+#define DEFAULT_SEED 4343543
+
+static double now_s(void) {
+  struct timeval t;
+  gettimeofday(&t, NULL);
+  return (double)t.tv_sec + 1e-6 * (double)t.tv_usec;
+}
+
+static void die(const char *fmt, const char *arg) {
+  fprintf(stderr, fmt, arg);
+  fputc('\n', stderr);
+  exit(EXIT_FAILURE);
+}
+
+static void check(int rc, const char *what) {
+  if (rc != AXBC_OK) {
+    fprintf(stderr, "%s failed (%d): %s\n", what, rc, axbc_last_error());
+    exit(EXIT_FAILURE);
+  }
+}
+
+static void usage(const char *p) {
+  fprintf(stderr,
+          "Usage:\n"
+          "\t %s -p 1x1 [-o outfile] [-D] [-d] [-N <# of searches>]\n"
+          "\t -> to visit a graph read from file:\n"
+          "\t\t -f <graph file> -n <# vertices> [-r <start vert>]\n"
+          "\t -> to visit an RMAT graph:\n"
+          "\t\t -S <scale> [-E <edge factor>]\n"
+          "\t Where:\n"
+          "\t\t -c <c> -x <x>  adaptive stop: bc/2 >= c*N on x check vertices\n"
+          "\t\t -z <0..5>      pivot distribution\n"
+          "\t\t -H <0|1>       1-degree reduction\n"
+          "\t\t -U             DO NOT make graph Undirected\n"
+          "\t\t -R <0|1>       fixed / time-based seed\n"
+          "\t\t -B <0|64|128>  roots per GPU batch\n",
+          p);
+  exit(EXIT_FAILURE);
+}
+
+/* uniform_int(max, min) = lrand48() % (max - min + 1) + min in uint32
+ * (src/axbc2d.c:2156-2164); an empty modulus uses the raw draw. */
+static uint32_t uniform_int(uint32_t max, uint32_t min) {
+  uint32_t m = max - min + 1u;
+  long r = lrand48();
+  return m ? (uint32_t)((uint32_t)r % m) + min : (uint32_t)r + min;
+}
+
+/* first i with pmax[i] >= r; N-1 when none (findsample_LOCINT + fallback) */
+static uint32_t first_geq(const uint32_t *pmax, uint32_t n, uint32_t r) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (pmax[mid] >= r)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo < n ? lo : n - 1;
+}
+
+typedef struct {
+  uint32_t v0;
+  double p;
+  uint64_t ui;
+  uint32_t mix;
+} round_t;
+
+int main(int argc, char **argv) {
+  int R = 0, C = 0, scale = 21, edgef = 16, gread = -1, heuristic = 0, undirected = 1, random_mode = 0;
+  int dump = 0, debug = 0, analyze = 0, quiet = 0, width = 0, nthreads = 0;
+  uint64_t N = 0, nbfs = 1, startv = 0, seed_s = 0;
+  uint32_t bader = 1, approx_n = 1;
+  int distribution = 0, rootset = 0, have_seed = 0;
+  char *gfile = NULL, *ofile = NULL, *gout = NULL;
+  int c;
+  if (argc == 1) usage(argv[0]);
+  setvbuf(stdout, NULL, _IOLBF, 0);
+  printf("MPI Tasks 1\n");
+  for (int i = 0; i < argc; ++i) printf(" %s", argv[i]);
+  printf("\n\n");
+  while ((c = getopt(argc, argv, "o:p:amhDR:dUf:n:r:S:E:N:H:c:x:z:B:s:T:qG:")) != -1) {
+    switch (c) {
+      case 'c': bader = (uint32_t)strtoul(optarg, NULL, 10); break;
+      case 'x': approx_n = (uint32_t)strtoul(optarg, NULL, 10); break;
+      case 'z': distribution = atoi(optarg); break;
+      case 'H':
+        heuristic = atoi(optarg);
+        if (heuristic >= 2) die("H2 and H3 are not implemented in Approx BC(-H): %s", optarg);
+        break;
+      case 'o': ofile = strdup(optarg); break;
+      case 'p':
+        if (sscanf(optarg, "%dx%d", &R, &C) != 2) die("Invalid proc mesh field: %s", optarg);
+        break;
+      case 'd': dump = 1; break;
+      case 'R':
+        random_mode = atoi(optarg);
+        if (random_mode < 0 || random_mode > 3) die("Invalid random option (-R): %s", optarg);
+        break;
+      case 'D': debug = 1; break;
+      case 'm': die("%s", "-m selects the reference's 2D multi-GPU path; this engine runs the 1x1 path"); break;
+      case 'f':
+        if (gread == 0) die("Unexpected option -%s!", "f");
+        gread = 1;
+        gfile = strdup(optarg);
+        break;
+      case 'n':
+        if (gread == 0) die("Unexpected option -%s!", "n");
+        gread = 1;
+        N = strtoull(optarg, NULL, 10);
+        break;
+      case 'r':
+        startv = strtoull(optarg, NULL, 10);
+        rootset = 1;
+        break;
+      case 'S':
+        if (gread == 1) die("Unexpected option -%s!", "S");
+        gread = 0;
+        scale = atoi(optarg);
+        N = 1ull << scale;
+        break;
+      case 'E':
+        if (gread == 1) die("Unexpected option -%s!", "E");
+        gread = 0;
+        edgef = atoi(optarg);
+        break;
+      case 'N': nbfs = strtoull(optarg, NULL, 10); break;
+      case 'U': undirected = 0; break;
+      case 'a': analyze = 1; break;
+      case 'B': width = atoi(optarg); break;
+      case 's':
+        seed_s = strtoull(optarg, NULL, 10);
+        have_seed = 1;
+        break;
+      case 'T': nthreads = atoi(optarg); break;
+      case 'q': quiet = 1; break;
+      case 'G': gout = strdup(optarg); break;
+      default: usage(argv[0]);
+    }
+  }
+  (void)debug;
+  if (gread == -1) {  /* defaults: -S 21 -E 16 */
+    gread = 0;
+    N = 1ull << scale;
+  }
+  if (approx_n > N) die("%s", "Number of vertices required is lager than N");
+  if (distribution > 6 || distribution < 0) die("%s", "Select a strategy between 0 and 6");
+  if (distribution == 2 || distribution == 6)
+    die("%s", "the LCC-based strategies (-z 2, -z 6) are not available in this engine yet");
+  if (gread && (!gfile || !N)) die("%s", "Graph file (-f) and number of vertices (-n) must be specified for file based bfs.");
+  if (R != 1 || C != 1) die("%s", "R and C must be 1: this engine runs the 1x1 path on one GPU (-p 1x1)");
+  if (startv >= N) die("Invalid start vertex: %s", "");
+  if (N > 0xFFFFFFF0ull) die("%s", "Number of vertices too big for 32-bit ids");
+  if (width != 0 && width != 64 && width != 128) die("%s", "-B must be 0, 64 or 128");
+  if (width && !(distribution == 0 || distribution == 1))
+    die("%s", "-B batches need a static distribution (-z 0 or -z 1)");
+  if (rootset) random_mode = 0;
+  if (nthreads <= 0) {
+#ifdef _OPENMP
+    nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+  }
+  uint64_t s1 = 23, s2 = 24;  /* gen_graph's seeds (src/axbc2d.c:524) */
+  if (have_seed) {
+    s1 = seed_s;
+    s2 = seed_s + 1;
+  }
+  printf("Total number of vertices (N): %" PRIu64 "\n", N);
+  printf("Processor mesh rows (R): 1\nProcessor mesh columns (C): 1\n");
+  if (gread)
+    printf("Reading graph from file: %s\n", gfile);
+  else
+    printf("RMAT graph scale: %d\nRMAT graph edge factor: %d\nNumber of bc rounds: %" PRIu64 "\n", scale, edgef, nbfs);
+  printf("HEURISTICs: %s: %d\n\n", heuristic ? "1-degree reduction ON" : "OFF", heuristic);
+
+  /* ---- graph ---------------------------------------------------------- */
+  double t0 = now_s();
+  axbc_graph g;
+  uint64_t generated = 0;
+  if (gread) {
+    char err[512];
+    uint64_t ned = 0;
+    uint64_t *ed = axbc_read_edge_file(gfile, N, undirected, &ned, err, sizeof(err));
+    if (!ed) die("%s", err);
+    generated = ned;
+    if (axbc_graph_from_edges(ed, ned, N, heuristic, nthreads, &g)) die("%s", "graph construction failed");
+  } else {
+    if (axbc_graph_kronecker(scale, edgef, s1, s2, undirected, heuristic, nthreads, &g))
+      die("%s", "graph construction failed");
+    generated = g.generated;
+  }
+  printf("Edges %s: %" PRIu64 "\n", gread ? "read from file (after mirror)" : "generated", generated);
+  printf("Edges in the CSC: %" PRIu64 " (built in %.2f s)\n", g.ned, now_s() - t0);
+  if (heuristic == 1) printf("task 0 Total edges removed %u\n", g.rem_ed);
+  if (gout) {
+    FILE *f = fopen(gout, "w");
+    if (!f) die("cannot open %s", gout);
+    for (uint64_t v = 0; v < N; ++v)
+      for (uint32_t k = g.col[v]; k < g.col[v + 1]; ++k) fprintf(f, "%u\t%" PRIu64 "\n", g.row[k], v);
+    fclose(f);
+    return 0;
+  }
+  if (dump && !gread) {
+    char fname[256];
+    snprintf(fname, sizeof(fname), "rmat_S%d_EF%d.txt", scale, edgef);
+    printf("DUMP THE GENERATED RMAT FILE IN %s\n", fname);
+    FILE *f = fopen(fname, "w");
+    if (f) {
+      for (uint64_t v = 0; v < N; ++v)
+        for (uint32_t k = g.col[v]; k < g.col[v + 1]; ++k) fprintf(f, "%u\t%" PRIu64 "\n", g.row[k], v);
+      fclose(f);
+    }
+  }
+  if (analyze) {
+    uint64_t hist[33] = {0};
+    for (uint64_t v = 0; v < N; ++v) {
+      uint32_t d = g.col[v + 1] - g.col[v];
+      int b = 0;
+      while (d >> b) ++b;
+      hist[b]++;
+    }
+    printf("Degree histogram (bin b holds degrees in [2^(b-1), 2^b)):\n");
+    for (int b = 0; b < 33; ++b)
+      if (hist[b]) printf("  %2d %" PRIu64 "\n", b, hist[b]);
+  }
+
+  /* ---- device --------------------------------------------------------- */
+  axbc_ctx *ctx = NULL;
+  size_t dev_bytes = 0;
+  check(axbc_ctx_create(0, N, g.ned, g.col, g.row, g.reach, heuristic, &ctx, &dev_bytes), "axbc_ctx_create");
+  if (width) check(axbc_set_batch_width(ctx, width), "axbc_set_batch_width");
+  printf("Device memory allocated (MB): %zu\n", dev_bytes >> 20);
+
+  /* ---- pivot loop (src/axbc2d.c:2700-3156) --------------------------------- */
+  srand48(random_mode == 1 ? (long)(getpid() + time(NULL)) : DEFAULT_SEED);
+  uint32_t *approx = (uint32_t *)calloc(approx_n ? approx_n : 1, sizeof(uint32_t));
+  for (uint32_t i = 0; i < approx_n; ++i) approx[i] = uniform_int((uint32_t)N, 0);
+  if (bader > 0) {
+    printf("Selected Verts to Approximate (TODO REMOVE PRINT): ");
+    for (uint32_t i = 0; i < approx_n; ++i) printf("%u ", approx[i]);
+    printf("\n");
+  } else {
+    printf("Stop computation after %" PRIu64 " good samples\n", nbfs);
+  }
+  check(axbc_set_checks(ctx, approx, approx_n), "axbc_set_checks");
+  uint32_t *pmax = NULL, *sdn = NULL, sum_deg = 0;
+  if (distribution == 0) {
+    (void)uniform_int((uint32_t)N, 0);
+    printf("UNIFORM sampling strategy\n");
+  } else if (distribution == 1) {
+    printf("High-Degree Neighborhood  sampling strategy\n");
+    pmax = (uint32_t *)malloc(sizeof(uint32_t) * N);
+    sdn = (uint32_t *)malloc(sizeof(uint32_t) * N);
+    check(axbc_hdn_build(ctx, pmax, sdn, &sum_deg), "axbc_hdn_build");
+  } else {
+    printf("%s sampling strategy\n", distribution == 3 ? "BC" : distribution == 4 ? "Delta" : "BC inverse");
+  }
+  nbfs = nbfs < N ? nbfs : N;
+  const uint32_t lim = (uint32_t)(N * N); /* int LIMxSAMPLE = N*N vs uint32 BSM */
+  const uint32_t dist_mix = (uint32_t)distribution;
+  uint8_t *vs = (uint8_t *)calloc(N, 1);
+  uint32_t nrounds = 0, skipped = 0;
+  uint64_t sum_er = 0;
+  double loop_t0 = now_s(), min_t = 1e30, max_t = 0;
+  int stopped = 0;
+  const int B = width ? width : 1;
+  round_t *pending = (round_t *)calloc((size_t)B, sizeof(round_t));
+  uint32_t *bv = (uint32_t *)calloc((size_t)B, sizeof(uint32_t));
+  double *bp = (double *)calloc((size_t)B, sizeof(double));
+  uint64_t *skip_at = (uint64_t *)calloc((size_t)B + 1, sizeof(uint64_t));
+  axbc_pivot_stats *bst = (axbc_pivot_stats *)calloc((size_t)B, sizeof(axbc_pivot_stats));
+  int npend = 0;
+  uint32_t skipped_pend = 0;
+  uint64_t ui;
+  for (ui = 0; ui < nbfs; ++ui) {
+    if (bader > 0 && npend == 0) {
+      int stop = 0;
+      check(axbc_stop_test(ctx, bader, &stop), "axbc_stop_test");
+      if (stop) {
+        printf("exit bader criterion\n");
+        stopped = 1;
+        break;
+      }
+    }
+    /* draw until a root not used before (or the retry cap) */
+    uint32_t v0 = 0, bsm = 0;
+    double p = 0;
+    int good = 0;
+    uint64_t wtotal = 0;
+    int wready = 0;
+    while (!good) {
+      if (bsm > lim) break;
+      const int dyn = distribution >= 3 && distribution <= 5;
+      if (distribution == 0 || (dyn && nrounds == 0)) {
+        v0 = uniform_int((uint32_t)N, 0);
+        p = 1.0 / (double)N;
+      } else if (distribution == 1) {
+        uint32_t r = uniform_int(sum_deg, 0);
+        uint32_t u = first_geq(pmax, (uint32_t)N, r);
+        uint32_t d = g.col[u + 1] - g.col[u];
+        if (d > 0) {
+          uint32_t nb = d == 1 ? 0 : uniform_int(d - 1, 0);
+          u = g.row[g.col[u] + nb];
+        }
+        v0 = u;
+        p = (double)sdn[v0] / (double)sum_deg;
+      } else {
+        if (!wready) {
+          check(axbc_weights(ctx, distribution, &wtotal), "axbc_weights");
+          wready = 1;
+        }
+        long draw = lrand48();
+        double rd = (double)draw / 2147483647.0 * (double)wtotal;
+        uint64_t r = rd >= (double)wtotal ? wtotal : (uint64_t)rd;
+        check(axbc_weights_pick(ctx, r, &v0, &p), "axbc_weights_pick");
+      }
+      if (v0 < N && vs[v0])
+        ++bsm;
+      else
+        good = 1;
+    }
+    if (bsm > lim) continue;
+    if (v0 >= N || g.col[v0 + 1] == g.col[v0]) { /* degree-0 root */
+      ++skipped_pend;
+      if (v0 < N) vs[v0] = 1;
+      if (!width) {
+        skipped += skipped_pend;
+        skipped_pend = 0;
+      }
+      continue;
+    }
+    vs[v0] = 1;
+    if (!quiet)
+      printf("Perform %u/%" PRIu64 " BC from %u (ui=%" PRIu64 " strategy (%d,%u) p =%lf)\n", nrounds + npend + 1,
+             nbfs, v0, ui, distribution, dist_mix, p);
+    if (!width) {
+      axbc_pivot_stats st;
+      double t1 = now_s();
+      check(axbc_pivot(ctx, v0, p, &st), "axbc_pivot");
+      double dt = now_s() - t1;
+      min_t = dt < min_t ? dt : min_t;
+      max_t = dt > max_t ? dt : max_t;
+      sum_er += st.e_r;
+      ++nrounds;
+      continue;
+    }
+    skip_at[npend] = skipped_pend;
+    pending[npend].v0 = v0;
+    pending[npend].p = p;
+    pending[npend].ui = ui;
+    bv[npend] = v0;
+    bp[npend] = p;
+    ++npend;
+    if (npend == B || ui + 1 == nbfs) {
+      int applied = 0;
+      double t1 = now_s();
+      check(axbc_batch(ctx, bv, bp, npend, bader, &applied, bst), "axbc_batch");
+      double dt = now_s() - t1;
+      min_t = dt < min_t ? dt : min_t;
+      max_t = dt > max_t ? dt : max_t;
+      for (int k = 0; k < applied; ++k) sum_er += bst[k].e_r;
+      nrounds += (uint32_t)applied;
+      if (applied < npend) { /* stopped after round applied-1 */
+        skipped += (uint32_t)skip_at[applied - 1];
+        ui = pending[applied - 1].ui + 1;
+        printf("exit bader criterion\n");
+        stopped = 1;
+        npend = 0;
+        break;
+      }
+      skipped += skipped_pend;
+      skipped_pend = 0;
+      npend = 0;
+    }
+  }
+  if (width && npend > 0) { /* loop ended with a partial batch */
+    int applied = 0;
+    check(axbc_batch(ctx, bv, bp, npend, bader, &applied, bst), "axbc_batch");
+    for (int k = 0; k < applied; ++k) sum_er += bst[k].e_r;
+    nrounds += (uint32_t)applied;
+    skipped += applied < npend ? (uint32_t)skip_at[applied - 1] : skipped_pend;
+  } else if (width) {
+    skipped += skipped_pend;
+  }
+  check(axbc_sync(ctx), "axbc_sync");
+  const double all_time = now_s() - loop_t0;
+
+  /* ---- results ---------------------------------------------------------- */
+  double *bc = (double *)malloc(sizeof(double) * N);
+  check(axbc_get_bc(ctx, bc), "axbc_get_bc");
+  if (ofile) {
+    char resname[1024];
+    snprintf(resname, sizeof(resname), "%s_1X1_0.log", ofile);
+    printf("Result written to file: %s\n", resname);
+    FILE *f = fopen(resname, "w");
+    if (f) {
+      for (uint64_t k = 0; k < N; ++k) fprintf(f, "%" PRIu64 "\t%.2f\n", k, nrounds ? bc[k] / 2.0 / nrounds : 0.0);
+      fclose(f);
+    }
+  }
+  printf("\n------------- RESULTS ---------------\n");
+  printf("ClusterId\tSkip\tRounds\tExecTime\tRoundsTime\tMax\tMin\tMean\n");
+  printf("0\t%u\t%u\t%lf\t%lf\t%lf\t%lf\t%lf\n", skipped, nrounds, all_time, all_time, max_t,
+         min_t > 1e29 ? 0.0 : min_t, nrounds ? all_time / nrounds : 0.0);
+  printf("task 0 BC skipped: %u \n", skipped);
+  printf("task 0 BC rounds: %u  \n", nrounds);
+  printf("task 0 BC rounds total time: %lf sec\n", all_time);
+  printf("task 0 pivots/s: %.3f  GTEPS (fwd+bwd): %.4f  stopped: %d  batch: %d\n",
+         all_time > 0 ? nrounds / all_time : 0.0, all_time > 0 ? 2.0 * (double)sum_er / all_time / 1e9 : 0.0, stopped,
+         width);
+  free(bc);
+  free(vs);
+  free(approx);
+  free(pmax);
+  free(sdn);
+  free(pending);
+  free(bv);
+  free(bp);
+  free(skip_at);
+  free(bst);
+  axbc_ctx_destroy(ctx);
+  axbc_graph_free(&g);
+  return 0;
+}

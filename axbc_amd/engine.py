@@ -71,16 +71,25 @@ class Engine:
         _check(_ffi.hip().axbc_pivot(self.ctx, int(v0), float(p), C.byref(st) if stats else None))
         return st.as_dict() if stats else None
 
-    def batch(self, v0s, ps, stats: bool = False):
-        """Bit-packed multi-source rounds (requires the batched kernels)."""
+    def set_batch_width(self, width: int) -> None:
+        _check(_ffi.hip().axbc_set_batch_width(self.ctx, int(width)))
+
+    def batch(self, v0s, ps, stop_c: int = 0, stats: bool = False):
+        """Bit-packed multi-source rounds: the same result as one pivot()
+        per root in order.  Returns (applied, stats or None)."""
         L = _ffi.hip()
-        if not hasattr(L, "axbc_batch"):
-            raise EngineError("batched kernels not built")
         v = np.ascontiguousarray(v0s, dtype=np.uint32)
         p = np.ascontiguousarray(ps, dtype=np.float64)
         st = (_ffi.PivotStats * len(v))()
-        _check(L.axbc_batch(self.ctx, v, p, len(v), C.cast(st, C.c_void_p) if stats else None))
-        return [s.as_dict() for s in st] if stats else None
+        ap = C.c_int(0)
+        _check(L.axbc_batch(self.ctx, v, p, len(v), int(stop_c), C.byref(ap),
+                            C.cast(st, C.c_void_p) if stats else None))
+        return ap.value, ([s.as_dict() for s in st] if stats else None)
+
+    def batch_delta(self, k: int) -> np.ndarray:
+        out = np.empty(self.N, np.float64)
+        _check(_ffi.hip().axbc_batch_delta(self.ctx, int(k), out))
+        return out
 
     def sync(self) -> None:
         _check(_ffi.hip().axbc_sync(self.ctx))

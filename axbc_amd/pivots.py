@@ -1,0 +1,184 @@
+"""The pivot loop of AxBC's main() (src/axbc2d.c:2700-3156) over an Engine.
+
+Same draw order as the reference (srand48(4343543), the -x check vertices,
+the discarded initial uniform draw for -z 0, one lrand48 per attempt plus
+the neighbour draw for -z 1), the same retry/skip rules (LIMxSAMPLE = N*N
+as uint32, degree-0 roots consume an attempt), and the adaptive stop test
+before every attempt.  The samplers' tables and the stop test run on the
+device.  With batch > 0 the rounds of a static distribution are grouped
+into bit-packed batches; the stop test is then evaluated exactly after
+every root inside the batch, so the result equals the one-root loop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _ffi
+from .engine import Engine, _check
+
+_libc = C.CDLL(None)
+_libc.lrand48.restype = C.c_long
+_libc.srand48.argtypes = [C.c_long]
+
+DEFAULT_SEED = 4343543
+
+
+def uniform_int(mx: int, mn: int = 0) -> int:
+    """lrand48() % (max - min + 1) + min in uint32 (src/axbc2d.c:2156-2164)."""
+    m = (mx - mn + 1) & 0xFFFFFFFF
+    r = _libc.lrand48()
+    return ((r & 0xFFFFFFFF) % m + mn) & 0xFFFFFFFF if m else (r + mn) & 0xFFFFFFFF
+
+
+@dataclass
+class LoopResult:
+    pivots: list = field(default_factory=list)
+    probs: list = field(default_factory=list)
+    nrounds: int = 0
+    skipped: int = 0
+    stopped: bool = False
+    attempts: int = 0
+    seconds: float = 0.0
+    e_r: int = 0
+    batch_seconds: list = field(default_factory=list)
+
+
+class PivotLoop:
+    """Stateful pivot loop: draws follow libc's lrand48 stream."""
+
+    def __init__(self, engine: Engine, nbfs: int, c: int = 1, x: int = 1, z: int = 0,
+                 seed: int = DEFAULT_SEED, batch: int = 0):
+        if z not in (0, 1, 3, 4, 5):
+            raise ValueError("supported distributions: 0, 1, 3, 4, 5")
+        if batch and z not in (0, 1):
+            raise ValueError("batches need a static distribution (-z 0 or -z 1)")
+        self.e, self.g = engine, engine.g
+        self.N = self.g.N
+        self.nbfs = min(nbfs, self.N)
+        self.c, self.x, self.z, self.batch = c, x, z, batch
+        self.col, self.row = self.g.col, self.g.row
+        self.deg = np.diff(self.col)
+        _libc.srand48(seed)
+        self.checks = np.array([uniform_int(self.N) for _ in range(x)], np.uint32)
+        engine.set_checks(self.checks)
+        if batch:
+            _check(_ffi.hip().axbc_set_batch_width(engine.ctx, batch))
+        if z == 0:
+            uniform_int(self.N)  # discarded initial draw (axbc2d.c:2740)
+        if z == 1:
+            self.pmax = np.empty(self.N, np.uint32)
+            self.sdn = np.empty(self.N, np.uint32)
+            s = C.c_uint32(0)
+            _check(_ffi.hip().axbc_hdn_build(engine.ctx, self.pmax, self.sdn, C.byref(s)))
+            self.sum_deg = int(s.value)
+        self.lim = (self.N * self.N) & 0xFFFFFFFF
+        self.vs = np.zeros(self.N, np.uint8)
+        self.ui = 0
+        self.res = LoopResult()
+
+    # -- sampling ---------------------------------------------------------
+    def _draw(self) -> tuple[int, float]:
+        z, N = self.z, self.N
+        if z == 0 or (z in (3, 4, 5) and self.res.nrounds == 0):
+            return uniform_int(N), 1.0 / N
+        if z == 1:
+            r = uniform_int(self.sum_deg)
+            u = int(np.searchsorted(self.pmax, r, side="left"))
+            if u >= N:
+                u = N - 1
+            d = int(self.deg[u])
+            if d > 0:
+                nb = 0 if d == 1 else uniform_int(d - 1)
+                u = int(self.row[self.col[u] + nb])
+            return u, float(self.sdn[u]) / float(self.sum_deg)
+        total = C.c_uint64(0)
+        _check(_ffi.hip().axbc_weights(self.e.ctx, z, C.byref(total)))
+        draw = _libc.lrand48()
+        rd = float(draw) / 2147483647.0 * float(total.value)
+        r = total.value if rd >= float(total.value) else int(rd)
+        v, p = C.c_uint32(0), C.c_double(0)
+        _check(_ffi.hip().axbc_weights_pick(self.e.ctx, r, C.byref(v), C.byref(p)))
+        return int(v.value), float(p.value)
+
+    def _next_root(self):
+        """One attempt: (v0, p) or None for a consumed attempt."""
+        bsm = 0
+        while True:
+            if bsm > self.lim:
+                return None
+            v0, p = self._draw()
+            if v0 < self.N and self.vs[v0]:
+                bsm += 1
+                continue
+            break
+        if v0 >= self.N or self.deg[v0] == 0:
+            self.res.skipped += 1
+            if v0 < self.N:
+                self.vs[v0] = 1
+            return None
+        self.vs[v0] = 1
+        return v0, p
+
+    # -- running ----------------------------------------------------------
+    def step(self, max_rounds: int | None = None) -> int:
+        """Run rounds until max_rounds (a batch for batched loops) or the
+        end; returns the rounds applied (0 when the loop is over)."""
+        res, e = self.res, self.e
+        if res.stopped or self.ui >= self.nbfs:
+            return 0
+        if self.c > 0 and e.stop_test(self.c):
+            res.stopped = True
+            return 0
+        want = max_rounds or (self.batch or 1)
+        pend, pend_ui, pend_skip = [], [], []
+        while self.ui < self.nbfs and len(pend) < want:
+            skipped_before = res.skipped
+            r = self._next_root()
+            self.ui += 1
+            if r is None:
+                continue
+            pend.append(r)
+            pend_ui.append(self.ui)
+            pend_skip.append(skipped_before)
+            if not self.batch:
+                break
+        if not pend:
+            return 0
+        t = time.perf_counter()
+        if not self.batch:
+            v0, p = pend[0]
+            st = e.pivot(v0, p, stats=True)
+            applied = 1
+            res.e_r += st["e_r"]
+        else:
+            L = _ffi.hip()
+            v = np.array([a for a, _ in pend], np.uint32)
+            pp = np.array([b for _, b in pend], np.float64)
+            st = (_ffi.PivotStats * len(v))()
+            ap = C.c_int(0)
+            _check(L.axbc_batch(e.ctx, v, pp, len(v), self.c, C.byref(ap), C.cast(st, C.c_void_p)))
+            applied = ap.value
+            res.e_r += sum(int(st[k].e_r) for k in range(applied))
+            if applied < len(pend):  # stopped inside the batch
+                res.stopped = True
+                res.skipped = pend_skip[applied - 1]
+                self.ui = pend_ui[applied - 1]
+        res.batch_seconds.append(time.perf_counter() - t)
+        for k in range(applied):
+            res.pivots.append(pend[k][0])
+            res.probs.append(pend[k][1])
+        res.nrounds += applied
+        res.attempts = self.ui
+        return applied
+
+    def run(self) -> LoopResult:
+        t = time.perf_counter()
+        while self.step():
+            pass
+        self.e.sync()
+        self.res.seconds = time.perf_counter() - t
+        return self.res

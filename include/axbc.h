@@ -69,6 +69,12 @@ int axbc_device_count(void);
 int axbc_ctx_create(int device, uint64_t N, uint64_t ned, const uint32_t *col,
                     const uint32_t *row, const uint32_t *reach, int heuristic,
                     axbc_ctx **out, size_t *dev_bytes);
+/* Replace the pendant counts (init_bc_1degree_device, cu:1515-1519) and
+ * reset the round state; bc is cleared too. */
+int axbc_set_reach(axbc_ctx *ctx, const uint32_t *reach);
+/* Pinned host memory (CudaMallocHostSet / CudaFreeHost, cu:825-846). */
+void *axbc_host_alloc(size_t bytes);
+void axbc_host_free(void *p);
 /* Release everything (fincuda(), cu:1659-1693). */
 int axbc_ctx_destroy(axbc_ctx *ctx);
 
@@ -165,6 +171,22 @@ int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);
 int axbc_batch_delta(axbc_ctx *ctx, int k, double *delta);
+
+/*
+ * Per-kernel instrumentation for roofline figures (no reference
+ * counterpart).  While enabled, every launch of the main kernels is
+ * bracketed by HIP events on the context stream and the kernels add their
+ * algorithmic byte counts to device counters.  Kinds: 0 batched push,
+ * 1 batched path-count pull, 2 batched level close, 3 batched dependency
+ * pull, 4 batched bc update, 5 plain forward, 6 plain dependency pull,
+ * 7 calibration copy.  axbc_kernel_stats returns the summed milliseconds,
+ * the launch count and the byte count of one kind since enabling.
+ * axbc_copy_kernel runs a plain device copy of `bytes` bytes (kind 7) to
+ * calibrate hardware counters against a known traffic.
+ */
+int axbc_kernel_timing(axbc_ctx *ctx, int enable);
+int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, uint64_t *bytes);
+int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 
 /* Last round's counters (waits for the context). */
 int axbc_last_stats(axbc_ctx *ctx, axbc_pivot_stats *stats);
