@@ -88,6 +88,9 @@ def hip() -> C.CDLL:
         "axbc_hdn_pick": (C.c_int, [ctxp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "axbc_weights": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_uint64)]),
         "axbc_weights_pick": (C.c_int, [ctxp, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+        "axbc_weights_at": (C.c_int, [ctxp, C.c_uint32, C.POINTER(C.c_uint64)]),
+        "axbc_lcc_build": (C.c_int, [ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+        "axbc_set_reach": (C.c_int, [ctxp, C.c_void_p]),
         "axbc_kernel_timing": (C.c_int, [ctxp, C.c_int]),
         "axbc_kernel_stats": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),

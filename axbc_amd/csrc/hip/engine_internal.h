@@ -41,6 +41,12 @@ struct SamplerBufs {
   uint64_t *host64 = nullptr;     // pinned, 2 values
   uint64_t total = 0;
   int hdn_ready = 0, weights_ready = 0;
+  // static LCC distribution (-z 2 / -z 6)
+  float *lcc = nullptr;
+  uint32_t *lcc_cnt = nullptr;
+  uint64_t *lw = nullptr, *lp = nullptr;  // fixed-point weights and their prefix
+  uint64_t ltotal = 0;
+  int lcc_ready = 0;
 };
 
 struct BatchState;  // batch.hip

@@ -18,6 +18,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../../include/axbc.h"
 #include "engine_internal.h"
 #include "wave.h"
@@ -142,11 +144,92 @@ __global__ void k_hdn_terms(const uint32_t *__restrict__ col, const uint32_t *__
   }
 }
 
+// ---- local clustering coefficient (-z 2, -z 6) ------------------------------
+//
+// lcc_func (src/axbc2d.c:1847-1935): for vertex i of degree d >= 2,
+// counter = #{(j, x): j in adj(i), x in adj(j), x != i, x in adj(i)} as a
+// wrapping uint32, lcc = (float)counter / (float)(uint32)(d*(d-1)).  Each
+// (i, j) pair intersects two sorted adjacency lists: lanes walk the shorter
+// one and binary-search the longer one.
+
+__device__ __forceinline__ bool in_sorted(const uint32_t *__restrict__ a, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && a[lo] == x;
+}
+
+// |adj(i) cap adj(j) minus {i}|, one partial count per lane
+__device__ __forceinline__ uint32_t pair_count(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                               uint32_t i, uint32_t j) {
+  const uint32_t *ai = row + col[i], *aj = row + col[j];
+  const uint32_t di = col[i + 1] - col[i], dj = col[j + 1] - col[j];
+  const uint32_t *walk = dj <= di ? aj : ai, *look = dj <= di ? ai : aj;
+  const uint32_t nw = dj <= di ? dj : di, nl = dj <= di ? di : dj;
+  uint32_t c = 0;
+  for (uint32_t k = lane_id(); k < nw; k += kWave) {
+    const uint32_t x = walk[k];
+    if (x != i && in_sorted(look, nl, x)) ++c;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_lcc_small(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                   uint32_t n, uint32_t hub_deg, uint32_t *counter) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t i = gw; i < n; i += nwv) {
+    const uint32_t d = col[i + 1] - col[i];
+    if (d < 2 || d > hub_deg) continue;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < d; ++k) c += pair_count(col, row, i, row[col[i] + k]);
+    c = (uint32_t)wave_sum_u64(c);
+    if (lane_id() == 0) counter[i] = c;
+  }
+}
+
+// Hubs: one wave per (hub, edge) item; items enumerated by the prefix of
+// the hubs' degrees.
+__global__ __launch_bounds__(256) void k_lcc_hubs(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ hubs, const uint64_t *__restrict__ hoff,
+                                                  uint32_t nh, uint32_t *counter) {
+  const uint64_t total = hoff[nh];
+  const uint64_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint64_t t = gw; t < total; t += nwv) {
+    uint32_t lo = 0, hi = nh;  // hoff[lo] <= t < hoff[lo+1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (hoff[mid] <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t i = hubs[lo];
+    const uint32_t j = row[col[i] + (uint32_t)(t - hoff[lo])];
+    const uint32_t c = (uint32_t)wave_sum_u64(pair_count(col, row, i, j));
+    if (lane_id() == 0 && c) atomicAdd(&counter[i], c);
+  }
+}
+
+__global__ void k_lcc_value(const uint32_t *__restrict__ col, uint32_t n, const uint32_t *__restrict__ counter,
+                            float *lcc) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t d = col[i + 1] - col[i];
+    lcc[i] = d < 2 ? 0.0f : (float)counter[i] / (float)(uint32_t)(d * (d - 1u));
+  }
+}
+
 // ---- dynamic fixed-point weights -----------------------------------------
 
 __device__ __forceinline__ double weight_of(int kind, uint32_t v, const double *__restrict__ bc,
                                             const double *__restrict__ delta) {
-  double w = kind == 3 ? bc[v] : kind == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
+  // kind 2 passes the LCC values (float) through `delta` reinterpreted
+  double w = kind == 2 ? (double)((const float *)delta)[v]
+                       : kind == 3 ? bc[v] : kind == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
   return (isfinite(w) && w > 0.0) ? w : 0.0;
 }
 
@@ -226,7 +309,8 @@ int sampler_bufs(axbc_ctx *c, SamplerBufs **out) {
 void sampler_free(axbc_ctx *c) {
   SamplerBufs *sb = c->sampler;
   if (!sb) return;
-  void *d[] = {sb->tmp, sb->u32a, sb->u32b, sb->u32c, sb->u64a, sb->tiles, sb->wmax, sb->pick};
+  void *d[] = {sb->tmp, sb->u32a, sb->u32b, sb->u32c, sb->u64a, sb->tiles, sb->wmax, sb->pick,
+               sb->lcc, sb->lcc_cnt, sb->lw, sb->lp};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (sb->pick_host) (void)hipHostFree(sb->pick_host);
@@ -300,6 +384,74 @@ int axbc_weights(axbc_ctx *c, int kind, uint64_t *total) {
   sb->total = sb->host64[0];
   if (total) *total = sb->total;
   sb->weights_ready = 1;
+  return AXBC_OK;
+}
+
+int axbc_lcc_build(axbc_ctx *c, float *lcc_host, uint64_t *lw_host, uint64_t *lp_host, uint64_t *total) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  const uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  if (!sb->lcc) {
+    HIP_TRY(hipMalloc((void **)&sb->lcc, sizeof(float) * n));
+    HIP_TRY(hipMalloc((void **)&sb->lcc_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(hipMalloc((void **)&sb->lw, sizeof(uint64_t) * n));
+    HIP_TRY(hipMalloc((void **)&sb->lp, sizeof(uint64_t) * n));
+    c->bytes += 24ull * n;
+  }
+  constexpr uint32_t kLccHub = 4096;
+  std::vector<uint32_t> hubs;
+  std::vector<uint64_t> hoff(1, 0);
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t d = c->h_col[v + 1] - c->h_col[v];
+    if (d > kLccHub) {
+      hubs.push_back(v);
+      hoff.push_back(hoff.back() + d);
+    }
+  }
+  uint32_t *dh = nullptr;
+  uint64_t *doff = nullptr;
+  HIP_TRY(hipMalloc((void **)&dh, sizeof(uint32_t) * (hubs.size() + 1)));
+  HIP_TRY(hipMalloc((void **)&doff, sizeof(uint64_t) * hoff.size()));
+  if (!hubs.empty()) HIP_TRY(hipMemcpyAsync(dh, hubs.data(), sizeof(uint32_t) * hubs.size(), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(doff, hoff.data(), sizeof(uint64_t) * hoff.size(), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(sb->lcc_cnt, 0, sizeof(uint32_t) * n, s));
+  hipLaunchKernelGGL(k_lcc_small, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccHub, sb->lcc_cnt);
+  if (!hubs.empty())
+    hipLaunchKernelGGL(k_lcc_hubs, dim3(4096), dim3(256), 0, s, c->col, c->row, dh, doff, (uint32_t)hubs.size(),
+                       sb->lcc_cnt);
+  hipLaunchKernelGGL(k_lcc_value, dim3(grid_n(n)), dim3(256), 0, s, c->col, n, sb->lcc_cnt, sb->lcc);
+  // fixed-point weights of the LCC values and their prefix
+  HIP_TRY(hipMemsetAsync(sb->wmax, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_weight_max, dim3(grid_n(n)), dim3(256), 0, s, 2, c->bc, (const double *)sb->lcc, n, sb->wmax);
+  hipLaunchKernelGGL(k_weight_fixed, dim3(grid_n(n)), dim3(256), 0, s, 2, c->bc, (const double *)sb->lcc, n, sb->wmax,
+                     sb->lw);
+  if (device_scan<uint64_t, AddU64>(sb->lw, sb->lp, n, (uint64_t *)sb->tiles, s))
+    return fail(AXBC_EHIP, "scan launch failed%s (line %d)", "", __LINE__);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&sb->host64[0], sb->lp + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  if (lcc_host) HIP_TRY(hipMemcpyAsync(lcc_host, sb->lcc, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  if (lw_host) HIP_TRY(hipMemcpyAsync(lw_host, sb->lw, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, s));
+  if (lp_host) HIP_TRY(hipMemcpyAsync(lp_host, sb->lp, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipFree(dh));
+  HIP_TRY(hipFree(doff));
+  sb->ltotal = sb->host64[0];
+  if (total) *total = sb->ltotal;
+  sb->lcc_ready = 1;
+  return AXBC_OK;
+}
+
+int axbc_weights_at(axbc_ctx *c, uint32_t v, uint64_t *w) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  if (!sb->weights_ready || !w || v >= c->n) return fail(AXBC_ESTATE, "weights not built%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(&sb->host64[1], (const uint64_t *)sb->tmp + v, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *w = sb->host64[1];
   return AXBC_OK;
 }
 

@@ -64,7 +64,7 @@ static void usage(const char *p) {
           "\t\t -S <scale> [-E <edge factor>]\n"
           "\t Where:\n"
           "\t\t -c <c> -x <x>  adaptive stop: bc/2 >= c*N on x check vertices\n"
-          "\t\t -z <0..5>      pivot distribution\n"
+          "\t\t -z <0..6>      pivot distribution\n"
           "\t\t -H <0|1>       1-degree reduction\n"
           "\t\t -U             DO NOT make graph Undirected\n"
           "\t\t -R <0|1>       fixed / time-based seed\n"
@@ -79,6 +79,13 @@ static uint32_t uniform_int(uint32_t max, uint32_t min) {
   uint32_t m = max - min + 1u;
   long r = lrand48();
   return m ? (uint32_t)((uint32_t)r % m) + min : (uint32_t)r + min;
+}
+
+/* r = lrand48()/RAND_MAX * total (uniform_double, src/axbc2d.c:2167-2172)
+ * on the fixed-point scale of the weights, clamped to total */
+static uint64_t fixed_draw(uint64_t total) {
+  double rd = (double)lrand48() / 2147483647.0 * (double)total;
+  return rd >= (double)total ? total : (uint64_t)rd;
 }
 
 /* first i with pmax[i] >= r; N-1 when none (findsample_LOCINT + fallback) */
@@ -180,8 +187,6 @@ int main(int argc, char **argv) {
   }
   if (approx_n > N) die("%s", "Number of vertices required is lager than N");
   if (distribution > 6 || distribution < 0) die("%s", "Select a strategy between 0 and 6");
-  if (distribution == 2 || distribution == 6)
-    die("%s", "the LCC-based strategies (-z 2, -z 6) are not available in this engine yet");
   if (gread && (!gfile || !N)) die("%s", "Graph file (-f) and number of vertices (-n) must be specified for file based bfs.");
   if (R != 1 || C != 1) die("%s", "R and C must be 1: this engine runs the 1x1 path on one GPU (-p 1x1)");
   if (startv >= N) die("Invalid start vertex: %s", "");
@@ -281,20 +286,32 @@ int main(int argc, char **argv) {
   }
   check(axbc_set_checks(ctx, approx, approx_n), "axbc_set_checks");
   uint32_t *pmax = NULL, *sdn = NULL, sum_deg = 0;
+  uint64_t *lw = NULL, *lp = NULL, ltotal = 0;
+  uint32_t dist_mix = (uint32_t)distribution;
   if (distribution == 0) {
     (void)uniform_int((uint32_t)N, 0);
     printf("UNIFORM sampling strategy\n");
-  } else if (distribution == 1) {
-    printf("High-Degree Neighborhood  sampling strategy\n");
+  } else if (distribution == 6) {
+    dist_mix = uniform_int(4, 0);
+    printf("Mixture sampling strategy: %u\n", dist_mix);
+  } else {
+    static const char *names[] = {"", "High-Degree Neighborhood ", "LCC ", "BC", "Delta", "BC inverse"};
+    printf("%s sampling strategy\n", names[distribution]);
+  }
+  if (distribution == 2 || distribution == 6) {
+    double tl = now_s();
+    lw = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    lp = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    check(axbc_lcc_build(ctx, NULL, lw, lp, &ltotal), "axbc_lcc_build");
+    printf("LCC done in %f secs\n", now_s() - tl);
+  }
+  if (distribution == 1 || distribution == 6) {
     pmax = (uint32_t *)malloc(sizeof(uint32_t) * N);
     sdn = (uint32_t *)malloc(sizeof(uint32_t) * N);
     check(axbc_hdn_build(ctx, pmax, sdn, &sum_deg), "axbc_hdn_build");
-  } else {
-    printf("%s sampling strategy\n", distribution == 3 ? "BC" : distribution == 4 ? "Delta" : "BC inverse");
   }
   nbfs = nbfs < N ? nbfs : N;
   const uint32_t lim = (uint32_t)(N * N); /* int LIMxSAMPLE = N*N vs uint32 BSM */
-  const uint32_t dist_mix = (uint32_t)distribution;
   uint8_t *vs = (uint8_t *)calloc(N, 1);
   uint32_t nrounds = 0, skipped = 0;
   uint64_t sum_er = 0;
@@ -327,11 +344,12 @@ int main(int argc, char **argv) {
     int wready = 0;
     while (!good) {
       if (bsm > lim) break;
-      const int dyn = distribution >= 3 && distribution <= 5;
-      if (distribution == 0 || (dyn && nrounds == 0)) {
+      const uint32_t k = distribution == 6 ? dist_mix : (uint32_t)distribution;
+      const int dyn = k >= 3 && k <= 5;
+      if (k == 0 || (dyn && nrounds == 0)) {
         v0 = uniform_int((uint32_t)N, 0);
         p = 1.0 / (double)N;
-      } else if (distribution == 1) {
+      } else if (k == 1) {
         uint32_t r = uniform_int(sum_deg, 0);
         uint32_t u = first_geq(pmax, (uint32_t)N, r);
         uint32_t d = g.col[u + 1] - g.col[u];
@@ -341,15 +359,46 @@ int main(int argc, char **argv) {
         }
         v0 = u;
         p = (double)sdn[v0] / (double)sum_deg;
+      } else if (k == 2) {
+        uint64_t r = fixed_draw(ltotal);
+        uint32_t lo = 0, hi = (uint32_t)N;
+        while (lo < hi) {
+          uint32_t mid = lo + (hi - lo) / 2;
+          if (lp[mid] >= r)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        v0 = lo < N ? lo : (uint32_t)N - 1;
+        p = (double)lw[v0] / (double)ltotal;
       } else {
-        if (!wready) {
-          check(axbc_weights(ctx, distribution, &wtotal), "axbc_weights");
+        if (!wready || distribution == 6) {
+          check(axbc_weights(ctx, (int)k, &wtotal), "axbc_weights");
           wready = 1;
         }
-        long draw = lrand48();
-        double rd = (double)draw / 2147483647.0 * (double)wtotal;
-        uint64_t r = rd >= (double)wtotal ? wtotal : (uint64_t)rd;
+        uint64_t r = fixed_draw(wtotal);
         check(axbc_weights_pick(ctx, r, &v0, &p), "axbc_weights_pick");
+      }
+      if (distribution == 6) { /* mixture probability, next component (3030-3051) */
+        const int in = v0 < N;
+        p = 1.0 / (double)N;
+        if (nrounds == 0) {
+          p = p + 1.0 / (double)N;
+          p = p + 1.0 / (double)N;
+        } else {
+          uint64_t t = 0, w = 0;
+          check(axbc_weights(ctx, 3, &t), "axbc_weights");
+          if (in) check(axbc_weights_at(ctx, v0, &w), "axbc_weights_at");
+          p = p + (double)w / (double)t;
+          w = 0;
+          check(axbc_weights(ctx, 4, &t), "axbc_weights");
+          if (in) check(axbc_weights_at(ctx, v0, &w), "axbc_weights_at");
+          p = p + (double)w / (double)t;
+        }
+        p = p + (in ? (double)sdn[v0] : 0.0) / (double)sum_deg;
+        p = p + (in ? (double)lw[v0] : 0.0) / (double)ltotal;
+        p = p / 5.0;
+        dist_mix = uniform_int(2, 0);
       }
       if (v0 < N && vs[v0])
         ++bsm;
@@ -450,6 +499,8 @@ int main(int argc, char **argv) {
   free(approx);
   free(pmax);
   free(sdn);
+  free(lw);
+  free(lp);
   free(pending);
   free(bv);
   free(bp);

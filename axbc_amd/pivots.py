@@ -52,8 +52,8 @@ class PivotLoop:
 
     def __init__(self, engine: Engine, nbfs: int, c: int = 1, x: int = 1, z: int = 0,
                  seed: int = DEFAULT_SEED, batch: int = 0):
-        if z not in (0, 1, 3, 4, 5):
-            raise ValueError("supported distributions: 0, 1, 3, 4, 5")
+        if z not in range(7):
+            raise ValueError("distributions are 0..6")
         if batch and z not in (0, 1):
             raise ValueError("batches need a static distribution (-z 0 or -z 1)")
         self.e, self.g = engine, engine.g
@@ -67,9 +67,20 @@ class PivotLoop:
         engine.set_checks(self.checks)
         if batch:
             _check(_ffi.hip().axbc_set_batch_width(engine.ctx, batch))
+        self.mix = z
         if z == 0:
             uniform_int(self.N)  # discarded initial draw (axbc2d.c:2740)
-        if z == 1:
+        if z == 6:
+            self.mix = uniform_int(4)  # initial mixture component (axbc2d.c:2798)
+        if z in (2, 6):
+            self.lcc = np.empty(self.N, np.float32)
+            self.lw = np.empty(self.N, np.uint64)
+            self.lp = np.empty(self.N, np.uint64)
+            t = C.c_uint64(0)
+            _check(_ffi.hip().axbc_lcc_build(engine.ctx, self.lcc.ctypes.data, self.lw.ctypes.data,
+                                             self.lp.ctypes.data, C.byref(t)))
+            self.ltotal = int(t.value)
+        if z in (1, 6):
             self.pmax = np.empty(self.N, np.uint32)
             self.sdn = np.empty(self.N, np.uint32)
             s = C.c_uint32(0)
@@ -81,10 +92,52 @@ class PivotLoop:
         self.res = LoopResult()
 
     # -- sampling ---------------------------------------------------------
+    @staticmethod
+    def _fixed_draw(total: int) -> int:
+        """r = lrand48()/RAND_MAX * total (uniform_double), clamped."""
+        rd = float(_libc.lrand48()) / 2147483647.0 * float(total)
+        return total if rd >= float(total) else int(rd)
+
+    def _device_weight(self, kind: int, v: int) -> tuple[int, int]:
+        L = _ffi.hip()
+        total, w = C.c_uint64(0), C.c_uint64(0)
+        _check(L.axbc_weights(self.e.ctx, kind, C.byref(total)))
+        if v < self.N:
+            _check(L.axbc_weights_at(self.e.ctx, v, C.byref(w)))
+        return int(w.value), int(total.value)
+
     def _draw(self) -> tuple[int, float]:
-        z, N = self.z, self.N
+        if self.z != 6:
+            return self._draw_component(self.z)
+        N = self.N
+        v0, _ = self._draw_component(self.mix)
+        # mixture probability (axbc2d.c:3030-3047), then the next component
+        p = 1.0 / N
+        if self.res.nrounds == 0:
+            p = p + 1.0 / N
+            p = p + 1.0 / N
+        else:
+            wb, tb = self._device_weight(3, v0)
+            p = p + float(wb) / float(tb)
+            wd, td = self._device_weight(4, v0)
+            p = p + float(wd) / float(td)
+        inside = v0 < N
+        p = p + (float(self.sdn[v0]) if inside else 0.0) / float(self.sum_deg)
+        p = p + (float(self.lw[v0]) if inside else 0.0) / float(self.ltotal)
+        p = p / 5.0
+        self.mix = uniform_int(2)
+        return v0, p
+
+    def _draw_component(self, z: int) -> tuple[int, float]:
+        N = self.N
         if z == 0 or (z in (3, 4, 5) and self.res.nrounds == 0):
             return uniform_int(N), 1.0 / N
+        if z == 2:
+            r = self._fixed_draw(self.ltotal)
+            u = int(np.searchsorted(self.lp, np.uint64(r), side="left"))
+            if u >= N:
+                u = N - 1
+            return u, float(self.lw[u]) / float(self.ltotal)
         if z == 1:
             r = uniform_int(self.sum_deg)
             u = int(np.searchsorted(self.pmax, r, side="left"))
@@ -97,9 +150,7 @@ class PivotLoop:
             return u, float(self.sdn[u]) / float(self.sum_deg)
         total = C.c_uint64(0)
         _check(_ffi.hip().axbc_weights(self.e.ctx, z, C.byref(total)))
-        draw = _libc.lrand48()
-        rd = float(draw) / 2147483647.0 * float(total.value)
-        r = total.value if rd >= float(total.value) else int(rd)
+        r = self._fixed_draw(int(total.value))
         v, p = C.c_uint32(0), C.c_double(0)
         _check(_ffi.hip().axbc_weights_pick(self.e.ctx, r, C.byref(v), C.byref(p)))
         return int(v.value), float(p.value)

@@ -153,6 +153,17 @@ int axbc_hdn_pick(axbc_ctx *ctx, uint32_t r, uint32_t *u, uint32_t *sdn_u);
  */
 int axbc_weights(axbc_ctx *ctx, int kind, uint64_t *total);
 int axbc_weights_pick(axbc_ctx *ctx, uint64_t r, uint32_t *v, double *p);
+/* W[v] of the last axbc_weights call (mixture probabilities, -z 6). */
+int axbc_weights_at(axbc_ctx *ctx, uint32_t v, uint64_t *w);
+
+/*
+ * Local clustering coefficients (lcc_func, src/axbc2d.c:1847-1935) on the
+ * device: float lcc[v] = counter / (uint32)(d*(d-1)) with the wrapping
+ * uint32 count of (j, x) pairs, j in adj(v), x in adj(j) cap adj(v), x != v;
+ * then the static LCC distribution (-z 2) as fixed-point weights and their
+ * prefix.  Host copies (N entries each) may be NULL.
+ */
+int axbc_lcc_build(axbc_ctx *ctx, float *lcc_host, uint64_t *lw_host, uint64_t *lp_host, uint64_t *total);
 
 /*
  * Bit-packed multi-source rounds: nb (1..width) roots in one sweep, width

@@ -239,11 +239,43 @@ uint32_t oracle_weighted_sample(uint32_t N, const uint64_t *W, uint64_t total, l
   return v;
 }
 
+/* lcc_func (axbc2d.c:1847-1935) at 1x1: for a vertex i of degree d >= 2,
+ * counter = number of pairs (j, x) with j in adj(i), x in adj(j), x != i,
+ * x in adj(i) (uint32, wrapping); lcc = (float)counter /
+ * (float)(uint32)(d*(d-1)); 0 for degree < 2.  Adjacency lists are sorted
+ * and duplicate-free, so each membership test is a merge step. */
+void oracle_lcc(uint32_t N, const uint32_t *col, const uint32_t *row, float *lcc) {
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t d = col[i + 1] - col[i];
+    if (d < 2) {
+      lcc[i] = 0.0f;
+      continue;
+    }
+    uint32_t counter = 0;
+    for (uint32_t k = col[i]; k < col[i + 1]; ++k) {
+      const uint32_t j = row[k];
+      uint32_t a = col[j], b = col[i];
+      while (a < col[j + 1] && b < col[i + 1]) {
+        if (row[a] < row[b])
+          ++a;
+        else if (row[a] > row[b])
+          ++b;
+        else {
+          if (row[a] != i) ++counter;
+          ++a;
+          ++b;
+        }
+      }
+    }
+    lcc[i] = (float)counter / (float)(uint32_t)(d * (d - 1u));
+  }
+}
+
 int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
                const oracle_cfg *cfg, uint32_t *pivots, double *probs, double *bc,
                oracle_run_stats *rs) {
   const int z = cfg->distribution;
-  if (!(z == 0 || z == 1 || z == 3 || z == 4 || z == 5)) return -1;
+  if (z < 0 || z > 6) return -1;
   memset(rs, 0, sizeof(*rs));
   memset(bc, 0, sizeof(double) * N);
   uint64_t nbfs = cfg->nbfs < N ? cfg->nbfs : N;
@@ -252,8 +284,21 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
   uint32_t *approx = (uint32_t *)xcalloc(cfg->approx_n, sizeof(uint32_t));
   for (uint32_t i = 0; i < cfg->approx_n; ++i) approx[i] = select_root1(N);
   hdn_tables hdn = {0};
+  uint32_t mix = (uint32_t)z;
+  float *lcc = NULL;
+  uint64_t *Wl = NULL, Tl = 0;
   if (z == 0) (void)select_root1(N); /* discarded initial draw (2740) */
-  if (z == 1) hdn_build(N, col, row, &hdn);
+  if (z == 6) mix = uniform_int(4, 0); /* initial mixture component (2798) */
+  if (z == 1 || z == 6) hdn_build(N, col, row, &hdn);
+  if (z == 2 || z == 6) {
+    lcc = (float *)xcalloc(N, sizeof(float));
+    oracle_lcc(N, col, row, lcc);
+    double *w = (double *)xcalloc(N, sizeof(double));
+    for (uint32_t v = 0; v < N; ++v) w[v] = (double)lcc[v];
+    Wl = (uint64_t *)xcalloc(N, sizeof(uint64_t));
+    Tl = oracle_fixed_weights(N, w, Wl);
+    free(w);
+  }
   int32_t *lvl = (int32_t *)xcalloc(N, sizeof(int32_t));
   uint32_t *sigma = (uint32_t *)xcalloc(N, sizeof(uint32_t));
   double *delta = (double *)xcalloc(N, sizeof(double));
@@ -281,17 +326,40 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
     int good = 0;
     while (!good) {
       if (bsm > lim) break;
-      if (z == 0 || ((z == 3 || z == 4 || z == 5) && rs->nrounds == 0)) {
+      /* component: the distribution itself, or the mixture's current one */
+      const uint32_t k = z == 6 ? mix : (uint32_t)z;
+      if (k == 0 || ((k == 3 || k == 4 || k == 5) && rs->nrounds == 0)) {
         v0 = select_root1(N);
         p = 1.0 / (double)N;
-      } else if (z == 1) {
+      } else if (k == 1) {
         uint32_t r = uniform_int(hdn.sum, 0);
         v0 = hdn_pick(N, col, row, &hdn, r, &p);
+      } else if (k == 2) {
+        v0 = oracle_weighted_sample(N, Wl, Tl, lrand48(), &p);
       } else {
         for (uint32_t v = 0; v < N; ++v)
-          wt[v] = z == 3 ? bc[v] : z == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
+          wt[v] = k == 3 ? bc[v] : k == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
         uint64_t total = oracle_fixed_weights(N, wt, W);
         v0 = oracle_weighted_sample(N, W, total, lrand48(), &p);
+      }
+      if (z == 6) { /* mixture probability and next component (3030-3051) */
+        const int in = v0 < N;
+        p = 1.0 / (double)N;
+        if (rs->nrounds == 0) {
+          p = p + 1.0 / (double)N;
+          p = p + 1.0 / (double)N;
+        } else {
+          for (uint32_t v = 0; v < N; ++v) wt[v] = bc[v];
+          uint64_t tb = oracle_fixed_weights(N, wt, W);
+          p = p + (in ? (double)W[v0] : 0.0) / (double)tb;
+          for (uint32_t v = 0; v < N; ++v) wt[v] = delta[v];
+          uint64_t td = oracle_fixed_weights(N, wt, W);
+          p = p + (in ? (double)W[v0] : 0.0) / (double)td;
+        }
+        p = p + (in ? (double)hdn.sdn[v0] : 0.0) / (double)hdn.sum;
+        p = p + (in ? (double)Wl[v0] : 0.0) / (double)Tl;
+        p = p / 5.0;
+        mix = uniform_int(2, 0);
       }
       if (v0 < N && vs[v0])
         ++bsm;
@@ -311,6 +379,8 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
     rs->nrounds++;
   }
   free(approx);
+  free(lcc);
+  free(Wl);
   free(hdn.prefix);
   free(hdn.sdn);
   free(lvl);

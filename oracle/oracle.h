@@ -40,7 +40,7 @@ typedef struct {
   uint64_t nbfs;        /* -N */
   uint32_t bader;       /* -c */
   uint32_t approx_n;    /* -x */
-  int distribution;     /* -z: 0,1,3,4,5 supported */
+  int distribution;     /* -z: 0..6 */
   int heuristic;        /* -H */
   int random_seed_mode; /* -R: 0 fixed seed */
   int64_t seed;         /* seed for srand48 when random_seed_mode == 0 */
@@ -59,6 +59,10 @@ typedef struct {
 int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
                const oracle_cfg *cfg, uint32_t *pivots, double *probs, double *bc,
                oracle_run_stats *rs);
+
+/* Local clustering coefficient of every vertex as the reference's
+ * lcc_func computes it (float). */
+void oracle_lcc(uint32_t N, const uint32_t *col, const uint32_t *row, float *lcc);
 
 /* Sampling helpers shared by the loop (exposed for the tests). */
 uint32_t oracle_lim_x_sample(uint64_t N);

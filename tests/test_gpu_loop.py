@@ -19,6 +19,7 @@ def test_require(require_gpu):
 @pytest.mark.parametrize("z,c,x,batch,heur", [
     (0, 0, 1, 0, 0), (0, 1, 1, 0, 1), (1, 0, 4, 0, 1), (3, 0, 1, 0, 0), (4, 0, 1, 0, 1), (5, 0, 1, 0, 0),
     (0, 0, 1, 64, 0), (1, 1, 8, 64, 1), (1, 0, 1, 128, 1), (0, 2, 3, 64, 0),
+    (2, 0, 1, 0, 0), (6, 0, 2, 0, 1),
 ])
 def test_loop_matches_oracle(require_gpu, z, c, x, batch, heur):
     g = Graph.kronecker(11, 16, seed=(1, 2), heuristic=heur)
@@ -33,3 +34,24 @@ def test_loop_matches_oracle(require_gpu, z, c, x, batch, heur):
         assert res.stopped == rs["stopped"]
         ok, err = close(e.bc(), bc_ref)
         assert ok, err
+
+
+def test_lcc_matches_oracle(require_gpu):
+    import ctypes as C
+    from axbc_amd import _ffi
+    for g in (Graph.kronecker(12, 16, seed=(1, 2)), Graph.kronecker(11, 32, seed=(7, 8), undirected=False)):
+        lcc_ref = np.zeros(g.N, np.float32)
+        L = oracle.lib()
+        L.oracle_lcc.argtypes = [C.c_uint32, oracle.u32p, oracle.u32p,
+                                 np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
+        L.oracle_lcc(g.N, g.col, g.row, lcc_ref)
+        with Engine(g) as e:
+            lcc = np.empty(g.N, np.float32)
+            lw = np.empty(g.N, np.uint64)
+            t = C.c_uint64(0)
+            assert _ffi.hip().axbc_lcc_build(e.ctx, lcc.ctypes.data, lw.ctypes.data, None, C.byref(t)) == 0
+        np.testing.assert_array_equal(lcc, lcc_ref)
+        W = np.zeros(g.N, np.uint64)
+        total = L.oracle_fixed_weights(g.N, lcc_ref.astype(np.float64), W)
+        np.testing.assert_array_equal(lw, W)
+        assert t.value == total

@@ -239,3 +239,41 @@ def test_stop_test_stops():
     # stopped the criterion holds
     if rs["stopped"]:
         assert math.isfinite(bc.sum())
+
+
+def test_lcc_closed_forms():
+    # triangle with a pendant: lcc(0)=lcc(1)=1 ... vertex 2 (deg 3): 2 of 6
+    g = Graph.from_edges(4, np.array([(0, 1), (1, 2), (0, 2), (2, 3)]))
+    lcc = np.zeros(4, np.float32)
+    oracle.lib().oracle_lcc.argtypes = [oracle.C.c_uint32, oracle.u32p, oracle.u32p,
+                                        np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
+    oracle.lib().oracle_lcc(4, g.col, g.row, lcc)
+    assert list(lcc) == [1.0, 1.0, np.float32(2) / np.float32(6), 0.0]
+    # K_5: every vertex has lcc 1; a star centre has 0
+    k5 = Graph.from_edges(5, np.array([(i, j) for i in range(5) for j in range(i + 1, 5)]))
+    l5 = np.zeros(5, np.float32)
+    oracle.lib().oracle_lcc(5, k5.col, k5.row, l5)
+    assert (l5 == 1.0).all()
+
+
+def test_lcc_matches_triangle_count():
+    # lcc(v) * d(d-1) = 2 * triangles through v on a random undirected graph
+    rng = np.random.default_rng(2)
+    n = 60
+    g = Graph.from_edges(n, rng.integers(0, n, size=(400, 2)))
+    adj = [set(g.row[g.col[v]:g.col[v + 1]]) - {v} for v in range(n)]
+    lcc = np.zeros(n, np.float32)
+    oracle.lib().oracle_lcc(n, g.col, g.row, lcc)
+    for v in range(n):
+        d = g.col[v + 1] - g.col[v]
+        tri = sum(1 for a in adj[v] for b in adj[v] if a < b and b in adj[a])
+        exp = np.float32(0) if d < 2 else np.float32(2 * tri) / np.float32(d * (d - 1))
+        assert lcc[v] == exp
+
+
+@pytest.mark.parametrize("z", [2, 6])
+def test_run_lcc_and_mixture(z):
+    g = Graph.kronecker(9, 16, seed=(1, 2))
+    pv, pr, bc, rs = oracle.run(g, 60, c=0, x=2, z=z)
+    assert rs["nrounds"] > 10
+    assert np.isfinite(pr).all() and (pr > 0).all()
