@@ -21,17 +21,28 @@ from . import _ffi
 from .engine import Engine, _check
 
 _libc = C.CDLL(None)
-_libc.lrand48.restype = C.c_long
-_libc.srand48.argtypes = [C.c_long]
+_libc.nrand48.restype = C.c_long
+_libc.nrand48.argtypes = [C.POINTER(C.c_ushort * 3)]
 
 DEFAULT_SEED = 4343543
 
 
-def uniform_int(mx: int, mn: int = 0) -> int:
-    """lrand48() % (max - min + 1) + min in uint32 (src/axbc2d.c:2156-2164)."""
-    m = (mx - mn + 1) & 0xFFFFFFFF
-    r = _libc.lrand48()
-    return ((r & 0xFFFFFFFF) % m + mn) & 0xFFFFFFFF if m else (r + mn) & 0xFFFFFFFF
+class Rand48:
+    """The lrand48 stream of srand48(seed), with its own state (nrand48), so
+    that concurrent loops do not share libc's global generator."""
+
+    def __init__(self, seed: int):
+        s = seed & 0xFFFFFFFF
+        self.x = (C.c_ushort * 3)(0x330E, s & 0xFFFF, (s >> 16) & 0xFFFF)
+
+    def lrand48(self) -> int:
+        return int(_libc.nrand48(C.byref(self.x)))
+
+    def uniform_int(self, mx: int, mn: int = 0) -> int:
+        """lrand48() % (max - min + 1) + min in uint32 (src/axbc2d.c:2156-2164)."""
+        m = (mx - mn + 1) & 0xFFFFFFFF
+        r = self.lrand48()
+        return ((r & 0xFFFFFFFF) % m + mn) & 0xFFFFFFFF if m else (r + mn) & 0xFFFFFFFF
 
 
 @dataclass
@@ -62,7 +73,8 @@ class PivotLoop:
         self.c, self.x, self.z, self.batch = c, x, z, batch
         self.col, self.row = self.g.col, self.g.row
         self.deg = np.diff(self.col)
-        _libc.srand48(seed)
+        self.rng = Rand48(seed)
+        uniform_int = self.rng.uniform_int
         self.checks = np.array([uniform_int(self.N) for _ in range(x)], np.uint32)
         engine.set_checks(self.checks)
         if batch:
@@ -92,10 +104,9 @@ class PivotLoop:
         self.res = LoopResult()
 
     # -- sampling ---------------------------------------------------------
-    @staticmethod
-    def _fixed_draw(total: int) -> int:
+    def _fixed_draw(self, total: int) -> int:
         """r = lrand48()/RAND_MAX * total (uniform_double), clamped."""
-        rd = float(_libc.lrand48()) / 2147483647.0 * float(total)
+        rd = float(self.rng.lrand48()) / 2147483647.0 * float(total)
         return total if rd >= float(total) else int(rd)
 
     def _device_weight(self, kind: int, v: int) -> tuple[int, int]:
@@ -125,11 +136,12 @@ class PivotLoop:
         p = p + (float(self.sdn[v0]) if inside else 0.0) / float(self.sum_deg)
         p = p + (float(self.lw[v0]) if inside else 0.0) / float(self.ltotal)
         p = p / 5.0
-        self.mix = uniform_int(2)
+        self.mix = self.rng.uniform_int(2)
         return v0, p
 
     def _draw_component(self, z: int) -> tuple[int, float]:
         N = self.N
+        uniform_int = self.rng.uniform_int
         if z == 0 or (z in (3, 4, 5) and self.res.nrounds == 0):
             return uniform_int(N), 1.0 / N
         if z == 2:

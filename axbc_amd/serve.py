@@ -1,0 +1,216 @@
+"""Serving: BC-approximation jobs on device-resident graphs, admitted as they
+arrive.
+
+A job names a graph (Kronecker parameters or a prebuilt Graph), a pivot
+budget and the pivot-loop flags.  The server keeps `slots` device workers;
+each worker owns one engine context (its own HIP stream) at a time, so the
+rounds of different jobs run concurrently on the GPU.  A job is admitted to
+a free slot when its device footprint fits under the memory cap next to the
+jobs already resident (jobs that do not fit wait; smaller jobs behind them
+may pass).  Graph construction runs on host threads ahead of admission.
+Latency of a job = completion time - arrival time.
+
+No reference counterpart: AxBC runs one job per process.
+"""
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import Engine
+from .graph import Graph
+from .pivots import PivotLoop
+
+PLAIN_STATE_BYTES = 4 + 4 + 8 + 8 + 4 + 4 + 4  # lvl, sigma, delta, bc, queue, qoff, reach per vertex
+SAMPLER_BYTES = 28 + 24 + 8                    # sampler scratch, LCC tables, slack per vertex
+
+
+def footprint_bytes(N: int, ned: int, batch: int = 0) -> int:
+    """Device bytes a job needs: CSC + per-vertex state + sampler tables
+    (+ the batch rows when batched)."""
+    b = 4 * (N + 1) + 4 * ned + N * (PLAIN_STATE_BYTES + SAMPLER_BYTES)
+    if batch:
+        words = batch // 64
+        b += N * (batch * (4 + 8 + 8) + 8 * 4 * words + 8) + 4 * N * (4 + 8 * words)
+    return b
+
+
+@dataclass
+class Job:
+    jid: int
+    budget: int                   # pivot attempts (-N)
+    z: int = 6
+    c: int = 0
+    x: int = 1
+    heuristic: int = 1
+    batch: int = 0
+    scale: int = 0
+    edgefactor: int = 16
+    seed: tuple = (1, 2)
+    graph: Graph | None = None
+    arrival: float = 0.0
+    # results
+    admitted: float = 0.0
+    finished: float = 0.0
+    nrounds: int = 0
+    bc: np.ndarray | None = None
+    pivots: list = field(default_factory=list)
+    error: str | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class Server:
+    def __init__(self, slots: int = 4, hbm_cap: float = 200e9, device: int = 0, build_threads: int = 4,
+                 keep_bc: bool = True):
+        self.slots, self.cap, self.device, self.keep_bc = slots, hbm_cap, device, keep_bc
+        self._ready: list[Job] = []      # built, waiting for admission (arrival order)
+        self._lock = threading.Condition()
+        self._used = 0
+        self._active = 0
+        self._done: list[Job] = []
+        self._pending = 0
+        self._stop = False
+        self._builder = ThreadPoolExecutor(max_workers=build_threads)
+        self._workers = [threading.Thread(target=self._work, daemon=True) for _ in range(slots)]
+        for w in self._workers:
+            w.start()
+
+    # -- client side ------------------------------------------------------
+    def submit(self, job: Job) -> None:
+        job.arrival = job.arrival or time.perf_counter()
+        with self._lock:
+            self._pending += 1
+        self._builder.submit(self._build, job)
+
+    def drain(self, timeout: float | None = None) -> list[Job]:
+        t0 = time.perf_counter()
+        with self._lock:
+            while self._pending:
+                left = None if timeout is None else timeout - (time.perf_counter() - t0)
+                if left is not None and left <= 0:
+                    break
+                self._lock.wait(left if left is not None else 1.0)
+            return list(self._done)
+
+    def close(self) -> None:
+        with self._lock:
+            self._stop = True
+            self._lock.notify_all()
+        for w in self._workers:
+            w.join()
+        self._builder.shutdown(wait=True)
+
+    # -- internals ----------------------------------------------------------
+    def _build(self, job: Job) -> None:
+        try:
+            if job.graph is None:
+                job.graph = Graph.kronecker(job.scale, job.edgefactor, seed=job.seed, heuristic=job.heuristic,
+                                            threads=2)
+        except Exception as ex:  # the job fails, the server goes on
+            job.error = f"graph build failed: {ex}"
+        with self._lock:
+            self._ready.append(job)
+            self._lock.notify_all()
+
+    def _admit(self) -> Job | None:
+        """First built job (arrival order) whose footprint fits; a job larger
+        than the whole cap is failed immediately."""
+        for i, job in enumerate(self._ready):
+            if job.error:
+                return self._ready.pop(i)
+            need = footprint_bytes(job.graph.N, job.graph.ned, job.batch)
+            if need > self.cap:
+                job.error = f"footprint {need} B exceeds the memory cap {int(self.cap)} B"
+                return self._ready.pop(i)
+            if self._used + need <= self.cap:
+                job._need = need
+                self._used += need
+                return self._ready.pop(i)
+        return None
+
+    def _work(self) -> None:
+        while True:
+            with self._lock:
+                job = None
+                while job is None:
+                    if self._stop:
+                        return
+                    job = self._admit()
+                    if job is None:
+                        self._lock.wait(0.05)
+                self._active += 1
+            job.admitted = time.perf_counter()
+            try:
+                if not job.error:
+                    self._run(job)
+            except Exception as ex:
+                job.error = str(ex)
+            job.finished = time.perf_counter()
+            with self._lock:
+                self._used -= getattr(job, "_need", 0)
+                self._active -= 1
+                self._pending -= 1
+                self._done.append(job)
+                self._lock.notify_all()
+
+    def _run(self, job: Job) -> None:
+        with Engine(job.graph, self.device) as e:
+            loop = PivotLoop(e, job.budget, c=job.c, x=job.x, z=job.z, batch=job.batch)
+            res = loop.run()
+            job.nrounds = res.nrounds
+            job.pivots = res.pivots
+            if self.keep_bc:
+                job.bc = e.bc()
+        if not self.keep_bc:
+            job.graph = None
+
+
+def poisson_jobs(n: int, rate: float, seed: int = 0, scales=(12, 20), efs=(8, 16, 32), budget=(64, 2048),
+                 z: int = 6) -> list[tuple[float, Job]]:
+    """The serving workload: Poisson arrivals at `rate` per second; per job a
+    Kronecker SCALE drawn uniformly from `scales`, an edge factor from `efs`,
+    the job id as seed, and a pivot budget log-uniform in `budget`."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for j in range(n):
+        t += rng.exponential(1.0 / rate)
+        s = int(rng.integers(scales[0], scales[1] + 1))
+        ef = int(rng.choice(efs))
+        b = int(round(np.exp(rng.uniform(np.log(budget[0]), np.log(budget[1])))))
+        out.append((t, Job(jid=j, budget=b, z=z, scale=s, edgefactor=ef, seed=(j + 1, j + 2))))
+    return out
+
+
+def run_workload(jobs: list[tuple[float, Job]], slots: int = 4, hbm_cap: float = 200e9,
+                 keep_bc: bool = False) -> dict:
+    """Replay arrivals in real time and report latency percentiles and the
+    aggregate pivot throughput."""
+    srv = Server(slots=slots, hbm_cap=hbm_cap, keep_bc=keep_bc)
+    t0 = time.perf_counter()
+    for at, job in jobs:
+        now = time.perf_counter() - t0
+        if at > now:
+            time.sleep(at - now)
+        job.arrival = time.perf_counter()
+        srv.submit(job)
+    done = srv.drain()
+    t1 = time.perf_counter()
+    srv.close()
+    ok = [j for j in done if not j.error]
+    lat = np.array([j.latency for j in ok]) if ok else np.zeros(1)
+    rounds = sum(j.nrounds for j in ok)
+    return {
+        "jobs": len(done), "failed": len(done) - len(ok),
+        "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+        "mean_s": float(lat.mean()), "pivots": rounds, "wall_s": t1 - t0,
+        "pivots_per_s": rounds / (t1 - t0) if t1 > t0 else 0.0,
+    }

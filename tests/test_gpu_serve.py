@@ -1,0 +1,82 @@
+"""Serving: each job's result is the same alone and in a concurrent mix
+(streams overlap on the device), admission respects the memory cap, and
+oversize jobs are rejected."""
+import numpy as np
+import pytest
+
+from axbc_amd.serve import Job, Server, footprint_bytes
+from tests.test_gpu_plain import close
+
+pytestmark = pytest.mark.gpu
+
+
+def test_require(require_gpu):
+    pass
+
+
+def jobs():
+    return [Job(jid=0, budget=60, z=6, scale=11, edgefactor=16, seed=(1, 2)),
+            Job(jid=1, budget=80, z=1, scale=12, edgefactor=8, seed=(3, 4), batch=64, x=4, c=0),
+            Job(jid=2, budget=50, z=0, scale=10, edgefactor=32, seed=(5, 6)),
+            Job(jid=3, budget=70, z=4, scale=11, edgefactor=16, seed=(7, 8))]
+
+
+def test_alone_equals_mix(require_gpu):
+    print("[serve] alone", flush=True)
+    alone = {}
+    for j in jobs():
+        srv = Server(slots=1)
+        srv.submit(j)
+        (d,) = srv.drain(timeout=300)
+        srv.close()
+        assert d.error is None, d.error
+        alone[d.jid] = (d.bc, d.pivots)
+    print("[serve] mix", flush=True)
+    srv = Server(slots=4)
+    for j in jobs():
+        srv.submit(j)
+    done = srv.drain(timeout=300)
+    srv.close()
+    assert len(done) == 4
+    for d in done:
+        assert d.error is None, d.error
+        bc, piv = alone[d.jid]
+        assert d.pivots == piv
+        ok, err = close(d.bc, bc)
+        assert ok, (d.jid, err)
+
+
+def test_admission_by_footprint(require_gpu):
+    js = jobs()[:3]
+    for j in js:
+        j.graph = None
+    # cap that holds the largest job but not two of them at once
+    from axbc_amd import Graph
+    sizes = []
+    for j in js:
+        g = Graph.kronecker(j.scale, j.edgefactor, seed=j.seed, heuristic=j.heuristic)
+        j.graph = g
+        sizes.append(footprint_bytes(g.N, g.ned, j.batch))
+    cap = max(sizes) + min(sizes) // 2
+    srv = Server(slots=3, hbm_cap=cap)
+    peak = [0]
+    orig = srv._admit
+
+    def watch():
+        job = orig()
+        peak[0] = max(peak[0], srv._used)
+        return job
+
+    srv._admit = watch
+    for j in js:
+        srv.submit(j)
+    done = srv.drain(timeout=300)
+    srv.close()
+    assert all(d.error is None for d in done) and len(done) == 3
+    assert peak[0] <= cap
+    big = Job(jid=9, budget=10, z=0, scale=12, edgefactor=16, seed=(1, 2))
+    srv = Server(slots=1, hbm_cap=1e6)
+    srv.submit(big)
+    (d,) = srv.drain(timeout=120)
+    srv.close()
+    assert d.error and "exceeds" in d.error

@@ -136,3 +136,16 @@ def test_lattice_properties():
     assert len(seen) == int((deg > 0).sum())
     full = 2 * 64 * 63
     assert 0.9 * full < len(e) / 2 <= full
+
+
+def test_serving_footprint_and_workload():
+    from axbc_amd.serve import footprint_bytes, poisson_jobs
+    assert footprint_bytes(1 << 20, 30_000_000) > 4 * 30_000_000
+    assert footprint_bytes(1 << 20, 1000, batch=64) > footprint_bytes(1 << 20, 1000)
+    js = poisson_jobs(200, 20.0, seed=1)
+    t = np.array([a for a, _ in js])
+    assert (np.diff(t) > 0).all() and 5 < t[-1] < 20  # ~10 s of arrivals at 20/s
+    s = np.array([j.scale for _, j in js])
+    b = np.array([j.budget for _, j in js])
+    assert s.min() >= 12 and s.max() <= 20 and b.min() >= 64 and b.max() <= 2048
+    assert {j.edgefactor for _, j in js} <= {8, 16, 32}
