@@ -45,7 +45,7 @@ using namespace axbc;
 
 namespace axbc {
 
-constexpr uint32_t kChunk = 256;  // edges per wave work item for split vertices
+constexpr uint32_t kChunkMin = 64;  // smallest edges-per-work-item for split vertices
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
 
 struct BCtl {
@@ -92,6 +92,7 @@ struct BatchState {
   uint32_t epoch = 0;  // level stamps for `mark`, unique across batches
   bool have = false;
   int lev_guess = 12;
+  uint32_t chunk = 256;  // edges per work item of split vertices (tunable)
   size_t bytes = 0;
 };
 
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
                                                   uint64_t *fr_clear, uint32_t *q, const uint64_t *__restrict__ qw,
                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                   uint32_t *sp_v, uint32_t sp_cap, uint32_t qcap, int L,
-                                                  uint32_t mark_id, unsigned long long *kb) {
+                                                  uint32_t mark_id, unsigned long long *kb, uint32_t kChunk) {
   if (ldg_acq(&ctl->done)) return;
   const int lane = lane_id();
   unsigned long long bytes = 0;
@@ -357,7 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restri
                                                         const uint64_t *__restrict__ vis, uint64_t *nx,
                                                         uint32_t *mark, uint32_t *q, const uint64_t *__restrict__ qw,
                                                         axbc::BCtl *ctl, const uint32_t *__restrict__ sp_v,
-                                                        uint32_t sp_cap, uint32_t qcap, uint32_t mark_id) {
+                                                        uint32_t sp_cap, uint32_t qcap, uint32_t mark_id,
+                                                        uint32_t kChunk) {
   if (ldg_acq(&ctl->done)) return;
   const uint32_t ns = min(ctl->nsplit, sp_cap);
   const int lane = lane_id();
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
                                                    uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t *sp_e, uint32_t sp_cap, int L,
-                                                   uint32_t qcap, unsigned long long *kb) {
+                                                   uint32_t qcap, unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
   unsigned long long bytes = 0;
@@ -501,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
                                                          const uint64_t *__restrict__ frp, uint32_t *sig,
                                                          const uint32_t *__restrict__ q, axbc::BCtl *ctl,
                                                          unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
-                                                         uint32_t sp_cap) {
+                                                         uint32_t sp_cap, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
   const uint32_t ns = min(ctl->nsplit, sp_cap);
@@ -672,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
                                                  int depth, uint64_t root_mask0, uint64_t root_mask1,
-                                                 unsigned long long *kb) {
+                                                 unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   const uint32_t b = qoff[depth], e = qoff[depth + 1];
   unsigned long long bytes = 0;
@@ -734,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restric
                                                        axbc::BCtl *ctl, const uint32_t *__restrict__ sp_e,
                                                        const uint64_t *__restrict__ sp_base, uint32_t *sp_cnt,
                                                        uint32_t sp_cap, double *part, int depth, uint64_t root_mask0,
-                                                       uint64_t root_mask1) {
+                                                       uint64_t root_mask1, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   const unsigned long long claim = ctl->split_claim;
   const uint32_t ns = min((uint32_t)(claim >> 40), sp_cap);
@@ -976,9 +978,11 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   // split entries: vertices with more than kChunk edges, per level at most
   // all of them; chunk rows: at most ned / kChunk + number of them
   uint32_t nsplit = 0;
-  for (size_t v = 0; v < n; ++v) nsplit += (c->h_col[v + 1] - c->h_col[v]) > kChunk;
+  // sized for the smallest chunk so that the chunk can be tuned later
+  for (size_t v = 0; v < n; ++v) nsplit += (c->h_col[v + 1] - c->h_col[v]) > kChunkMin;
   b->sp_cap = nsplit + 1;
-  b->part_rows = c->ned / kChunk + nsplit + 1;
+  b->part_rows = c->ned / kChunkMin + nsplit + 1;
+  b->chunk = c->batch_chunk;
   b->qcap = qcap;
 #define BA(p, cnt)                         \
   if ((rc = balloc(b, &(p), (cnt))) != 0) { \
@@ -1070,17 +1074,18 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       const uint32_t mark_id = b->epoch + (uint32_t)lv;
       kt_begin(c, KK_PUSH);
       hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, frc,
-                         b->q, b->qw, b->qoff, b->ctl, b->sp_v, b->sp_cap, b->qcap, lv, mark_id, kt_bytes(c, KK_PUSH));
+                         b->q, b->qw, b->qoff, b->ctl, b->sp_v, b->sp_cap, b->qcap, lv, mark_id, kt_bytes(c, KK_PUSH),
+                         b->chunk);
       kt_end(c, KK_PUSH);
       hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
-                         b->mark, b->q, b->qw, b->ctl, b->sp_v, b->sp_cap, b->qcap, mark_id);
+                         b->mark, b->q, b->qw, b->ctl, b->sp_v, b->sp_cap, b->qcap, mark_id, b->chunk);
       hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
       kt_begin(c, KK_SIGMA);
       hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp, b->sig, b->q,
-                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_cap, lv, b->qcap, kt_bytes(c, KK_SIGMA));
+                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_cap, lv, b->qcap, kt_bytes(c, KK_SIGMA), b->chunk);
       kt_end(c, KK_SIGMA);
       hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
-                         b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_cap);
+                         b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_cap, b->chunk);
       kt_begin(c, KK_CLOSE);
       hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, frc,
                          b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap);
@@ -1118,11 +1123,11 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     kt_begin(c, KK_DEP);
     hipLaunchKernelGGL(k_bdep<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
                        c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, d,
-                       rootmask[0], NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP));
+                       rootmask[0], NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP), b->chunk);
     kt_end(c, KK_DEP);
     hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt,
                        b->xr, c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cnt, b->sp_cap, b->part, d,
-                       rootmask[0], NW > 1 ? rootmask[1] : 0ull);
+                       rootmask[0], NW > 1 ? rootmask[1] : 0ull, b->chunk);
     hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
   }
   HIP_TRY(hipGetLastError());
@@ -1148,6 +1153,12 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
                      b->prob, b->r0, b->dlt, b->cnt, c->bc);
   kt_end(c, KK_UPDATE);
   HIP_TRY(hipGetLastError());
+  // the context's delta becomes the last applied root's (the -z 4
+  // distribution of the next draw); plain rounds re-initialise it
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_bcopy_delta<NW>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, nr - 1, b->dlt, c->delta);
+    c->dirty_delta = true;
+  }
   if (applied) *applied = nr;
   if (stats) {
     HIP_TRY(hipMemcpyAsync(b->h_cnt, b->cnt, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost, s));
@@ -1175,6 +1186,14 @@ int axbc_set_batch_width(axbc_ctx *c, int width) {
 }
 
 int axbc_batch_width(axbc_ctx *c) { return c ? c->batch_width : 0; }
+
+int axbc_set_batch_chunk(axbc_ctx *c, uint32_t chunk) {
+  if (!c || chunk < kChunkMin || chunk > (1u << 20))
+    return fail(AXBC_EINVAL, "chunk must be in [64, 2^20]%s (line %d)", "", __LINE__);
+  c->batch_chunk = chunk;
+  if (c->batch) c->batch->chunk = chunk;
+  return AXBC_OK;
+}
 
 int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
                axbc_pivot_stats *stats) {

@@ -136,6 +136,15 @@ __global__ void k_reset_visited(const uint32_t *__restrict__ queue, const DevCtl
   }
 }
 
+__global__ void k_reset_all(uint32_t n, const uint32_t *__restrict__ reach, int *lvl, uint32_t *sigma,
+                            double *delta) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    lvl[v] = -1;
+    sigma[v] = 0;
+    delta[v] = (double)reach[v];
+  }
+}
+
 __global__ void k_set_root(uint32_t v0, int *lvl, uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
                            DevCtl *ctl) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -510,7 +519,11 @@ int enqueue_backward(axbc_ctx *c, int depth) {
 }
 
 int enqueue_reset_and_root(axbc_ctx *c, uint32_t v0) {
-  if (c->have_round)
+  if (c->dirty_delta) {
+    hipLaunchKernelGGL(k_reset_all, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->reach, c->lvl,
+                       c->sigma, c->delta);
+    c->dirty_delta = false;
+  } else if (c->have_round)
     hipLaunchKernelGGL(k_reset_visited, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->queue,
                        c->ctl, c->reach, c->lvl, c->sigma, c->delta);
   hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, c->stream, v0, c->lvl, c->sigma, c->queue,
@@ -884,7 +897,11 @@ int axbc_stop_test(axbc_ctx *c, uint32_t cc, int *stop) {
 
 int axbc_begin_round(axbc_ctx *c) {
   if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
-  if (c->have_round)
+  if (c->dirty_delta) {
+    hipLaunchKernelGGL(k_reset_all, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->reach, c->lvl,
+                       c->sigma, c->delta);
+    c->dirty_delta = false;
+  } else if (c->have_round)
     hipLaunchKernelGGL(k_reset_visited, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->queue,
                        c->ctl, c->reach, c->lvl, c->sigma, c->delta);
   HIP_TRY(hipMemsetAsync(c->ctl, 0, sizeof(DevCtl), c->stream));

@@ -116,12 +116,14 @@ struct axbc_ctx {
   // round bookkeeping
   int lev_guess = 8;
   bool have_round = false;
+  bool dirty_delta = false;  // delta holds a batch root's values: full reset before the next round
   uint32_t last_v0 = 0;
   int last_nlev = 0;
   // samplers and batches
   axbc::SamplerBufs *sampler = nullptr;
   axbc::BatchState *batch = nullptr;
   int batch_width = 64;
+  uint32_t batch_chunk = 256;
   size_t budget = 0;  // 0: no limit
   axbc::KernelTimer ktimer;
 };

@@ -65,8 +65,8 @@ class PivotLoop:
                  seed: int = DEFAULT_SEED, batch: int = 0):
         if z not in range(7):
             raise ValueError("distributions are 0..6")
-        if batch and z not in (0, 1):
-            raise ValueError("batches need a static distribution (-z 0 or -z 1)")
+        if batch and z not in (0, 1, 3, 4, 5):
+            raise ValueError("batches need -z 0, 1 (exact) or 3, 4, 5 (batch-synchronous)")
         self.e, self.g = engine, engine.g
         self.N = self.g.N
         self.nbfs = min(nbfs, self.N)
@@ -78,7 +78,9 @@ class PivotLoop:
         self.checks = np.array([uniform_int(self.N) for _ in range(x)], np.uint32)
         engine.set_checks(self.checks)
         if batch:
-            _check(_ffi.hip().axbc_set_batch_width(engine.ctx, batch))
+            if batch not in (32, 64, 128):
+                raise ValueError("roots per batch: 32, 64 or 128")
+            _check(_ffi.hip().axbc_set_batch_width(engine.ctx, 128 if batch == 128 else 64))
         self.mix = z
         if z == 0:
             uniform_int(self.N)  # discarded initial draw (axbc2d.c:2740)
@@ -142,8 +144,15 @@ class PivotLoop:
     def _draw_component(self, z: int) -> tuple[int, float]:
         N = self.N
         uniform_int = self.rng.uniform_int
-        if z == 0 or (z in (3, 4, 5) and self.res.nrounds == 0):
+        first = self.res.nrounds < self.batch if self.batch else self.res.nrounds == 0
+        if z == 0 or (z in (3, 4, 5) and first):
             return uniform_int(N), 1.0 / N
+        if z in (3, 4, 5) and self.batch:
+            # batch-synchronous: the distribution frozen at the batch start
+            r = self._fixed_draw(self._frozen_total)
+            v, p = C.c_uint32(0), C.c_double(0)
+            _check(_ffi.hip().axbc_weights_pick(self.e.ctx, r, C.byref(v), C.byref(p)))
+            return int(v.value), float(p.value)
         if z == 2:
             r = self._fixed_draw(self.ltotal)
             u = int(np.searchsorted(self.lp, np.uint64(r), side="left"))
@@ -197,6 +206,10 @@ class PivotLoop:
             res.stopped = True
             return 0
         want = max_rounds or (self.batch or 1)
+        if self.batch and self.z in (3, 4, 5) and self.res.nrounds >= self.batch:
+            t = C.c_uint64(0)
+            _check(_ffi.hip().axbc_weights(e.ctx, self.z, C.byref(t)))
+            self._frozen_total = int(t.value)
         pend, pend_ui, pend_skip = [], [], []
         while self.ui < self.nbfs and len(pend) < want:
             skipped_before = res.skipped

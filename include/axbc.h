@@ -178,6 +178,9 @@ int axbc_lcc_build(axbc_ctx *ctx, float *lcc_host, uint64_t *lw_host, uint64_t *
  * No reference counterpart: the reference runs one root per round.
  */
 int axbc_set_batch_width(axbc_ctx *ctx, int width);
+/* Edges per work item for vertices split across waves (64 .. 2^20;
+ * default 256): the degree bucket boundary the autotuner sets. */
+int axbc_set_batch_chunk(axbc_ctx *ctx, uint32_t chunk);
 int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);

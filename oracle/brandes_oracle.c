@@ -306,6 +306,9 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
   uint64_t *W = (uint64_t *)xcalloc(N, sizeof(uint64_t));
   uint8_t *vs = (uint8_t *)xcalloc(N, 1);
   const uint32_t lim = oracle_lim_x_sample(N);
+  uint64_t frozen_total = 0, frozen_at = 0;
+  int frozen_ok = 0, frozen_uniform = 0;
+  (void)frozen_uniform;
   const double cN = (double)((uint64_t)cfg->bader * (uint64_t)N);
   for (uint64_t ui = 0; ui < nbfs; ++ui) {
     rs->attempts = ui + 1;
@@ -328,7 +331,8 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
       if (bsm > lim) break;
       /* component: the distribution itself, or the mixture's current one */
       const uint32_t k = z == 6 ? mix : (uint32_t)z;
-      if (k == 0 || ((k == 3 || k == 4 || k == 5) && rs->nrounds == 0)) {
+      const int first_group = cfg->sync_batch ? rs->nrounds < cfg->sync_batch : rs->nrounds == 0;
+      if (k == 0 || ((k == 3 || k == 4 || k == 5) && first_group)) {
         v0 = select_root1(N);
         p = 1.0 / (double)N;
       } else if (k == 1) {
@@ -336,6 +340,17 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
         v0 = hdn_pick(N, col, row, &hdn, r, &p);
       } else if (k == 2) {
         v0 = oracle_weighted_sample(N, Wl, Tl, lrand48(), &p);
+      } else if (cfg->sync_batch && (k == 3 || k == 4 || k == 5)) {
+        /* batch-synchronous: the distribution of the group's first round */
+        if (frozen_at != rs->nrounds / cfg->sync_batch || !frozen_ok) {
+          frozen_at = rs->nrounds / cfg->sync_batch;
+          frozen_uniform = (rs->nrounds % cfg->sync_batch == 0 && rs->nrounds == 0);
+          for (uint32_t v = 0; v < N; ++v)
+            wt[v] = k == 3 ? bc[v] : k == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
+          frozen_total = oracle_fixed_weights(N, wt, W);
+          frozen_ok = 1;
+        }
+        v0 = oracle_weighted_sample(N, W, frozen_total, lrand48(), &p);
       } else {
         for (uint32_t v = 0; v < N; ++v)
           wt[v] = k == 3 ? bc[v] : k == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);

@@ -44,6 +44,9 @@ typedef struct {
   int heuristic;        /* -H */
   int random_seed_mode; /* -R: 0 fixed seed */
   int64_t seed;         /* seed for srand48 when random_seed_mode == 0 */
+  uint32_t sync_batch;  /* > 0: dynamic distributions (-z 3/4/5) frozen for
+                           every group of sync_batch rounds (batch-synchronous
+                           sampling); 0: the reference's per-round update */
 } oracle_cfg;
 
 typedef struct {

@@ -28,7 +28,7 @@ class Stats(C.Structure):
 class Cfg(C.Structure):
     _fields_ = [("nbfs", C.c_uint64), ("bader", C.c_uint32), ("approx_n", C.c_uint32),
                 ("distribution", C.c_int), ("heuristic", C.c_int), ("random_seed_mode", C.c_int),
-                ("seed", C.c_int64)]
+                ("seed", C.c_int64), ("sync_batch", C.c_uint32)]
 
 
 class RunStats(C.Structure):
@@ -84,10 +84,11 @@ def pivot(g, v0: int, p: float, bc: np.ndarray | None = None):
     return lvl, sig, dl, bc, {k: int(getattr(st, k)) for k, _ in Stats._fields_}
 
 
-def run(g, nbfs: int, c: int = 1, x: int = 1, z: int = 0, seed: int = 4343543):
-    """Full pivot loop; returns (pivots, probs, bc, stats dict)."""
+def run(g, nbfs: int, c: int = 1, x: int = 1, z: int = 0, seed: int = 4343543, sync_batch: int = 0):
+    """Full pivot loop; returns (pivots, probs, bc, stats dict).  sync_batch
+    > 0 freezes dynamic distributions for every group of that many rounds."""
     N = g.N
-    cfg = Cfg(nbfs, c, x, z, int(g.heuristic), 0, seed)
+    cfg = Cfg(nbfs, c, x, z, int(g.heuristic), 0, seed, sync_batch)
     pv = np.zeros(max(1, min(nbfs, N)), np.uint32)
     pr = np.zeros(max(1, min(nbfs, N)), np.float64)
     bc = np.zeros(N, np.float64)

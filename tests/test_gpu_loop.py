@@ -19,17 +19,21 @@ def test_require(require_gpu):
 @pytest.mark.parametrize("z,c,x,batch,heur", [
     (0, 0, 1, 0, 0), (0, 1, 1, 0, 1), (1, 0, 4, 0, 1), (3, 0, 1, 0, 0), (4, 0, 1, 0, 1), (5, 0, 1, 0, 0),
     (0, 0, 1, 64, 0), (1, 1, 8, 64, 1), (1, 0, 1, 128, 1), (0, 2, 3, 64, 0),
-    (2, 0, 1, 0, 0), (6, 0, 2, 0, 1),
+    (2, 0, 1, 0, 0), (6, 0, 2, 0, 1), (4, 0, 1, 64, 0), (3, 1, 2, 32, 1), (5, 0, 1, 128, 0),
 ])
 def test_loop_matches_oracle(require_gpu, z, c, x, batch, heur):
     g = Graph.kronecker(11, 16, seed=(1, 2), heuristic=heur)
     nbfs = 300
     print(f"[loop z{z} c{c} x{x} B{batch} H{heur}]", flush=True)
-    pv, pr, bc_ref, rs = oracle.run(g, nbfs, c=c, x=x, z=z)
+    sync = batch if z in (3, 4, 5) else 0
+    pv, pr, bc_ref, rs = oracle.run(g, nbfs, c=c, x=x, z=z, sync_batch=sync)
     with Engine(g) as e:
         res = PivotLoop(e, nbfs, c=c, x=x, z=z, batch=batch).run()
         assert res.pivots == [int(v) for v in pv]
-        np.testing.assert_array_equal(np.array(res.probs), pr)
+        # probabilities of static distributions are exact; dynamic ones are
+        # ratios of fixed-point weights of the FP64 bc/delta, equal within
+        # the FP64 tolerance
+        np.testing.assert_allclose(np.array(res.probs), pr, rtol=0 if z in (0, 1, 2) else 1e-9)
         assert res.nrounds == rs["nrounds"] and res.skipped == rs["skipped"]
         assert res.stopped == rs["stopped"]
         ok, err = close(e.bc(), bc_ref)

@@ -277,3 +277,14 @@ def test_run_lcc_and_mixture(z):
     pv, pr, bc, rs = oracle.run(g, 60, c=0, x=2, z=z)
     assert rs["nrounds"] > 10
     assert np.isfinite(pr).all() and (pr > 0).all()
+
+
+def test_sync_batch_freezes_dynamic_distribution():
+    g = Graph.kronecker(9, 16, seed=(1, 2))
+    pv0, _, _, rs0 = oracle.run(g, 80, c=0, x=1, z=4)
+    pv1, _, _, rs1 = oracle.run(g, 80, c=0, x=1, z=4, sync_batch=1)
+    # groups of one round are the per-round rule
+    assert list(pv0) == list(pv1)
+    pv8, pr8, _, _ = oracle.run(g, 80, c=0, x=1, z=4, sync_batch=8)
+    assert (pr8[:8] == 1.0 / g.N).all()  # first group: uniform
+    assert list(pv8[:8]) == list(pv0[:1]) + list(pv8[1:8])
