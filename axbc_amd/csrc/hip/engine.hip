@@ -642,6 +642,7 @@ int axbc_ctx_destroy(axbc_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   sampler_free(c);
   batch_free(c);
+  mlp_free(c);
   for (auto &pr : c->ktimer.pending) {
     (void)hipEventDestroy(pr.second.first);
     (void)hipEventDestroy(pr.second.second);

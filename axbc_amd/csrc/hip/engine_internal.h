@@ -122,6 +122,7 @@ struct axbc_ctx {
   // samplers and batches
   axbc::SamplerBufs *sampler = nullptr;
   axbc::BatchState *batch = nullptr;
+  void *mlp = nullptr;  // learned proposal buffers (mlp.hip)
   int batch_width = 64;
   uint32_t batch_chunk = 256;
   size_t budget = 0;  // 0: no limit
@@ -132,6 +133,7 @@ namespace axbc {
 int sampler_bufs(axbc_ctx *c, SamplerBufs **out);
 void sampler_free(axbc_ctx *c);
 void batch_free(axbc_ctx *c);
+void mlp_free(axbc_ctx *c);
 int grid_for(uint64_t n, int cap = 2048);
 // Event pair around one launch of kind k when instrumentation is on.
 void kt_begin(axbc_ctx *c, int kind);

@@ -1,0 +1,389 @@
+// Learned pivot proposal: a 3-layer MLP over per-vertex features
+// (log degree, LCC, running bc, last delta) -> score, with BF16 weights,
+// FP32 accumulation, hand-written forward, backward and Adam kernels.
+//
+// Network:  x (4) -> h1 = relu(W1 x + b1) (64) -> h2 = relu(W2 h1 + b2) (64)
+//           -> s = w3 . h2 + b3.
+// Training: mean squared error between s and a target per vertex, minibatches
+// of vertices drawn by the host; gradients summed per workgroup in LDS and
+// added to FP32 gradient buffers; Adam on FP32 master weights, BF16 copies
+// used by the forward passes.
+// Proposal: w(v) = softplus(s(v)) + eps * mean softplus, turned into the
+// fixed-point distribution of the dynamic samplers (kind 7).
+//
+// No reference counterpart: it stands in for the bc/delta heuristics
+// (-z 3/4, src/axbc2d.c:2941-2994) as a pivot proposal.
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../../include/axbc.h"
+#include "engine_internal.h"
+#include "wave.h"
+
+using namespace axbc;
+
+namespace {
+
+constexpr int kIn = 4, kH = 64;
+// parameter layout (floats): W1[64][4], b1[64], W2[64][64], b2[64], w3[64], b3
+constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
+constexpr int kParams = oB3 + 1;  // 4481
+
+struct MlpBufs {
+  float *x = nullptr;        // N x 4 features
+  float *y = nullptr;        // N targets
+  float *score = nullptr;    // N scores
+  float *master = nullptr;   // FP32 master weights
+  __hip_bfloat16 *wb = nullptr;  // BF16 weights
+  float *grad = nullptr, *m = nullptr, *v = nullptr;
+  float *loss = nullptr;
+  double *prop = nullptr;    // N proposal weights
+  uint32_t *idx = nullptr;
+  uint32_t idx_cap = 0;
+  int step = 0;
+};
+
+MlpBufs *bufs_of(axbc_ctx *c) { return reinterpret_cast<MlpBufs *>(c->mlp); }
+
+
+__device__ __forceinline__ float bf(const __hip_bfloat16 *w, int i) { return __bfloat162float(w[i]); }
+
+__global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const float *__restrict__ lcc,
+                           const double *__restrict__ bc, const double *__restrict__ delta, double bc_scale,
+                           double delta_scale, float *x) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const float d = (float)(col[v + 1] - col[v]);
+    x[4 * v + 0] = log1pf(d);
+    x[4 * v + 1] = lcc ? lcc[v] : 0.0f;
+    x[4 * v + 2] = (float)log1p(bc[v] * bc_scale);
+    x[4 * v + 3] = (float)log1p(delta[v] * delta_scale);
+  }
+}
+
+// weights of the workgroup in LDS as FP32
+struct SharedW {
+  float w[kParams];
+};
+
+__device__ void load_weights(const __hip_bfloat16 *__restrict__ wb, SharedW &sw) {
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) sw.w[i] = bf(wb, i);
+  __syncthreads();
+}
+
+__device__ __forceinline__ float forward_one(const SharedW &sw, const float *x, float h1[kH], float h2[kH]) {
+#pragma unroll
+  for (int j = 0; j < kH; ++j) {
+    float a = sw.w[oB1 + j];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) a += sw.w[oW1 + j * kIn + k] * x[k];
+    h1[j] = a > 0.0f ? a : 0.0f;
+  }
+  float s = sw.w[oB3];
+  for (int j = 0; j < kH; ++j) {
+    float a = sw.w[oB2 + j];
+#pragma unroll 16
+    for (int k = 0; k < kH; ++k) a += sw.w[oW2 + j * kH + k] * h1[k];
+    h2[j] = a > 0.0f ? a : 0.0f;
+    s += sw.w[oW3 + j] * h2[j];
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(uint32_t n, const float *__restrict__ x,
+                                                     const __hip_bfloat16 *__restrict__ wb, float *score) {
+  __shared__ SharedW sw;
+  load_weights(wb, sw);
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    float h1[kH], h2[kH], xv[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) xv[k] = x[4 * v + k];
+    score[v] = forward_one(sw, xv, h1, h2);
+  }
+}
+
+// Gradient of 1/M sum (s - y)^2 over the samples idx[0..m).  A workgroup is
+// one wave of 64 threads and takes tiles of 64 samples: thread t runs sample
+// t of the tile forward and backward and leaves its activations and
+// pre-activation gradients in LDS; then thread t accumulates the gradient
+// of the parameters it owns (row t of W2, entries t of b1, b2, w3, row t of
+// W1, and b3 for thread 0) over the tile's samples in registers -- a
+// (64 x 64) x (64 x 64) product per tile, no atomics inside the workgroup.
+// At the end one atomic add per owned parameter.
+constexpr int kTile = 64;
+
+__global__ __launch_bounds__(64) void k_mlp_grad(uint32_t m, const uint32_t *__restrict__ idx,
+                                                 const float *__restrict__ x, const float *__restrict__ y,
+                                                 const __hip_bfloat16 *__restrict__ wb, float *grad, float *loss) {
+  __shared__ SharedW sw;
+  __shared__ float H1[kTile][kH + 1], H2[kTile][kH + 1], DZ1[kTile][kH + 1], DZ2[kTile][kH + 1];
+  __shared__ float X[kTile][kIn], DS[kTile];
+  load_weights(wb, sw);
+  const int t = threadIdx.x;
+  float gW2[kH], gW1[kIn], gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+  for (int k = 0; k < kH; ++k) gW2[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < kIn; ++k) gW1[k] = 0.f;
+  const uint32_t ntiles = (m + kTile - 1) / kTile;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t si = tile * kTile + t;
+    // per-sample forward and backward
+    float h1[kH], h2[kH], xv[kIn] = {0.f, 0.f, 0.f, 0.f}, ds = 0.f;
+    const bool live = si < m;
+    if (live) {
+      const uint32_t v = idx[si];
+#pragma unroll
+      for (int k = 0; k < kIn; ++k) xv[k] = x[4 * v + k];
+      const float s = forward_one(sw, xv, h1, h2);
+      const float r = s - y[v];
+      lsum += r * r;
+      ds = 2.0f * r / (float)m;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kH; ++k) h1[k] = h2[k] = 0.f;
+    }
+    float dh1[kH];
+#pragma unroll
+    for (int k = 0; k < kH; ++k) dh1[k] = 0.f;
+    for (int j = 0; j < kH; ++j) {
+      const float dz2 = h2[j] > 0.f ? ds * sw.w[oW3 + j] : 0.f;
+      DZ2[t][j] = dz2;
+      H1[t][j] = h1[j];
+      H2[t][j] = h2[j];
+#pragma unroll 16
+      for (int k = 0; k < kH; ++k) dh1[k] += dz2 * sw.w[oW2 + j * kH + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kH; ++k) DZ1[t][k] = h1[k] > 0.f ? dh1[k] : 0.f;
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) X[t][k] = xv[k];
+    DS[t] = ds;
+    __syncthreads();
+    // thread t: parameters of hidden unit t
+    for (int s2 = 0; s2 < kTile; ++s2) {
+      const float dz2 = DZ2[s2][t], dz1 = DZ1[s2][t];
+#pragma unroll 16
+      for (int k = 0; k < kH; ++k) gW2[k] += dz2 * H1[s2][k];
+      gb2 += dz2;
+      gw3 += DS[s2] * H2[s2][t];
+      gb1 += dz1;
+#pragma unroll
+      for (int k = 0; k < kIn; ++k) gW1[k] += dz1 * X[s2][k];
+      if (t == 0) gb3 += DS[s2];
+    }
+    __syncthreads();
+  }
+#pragma unroll 16
+  for (int k = 0; k < kH; ++k)
+    if (gW2[k] != 0.f) atomicAdd(&grad[oW2 + t * kH + k], gW2[k]);
+#pragma unroll
+  for (int k = 0; k < kIn; ++k)
+    if (gW1[k] != 0.f) atomicAdd(&grad[oW1 + t * kIn + k], gW1[k]);
+  if (gb1 != 0.f) atomicAdd(&grad[oB1 + t], gb1);
+  if (gb2 != 0.f) atomicAdd(&grad[oB2 + t], gb2);
+  if (gw3 != 0.f) atomicAdd(&grad[oW3 + t], gw3);
+  if (t == 0 && gb3 != 0.f) atomicAdd(&grad[oB3], gb3);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+  if (t == 0) atomicAdd(loss, lsum / (float)m);
+}
+
+__global__ void k_adam(int n, float *master, __hip_bfloat16 *wb, float *grad, float *m, float *v, float lr,
+                       float b1, float b2, float eps, int step) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gi = grad[i];
+    m[i] = b1 * m[i] + (1.0f - b1) * gi;
+    v[i] = b2 * v[i] + (1.0f - b2) * gi * gi;
+    master[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+    wb[i] = __float2bfloat16(master[i]);
+    grad[i] = 0.0f;
+  }
+}
+
+__global__ void k_to_bf16(int n, const float *__restrict__ src, __hip_bfloat16 *dst) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = __float2bfloat16(src[i]);
+}
+
+// proposal weights: softplus(score) + eps * mean softplus (mean passed in)
+__global__ void k_proposal(uint32_t n, const float *__restrict__ score, double floor_w, double *w) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const double s = (double)score[v];
+    const double sp = s > 30.0 ? s : log1p(exp(s));
+    w[v] = sp + floor_w;
+  }
+}
+
+__global__ void k_softplus_sum(uint32_t n, const float *__restrict__ score, double *sum) {
+  double a = 0.0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const double s = (double)score[v];
+    a += s > 30.0 ? s : log1p(exp(s));
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) a += __shfl_xor(a, off);
+  if (lane_id() == 0) atomicAdd(sum, a);
+}
+
+int grid_of(uint64_t n) {
+  uint64_t g = (n + 255) / 256;
+  return (int)(g < 1 ? 1 : g > 4096 ? 4096 : g);
+}
+
+}  // namespace
+
+namespace axbc {
+void mlp_free(axbc_ctx *c) {
+  MlpBufs *b = bufs_of(c);
+  if (!b) return;
+  void *d[] = {b->x, b->y, b->score, b->master, b->wb, b->grad, b->m, b->v, b->loss, b->idx, b->prop};
+  for (void *p : d)
+    if (p) (void)hipFree(p);
+  delete b;
+  c->mlp = nullptr;
+}
+}  // namespace axbc
+
+extern "C" {
+
+int axbc_mlp_init(axbc_ctx *c, const float *params) {
+  if (!c || !params) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  HIP_TRY(hipSetDevice(c->device));
+  MlpBufs *b = bufs_of(c);
+  if (!b) {
+    b = new MlpBufs();
+    c->mlp = b;
+    const size_t n = c->n;
+    HIP_TRY(hipMalloc((void **)&b->x, sizeof(float) * 4 * n));
+    HIP_TRY(hipMalloc((void **)&b->y, sizeof(float) * n));
+    HIP_TRY(hipMalloc((void **)&b->score, sizeof(float) * n));
+    HIP_TRY(hipMalloc((void **)&b->master, sizeof(float) * kParams));
+    HIP_TRY(hipMalloc((void **)&b->wb, sizeof(__hip_bfloat16) * kParams));
+    HIP_TRY(hipMalloc((void **)&b->grad, sizeof(float) * kParams));
+    HIP_TRY(hipMalloc((void **)&b->m, sizeof(float) * kParams));
+    HIP_TRY(hipMalloc((void **)&b->v, sizeof(float) * kParams));
+    HIP_TRY(hipMalloc((void **)&b->loss, sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&b->prop, sizeof(double) * n));
+    c->bytes += 32 * n;
+  }
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(b->master, params, sizeof(float) * kParams, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(b->grad, 0, sizeof(float) * kParams, s));
+  HIP_TRY(hipMemsetAsync(b->m, 0, sizeof(float) * kParams, s));
+  HIP_TRY(hipMemsetAsync(b->v, 0, sizeof(float) * kParams, s));
+  hipLaunchKernelGGL(k_to_bf16, dim3(32), dim3(256), 0, s, kParams, b->master, b->wb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  b->step = 0;
+  return AXBC_OK;
+}
+
+int axbc_mlp_param_count(void) { return kParams; }
+
+int axbc_mlp_features(axbc_ctx *c, double bc_scale, double delta_scale) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  const float *lcc = c->sampler && c->sampler->lcc_ready ? c->sampler->lcc : nullptr;
+  hipLaunchKernelGGL(k_features, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, lcc, c->bc, c->delta,
+                     bc_scale, delta_scale, b->x);
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+int axbc_mlp_set_data(axbc_ctx *c, const float *x, const float *y) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  if (x) HIP_TRY(hipMemcpyAsync(b->x, x, sizeof(float) * 4 * c->n, hipMemcpyHostToDevice, c->stream));
+  if (y) HIP_TRY(hipMemcpyAsync(b->y, y, sizeof(float) * c->n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_mlp_get_data(axbc_ctx *c, float *x, float *score) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  if (x) HIP_TRY(hipMemcpyAsync(x, b->x, sizeof(float) * 4 * c->n, hipMemcpyDeviceToHost, c->stream));
+  if (score) HIP_TRY(hipMemcpyAsync(score, b->score, sizeof(float) * c->n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+// Gradients (summed into an FP32 host array) of the MSE over the samples.
+int axbc_mlp_gradient(axbc_ctx *c, const uint32_t *idx, uint32_t m, float *grad_host, float *loss_host) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b || !idx || m == 0) return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  hipStream_t s = c->stream;
+  if (m > b->idx_cap) {
+    if (b->idx) HIP_TRY(hipFree(b->idx));
+    HIP_TRY(hipMalloc((void **)&b->idx, sizeof(uint32_t) * m));
+    b->idx_cap = m;
+  }
+  HIP_TRY(hipMemcpyAsync(b->idx, idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(b->loss, 0, sizeof(float), s));
+  const uint32_t tiles = (m + 63) / 64;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(tiles < 1024 ? tiles : 1024), dim3(64), 0, s, m, b->idx, b->x, b->y, b->wb,
+                     b->grad, b->loss);
+  HIP_TRY(hipGetLastError());
+  if (grad_host) HIP_TRY(hipMemcpyAsync(grad_host, b->grad, sizeof(float) * kParams, hipMemcpyDeviceToHost, s));
+  if (loss_host) HIP_TRY(hipMemcpyAsync(loss_host, b->loss, sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return AXBC_OK;
+}
+
+// One Adam step with the gradients accumulated by axbc_mlp_gradient.
+int axbc_mlp_adam(axbc_ctx *c, float lr) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  b->step += 1;
+  hipLaunchKernelGGL(k_adam, dim3(32), dim3(256), 0, c->stream, kParams, b->master, b->wb, b->grad, b->m, b->v, lr,
+                     0.9f, 0.999f, 1e-8f, b->step);
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+int axbc_mlp_params(axbc_ctx *c, float *master_host) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b || !master_host) return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(master_host, b->master, sizeof(float) * kParams, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_mlp_forward(axbc_ctx *c) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid_of(c->n) < 2048 ? grid_of(c->n) : 2048), dim3(256), 0, c->stream, c->n,
+                     b->x, b->wb, b->score);
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+// Proposal distribution from the scores: w = softplus(s) + eps * mean, as
+// fixed-point weights scanned for axbc_weights_pick (kind 7).
+int axbc_mlp_proposal(axbc_ctx *c, double eps, uint64_t *total) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  double *dsum = nullptr, hsum = 0;
+  HIP_TRY(hipMalloc((void **)&dsum, sizeof(double)));
+  HIP_TRY(hipMemsetAsync(dsum, 0, sizeof(double), c->stream));
+  hipLaunchKernelGGL(k_softplus_sum, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, b->score, dsum);
+  HIP_TRY(hipMemcpyAsync(&hsum, dsum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(dsum));
+  double *w = b->prop;
+  hipLaunchKernelGGL(k_proposal, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, b->score,
+                     eps * hsum / (double)c->n, w);
+  HIP_TRY(hipGetLastError());
+  return axbc_weights_from(c, w, total);
+}
+
+}  // extern "C"

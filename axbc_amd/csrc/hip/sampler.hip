@@ -227,9 +227,10 @@ __global__ void k_lcc_value(const uint32_t *__restrict__ col, uint32_t n, const 
 
 __device__ __forceinline__ double weight_of(int kind, uint32_t v, const double *__restrict__ bc,
                                             const double *__restrict__ delta) {
-  // kind 2 passes the LCC values (float) through `delta` reinterpreted
+  // kind 2 passes the LCC values (float) through `delta` reinterpreted,
+  // kind 7 passes arbitrary weights (double) through `delta`
   double w = kind == 2 ? (double)((const float *)delta)[v]
-                       : kind == 3 ? bc[v] : kind == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
+             : kind == 3 ? bc[v] : kind == 4 || kind == 7 ? delta[v] : 1.0 / (1.0 + bc[v]);
   return (isfinite(w) && w > 0.0) ? w : 0.0;
 }
 
@@ -366,8 +367,18 @@ int axbc_hdn_pick(axbc_ctx *c, uint32_t r, uint32_t *u, uint32_t *sdn_u) {
   return AXBC_OK;
 }
 
+static int weights_impl(axbc_ctx *c, int kind, const double *src, uint64_t *total);
+
+int axbc_weights_from(axbc_ctx *c, const double *w_device, uint64_t *total) {
+  return weights_impl(c, 7, w_device, total);
+}
+
 int axbc_weights(axbc_ctx *c, int kind, uint64_t *total) {
   if (kind != 3 && kind != 4 && kind != 5) return fail(AXBC_EINVAL, "weight kind%s (line %d)", "", __LINE__);
+  return weights_impl(c, kind, c ? c->delta : nullptr, total);
+}
+
+static int weights_impl(axbc_ctx *c, int kind, const double *src, uint64_t *total) {
   SamplerBufs *sb = nullptr;
   int rc = sampler_bufs(c, &sb);
   if (rc) return rc;
@@ -375,8 +386,8 @@ int axbc_weights(axbc_ctx *c, int kind, uint64_t *total) {
   hipStream_t s = c->stream;
   uint64_t *W = (uint64_t *)sb->tmp, *P = (uint64_t *)sb->u64a;
   HIP_TRY(hipMemsetAsync(sb->wmax, 0, sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_weight_max, dim3(grid_n(n)), dim3(256), 0, s, kind, c->bc, c->delta, n, sb->wmax);
-  hipLaunchKernelGGL(k_weight_fixed, dim3(grid_n(n)), dim3(256), 0, s, kind, c->bc, c->delta, n, sb->wmax, W);
+  hipLaunchKernelGGL(k_weight_max, dim3(grid_n(n)), dim3(256), 0, s, kind, c->bc, src, n, sb->wmax);
+  hipLaunchKernelGGL(k_weight_fixed, dim3(grid_n(n)), dim3(256), 0, s, kind, c->bc, src, n, sb->wmax, W);
   if (device_scan<uint64_t, AddU64>(W, P, n, (uint64_t *)sb->tiles, s))
     return fail(AXBC_EHIP, "scan launch failed%s (line %d)", "", __LINE__);
   HIP_TRY(hipMemcpyAsync(&sb->host64[0], P + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));

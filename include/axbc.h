@@ -202,6 +202,36 @@ int axbc_kernel_timing(axbc_ctx *ctx, int enable);
 int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, uint64_t *bytes);
 int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 
+/*
+ * Learned pivot proposal (no reference counterpart; stands in for the
+ * bc/delta heuristics -z 3/4): a 4 -> 64 -> 64 -> 1 MLP with ReLU, BF16
+ * weights and FP32 accumulation, over per-vertex features (log(1+deg), LCC,
+ * log(1+bc*bc_scale), log(1+delta*delta_scale)).
+ *   axbc_mlp_init        FP32 master parameters (axbc_mlp_param_count()
+ *                        floats: W1[64][4], b1, W2[64][64], b2, w3, b3)
+ *   axbc_mlp_features    features from the device state (LCC when built)
+ *   axbc_mlp_set_data    host features / targets (either may be NULL)
+ *   axbc_mlp_gradient    MSE gradient over the sample ids (accumulated on the
+ *                        device; copied out when grad_host is given)
+ *   axbc_mlp_adam        one Adam step with the accumulated gradient
+ *   axbc_mlp_forward     scores of all vertices
+ *   axbc_mlp_proposal    proposal weights softplus(score) + eps * mean as the
+ *                        fixed-point distribution of axbc_weights_pick
+ * axbc_weights_from      fixed-point distribution from any device array of N
+ *                        non-negative doubles.
+ */
+int axbc_mlp_param_count(void);
+int axbc_mlp_init(axbc_ctx *ctx, const float *params);
+int axbc_mlp_features(axbc_ctx *ctx, double bc_scale, double delta_scale);
+int axbc_mlp_set_data(axbc_ctx *ctx, const float *x, const float *y);
+int axbc_mlp_get_data(axbc_ctx *ctx, float *x, float *score);
+int axbc_mlp_gradient(axbc_ctx *ctx, const uint32_t *idx, uint32_t m, float *grad_host, float *loss_host);
+int axbc_mlp_adam(axbc_ctx *ctx, float lr);
+int axbc_mlp_params(axbc_ctx *ctx, float *master_host);
+int axbc_mlp_forward(axbc_ctx *ctx);
+int axbc_mlp_proposal(axbc_ctx *ctx, double eps, uint64_t *total);
+int axbc_weights_from(axbc_ctx *ctx, const double *w_device, uint64_t *total);
+
 /* Last round's counters (waits for the context). */
 int axbc_last_stats(axbc_ctx *ctx, axbc_pivot_stats *stats);
 

@@ -288,3 +288,29 @@ def test_sync_batch_freezes_dynamic_distribution():
     pv8, pr8, _, _ = oracle.run(g, 80, c=0, x=1, z=4, sync_batch=8)
     assert (pr8[:8] == 1.0 / g.N).all()  # first group: uniform
     assert list(pv8[:8]) == list(pv0[:1]) + list(pv8[1:8])
+
+
+def test_mlp_oracle_gradient_by_finite_differences():
+    from oracle import mlp_oracle as mo
+    rng = np.random.default_rng(0)
+    p = rng.normal(0, 0.3, mo.N_PARAMS)
+    x = rng.normal(size=(40, 4))
+    y = rng.normal(size=40)
+    loss, g = mo.loss_and_grad(p, x, y)
+    for i in rng.choice(mo.N_PARAMS, 60, replace=False):
+        h = 1e-6
+        q = p.copy()
+        q[i] += h
+        lp, _ = mo.loss_and_grad(q, x, y)
+        q[i] -= 2 * h
+        lm, _ = mo.loss_and_grad(q, x, y)
+        assert (lp - lm) / (2 * h) == pytest.approx(g[i], rel=1e-4, abs=1e-7)
+
+
+def test_bf16_rounding():
+    from oracle import mlp_oracle as mo
+    a = np.array([1.0, 1.00390625, 1.0 + 2 ** -8 + 2 ** -10, -3.14159], np.float32)
+    b = mo.to_bf16(a)
+    assert b[0] == 1.0 and b[1] == 1.0  # tie to even
+    assert b[2] == np.float32(1.0 + 2 ** -7)
+    assert abs(b[3] + 3.140625) < 1e-6
