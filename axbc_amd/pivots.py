@@ -161,7 +161,7 @@ class PivotLoop:
             return u, float(self.lw[u]) / float(self.ltotal)
         if z == 1:
             r = uniform_int(self.sum_deg)
-            u = int(np.searchsorted(self.pmax, r, side="left"))
+            u = int(np.searchsorted(self.pmax, np.uint32(r), side="left"))
             if u >= N:
                 u = N - 1
             d = int(self.deg[u])

@@ -145,12 +145,19 @@ def main():
         ploop.step()
     e.sync()
     r0p, er0p = ploop.res.nrounds, ploop.res.e_r
+    set_timing(e, True)
     t = time.perf_counter()
     for _ in range(args.plain_rounds):
         if not ploop.step():
             break
     e.sync()
     dtp = time.perf_counter() - t
+    for k in (5, 6):
+        ms, n, b = kernel_stats(e, k)
+        if n:
+            log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us, "
+                f"{b / 1e9:.3f} GB -> {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s")
+    set_timing(e, False)
     prounds = ploop.res.nrounds - r0p
     plain_s = prounds / dtp
     plain_gteps = 2.0 * (ploop.res.e_r - er0p) / dtp / 1e9

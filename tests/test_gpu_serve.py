@@ -75,7 +75,7 @@ def test_admission_by_footprint(require_gpu):
     assert all(d.error is None for d in done) and len(done) == 3
     assert peak[0] <= cap
     big = Job(jid=9, budget=10, z=0, scale=12, edgefactor=16, seed=(1, 2))
-    srv = Server(slots=1, hbm_cap=1e6)
+    srv = Server(slots=1, hbm_cap=1e5)
     srv.submit(big)
     (d,) = srv.drain(timeout=120)
     srv.close()
