@@ -46,6 +46,7 @@ using namespace axbc;
 namespace axbc {
 
 constexpr uint32_t kChunkMin = 64;  // smallest edges-per-work-item for split vertices
+constexpr int kBurst = 8;           // row loads in flight per wave in the pull loops
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
 
 struct BCtl {
@@ -53,7 +54,7 @@ struct BCtl {
   uint32_t done;
   uint32_t empty_lev;
   uint32_t overflow;
-  uint32_t nsplit;     // split vertices in the current pass
+  uint32_t unused;
   uint32_t wg_done;
   uint32_t stop_round; // first round after which the stop test holds (nb if none)
   uint32_t pad;
@@ -75,7 +76,7 @@ struct BatchState {
   uint32_t *qoff = nullptr;
   uint64_t *act = nullptr;   // per level: OR of the level's words
   // split vertices: entry list index, vertex, chunk base per split entry
-  uint32_t *sp_v = nullptr, *sp_e = nullptr, *sp_cnt = nullptr;
+  uint32_t *sp_e = nullptr;
   uint64_t *sp_base = nullptr;
   uint32_t sp_cap = 0;
   double *part = nullptr;
@@ -146,6 +147,70 @@ __device__ __forceinline__ bool last_wg(uint32_t *counter) {
   return is_last;
 }
 
+// Split vertices.  An entry with more than `ch` edges is claimed with one
+// 64-bit atomic that returns both its slot k and the first of its
+// ceil(d/ch) chunk items (split_claim = entries << 40 | items), so sp_base[]
+// increases with k.  The split kernels give each wave one contiguous range
+// of items: a binary search over sp_base for the first, then a forward walk.
+__device__ __forceinline__ void claim_split(axbc::BCtl *ctl, uint32_t i, uint32_t nch, uint32_t *sp_e,
+                                            uint64_t *sp_base, uint32_t sp_cap, uint64_t item_cap) {
+  const unsigned long long old = atomicAdd(&ctl->split_claim, (1ull << 40) + nch);
+  const uint32_t k = (uint32_t)(old >> 40);
+  const uint64_t base = old & ((1ull << 40) - 1);
+  if (k < sp_cap && base + nch <= item_cap) {
+    sp_e[k] = i;
+    sp_base[k] = base;
+  } else {
+    atomicOr(&ctl->overflow, 1u);
+  }
+}
+
+struct ItemWalk {
+  uint32_t ns, k;
+  uint64_t lo, hi, next;
+};
+
+// This wave's item range [lo, hi) and the entry holding lo.
+__device__ __forceinline__ ItemWalk walk_begin(const axbc::BCtl *ctl, const uint64_t *__restrict__ sp_base,
+                                               uint32_t sp_cap, uint64_t item_cap) {
+  const unsigned long long claim = ctl->split_claim;
+  ItemWalk r;
+  r.ns = min((uint32_t)(claim >> 40), sp_cap);
+  const uint64_t total = min((uint64_t)(claim & ((1ull << 40) - 1)), item_cap);
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  const uint64_t per = (total + nwv - 1) / nwv;
+  r.lo = min(total, (uint64_t)gw * per);
+  r.hi = min(total, r.lo + per);
+  r.k = 0;
+  r.next = ~0ull;
+  if (r.lo < r.hi && r.ns > 0) {
+    uint32_t lo = 0, hi = r.ns;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (sp_base[mid] <= r.lo)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    r.k = lo;
+    r.next = lo + 1 < r.ns ? sp_base[lo + 1] : ~0ull;
+  } else {
+    r.hi = r.lo;
+  }
+  return r;
+}
+
+// Advance to the entry holding `item`; true when the entry changed.
+__device__ __forceinline__ bool walk_to(ItemWalk &r, const uint64_t *__restrict__ sp_base, uint64_t item) {
+  bool moved = false;
+  while (item >= r.next) {
+    ++r.k;
+    r.next = r.k + 1 < r.ns ? sp_base[r.k + 1] : ~0ull;
+    moved = true;
+  }
+  return moved;
+}
+
 // ---------------------------------------------------------------------------
 // Batch set-up
 // ---------------------------------------------------------------------------
@@ -171,29 +236,35 @@ __global__ void k_bfill(uint32_t n, const uint32_t *__restrict__ reach, uint64_t
   }
 }
 
-// Undo the previous batch on the vertices it listed (a vertex may be
-// listed at several levels; the writes are idempotent).
+// Undo the previous batch on the vertices it reached (seen == its stamp),
+// each once: coalesced row stores, a wave per 64 vertices.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_breset(const uint32_t *__restrict__ q, const axbc::BCtl *ctl,
+__global__ __launch_bounds__(kBlock) void k_breset(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
                                                    const uint32_t *__restrict__ reach, uint64_t *vis,
-                                                   uint64_t *fr0, uint64_t *fr1, uint32_t *sig, double *dlt,
-                                                   uint32_t qcap) {
+                                                   uint64_t *fr0, uint64_t *fr1, uint32_t *sig, double *dlt) {
   constexpr int B = 64 * NW;
-  const uint32_t n = min(ctl->tail, qcap);
   const int lane = lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nw = (gridDim.x * blockDim.x) / kWave;
-  for (uint32_t i = wid; i < n; i += nw) {
-    const uint32_t v = q[i];
-    const double r = (double)reach[v];
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
+    const uint32_t v = vb + (uint32_t)lane;
+    const bool hit = v < n && seen[v] == stamp;
+    const double r = hit ? (double)reach[v] : 0.0;
+    unsigned long long todo = __ballot(hit);
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t vv = vb + (uint32_t)i;
+      const double rr = __shfl(r, i);
 #pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      sig[(size_t)v * B + lane + 64 * j] = 0;
-      dlt[(size_t)v * B + lane + 64 * j] = r;
-    }
-    if (lane < NW) {
-      vis[(size_t)v * NW + lane] = 0;
-      fr0[(size_t)v * NW + lane] = 0;
-      fr1[(size_t)v * NW + lane] = 0;
+      for (int j = 0; j < NW; ++j) {
+        sig[(size_t)vv * B + lane + 64 * j] = 0;
+        dlt[(size_t)vv * B + lane + 64 * j] = rr;
+      }
+      if (lane < NW) {
+        vis[(size_t)vv * NW + lane] = 0;
+        fr0[(size_t)vv * NW + lane] = 0;
+        fr1[(size_t)vv * NW + lane] = 0;
+      }
     }
   }
 }
@@ -231,7 +302,6 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     ctl->done = 0;
     ctl->empty_lev = 0;
     ctl->overflow = 0;
-    ctl->nsplit = 0;
     ctl->wg_done = 0;
     ctl->split_claim = 0;
   }
@@ -247,17 +317,26 @@ template <int NW>
 __device__ __forceinline__ bool push_edge(uint32_t w, const Words<NW> &f, const uint64_t *__restrict__ vis,
                                           uint64_t *nx, uint32_t *mark, uint32_t L) {
   bool first = false;
+  // bits already in NX need no atomic (NX only grows during the level, and
+  // whoever set them has decided the append)
   if (NW == 1) {
-    const uint64_t nw = f.w[0] & ~vis[w];
+    const uint64_t vw = vis[w], xw = nx[w];
+    const uint64_t nw = f.w[0] & ~(vw | xw);
     if (nw) {
       const uint64_t old = atomicOr((unsigned long long *)&nx[w], (unsigned long long)nw);
       first = old == 0ull;
     }
   } else {
     bool any = false;
+    uint64_t vw[NW], xw[NW];
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-      const uint64_t nw = f.w[j] & ~vis[(size_t)w * NW + j];
+      vw[j] = vis[(size_t)w * NW + j];
+      xw[j] = nx[(size_t)w * NW + j];
+    }
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      const uint64_t nw = f.w[j] & ~(vw[j] | xw[j]);
       if (nw) {
         atomicOr((unsigned long long *)&nx[(size_t)w * NW + j], (unsigned long long)nw);
         any = true;
@@ -273,8 +352,9 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
                                                   const uint64_t *__restrict__ vis, uint64_t *nx, uint32_t *mark,
                                                   uint64_t *fr_clear, uint32_t *q, const uint64_t *__restrict__ qw,
                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                  uint32_t *sp_v, uint32_t sp_cap, uint32_t qcap, int L,
-                                                  uint32_t mark_id, unsigned long long *kb, uint32_t kChunk) {
+                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
+                                                  uint64_t item_cap, uint32_t qcap, int L, uint32_t mark_id,
+                                                  unsigned long long *kb, uint32_t kChunk) {
   if (ldg_acq(&ctl->done)) return;
   const int lane = lane_id();
   unsigned long long bytes = 0;
@@ -301,12 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
       d = col[u + 1] - st;
       f = ldw<NW>(qw, i);
     }
-    if (d > kChunk) {  // split: handled by k_bpush_split, one chunk per wave
-      const uint32_t k = atomicAdd(&ctl->nsplit, 1u);
-      if (k < sp_cap)
-        sp_v[k] = i;
-      else
-        atomicOr(&ctl->overflow, 1u);
+    if (d > kChunk) {  // split: its chunks go to k_bpush_split
+      claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
       d = 0;
     }
     const uint32_t incl = wave_incl_scan(d);
@@ -348,60 +424,64 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
-// Frontier vertices with more than kChunk edges: every wave takes one chunk
-// of kChunk edges of one of them (entries in sp_v, chunk counts derived from
-// the degrees, scanned implicitly by a linear walk that every wave repeats
-// only over the split entries).
+// Frontier entries with more than kChunk edges, kChunk edges per item.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restrict__ col,
                                                         const uint32_t *__restrict__ row,
                                                         const uint64_t *__restrict__ vis, uint64_t *nx,
                                                         uint32_t *mark, uint32_t *q, const uint64_t *__restrict__ qw,
-                                                        axbc::BCtl *ctl, const uint32_t *__restrict__ sp_v,
-                                                        uint32_t sp_cap, uint32_t qcap, uint32_t mark_id,
-                                                        uint32_t kChunk) {
+                                                        axbc::BCtl *ctl, const uint32_t *__restrict__ sp_e,
+                                                        const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
+                                                        uint64_t item_cap, uint32_t qcap, uint32_t mark_id,
+                                                        unsigned long long *kb, uint32_t kChunk) {
   if (ldg_acq(&ctl->done)) return;
-  const uint32_t ns = min(ctl->nsplit, sp_cap);
   const int lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  uint32_t wave_item = gw;  // global chunk index this wave works on next
-  uint32_t acc_chunks = 0;
-  for (uint32_t h = 0; h < ns; ++h) {
-    const uint32_t i = sp_v[h];
-    const uint32_t u = q[i];
-    const uint32_t st = col[u], d = col[u + 1] - st;
-    const uint32_t nch = (d + kChunk - 1) / kChunk;
-    const Words<NW> f = ldw<NW>(qw, i);
-    while (wave_item < acc_chunks + nch) {
-      const uint32_t c = wave_item - acc_chunks;
-      const uint32_t cb = c * kChunk, ce = min(d, cb + kChunk);
-      for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
-        const uint32_t k = k0 + lane;
-        bool first = false;
-        uint32_t w = 0;
-        if (k < ce) {
-          w = row[st + k];
-          first = push_edge<NW>(w, f, vis, nx, mark, mark_id);
-        }
-        const unsigned long long m = __ballot(first);
-        if (m) {
-          const int leader = __ffsll((long long)m) - 1;
-          uint32_t b = 0;
-          if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
-          b = __shfl(b, leader);
-          const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-          if (first) {
-            if (slot < qcap)
-              q[slot] = w;
-            else
-              atomicOr(&ctl->overflow, 1u);
-          }
-        }
-      }
-      wave_item += nwv;
+  ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
+  if (r.lo >= r.hi) return;
+  unsigned long long bytes = 0;
+  uint32_t i = sp_e[r.k];
+  uint32_t u = q[i];
+  uint32_t st = col[u], d = col[u + 1] - st;
+  Words<NW> f = ldw<NW>(qw, i);
+  uint64_t base = sp_base[r.k];
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    if (walk_to(r, sp_base, item)) {
+      i = sp_e[r.k];
+      u = q[i];
+      st = col[u];
+      d = col[u + 1] - st;
+      f = ldw<NW>(qw, i);
+      base = sp_base[r.k];
     }
-    acc_chunks += nch;
+    const uint32_t cb = (uint32_t)(item - base) * kChunk, ce = min(d, cb + kChunk);
+    if (cb >= ce) continue;
+    bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW);
+    for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      bool first = false;
+      uint32_t w = 0;
+      if (k < ce) {
+        w = row[st + k];
+        first = push_edge<NW>(w, f, vis, nx, mark, mark_id);
+      }
+      const unsigned long long m = __ballot(first);
+      if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
+        b = __shfl(b, leader);
+        const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (first) {
+          if (slot < qcap)
+            q[slot] = w;
+          else
+            atomicOr(&ctl->overflow, 1u);
+        }
+        bytes += 4ull * (unsigned long long)__popcll(m);
+      }
+    }
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
 // sigma pull for the vertices new at level L: q[qoff[L] .. tail).  A wave
@@ -426,20 +506,29 @@ __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t c
 #pragma unroll
       for (int j = 0; j < NW; ++j) m.w[j] = f.w[j] & nw.w[j];
     }
+    // edges with roots to add, kBurst at a time: all row loads of a burst
+    // are issued before any is used
     unsigned long long todo = __ballot(m.any());
     while (todo) {
-      const int i = __ffsll((long long)todo) - 1;
-      todo &= todo - 1;
-      const uint32_t uu = __shfl(u, i);
+      uint32_t got[kBurst][NW];
 #pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        const uint64_t mm = __shfl(m.w[j], i);
-        *contrib += (uint32_t)__popcll(mm);
-        if (bit(mm, lane)) {
-          acc[j] += sig[(size_t)uu * B + lane + 64 * j];
-          ++dag[j];
+      for (int t = 0; t < kBurst; ++t) {
+        const int i = todo ? __ffsll((long long)todo) - 1 : 0;
+        const bool live = todo != 0;
+        todo &= todo - 1;
+        const uint32_t uu = __shfl(u, i);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
+          *contrib += (uint32_t)__popcll(mm);
+          got[t][j] = bit(mm, lane) ? sig[(size_t)uu * B + lane + 64 * j] : 0u;
+          dag[j] += bit(mm, lane) ? 1u : 0u;
         }
       }
+#pragma unroll
+      for (int t = 0; t < kBurst; ++t)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc[j] += got[t][j];
     }
   }
 }
@@ -449,8 +538,9 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
                                                    const uint64_t *__restrict__ nx, const uint64_t *__restrict__ frp,
                                                    uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                   unsigned long long *cnt, uint32_t *sp_e, uint32_t sp_cap, int L,
-                                                   uint32_t qcap, unsigned long long *kb, uint32_t kChunk) {
+                                                   unsigned long long *cnt, uint32_t *sp_e, uint64_t *sp_base,
+                                                   uint32_t sp_cap, uint64_t item_cap, int L, uint32_t qcap,
+                                                   unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
   unsigned long long bytes = 0;
@@ -464,13 +554,8 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
     const uint32_t w = q[i];
     const uint32_t st = rcol[w], d = rcol[w + 1] - st;
     if (d > kChunk) {
-      if (lane == 0) {
-        const uint32_t k = atomicAdd(&ctl->nsplit, 1u);
-        if (k < sp_cap)
-          sp_e[k] = i;
-        else
-          atomicOr(&ctl->overflow, 1u);
-      }
+      if (lane == 0) claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
+      bytes += 12;
       continue;
     }
     const Words<NW> nw = ldw<NW>(nx, w);
@@ -503,37 +588,53 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
                                                          const uint64_t *__restrict__ frp, uint32_t *sig,
                                                          const uint32_t *__restrict__ q, axbc::BCtl *ctl,
                                                          unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
-                                                         uint32_t sp_cap, uint32_t kChunk) {
+                                                         const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
+                                                         uint64_t item_cap, unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
-  const uint32_t ns = min(ctl->nsplit, sp_cap);
   const int lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  uint32_t wave_item = gw, acc_chunks = 0;
+  ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
+  if (r.lo >= r.hi) return;
+  unsigned long long bytes = 0;
   uint32_t dag_tot[NW];
 #pragma unroll
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
-  for (uint32_t h = 0; h < ns; ++h) {
-    const uint32_t w = q[sp_e[h]];
-    const uint32_t st = rcol[w], d = rcol[w + 1] - st;
-    const uint32_t nch = (d + kChunk - 1) / kChunk;
-    const Words<NW> nw = ldw<NW>(nx, w);
-    while (wave_item < acc_chunks + nch) {
-      const uint32_t c = wave_item - acc_chunks;
-      uint32_t acc[NW], dag[NW];
-#pragma unroll
-      for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
-      uint32_t contrib = 0;
-      sigma_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), rrow, frp, nw, sig, acc, dag, &contrib);
-#pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        if (acc[j]) atomicAdd(&sig[(size_t)w * B + lane + 64 * j], acc[j]);
-        dag_tot[j] += dag[j];
-      }
-      wave_item += nwv;
+  uint32_t w = q[sp_e[r.k]];
+  uint32_t st = rcol[w], d = rcol[w + 1] - st;
+  Words<NW> nw = ldw<NW>(nx, w);
+  uint64_t base = sp_base[r.k];
+  bool first_item = r.lo == base;
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    if (walk_to(r, sp_base, item)) {
+      w = q[sp_e[r.k]];
+      st = rcol[w];
+      d = rcol[w + 1] - st;
+      nw = ldw<NW>(nx, w);
+      base = sp_base[r.k];
+      first_item = true;
     }
-    acc_chunks += nch;
+    const uint32_t c = (uint32_t)(item - base);
+    uint32_t acc[NW], dag[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
+    uint32_t contrib = 0;
+    const uint32_t cb = c * kChunk, ce = min(d, cb + kChunk);
+    if (cb >= ce) continue;
+    sigma_chunk<NW>(st, cb, ce, rrow, frp, nw, sig, acc, dag, &contrib);
+    uint32_t roots = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      if (acc[j]) atomicAdd(&sig[(size_t)w * B + lane + 64 * j], acc[j]);
+      dag_tot[j] += dag[j];
+      roots += (uint32_t)__popcll(nw.w[j]);
+    }
+    // per item: edges id 4 + frontier word 8NW, terms 4; per entry (once):
+    // word 8NW + sigma row write 4 per root (id and offsets: k_bsigma)
+    bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW) + 4ull * contrib;
+    if (first_item) bytes += 8 * NW + 4ull * roots;
+    first_item = false;
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 #pragma unroll
   for (int j = 0; j < NW; ++j)
     if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
@@ -546,9 +647,10 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
                                                    double *xr, const uint32_t *__restrict__ q, uint64_t *qw,
                                                    uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t stamp, uint32_t *seen, int L,
-                                                   uint32_t qcap) {
+                                                   uint32_t qcap, unsigned long long *kb) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
+  unsigned long long bytes = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -580,7 +682,14 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
       nx[(size_t)w * NW + lane] = 0;
     }
     if (lane == 0) seen[w] = stamp;
+    // entry: id 4 + NX 8NW + reach 4 + offsets 8 + seen 4; writes FR, VIS
+    // (read+write), list word, NX: 5 * 8NW; per root reached: sigma 4 + x 8
+    uint32_t roots = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) roots += (uint32_t)__popcll(nw.w[j]);
+    bytes += 20 + 48 * NW + 12ull * roots;
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 #pragma unroll
   for (int j = 0; j < NW; ++j) {
     const int s = lane + 64 * j;
@@ -602,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
       ctl->empty_lev = (uint32_t)L;
       __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    ctl->nsplit = 0;
+    ctl->split_claim = 0;
     ctl->wg_done = 0;
   }
 }
@@ -650,17 +759,30 @@ __device__ __forceinline__ uint32_t dep_chunk(uint32_t st, uint32_t cb, uint32_t
 #pragma unroll
       for (int j = 0; j < NW; ++j) m.w[j] = f.w[j] & ~vw.w[j];
     }
+    // kBurst edges at a time: loads first, then the adds in edge order
     unsigned long long todo = __ballot(m.any());
     while (todo) {
-      const int i = __ffsll((long long)todo) - 1;
-      todo &= todo - 1;
-      const uint32_t ww = __shfl(w, i);
+      double got[kBurst][NW];
+      bool use[kBurst][NW];
 #pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        const uint64_t mm = __shfl(m.w[j], i);
-        contrib += (uint32_t)__popcll(mm);
-        if (bit(mm, lane)) acc[j] += xr[(size_t)ww * B + lane + 64 * j] * sv[j];
+      for (int t = 0; t < kBurst; ++t) {
+        const int i = todo ? __ffsll((long long)todo) - 1 : 0;
+        const bool live = todo != 0;
+        todo &= todo - 1;
+        const uint32_t ww = __shfl(w, i);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
+          contrib += (uint32_t)__popcll(mm);
+          use[t][j] = bit(mm, lane);
+          got[t][j] = use[t][j] ? xr[(size_t)ww * B + lane + 64 * j] : 0.0;
+        }
       }
+#pragma unroll
+      for (int t = 0; t < kBurst; ++t)
+#pragma unroll
+        for (int j = 0; j < NW; ++j)
+          if (use[t][j]) acc[j] += got[t][j] * sv[j];
     }
   }
   return contrib;
@@ -690,18 +812,8 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
     if (!f.any()) continue;
     const uint32_t st = col[v], d = col[v + 1] - st;
     if (d > kChunk) {
-      if (lane == 0) {
-        const unsigned long long nch = (d + kChunk - 1) / kChunk;
-        const unsigned long long old = atomicAdd(&ctl->split_claim, (1ull << 40) + nch);
-        const uint32_t k = (uint32_t)(old >> 40);
-        const uint64_t base = old & ((1ull << 40) - 1);
-        if (k < sp_cap && base + nch <= part_rows) {
-          sp_e[k] = i;
-          sp_base[k] = base;
-        } else {
-          atomicOr(&ctl->overflow, 1u);
-        }
-      }
+      if (lane == 0) claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, part_rows);
+      bytes += 16 + 8 * NW;
       continue;
     }
     double sv[NW], acc[NW];
@@ -728,78 +840,109 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
+// Chunks of the split entries: per item the partial sums of its kChunk
+// edges (left fold from 0.0) go to its row of `part`.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                        const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
-                                                       double *dlt, double *xr, const uint32_t *__restrict__ reach,
-                                                       const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
-                                                       axbc::BCtl *ctl, const uint32_t *__restrict__ sp_e,
-                                                       const uint64_t *__restrict__ sp_base, uint32_t *sp_cnt,
-                                                       uint32_t sp_cap, double *part, int depth, uint64_t root_mask0,
-                                                       uint64_t root_mask1, uint32_t kChunk) {
+                                                       const double *__restrict__ xr, const uint32_t *__restrict__ q,
+                                                       const uint64_t *__restrict__ qw, const axbc::BCtl *ctl,
+                                                       const uint32_t *__restrict__ sp_e,
+                                                       const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
+                                                       uint64_t part_rows, double *part, int depth,
+                                                       uint64_t root_mask0, uint64_t root_mask1,
+                                                       unsigned long long *kb, uint32_t kChunk) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  ItemWalk r = walk_begin(ctl, sp_base, sp_cap, part_rows);
+  if (r.lo >= r.hi) return;
+  unsigned long long bytes = 0;
+  uint32_t v = 0, st = 0, d = 0;
+  uint64_t base = 0;
+  Words<NW> f;
+  double sv[NW];
+  bool fresh = true;
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    if (walk_to(r, sp_base, item) || fresh) {
+      const uint32_t i = sp_e[r.k];
+      v = q[i];
+      st = col[v];
+      d = col[v + 1] - st;
+      base = sp_base[r.k];
+      f = ldw<NW>(qw, i);
+      if (depth == 0) {
+        f.w[0] &= root_mask0;
+        if (NW > 1) f.w[NW - 1] &= root_mask1;
+      }
+#pragma unroll
+      for (int j = 0; j < NW; ++j) sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
+      fresh = false;
+    }
+    const uint32_t cb = (uint32_t)(item - base) * kChunk, ce = min(d, cb + kChunk);
+    if (cb >= ce) continue;
+    double acc[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) acc[j] = 0.0;
+    const uint32_t contrib = dep_chunk<NW>(st, cb, ce, row, vis, f, xr, sv, acc);
+#pragma unroll
+    for (int j = 0; j < NW; ++j) part[(size_t)item * B + lane + 64 * j] = acc[j];
+    // per edge: id 4 + VIS 8NW; per term: x 8 (partials are scratch)
+    bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW) + 8ull * contrib;
+  }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+}
+
+// One wave per split entry: delta = reach + partials in chunk order.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bdep_combine(const uint32_t *__restrict__ col,
+                                                         const uint32_t *__restrict__ sig, double *dlt, double *xr,
+                                                         const uint32_t *__restrict__ reach,
+                                                         const uint32_t *__restrict__ q,
+                                                         const uint64_t *__restrict__ qw, const axbc::BCtl *ctl,
+                                                         const uint32_t *__restrict__ sp_e,
+                                                         const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
+                                                         const double *__restrict__ part, int depth,
+                                                         uint64_t root_mask0, uint64_t root_mask1,
+                                                         unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   const unsigned long long claim = ctl->split_claim;
   const uint32_t ns = min((uint32_t)(claim >> 40), sp_cap);
-  const uint64_t total = claim & ((1ull << 40) - 1);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  for (uint64_t item = gw; item < total; item += nwv) {
-    // entry whose chunk range holds `item` (bases are increasing in k)
-    uint32_t lo = 0, hi = ns;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (sp_base[mid] <= item)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const uint32_t k = lo;
+  unsigned long long bytes = 0;
+  for (uint32_t k = gw; k < ns; k += nwv) {
     const uint32_t i = sp_e[k];
     const uint32_t v = q[i];
+    const uint32_t d = col[v + 1] - col[v];
+    const uint32_t nch = (d + kChunk - 1) / kChunk;
+    const uint64_t base = sp_base[k];
     Words<NW> f = ldw<NW>(qw, i);
     if (depth == 0) {
       f.w[0] &= root_mask0;
       if (NW > 1) f.w[NW - 1] &= root_mask1;
     }
-    const uint32_t st = col[v], d = col[v + 1] - st;
-    const uint32_t nch = (d + kChunk - 1) / kChunk;
-    const uint32_t c = (uint32_t)(item - sp_base[k]);
-    double sv[NW], acc[NW];
+    const double r = (double)reach[v];
+    uint32_t roots = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-      sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
-      acc[j] = 0.0;
-    }
-    (void)dep_chunk<NW>(st, c * kChunk, min(d, c * kChunk + kChunk), row, vis, f, xr, sv, acc);
-#pragma unroll
-    for (int j = 0; j < NW; ++j) part[(size_t)item * B + lane + 64 * j] = acc[j];
-    __threadfence();
-    uint32_t prev = 0;
-    if (lane == 0) prev = atomicAdd(&sp_cnt[k], 1u);
-    prev = __shfl(prev, 0);
-    if (prev == nch - 1) {  // last chunk of this vertex: combine in chunk order
-      __threadfence();
-      const double r = (double)reach[v];
-#pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        double t = r;
-        for (uint32_t cc = 0; cc < nch; ++cc)
-          t += __hip_atomic_load(&part[(size_t)(sp_base[k] + cc) * B + lane + 64 * j], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-        if (bit(f.w[j], lane)) {
-          dlt[(size_t)v * B + lane + 64 * j] = t;
-          xr[(size_t)v * B + lane + 64 * j] = (t + 1.0) / sv[j];
-        }
+      double t = r;
+      const double *pp = part + (size_t)base * B + lane + 64 * j;
+      for (uint32_t cc = 0; cc < nch; ++cc) t += pp[(size_t)cc * B];
+      roots += (uint32_t)__popcll(f.w[j]);
+      if (bit(f.w[j], lane)) {
+        dlt[(size_t)v * B + lane + 64 * j] = t;
+        xr[(size_t)v * B + lane + 64 * j] = (t + 1.0) / (double)sig[(size_t)v * B + lane + 64 * j];
       }
-      if (lane == 0) sp_cnt[k] = 0;
     }
+    // per root of the entry: sigma 4 + delta 8 + x 8
+    bytes += 20ull * roots;
   }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
 __global__ void k_bsplit_reset(axbc::BCtl *ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->split_claim = 0;
-    ctl->nsplit = 0;
   }
 }
 
@@ -854,44 +997,69 @@ __global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t
   }
 }
 
-// bc += the batch's terms in root order, roots [0, nr).
+// bc += the batch's terms in root order, roots [0, nr).  A workgroup takes
+// kBlock vertices at a time.  The terms of kUpdRoots roots at a time are
+// computed with lanes over (vertex, root), so every wave reads whole
+// 128-byte segments of the delta rows, and staged in LDS; then each thread
+// folds its own vertex's terms in root order (the sequence of one-root
+// updates, bit for bit).
+constexpr int kUpdRoots = 16;
+
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bupdate(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
                                                     const uint32_t *__restrict__ reach, int nr,
                                                     const uint32_t *__restrict__ src, const double *__restrict__ prob,
                                                     const uint32_t *__restrict__ r0, const double *__restrict__ dlt,
-                                                    const unsigned long long *__restrict__ cnt, double *bc) {
-  const int lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  for (uint32_t v0 = gw * kWave; v0 < n; v0 += nwv * kWave) {
-    const uint32_t v = v0 + lane;
-    const bool want = v < n && (seen[v] == stamp || reach[v] != 0);
-    unsigned long long todo = __ballot(want);
-    while (todo) {
-      const int i = __ffsll((long long)todo) - 1;
-      todo &= todo - 1;
-      const uint32_t vv = v0 + (uint32_t)i;
-      double t[NW];
-      bool h[NW];
-#pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        const int s = lane + 64 * j;
-        t[j] = 0.0;
-        h[j] = false;
-        if (s < nr) t[j] = bc_term<NW>(vv, s, src, prob, r0, dlt, cnt, &h[j]);
-      }
-      // sequential in root order
-      double acc = 0.0;
-      if (lane == 0) acc = bc[vv];
-#pragma unroll
-      for (int j = 0; j < NW; ++j)
-        for (int s = 0; s < 64; ++s) {
-          const double ts = __shfl(t[j], s);
-          const int hs = __shfl((int)h[j], s);
-          if (lane == 0 && hs) acc += ts;
-        }
-      if (lane == 0) bc[vv] = acc;
+                                                    const unsigned long long *__restrict__ cnt, double *bc,
+                                                    unsigned long long *kb) {
+  constexpr int B = 64 * NW;
+  constexpr int G = kUpdRoots;
+  __shared__ double tile[kBlock][G + 1];
+  __shared__ double s_r1[B], s_p[B], s_root[B];
+  __shared__ uint32_t s_src[B];
+  __shared__ uint32_t s_want[kBlock];
+  for (int s = threadIdx.x; s < B; s += blockDim.x) {
+    if (s < nr) {
+      const uint32_t rr = r0[s];
+      s_src[s] = src[s];
+      s_r1[s] = (double)(uint32_t)(rr + 1u);
+      s_p[s] = prob[s];
+      // the root's own term (update_bc's v == v0 branch); absent when rr == 0
+      const uint32_t all = (uint32_t)cnt[4 * s + 1] + rr;
+      const unsigned long long nvt = cnt[4 * s + 0] + (unsigned long long)all;
+      s_root[s] = rr ? (double)((unsigned long long)rr * (nvt - 2ull)) : 0.0;
     }
+  }
+  unsigned long long bytes = 0;
+  for (uint32_t vb = blockIdx.x * kBlock; vb < n; vb += gridDim.x * kBlock) {
+    const uint32_t v = vb + threadIdx.x;
+    const bool want = v < n && (seen[v] == stamp || reach[v] != 0);
+    s_want[threadIdx.x] = want;
+    double acc = want ? bc[v] : 0.0;
+    bytes += v < n ? 8 + (want ? 16 + 8ull * (unsigned)nr : 0) : 0;
+    if (!__syncthreads_or(want)) continue;
+    for (int g = 0; g < nr; g += G) {
+#pragma unroll 4
+      for (int k = 0; k < G; ++k) {
+        const int flat = k * kBlock + threadIdx.x;
+        const int vi = flat / G, sl = flat % G, s = g + sl;
+        const uint32_t vv = vb + (uint32_t)vi;
+        double t = 0.0;
+        if (s < nr && s_want[vi]) t = vv == s_src[s] ? s_root[s] : dlt[(size_t)vv * B + s] * s_r1[s] / s_p[s];
+        tile[vi][sl] = t;
+      }
+      __syncthreads();
+      if (want) {
+        const int m = min(G, nr - g);
+        for (int sl = 0; sl < m; ++sl) acc += tile[threadIdx.x][sl];
+      }
+      __syncthreads();
+    }
+    if (want) bc[v] = acc;
+  }
+  if (kb) {
+    bytes = wave_sum_u64(bytes);
+    if (lane_id() == 0 && bytes) atomicAdd(kb, bytes);
   }
 }
 
@@ -914,7 +1082,7 @@ void batch_free(axbc_ctx *c) {
   BatchState *b = c->batch;
   if (!b) return;
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
-               b->act, b->sp_v, b->sp_e, b->sp_cnt, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
+               b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
                b->check, b->ctl};
   for (void *p : d)
     if (p) (void)hipFree(p);
@@ -1002,9 +1170,7 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   BA(b->qw, (size_t)qcap * nw);
   BA(b->qoff, n + 2);
   BA(b->act, (n + 2) * nw);
-  BA(b->sp_v, b->sp_cap);
   BA(b->sp_e, b->sp_cap);
-  BA(b->sp_cnt, b->sp_cap);
   BA(b->sp_base, b->sp_cap);
   BA(b->part, b->part_rows * B);
   BA(b->cnt, 4 * B);
@@ -1017,7 +1183,6 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   HIP_TRY(hipHostMalloc((void **)&b->h_cnt, sizeof(unsigned long long) * 4 * B, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&b->h_act, sizeof(uint64_t) * (n + 2) * nw, hipHostMallocDefault));
   HIP_TRY(hipMemsetAsync(b->ctl, 0, sizeof(BCtl), c->stream));
-  HIP_TRY(hipMemsetAsync(b->sp_cnt, 0, sizeof(uint32_t) * b->sp_cap, c->stream));
   HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (n + 2) * nw, c->stream));
   if (nw == 1)
     hipLaunchKernelGGL(k_bfill<1>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, (uint32_t)n, c->reach, b->vis,
@@ -1050,8 +1215,8 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   HIP_TRY(hipMemcpyAsync(b->r0, r0.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
   // reset the previous batch, then level 0
   if (b->have)
-    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, b->q, b->ctl, c->reach, b->vis, b->fr[0],
-                       b->fr[1], b->sig, b->dlt, b->qcap);
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, b->vis,
+                       b->fr[0], b->fr[1], b->sig, b->dlt);
   b->have = true;
   b->stamp += 1;
   HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)(b->lev_guess + 2) * NW, s));
@@ -1074,21 +1239,29 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       const uint32_t mark_id = b->epoch + (uint32_t)lv;
       kt_begin(c, KK_PUSH);
       hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, frc,
-                         b->q, b->qw, b->qoff, b->ctl, b->sp_v, b->sp_cap, b->qcap, lv, mark_id, kt_bytes(c, KK_PUSH),
-                         b->chunk);
+                         b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap, lv,
+                         mark_id, kt_bytes(c, KK_PUSH), b->chunk);
       kt_end(c, KK_PUSH);
-      hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
-                         b->mark, b->q, b->qw, b->ctl, b->sp_v, b->sp_cap, b->qcap, mark_id, b->chunk);
+      kt_begin(c, KK_PUSH_SPLIT);
+      hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
+                         b->mark, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                         mark_id, kt_bytes(c, KK_PUSH_SPLIT), b->chunk);
+      kt_end(c, KK_PUSH_SPLIT);
       hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
       kt_begin(c, KK_SIGMA);
       hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp, b->sig, b->q,
-                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_cap, lv, b->qcap, kt_bytes(c, KK_SIGMA), b->chunk);
+                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, lv, b->qcap,
+                         kt_bytes(c, KK_SIGMA), b->chunk);
       kt_end(c, KK_SIGMA);
-      hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
-                         b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_cap, b->chunk);
+      kt_begin(c, KK_SIGMA_SPLIT);
+      hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
+                         b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                         kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
+      kt_end(c, KK_SIGMA_SPLIT);
       kt_begin(c, KK_CLOSE);
       hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, frc,
-                         b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap);
+                         b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap,
+                         kt_bytes(c, KK_CLOSE));
       kt_end(c, KK_CLOSE);
     }
     HIP_TRY(hipGetLastError());
@@ -1125,9 +1298,16 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
                        c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, d,
                        rootmask[0], NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP), b->chunk);
     kt_end(c, KK_DEP);
-    hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt,
-                       b->xr, c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cnt, b->sp_cap, b->part, d,
-                       rootmask[0], NW > 1 ? rootmask[1] : 0ull, b->chunk);
+    kt_begin(c, KK_DEP_SPLIT);
+    hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->xr,
+                       b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->part, d, rootmask[0],
+                       NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP_SPLIT), b->chunk);
+    kt_end(c, KK_DEP_SPLIT);
+    kt_begin(c, KK_DEP_COMBINE);
+    hipLaunchKernelGGL(k_bdep_combine<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, b->sig, b->dlt, b->xr,
+                       c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part, d, rootmask[0],
+                       NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP_COMBINE), b->chunk);
+    kt_end(c, KK_DEP_COMBINE);
     hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
   }
   HIP_TRY(hipGetLastError());
@@ -1150,7 +1330,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   }
   kt_begin(c, KK_UPDATE);
   hipLaunchKernelGGL(k_bupdate<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, nr, b->src,
-                     b->prob, b->r0, b->dlt, b->cnt, c->bc);
+                     b->prob, b->r0, b->dlt, b->cnt, c->bc, kt_bytes(c, KK_UPDATE));
   kt_end(c, KK_UPDATE);
   HIP_TRY(hipGetLastError());
   // the context's delta becomes the last applied root's (the -z 4

@@ -7,15 +7,18 @@
 //   lvl[N] int32, sigma[N] uint32, delta[N] f64, bc[N] f64
 //   queue[N]                uint32, the frontiers of the round back to back;
 //                           qoff[L] = start of level L (qoff[0]=0, qoff[1]=1)
-//   hubs[], hub_part[]      deferred high-degree vertices of one level and
-//                           their per-workgroup partial sums
+//   hubs[], hub_base[]      deferred high-degree vertices of one pass and the
+//                           first of their chunk work items; hub_part[] holds
+//                           one partial sum per item (backward)
 //   ctl                     device counters (queue tail, done flag, ...)
 //
 // One round = reset of the previous round's visited vertices, root init,
 // per level a load-balanced push expansion (wave = 64 frontier vertices,
-// their edges dealt out 64 per step) plus a grid-wide pass over the
-// deferred hubs, then per depth an atomic-free FP64 pull over the level's
-// vertices (segmented wave reduction) plus the hub pass, then the bc update
+// their edges dealt out 64 per step) plus a pass over the deferred hubs in
+// chunks of kHubChunk edges (each wave one contiguous range of chunks), then
+// per depth an atomic-free FP64 pull over the level's vertices (segmented
+// wave reduction) plus the hub chunks, whose partial sums are combined in
+// chunk order, then the bc update
 // over the visited vertices and the reach>0 list.  Levels are enqueued
 // speculatively; kernels past the last level exit on the device done flag,
 // so the host waits once per round.
@@ -156,7 +159,7 @@ __global__ void k_set_root(uint32_t v0, int *lvl, uint32_t *sigma, uint32_t *que
   ctl->tail = 1;
   ctl->done = 0;
   ctl->empty_lev = 0;
-  ctl->nhub = 0;
+  ctl->hub_claim = 0;
   ctl->wg_done = 0;
   ctl->all = 0;
   ctl->e_r = 0;
@@ -186,11 +189,72 @@ __device__ __forceinline__ bool visit_edge(uint32_t w, uint32_t su, int L, int *
   return fresh;
 }
 
+// Hub u (more than kHubDeg edges) of the current pass: one 64-bit atomic
+// returns its slot k and its first chunk item (hub_claim = hubs << 40 |
+// items), so hub_base[] increases with k.
+__device__ __forceinline__ void claim_hub(DevCtl *ctl, uint32_t u, uint32_t d, uint32_t *hubs, uint64_t *hub_base,
+                                          uint32_t hub_cap, uint64_t item_cap) {
+  const uint32_t nch = (d + kHubChunk - 1) / kHubChunk;
+  const unsigned long long old = atomicAdd(&ctl->hub_claim, (1ull << 40) + nch);
+  const uint32_t k = (uint32_t)(old >> 40);
+  const uint64_t base = old & ((1ull << 40) - 1);
+  if (k < hub_cap && base + nch <= item_cap) {
+    hubs[k] = u;
+    hub_base[k] = base;
+  } else {
+    atomicOr(&ctl->overflow, 1u);
+  }
+}
+
+// This wave's contiguous range of hub chunk items and the hub holding the
+// first one.
+struct HubWalk {
+  uint32_t nh, k;
+  uint64_t lo, hi, next;
+};
+
+__device__ __forceinline__ HubWalk hub_walk(const DevCtl *ctl, const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                            uint64_t item_cap) {
+  const unsigned long long claim = ctl->hub_claim;
+  HubWalk r;
+  r.nh = min((uint32_t)(claim >> 40), hub_cap);
+  const uint64_t total = min((uint64_t)(claim & ((1ull << 40) - 1)), item_cap);
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  const uint64_t per = (total + nwv - 1) / nwv;
+  r.lo = min(total, (uint64_t)gw * per);
+  r.hi = min(total, r.lo + per);
+  r.k = 0;
+  r.next = ~0ull;
+  if (r.lo < r.hi && r.nh > 0) {
+    uint32_t lo = 0, hi = r.nh;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (hub_base[mid] <= r.lo)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    r.k = lo;
+    r.next = lo + 1 < r.nh ? hub_base[lo + 1] : ~0ull;
+  } else {
+    r.hi = r.lo;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void hub_step(HubWalk &r, const uint64_t *__restrict__ hub_base, uint64_t item) {
+  while (item >= r.next) {
+    ++r.k;
+    r.next = r.k + 1 < r.nh ? hub_base[r.k + 1] : ~0ull;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restrict__ col,
                                                        const uint32_t *__restrict__ row, int *lvl,
                                                        uint32_t *sigma, uint32_t *queue,
                                                        const uint32_t *__restrict__ qoff,
-                                                       DevCtl *ctl, uint32_t *hubs, int L) {
+                                                       DevCtl *ctl, uint32_t *hubs, uint64_t *hub_base,
+                                                       uint32_t hub_cap, uint64_t item_cap, int L) {
   if (ld_acq(&ctl->done)) return;
   const uint32_t fb = qoff[L - 1], fe = qoff[L];
   const int lane = lane_id();
@@ -208,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
     }
     er += d;
     if (d > kHubDeg) {
-      hubs[atomicAdd(&ctl->nhub, 1u)] = u;
+      claim_hub(ctl, u, d, hubs, hub_base, hub_cap, item_cap);
       d = 0;
     }
     const uint32_t incl = wave_incl_scan(d);
@@ -255,21 +319,22 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ row, int *lvl,
                                                      uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
                                                      DevCtl *ctl, const uint32_t *__restrict__ hubs,
-                                                     int L) {
+                                                     const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                                     uint64_t item_cap, int L) {
   if (ld_acq(&ctl->done)) return;
-  const uint32_t nh = ctl->nhub;
   const int lane = lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
   unsigned long long dag = 0;
-  for (uint32_t h = 0; h < nh; ++h) {
-    const uint32_t u = hubs[h];
+  HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    hub_step(r, hub_base, item);
+    const uint32_t u = hubs[r.k];
     const uint32_t st = col[u], d = col[u + 1] - st, su = sigma[u];
-    for (uint32_t k0 = wid * kWave; k0 < d; k0 += nw * kWave) {
+    const uint32_t cb = (uint32_t)(item - hub_base[r.k]) * kHubChunk, ce = min(d, cb + kHubChunk);
+    for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
       const uint32_t k = k0 + lane;
       bool fresh = false;
       uint32_t w = 0;
-      if (k < d) {
+      if (k < ce) {
         w = row[st + k];
         fresh = visit_edge(w, su, L, lvl, sigma, dag);
       }
@@ -285,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
       ctl->empty_lev = (uint32_t)L;
       __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    ctl->nhub = 0;
+    ctl->hub_claim = 0;
     ctl->wg_done = 0;
   }
 }
@@ -310,7 +375,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_pull(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ reach,
                                                      const uint32_t *__restrict__ queue,
                                                      const uint32_t *__restrict__ qoff, DevCtl *ctl,
-                                                     uint32_t *hubs, int depth) {
+                                                     uint32_t *hubs, uint64_t *hub_base, uint32_t hub_cap,
+                                                     uint64_t item_cap, int depth) {
   const uint32_t fb = qoff[depth], fe = qoff[depth + 1];
   const int dn = depth + 1;
   const int lane = lane_id();
@@ -328,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_pull(const uint32_t *__restrict_
     }
     bool hub = d > kHubDeg;
     if (hub) {
-      hubs[atomicAdd(&ctl->nhub, 1u)] = v;
+      claim_hub(ctl, v, d, hubs, hub_base, hub_cap, item_cap);
       d = 0;
     }
     const uint32_t incl = wave_incl_scan(d);
@@ -366,49 +432,51 @@ __global__ __launch_bounds__(kBlock) void k_bwd_pull(const uint32_t *__restrict_
   }
 }
 
+// Hub chunks of depth d: per item the sum of its terms (lane partials in
+// edge order, then a fixed wave tree) goes to hub_part[item].
 __global__ __launch_bounds__(kBlock) void k_bwd_hubs(const uint32_t *__restrict__ col,
                                                      const uint32_t *__restrict__ row,
                                                      const int *__restrict__ lvl,
-                                                     const uint32_t *__restrict__ sigma, double *delta,
-                                                     const uint32_t *__restrict__ reach, DevCtl *ctl,
-                                                     const uint32_t *__restrict__ hubs, double *part,
-                                                     uint32_t *hubcnt, int depth) {
-  __shared__ double red[kBlock / kWave];
-  __shared__ bool last_for_hub;
-  const uint32_t nh = ctl->nhub;
+                                                     const uint32_t *__restrict__ sigma,
+                                                     const double *__restrict__ delta, const DevCtl *ctl,
+                                                     const uint32_t *__restrict__ hubs,
+                                                     const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                                     uint64_t item_cap, double *part, int depth) {
   const int dn = depth + 1;
-  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, ng = gridDim.x * blockDim.x;
-  for (uint32_t h = 0; h < nh; ++h) {
-    const uint32_t v = hubs[h];
+  const int lane = lane_id();
+  HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    hub_step(r, hub_base, item);
+    const uint32_t v = hubs[r.k];
     const uint32_t st = col[v], d = col[v + 1] - st;
     const double sv = (double)sigma[v];
-    double s = 0.0;
-    for (uint32_t k = gt; k < d; k += ng) s += dep_term(row[st + k], dn, sv, lvl, sigma, delta);
-    // deterministic block reduction: wave tree, then waves in order
+    const uint32_t cb = (uint32_t)(item - hub_base[r.k]) * kHubChunk, ce = min(d, cb + kHubChunk);
+    double t = 0.0;
+    for (uint32_t k = cb + lane; k < ce; k += kWave) t += dep_term(row[st + k], dn, sv, lvl, sigma, delta);
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_down(s, off);
-    if (lane_id() == 0) red[threadIdx.x / kWave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double b = 0.0;
-      for (int j = 0; j < kBlock / kWave; ++j) b += red[j];
-      part[(size_t)h * gridDim.x + blockIdx.x] = b;
-      __threadfence();
-      last_for_hub = atomicAdd(&hubcnt[h], 1u) == gridDim.x - 1;
-      if (last_for_hub) {
-        __threadfence();
-        double tot = 0.0;
-        for (uint32_t j = 0; j < gridDim.x; ++j)
-          tot += __hip_atomic_load(&part[(size_t)h * gridDim.x + j], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        delta[v] = (double)reach[v] + tot;
-        hubcnt[h] = 0;
-      }
-    }
-    __syncthreads();
+    for (int off = kWave / 2; off > 0; off >>= 1) t += __shfl_down(t, off);
+    if (lane == 0) part[item] = t;
+  }
+}
+
+// One thread per hub: delta = reach + its chunk sums in chunk order.
+__global__ __launch_bounds__(kBlock) void k_bwd_hub_combine(const uint32_t *__restrict__ col, double *delta,
+                                                            const uint32_t *__restrict__ reach, DevCtl *ctl,
+                                                            const uint32_t *__restrict__ hubs,
+                                                            const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                                            const double *__restrict__ part) {
+  const unsigned long long claim = ctl->hub_claim;
+  const uint32_t nh = min((uint32_t)(claim >> 40), hub_cap);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += gridDim.x * blockDim.x) {
+    const uint32_t v = hubs[k];
+    const uint32_t nch = (col[v + 1] - col[v] + kHubChunk - 1) / kHubChunk;
+    const double *p = part + hub_base[k];
+    double tot = 0.0;
+    for (uint32_t c = 0; c < nch; ++c) tot += p[c];
+    delta[v] = (double)reach[v] + tot;
   }
   if (last_workgroup(ctl) && threadIdx.x == 0) {
-    ctl->nhub = 0;
+    ctl->hub_claim = 0;
     ctl->wg_done = 0;
   }
 }
@@ -499,10 +567,13 @@ int pull_ctl(axbc_ctx *c) {
 int enqueue_forward(axbc_ctx *c, int L) {
   kt_begin(c, KK_FWD);
   hipLaunchKernelGGL(k_fwd_expand, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
-                     c->lvl, c->sigma, c->queue, c->qoff, c->ctl, c->hubs, L);
+                     c->lvl, c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap,
+                     c->hub_items, L);
   kt_end(c, KK_FWD);
+  kt_begin(c, KK_FWD_HUBS);
   hipLaunchKernelGGL(k_fwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
-                     c->sigma, c->queue, c->qoff, c->ctl, c->hubs, L);
+                     c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
+  kt_end(c, KK_FWD_HUBS);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }
@@ -510,10 +581,16 @@ int enqueue_forward(axbc_ctx *c, int L) {
 int enqueue_backward(axbc_ctx *c, int depth) {
   kt_begin(c, KK_BWD);
   hipLaunchKernelGGL(k_bwd_pull, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
-                     c->lvl, c->sigma, c->delta, c->reach, c->queue, c->qoff, c->ctl, c->hubs, depth);
+                     c->lvl, c->sigma, c->delta, c->reach, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base,
+                     c->nhub_cap, c->hub_items, depth);
   kt_end(c, KK_BWD);
+  kt_begin(c, KK_BWD_HUBS);
   hipLaunchKernelGGL(k_bwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
-                     c->sigma, c->delta, c->reach, c->ctl, c->hubs, c->hub_part, c->hubcnt, depth);
+                     c->sigma, c->delta, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, c->hub_part,
+                     depth);
+  kt_end(c, KK_BWD_HUBS);
+  hipLaunchKernelGGL(k_bwd_hub_combine, dim3(64), dim3(kBlock), 0, c->stream, c->col, c->delta, c->reach,
+                     c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_part);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }
@@ -600,9 +677,11 @@ int axbc_ctx_create(int device, uint64_t N, uint64_t ned, const uint32_t *col, c
   TRY_ALLOC(c->bc, N);
   TRY_ALLOC(c->queue, N);
   TRY_ALLOC(c->qoff, N + 2);
+  // chunk items per pass: sum over distinct hubs of ceil(d / kHubChunk)
+  c->hub_items = ned / kHubChunk + c->nhub_cap + 1;
   TRY_ALLOC(c->hubs, c->nhub_cap);
-  TRY_ALLOC(c->hubcnt, c->nhub_cap);
-  TRY_ALLOC(c->hub_part, (size_t)c->nhub_cap * kHubGrid);
+  TRY_ALLOC(c->hub_base, c->nhub_cap);
+  TRY_ALLOC(c->hub_part, c->hub_items);
   TRY_ALLOC(c->ctl, 1);
 #undef TRY_ALLOC
   if (hipHostMalloc((void **)&c->h_ctl, sizeof(DevCtl), hipHostMallocDefault) != hipSuccess) {
@@ -623,7 +702,6 @@ int axbc_ctx_create(int device, uint64_t N, uint64_t ned, const uint32_t *col, c
                              : hipMemcpyAsync(c->rlist, rl.data(), sizeof(uint32_t) * rl.size(),
                                               hipMemcpyHostToDevice, c->stream);
   (void)hipMemsetAsync(c->ctl, 0, sizeof(DevCtl), c->stream);
-  (void)hipMemsetAsync(c->hubcnt, 0, sizeof(uint32_t) * c->nhub_cap, c->stream);
   hipLaunchKernelGGL(k_init_state, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, (uint32_t)N, c->reach,
                      c->lvl, c->sigma, c->delta, c->bc);
   hipError_t e5 = hipStreamSynchronize(c->stream);
@@ -654,7 +732,7 @@ int axbc_ctx_destroy(axbc_ctx *c) {
     if (c->rrow) (void)hipFree(c->rrow);
   }
   void *ptrs[] = {c->col,   c->row,  c->reach, c->rlist,  c->lvl,      c->sigma, c->delta, c->bc,
-                  c->queue, c->qoff, c->hubs,  c->hubcnt, c->hub_part, c->ctl,   c->check};
+                  c->queue, c->qoff, c->hubs,  c->hub_base, c->hub_part, c->ctl,   c->check};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);

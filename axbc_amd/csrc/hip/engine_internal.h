@@ -13,18 +13,20 @@ namespace axbc {
 constexpr int kBlock = 256;
 constexpr int kExpandGrid = 1024;   // workgroups of the level kernels
 constexpr int kHubGrid = 512;       // workgroups of the hub passes
-constexpr uint32_t kHubDeg = 2048;  // degree above which a vertex is a hub
+constexpr uint32_t kHubDeg = 1024;    // degree above which a vertex is a hub
+constexpr uint32_t kHubChunk = 1024;  // edges per work item of a hub
 
 struct DevCtl {
   uint32_t tail;       // queue length of the current round
   uint32_t done;       // forward finished
   uint32_t empty_lev;  // first empty level (valid when done)
-  uint32_t nhub;       // hubs deferred in the current level
+  uint32_t unused0;
   uint32_t wg_done;    // last-workgroup counter
   uint32_t all;        // reach sum of visited non-root vertices
   uint32_t stop_any;   // stop test: a check vertex below threshold
   uint32_t overflow;   // a bounded list ran out of room
   unsigned long long e_r, e_dag;
+  unsigned long long hub_claim;  // hubs of the current pass << 40 | chunk items
 };
 
 // Device buffers of the samplers (allocated on first use).
@@ -63,7 +65,13 @@ enum KernelKind {
   KK_FWD = 5,       // plain forward expansion (k_fwd_expand)
   KK_BWD = 6,       // plain dependency pull (k_bwd_pull)
   KK_COPY = 7,      // calibration copy kernel
-  KK_COUNT = 8
+  KK_PUSH_SPLIT = 8,   // batched forward marking, split vertices (k_bpush_split)
+  KK_SIGMA_SPLIT = 9,  // batched path-count pull, split vertices (k_bsigma_split)
+  KK_DEP_SPLIT = 10,   // batched dependency pull, split vertices (k_bdep_split)
+  KK_DEP_COMBINE = 11, // batched dependency chunk combine (k_bdep_combine)
+  KK_FWD_HUBS = 12,    // plain forward, hub chunks (k_fwd_hubs)
+  KK_BWD_HUBS = 13,    // plain dependency pull, hub chunks (k_bwd_hubs)
+  KK_COUNT = 14
 };
 
 struct KernelTimer {
@@ -106,9 +114,11 @@ struct axbc_ctx {
   int *lvl = nullptr;
   uint32_t *sigma = nullptr;
   double *delta = nullptr, *bc = nullptr;
-  uint32_t *queue = nullptr, *qoff = nullptr, *hubs = nullptr, *hubcnt = nullptr;
+  uint32_t *queue = nullptr, *qoff = nullptr, *hubs = nullptr;
+  uint64_t *hub_base = nullptr;
   double *hub_part = nullptr;
   uint32_t nhub_cap = 0;
+  uint64_t hub_items = 0;
   axbc::DevCtl *ctl = nullptr;
   axbc::DevCtl *h_ctl = nullptr;  // pinned mirror
   uint32_t *check = nullptr;

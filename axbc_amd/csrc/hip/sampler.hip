@@ -129,18 +129,42 @@ int device_scan(const T *in, T *out, uint64_t n, T *tiles, hipStream_t s) {
 
 // ---- high-degree-neighbourhood tables ------------------------------------
 
+// Thread per vertex; vertices with more than 64 edges are summed by the
+// whole wave (wrapping uint32 sums are order-free).
 __global__ void k_hdn_terms(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row, uint32_t n,
                             uint32_t *deg2, uint32_t *sdn) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const uint32_t b = col[v], e = col[v + 1];
-    const uint64_t d = e - b;
-    deg2[v] = (uint32_t)(d * d);  // (LOCINT)pow(deg, 2): exact square, low 32 bits
-    uint32_t s = 0;
-    for (uint32_t k = b; k < e; ++k) {
-      const uint32_t w = row[k];
-      s += col[w + 1] - col[w];
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
+    const uint32_t v = base + (uint32_t)lane;
+    uint32_t b = 0, e = 0;
+    if (v < n) {
+      b = col[v];
+      e = col[v + 1];
+      const uint64_t d = e - b;
+      deg2[v] = (uint32_t)(d * d);  // (LOCINT)pow(deg, 2): exact square, low 32 bits
     }
-    sdn[v] = s;
+    const bool big = e - b > 64;
+    uint32_t s = 0;
+    if (!big)
+      for (uint32_t k = b; k < e; ++k) {
+        const uint32_t w = row[k];
+        s += col[w + 1] - col[w];
+      }
+    unsigned long long todo = __ballot(big);
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t bb = __shfl(b, i), ee = __shfl(e, i);
+      uint32_t t = 0;
+      for (uint32_t k = bb + (uint32_t)lane; k < ee; k += 64) {
+        const uint32_t w = row[k];
+        t += col[w + 1] - col[w];
+      }
+      t = (uint32_t)wave_sum_u64(t);
+      if (lane == i) s = t;
+    }
+    if (v < n) sdn[v] = s;
   }
 }
 

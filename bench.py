@@ -32,8 +32,15 @@ WORKLOAD = {
     "pivots": "high-degree-neighbourhood sampling (-z 1), srand48(4343543), adaptive stop c=1 x=1024, max 4096",
 }
 KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_expand",
-              6: "k_bwd_pull", 7: "k_copy"}
+              6: "k_bwd_pull", 7: "k_copy", 8: "k_bpush_split", 9: "k_bsigma_split", 10: "k_bdep_split",
+              11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs"}
+BATCH_KINDS = (0, 8, 1, 9, 2, 3, 10, 11, 4)
+PLAIN_KINDS = (5, 12, 6, 13)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (KiB per dispatch) on this workload, scaled by the same counters' reading
+# for a copy kernel of known size.
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
 
 
 def log(*a):
@@ -58,6 +65,45 @@ def set_timing(e, on):
     L.axbc_kernel_timing.argtypes = [C.c_void_p, C.c_int]
     if L.axbc_kernel_timing(e.ctx, 1 if on else 0) != 0:
         raise RuntimeError(L.axbc_last_error())
+
+
+def log_kernels(kstats):
+    for k, (ms, n, b) in kstats.items():
+        if n:
+            log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us, "
+                f"{b / 1e9:.3f} GB -> {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary,
+    or None when the kernel was not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or t.get("workload") != WORKLOAD["workload"]:
+        return None
+    return int((k["fetch_kib"] + k["write_kib"]) * 1024 * t["copy_factor"])
+
+
+def copy_calibration(mib):
+    """The calibration copy kernel (reads and writes `mib` MiB per launch),
+    timed with HIP events; used to scale the hardware byte counters."""
+    from axbc_amd import Engine, Graph, _ffi
+    g = Graph.kronecker(10, 16, seed=(1, 2))
+    e = Engine(g)
+    L = _ffi.hip()
+    set_timing(e, True)
+    if L.axbc_copy_kernel(e.ctx, mib << 20, 8) != 0:
+        raise RuntimeError(L.axbc_last_error())
+    ms, n, b = kernel_stats(e, 7)
+    set_timing(e, False)
+    e.close()
+    print(json.dumps({"kernel": "k_copy", "bytes_read_per_launch": mib << 20, "bytes_written_per_launch": mib << 20,
+                      "launches": n, "avg_launch_us": round(ms / n * 1e3, 2),
+                      "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1)}))
 
 
 def cpu_baseline(g, roots, probs, seconds):
@@ -89,6 +135,8 @@ def main():
     ap.add_argument("--plain-rounds", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the calibration copy kernel over MIB MiB (8 launches) and print its rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs (pivot sharding with one "
@@ -97,6 +145,10 @@ def main():
 
     from axbc_amd import Engine, Graph
     from axbc_amd.pivots import PivotLoop
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     t0 = time.perf_counter()
     g = Graph.kronecker(args.scale, 16, seed=(2, 3), heuristic=1)
@@ -121,22 +173,20 @@ def main():
     dt = time.perf_counter() - t
     rounds = loop.res.nrounds - rounds0
     er = loop.res.e_r - er0
-    kstats = {k: kernel_stats(e, k) for k in range(5)}
+    kstats = {k: kernel_stats(e, k) for k in BATCH_KINDS}
     set_timing(e, False)
     pivots_s = rounds / dt
     gteps = 2.0 * er / dt / 1e9
     log(f"batched: {rounds} rounds in {dt:.3f}s -> {pivots_s:.1f} pivots/s, {gteps:.2f} GTEPS")
-    for k, (ms, n, b) in kstats.items():
-        if n:
-            log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us, "
-                f"{b / 1e9:.3f} GB -> {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s")
-    dom = max(range(5), key=lambda k: kstats[k][0])
+    log_kernels(kstats)
+    dom = max(kstats, key=lambda k: kstats[k][0])
     ms, n, b = kstats[dom]
     achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": KIND_NAMES[dom],
-                "avg_launch_us": round(ms / max(n, 1) * 1e3, 2), "launches": n,
-                "algorithmic_bytes_per_launch": int(b / max(n, 1))}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(KIND_NAMES[dom]),
+                "kernel": KIND_NAMES[dom], "avg_launch_us": round(ms / max(n, 1) * 1e3, 2), "launches": n,
+                "algorithmic_bytes_per_launch": int(b / max(n, 1)),
+                "kernel_ms_share": round(ms / max(sum(v[0] for v in kstats.values()), 1e-9), 3)}
 
     # ---- plain one-root-per-round loop on the same engine --------------------
     e.reset_bc()
@@ -152,11 +202,7 @@ def main():
             break
     e.sync()
     dtp = time.perf_counter() - t
-    for k in (5, 6):
-        ms, n, b = kernel_stats(e, k)
-        if n:
-            log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us, "
-                f"{b / 1e9:.3f} GB -> {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s")
+    log_kernels({k: kernel_stats(e, k) for k in PLAIN_KINDS})
     set_timing(e, False)
     prounds = ploop.res.nrounds - r0p
     plain_s = prounds / dtp

@@ -193,7 +193,10 @@ int axbc_batch_delta(axbc_ctx *ctx, int k, double *delta);
  * algorithmic byte counts to device counters.  Kinds: 0 batched push,
  * 1 batched path-count pull, 2 batched level close, 3 batched dependency
  * pull, 4 batched bc update, 5 plain forward, 6 plain dependency pull,
- * 7 calibration copy.  axbc_kernel_stats returns the summed milliseconds,
+ * 7 calibration copy, 8 batched push of split vertices, 9 batched
+ * path-count pull of split vertices, 10 batched dependency pull of split
+ * vertices, 11 their chunk combine, 12 plain forward hub chunks, 13 plain
+ * dependency pull hub chunks.  axbc_kernel_stats returns the summed milliseconds,
  * the launch count and the byte count of one kind since enabling.
  * axbc_copy_kernel runs a plain device copy of `bytes` bytes (kind 7) to
  * calibrate hardware counters against a known traffic.
