@@ -1,8 +1,10 @@
 """Autotuner for the batched sweep: picks, per graph, the number of roots
-per batch (32, 64 or 128; 32 runs in the 64-wide kernels) and the split
-threshold (edges per work item for high-degree vertices) by timing real
-batches of the graph's own roots.  bc is saved and restored around the
-trials, so tuning leaves the engine's result untouched.
+per batch (32, 64 or 128; 32 runs in the 64-wide kernels), the split
+threshold (edges per work item for high-degree vertices) and the
+bottom-up switch ratio by timing real batches of the graph's own roots.
+Width and threshold are searched jointly, the switch ratio afterwards at
+the chosen pair.  bc is saved and restored around the trials, so tuning
+leaves the engine's result untouched.
 """
 from __future__ import annotations
 
@@ -15,12 +17,14 @@ from .engine import Engine, _check
 
 WIDTHS = (32, 64, 128)
 CHUNKS = (128, 256, 512, 1024)
+PULLS = (0, 16, 32, 64)
 
 
-def _trial(e: Engine, roots: np.ndarray, width: int, chunk: int, reps: int) -> float:
+def _trial(e: Engine, roots: np.ndarray, width: int, chunk: int, reps: int, pull: int = 32) -> float:
     L = _ffi.hip()
     _check(L.axbc_set_batch_width(e.ctx, 128 if width == 128 else 64))
     _check(L.axbc_set_batch_chunk(e.ctx, chunk))
+    _check(L.axbc_set_batch_pull(e.ctx, pull))
     p = np.full(width, 1.0 / e.N)
     e.batch(roots[:width], p[:width])  # warm-up (allocation, caches)
     e.sync()
@@ -32,10 +36,11 @@ def _trial(e: Engine, roots: np.ndarray, width: int, chunk: int, reps: int) -> f
     return (time.perf_counter() - t) / (reps * width)
 
 
-def tune(e: Engine, roots: np.ndarray, reps: int = 2, widths=WIDTHS, chunks=CHUNKS) -> dict:
+def tune(e: Engine, roots: np.ndarray, reps: int = 2, widths=WIDTHS, chunks=CHUNKS, pulls=PULLS) -> dict:
     """Time every (width, chunk) pair on `roots` (distinct vertices of
-    positive degree, at least 128 * (reps + 1)); set the best on the engine
-    and return {"width", "chunk", "s_per_root", "table"}."""
+    positive degree, at least 128 * (reps + 1)), then every pull ratio at
+    the best pair; set the best on the engine and return {"width", "chunk",
+    "pull", "s_per_root", "table"} (table: ms per root)."""
     roots = np.ascontiguousarray(roots, np.uint32)
     need = max(widths) * (reps + 1)
     if len(roots) < need:
@@ -44,11 +49,16 @@ def tune(e: Engine, roots: np.ndarray, reps: int = 2, widths=WIDTHS, chunks=CHUN
     table = {}
     for w in widths:
         for ch in chunks:
-            table[(w, ch)] = _trial(e, roots, w, ch, reps)
-    (w, ch), best = min(table.items(), key=lambda kv: kv[1])
+            table[(w, ch, 32)] = _trial(e, roots, w, ch, reps, 32)
+    (w, ch, _), _best = min(table.items(), key=lambda kv: kv[1])
+    for pl in pulls:
+        if (w, ch, pl) not in table:
+            table[(w, ch, pl)] = _trial(e, roots, w, ch, reps, pl)
+    (w, ch, pl), best = min(table.items(), key=lambda kv: kv[1])
     L = _ffi.hip()
     _check(L.axbc_set_batch_width(e.ctx, 128 if w == 128 else 64))
     _check(L.axbc_set_batch_chunk(e.ctx, ch))
+    _check(L.axbc_set_batch_pull(e.ctx, pl))
     e.set_bc(saved)
-    return {"width": w, "chunk": ch, "s_per_root": best,
-            "table": {f"{k[0]}x{k[1]}": round(v * 1e3, 4) for k, v in table.items()}}
+    return {"width": w, "chunk": ch, "pull": pl, "s_per_root": best,
+            "table": {f"{k[0]}x{k[1]}x{k[2]}": round(v * 1e3, 4) for k, v in table.items()}}

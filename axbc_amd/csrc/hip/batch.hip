@@ -35,6 +35,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../../include/axbc.h"
@@ -54,11 +55,13 @@ struct BCtl {
   uint32_t done;
   uint32_t empty_lev;
   uint32_t overflow;
-  uint32_t unused;
+  uint32_t bu;         // the current level runs bottom-up (decided by the previous close)
   uint32_t wg_done;
   uint32_t stop_round; // first round after which the stop test holds (nb if none)
   uint32_t pad;
   unsigned long long split_claim;  // (entry index << 40) | chunk base
+  unsigned long long fe_edges;     // out-edges of the level's entries (next level's push work)
+  unsigned long long cand_edges;   // in-edges of vertices not yet reached by every root
 };
 
 struct BatchState {
@@ -94,6 +97,7 @@ struct BatchState {
   bool have = false;
   int lev_guess = 12;
   uint32_t chunk = 256;  // edges per work item of split vertices (tunable)
+  uint32_t pull16 = 32;  // bottom-up when candidate in-edges <= pull16/16 x push edges (0: never)
   size_t bytes = 0;
 };
 
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_breset(uint32_t n, const uint32_t *_
 template <int NW>
 __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis, uint64_t *fr0, uint32_t *sig,
                          uint32_t *q, uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
-                         unsigned long long *cnt, uint32_t stamp, uint32_t *seen) {
+                         unsigned long long *cnt, uint32_t stamp, uint32_t *seen, unsigned long long in_edges) {
   constexpr int B = 64 * NW;
   const int s = threadIdx.x + blockIdx.x * blockDim.x;
   if (s < nb) {
@@ -304,6 +308,9 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     ctl->overflow = 0;
     ctl->wg_done = 0;
     ctl->split_claim = 0;
+    ctl->bu = 0;
+    ctl->fe_edges = 0;
+    ctl->cand_edges = in_edges;
   }
 }
 
@@ -355,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
                                                   uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
                                                   uint64_t item_cap, uint32_t qcap, int L, uint32_t mark_id,
                                                   unsigned long long *kb, uint32_t kChunk) {
-  if (ldg_acq(&ctl->done)) return;
+  if (ldg_acq(&ctl->done) || ctl->bu) return;
   const int lane = lane_id();
   unsigned long long bytes = 0;
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -434,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restri
                                                         const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
                                                         uint64_t item_cap, uint32_t qcap, uint32_t mark_id,
                                                         unsigned long long *kb, uint32_t kChunk) {
-  if (ldg_acq(&ctl->done)) return;
+  if (ldg_acq(&ctl->done) || ctl->bu) return;
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
   if (r.lo >= r.hi) return;
@@ -491,7 +498,7 @@ template <int NW>
 __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t ce, const uint32_t *__restrict__ rrow,
                                             const uint64_t *__restrict__ frp, const Words<NW> &nw,
                                             const uint32_t *__restrict__ sig, uint32_t acc[NW], uint32_t dag[NW],
-                                            uint32_t *contrib) {
+                                            uint32_t *contrib, uint64_t found[NW]) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
   for (uint32_t k0 = cb; k0 < ce; k0 += kWave) {
@@ -521,6 +528,7 @@ __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t c
         for (int j = 0; j < NW; ++j) {
           const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
           *contrib += (uint32_t)__popcll(mm);
+          found[j] |= mm;
           got[t][j] = bit(mm, lane) ? sig[(size_t)uu * B + lane + 64 * j] : 0u;
           dag[j] += bit(mm, lane) ? 1u : 0u;
         }
@@ -542,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
                                                    uint32_t sp_cap, uint64_t item_cap, int L, uint32_t qcap,
                                                    unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done)) return;
+  if (ldg_acq(&ctl->done) || ctl->bu) return;
   unsigned long long bytes = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
@@ -563,7 +571,8 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
 #pragma unroll
     for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
     uint32_t contrib = 0;
-    sigma_chunk<NW>(st, 0, d, rrow, frp, nw, sig, acc, dag, &contrib);
+    uint64_t found[NW] = {};
+    sigma_chunk<NW>(st, 0, d, rrow, frp, nw, sig, acc, dag, &contrib, found);
     uint32_t roots = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
@@ -591,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
                                                          const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
                                                          uint64_t item_cap, unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done)) return;
+  if (ldg_acq(&ctl->done) || ctl->bu) return;
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
   if (r.lo >= r.hi) return;
@@ -620,7 +629,8 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
     uint32_t contrib = 0;
     const uint32_t cb = c * kChunk, ce = min(d, cb + kChunk);
     if (cb >= ce) continue;
-    sigma_chunk<NW>(st, cb, ce, rrow, frp, nw, sig, acc, dag, &contrib);
+    uint64_t found[NW] = {};
+    sigma_chunk<NW>(st, cb, ce, rrow, frp, nw, sig, acc, dag, &contrib, found);
     uint32_t roots = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
@@ -640,6 +650,190 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
     if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
 }
 
+// Bottom-up level L (direction-optimising BFS, one pass for discovery and
+// path counts): every vertex w not yet reached by all roots pulls over its
+// in-edges, m = FR[u] & ~VIS[w]; the roots found, NX[w] = OR of m, reach w
+// at level L with sigma_s[w] = sum of sigma_s[u] over u with s in m.  The
+// same sums as push + k_bsigma, so the result does not depend on the
+// direction.  A wave takes 64 consecutive vertices, one at a time, and
+// appends the new ones with one atomic.  Vertices with more than kChunk
+// in-edges go to k_bpull_split.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bpull(uint32_t n, const uint32_t *__restrict__ rcol,
+                                                  const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
+                                                  uint64_t *nx, const uint64_t *__restrict__ frp, uint64_t *fr_clear,
+                                                  uint32_t *sig, uint32_t *q, const uint32_t *__restrict__ qoff,
+                                                  axbc::BCtl *ctl, unsigned long long *cnt, uint32_t *sp_e,
+                                                  uint64_t *sp_base, uint32_t sp_cap, uint64_t item_cap, int L,
+                                                  uint32_t qcap, uint64_t all0, uint64_t all1, unsigned long long *kb,
+                                                  uint32_t kChunk) {
+  constexpr int B = 64 * NW;
+  if (ldg_acq(&ctl->done) || !ctl->bu) return;
+  const uint64_t all[2] = {all0, all1};
+  // clear the dense frontier buffer of level L-2 (it receives level L)
+  if (L >= 2) {
+    const uint32_t cb = qoff[L - 2], ce = qoff[L - 1];
+    for (uint32_t i = cb + blockIdx.x * blockDim.x + threadIdx.x; i < ce; i += gridDim.x * blockDim.x) {
+      const uint32_t v = q[i];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) fr_clear[(size_t)v * NW + j] = 0;
+    }
+  }
+  unsigned long long bytes = 0;
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  uint32_t dag_tot[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
+  for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
+    const uint32_t v = vb + (uint32_t)lane;
+    uint32_t st = 0, d = 0;
+    Words<NW> open;  // roots that have not reached v
+#pragma unroll
+    for (int j = 0; j < NW; ++j) open.w[j] = 0;
+    if (v < n) {
+      st = rcol[v];
+      d = rcol[v + 1] - st;
+      const Words<NW> vw = ldw<NW>(vis, v);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) open.w[j] = all[j] & ~vw.w[j];
+    }
+    bool cand = d > 0 && open.any();
+    bytes += v < n ? 8 + 8 * NW : 0;
+    if (cand && d > kChunk) {
+      claim_split(ctl, v, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
+      cand = false;
+    }
+    unsigned long long todo = __ballot(cand);
+    unsigned long long fresh = 0;
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t w = vb + (uint32_t)i;
+      const uint32_t sw = __shfl(st, i), dw = __shfl(d, i);
+      const Words<NW> ow = shflw<NW>(open, i);
+      uint32_t acc[NW], dag[NW];
+      uint64_t found[NW] = {};
+#pragma unroll
+      for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
+      uint32_t contrib = 0;
+      sigma_chunk<NW>(sw, 0, dw, rrow, frp, ow, sig, acc, dag, &contrib, found);
+      uint32_t roots = 0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        roots += (uint32_t)__popcll(found[j]);
+        if (bit(found[j], lane)) sig[(size_t)w * B + lane + 64 * j] = acc[j];
+        dag_tot[j] += dag[j];
+      }
+      if (roots) {
+        if (lane < NW) nx[(size_t)w * NW + lane] = found[lane < NW ? lane : 0];
+        fresh |= 1ull << i;
+      }
+      // per edge: id 4 + frontier word 8NW; per term: sigma 4; per root
+      // reached: sigma write 4; NX words written
+      bytes += (unsigned long long)dw * (4 + 8 * NW) + 4ull * contrib + 4ull * roots + (roots ? 8 * NW : 0);
+    }
+    if (fresh) {
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(fresh));
+      b = __shfl(b, 0);
+      if (bit(fresh, lane)) {
+        const uint32_t slot = b + (uint32_t)__popcll(fresh & ((1ull << lane) - 1ull));
+        if (slot < qcap)
+          q[slot] = v;
+        else
+          atomicOr(&ctl->overflow, 1u);
+      }
+      bytes += 4ull * (unsigned long long)__popcll(fresh);
+    }
+  }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+}
+
+// Bottom-up chunks of the vertices with more than kChunk in-edges: counts
+// and roots found are added atomically; the chunk that first makes NX[w]
+// non-empty appends w.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bpull_split(const uint32_t *__restrict__ rcol,
+                                                        const uint32_t *__restrict__ rrow,
+                                                        const uint64_t *__restrict__ vis, uint64_t *nx,
+                                                        const uint64_t *__restrict__ frp, uint32_t *sig, uint32_t *mark,
+                                                        uint32_t *q, axbc::BCtl *ctl, unsigned long long *cnt,
+                                                        const uint32_t *__restrict__ sp_e,
+                                                        const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
+                                                        uint64_t item_cap, uint32_t qcap, uint32_t mark_id,
+                                                        uint64_t all0, uint64_t all1, unsigned long long *kb,
+                                                        uint32_t kChunk) {
+  constexpr int B = 64 * NW;
+  if (ldg_acq(&ctl->done) || !ctl->bu) return;
+  const uint64_t all[2] = {all0, all1};
+  const int lane = lane_id();
+  ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
+  if (r.lo >= r.hi) return;
+  unsigned long long bytes = 0;
+  uint32_t dag_tot[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
+  uint32_t w = 0, st = 0, d = 0;
+  uint64_t base = 0;
+  Words<NW> open;
+  bool fresh_entry = true;
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    if (walk_to(r, sp_base, item) || fresh_entry) {
+      w = sp_e[r.k];
+      st = rcol[w];
+      d = rcol[w + 1] - st;
+      base = sp_base[r.k];
+      const Words<NW> vw = ldw<NW>(vis, w);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) open.w[j] = all[j] & ~vw.w[j];
+      fresh_entry = false;
+    }
+    const uint32_t cb = (uint32_t)(item - base) * kChunk, ce = min(d, cb + kChunk);
+    if (cb >= ce) continue;
+    uint32_t acc[NW], dag[NW];
+    uint64_t found[NW] = {};
+#pragma unroll
+    for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
+    uint32_t contrib = 0;
+    sigma_chunk<NW>(st, cb, ce, rrow, frp, open, sig, acc, dag, &contrib, found);
+    bool any = false, first = false;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      if (acc[j]) atomicAdd(&sig[(size_t)w * B + lane + 64 * j], acc[j]);
+      dag_tot[j] += dag[j];
+      any |= found[j] != 0;
+    }
+    if (any) {
+      if (lane == 0) {
+        if (NW == 1) {
+          first = atomicOr((unsigned long long *)&nx[w], (unsigned long long)found[0]) == 0ull;
+        } else {
+#pragma unroll
+          for (int j = 0; j < NW; ++j)
+            if (found[j]) atomicOr((unsigned long long *)&nx[(size_t)w * NW + j], (unsigned long long)found[j]);
+          if (mark[w] != mark_id) first = atomicExch(&mark[w], mark_id) != mark_id;
+        }
+        if (first) {
+          const uint32_t slot = atomicAdd(&ctl->tail, 1u);
+          if (slot < qcap)
+            q[slot] = w;
+          else
+            atomicOr(&ctl->overflow, 1u);
+        }
+      }
+    }
+    bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW) + 4ull * contrib;
+  }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+}
+
 // Close level L: frontier words, VIS, list words, counters, x rows.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
@@ -647,10 +841,12 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
                                                    double *xr, const uint32_t *__restrict__ q, uint64_t *qw,
                                                    uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t stamp, uint32_t *seen, int L,
-                                                   uint32_t qcap, unsigned long long *kb) {
+                                                   uint32_t qcap, const uint32_t *__restrict__ rcol, uint64_t all0,
+                                                   uint64_t all1, uint32_t pull16, unsigned long long *kb) {
   constexpr int B = 64 * NW;
   if (ldg_acq(&ctl->done)) return;
-  unsigned long long bytes = 0;
+  const uint64_t all[2] = {all0, all1};
+  unsigned long long bytes = 0, fe = 0, full = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -661,8 +857,19 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   for (uint32_t i = nb + gw; i < ne; i += nwv) {
     const uint32_t w = q[i];
     const Words<NW> nw = ldw<NW>(nx, w);
+    const Words<NW> vo = ldw<NW>(vis, w);
     const uint32_t rw = reach[w];
     const uint32_t dw = col[w + 1] - col[w];
+    // direction bookkeeping: out-edges of the level, and in-edges of the
+    // vertices every root has now reached
+    bool was_full = true, is_full = true;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      was_full &= (vo.w[j] & all[j]) == all[j];
+      is_full &= ((vo.w[j] | nw.w[j]) & all[j]) == all[j];
+    }
+    fe += dw;
+    if (is_full && !was_full) full += rcol[w + 1] - rcol[w];
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
       if (bit(nw.w[j], lane)) {
@@ -690,6 +897,8 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
     bytes += 20 + 48 * NW + 12ull * roots;
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  if (lane == 0 && fe) atomicAdd(&ctl->fe_edges, fe);
+  if (lane == 0 && full) atomicAdd(&ctl->cand_edges, 0ull - full);
 #pragma unroll
   for (int j = 0; j < NW; ++j) {
     const int s = lane + 64 * j;
@@ -711,6 +920,12 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
       ctl->empty_lev = (uint32_t)L;
       __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // next level bottom-up when the vertices still open to some root have
+    // at most pull16/16 times the in-edges that a push would scan
+    const unsigned long long fe_all = __hip_atomic_load(&ctl->fe_edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long cand = __hip_atomic_load(&ctl->cand_edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe_all ? 1u : 0u;
+    ctl->fe_edges = 0;
     ctl->split_claim = 0;
     ctl->wg_done = 0;
   }
@@ -1119,6 +1334,11 @@ int ensure_transpose(axbc_ctx *c) {
   std::vector<uint32_t> fill(rcol.begin(), rcol.end() - 1);
   for (uint32_t u = 0; u < c->n; ++u)
     for (uint32_t k = col[u]; k < col[u + 1]; ++k) rrow[fill[row[k]]++] = u;
+  c->max_out = c->max_in = 0;
+  for (uint32_t v = 0; v < c->n; ++v) {
+    c->max_out = std::max(c->max_out, col[v + 1] - col[v]);
+    c->max_in = std::max(c->max_in, rcol[v + 1] - rcol[v]);
+  }
   if (rcol == col && rrow == row) {
     c->symmetric = true;
     c->rcol = c->col;
@@ -1151,6 +1371,7 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   b->sp_cap = nsplit + 1;
   b->part_rows = c->ned / kChunkMin + nsplit + 1;
   b->chunk = c->batch_chunk;
+  b->pull16 = c->batch_pull16;
   b->qcap = qcap;
 #define BA(p, cnt)                         \
   if ((rc = balloc(b, &(p), (cnt))) != 0) { \
@@ -1221,7 +1442,12 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   b->stamp += 1;
   HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)(b->lev_guess + 2) * NW, s));
   hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
-                     b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen);
+                     b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, (unsigned long long)c->ned);
+  // roots of this batch, per word (the "reached by every root" test)
+  const uint64_t all0 = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
+  const uint64_t all1 = NW == 1 || nb <= 64 ? 0ull : (nb >= 128 ? ~0ull : (1ull << (nb - 64)) - 1ull);
+  // split kernels only where some vertex has more edges than a chunk
+  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
   hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
   HIP_TRY(hipGetLastError());
   // forward levels, speculatively in chunks
@@ -1242,26 +1468,44 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
                          b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap, lv,
                          mark_id, kt_bytes(c, KK_PUSH), b->chunk);
       kt_end(c, KK_PUSH);
-      kt_begin(c, KK_PUSH_SPLIT);
-      hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
-                         b->mark, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
-                         mark_id, kt_bytes(c, KK_PUSH_SPLIT), b->chunk);
-      kt_end(c, KK_PUSH_SPLIT);
-      hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+      if (split_out) {
+        kt_begin(c, KK_PUSH_SPLIT);
+        hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
+                           b->mark, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                           mark_id, kt_bytes(c, KK_PUSH_SPLIT), b->chunk);
+        kt_end(c, KK_PUSH_SPLIT);
+        hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+      }
       kt_begin(c, KK_SIGMA);
       hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp, b->sig, b->q,
                          b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, lv, b->qcap,
                          kt_bytes(c, KK_SIGMA), b->chunk);
       kt_end(c, KK_SIGMA);
-      kt_begin(c, KK_SIGMA_SPLIT);
-      hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
-                         b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                         kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
-      kt_end(c, KK_SIGMA_SPLIT);
+      if (split_in) {
+        kt_begin(c, KK_SIGMA_SPLIT);
+        hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
+                           b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                           kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
+        kt_end(c, KK_SIGMA_SPLIT);
+      }
+      if (b->pull16) {
+        kt_begin(c, KK_PULL);
+        hipLaunchKernelGGL(k_bpull<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, c->rcol, c->rrow, b->vis, b->nx, frp,
+                           frc, b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                           lv, b->qcap, all0, all1, kt_bytes(c, KK_PULL), b->chunk);
+        kt_end(c, KK_PULL);
+        if (split_in) {
+          kt_begin(c, KK_PULL_SPLIT);
+          hipLaunchKernelGGL(k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->vis, b->nx,
+                             frp, b->sig, b->mark, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap,
+                             b->part_rows, b->qcap, mark_id, all0, all1, kt_bytes(c, KK_PULL_SPLIT), b->chunk);
+          kt_end(c, KK_PULL_SPLIT);
+        }
+      }
       kt_begin(c, KK_CLOSE);
       hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, frc,
                          b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap,
-                         kt_bytes(c, KK_CLOSE));
+                         c->rcol, all0, all1, b->pull16, kt_bytes(c, KK_CLOSE));
       kt_end(c, KK_CLOSE);
     }
     HIP_TRY(hipGetLastError());
@@ -1298,6 +1542,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
                        c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, d,
                        rootmask[0], NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP), b->chunk);
     kt_end(c, KK_DEP);
+    if (!split_out) continue;
     kt_begin(c, KK_DEP_SPLIT);
     hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->xr,
                        b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->part, d, rootmask[0],
@@ -1372,6 +1617,13 @@ int axbc_set_batch_chunk(axbc_ctx *c, uint32_t chunk) {
     return fail(AXBC_EINVAL, "chunk must be in [64, 2^20]%s (line %d)", "", __LINE__);
   c->batch_chunk = chunk;
   if (c->batch) c->batch->chunk = chunk;
+  return AXBC_OK;
+}
+
+int axbc_set_batch_pull(axbc_ctx *c, uint32_t ratio16) {
+  if (!c || ratio16 > (1u << 16)) return fail(AXBC_EINVAL, "pull ratio must be in [0, 65536]%s (line %d)", "", __LINE__);
+  c->batch_pull16 = ratio16;
+  if (c->batch) c->batch->pull16 = ratio16;
   return AXBC_OK;
 }
 

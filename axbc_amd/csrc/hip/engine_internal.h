@@ -71,7 +71,9 @@ enum KernelKind {
   KK_DEP_COMBINE = 11, // batched dependency chunk combine (k_bdep_combine)
   KK_FWD_HUBS = 12,    // plain forward, hub chunks (k_fwd_hubs)
   KK_BWD_HUBS = 13,    // plain dependency pull, hub chunks (k_bwd_hubs)
-  KK_COUNT = 14
+  KK_PULL = 14,        // batched bottom-up level (k_bpull)
+  KK_PULL_SPLIT = 15,  // batched bottom-up level, split vertices (k_bpull_split)
+  KK_COUNT = 16
 };
 
 struct KernelTimer {
@@ -135,6 +137,8 @@ struct axbc_ctx {
   void *mlp = nullptr;  // learned proposal buffers (mlp.hip)
   int batch_width = 64;
   uint32_t batch_chunk = 256;
+  uint32_t batch_pull16 = 32;        // bottom-up ratio x16 for batched levels (0: top-down only)
+  uint32_t max_out = 0, max_in = 0;  // largest out- and in-degree (set with the transpose)
   size_t budget = 0;  // 0: no limit
   axbc::KernelTimer ktimer;
 };

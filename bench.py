@@ -33,8 +33,8 @@ WORKLOAD = {
 }
 KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_expand",
               6: "k_bwd_pull", 7: "k_copy", 8: "k_bpush_split", 9: "k_bsigma_split", 10: "k_bdep_split",
-              11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs"}
-BATCH_KINDS = (0, 8, 1, 9, 2, 3, 10, 11, 4)
+              11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split"}
+BATCH_KINDS = (0, 8, 1, 9, 14, 15, 2, 3, 10, 11, 4)
 PLAIN_KINDS = (5, 12, 6, 13)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
@@ -130,7 +130,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24, help="timed batches")
     ap.add_argument("--warmup", type=int, default=2, help="untimed batches")
-    ap.add_argument("--batch", type=int, default=64, choices=[64, 128])
+    ap.add_argument("--batch", type=int, default=0, choices=[0, 32, 64, 128],
+                    help="roots per batch; 0 = chosen by the autotuner (with the split and pull settings)")
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--plain-rounds", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -156,8 +157,22 @@ def main():
     e = Engine(g)
     log(f"device bytes {e.device_bytes >> 20} MB")
 
+    # ---- autotuning (untimed): batch width, split threshold, pull ratio ----
+    tuned = None
+    batch = args.batch
+    if not batch:
+        from axbc_amd.autotune import tune
+        rng = np.random.default_rng(0)
+        sample = rng.choice(np.flatnonzero(np.diff(g.col) > 0), 128 * 3, replace=False)
+        t = time.perf_counter()
+        tuned = tune(e, sample, reps=1)
+        batch = tuned["width"]
+        log(f"autotune {time.perf_counter() - t:.1f}s: width {tuned['width']} chunk {tuned['chunk']} "
+            f"pull {tuned['pull']} ({tuned['s_per_root'] * 1e3:.3f} ms/root)")
+        e.reset_bc()
+
     # ---- batched pivot loop ---------------------------------------------
-    loop = PivotLoop(e, 4096, c=1, x=1024, z=1, batch=args.batch)
+    loop = PivotLoop(e, 4096, c=1, x=1024, z=1, batch=batch)
     for _ in range(args.warmup):
         loop.step()
     e.sync()
@@ -228,7 +243,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 sigma / f64 delta,bc",
         "data": "synthetic (Kronecker generator identical to the reference's Graph500 generator)",
-        "config": dict(WORKLOAD, batch_width=args.batch, pivots_per_step=args.batch),
+        "config": dict(WORKLOAD, batch_width=batch, pivots_per_step=batch,
+                       autotune={k: tuned[k] for k in ("width", "chunk", "pull")} if tuned else None),
         "gteps": round(gteps, 3),
         "plain_pivots_per_s": round(plain_s, 3),
         "plain_gteps": round(plain_gteps, 3),

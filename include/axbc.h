@@ -181,6 +181,12 @@ int axbc_set_batch_width(axbc_ctx *ctx, int width);
 /* Edges per work item for vertices split across waves (64 .. 2^20;
  * default 256): the degree bucket boundary the autotuner sets. */
 int axbc_set_batch_chunk(axbc_ctx *ctx, uint32_t chunk);
+/* Direction of the batched levels: a level runs bottom-up (every vertex
+ * not yet reached by all roots pulls over its in-edges) when those
+ * vertices' in-edges are at most ratio16/16 times the out-edges the
+ * top-down push would scan; 0 keeps every level top-down.  Default 32.
+ * Results do not depend on it. */
+int axbc_set_batch_pull(axbc_ctx *ctx, uint32_t ratio16);
 int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);
@@ -196,7 +202,8 @@ int axbc_batch_delta(axbc_ctx *ctx, int k, double *delta);
  * 7 calibration copy, 8 batched push of split vertices, 9 batched
  * path-count pull of split vertices, 10 batched dependency pull of split
  * vertices, 11 their chunk combine, 12 plain forward hub chunks, 13 plain
- * dependency pull hub chunks.  axbc_kernel_stats returns the summed milliseconds,
+ * dependency pull hub chunks, 14 batched bottom-up level, 15 its split
+ * vertices.  axbc_kernel_stats returns the summed milliseconds,
  * the launch count and the byte count of one kind since enabling.
  * axbc_copy_kernel runs a plain device copy of `bytes` bytes (kind 7) to
  * calibrate hardware counters against a known traffic.

@@ -31,12 +31,14 @@ def delta_of(e, k):
     return out
 
 
-def check_batch(g, roots, width, label, batches=1):
-    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)}", flush=True)
+def check_batch(g, roots, width, label, batches=1, pull=None):
+    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)} pull={pull}", flush=True)
     bc_ref = np.zeros(g.N)
     probs = np.full(len(roots), 1.0 / g.N)
     with Engine(g) as e:
         assert _ffi.hip().axbc_set_batch_width(e.ctx, width) == 0
+        if pull is not None:
+            assert _ffi.hip().axbc_set_batch_pull(e.ctx, pull) == 0
         step = (len(roots) + batches - 1) // batches
         for b0 in range(0, len(roots), step):
             rb = roots[b0:b0 + step]
@@ -56,29 +58,44 @@ def test_require(require_gpu):
     pass
 
 
+# pull ratio (x16): 0 = every level top-down, 32 = the default switch,
+# 65536 = bottom-up from level 2 on
+@pytest.mark.parametrize("pull", [0, 32, 65536])
 @pytest.mark.parametrize("width", [64, 128])
 @pytest.mark.parametrize("scale,seed,undirected,heur", [
     (10, (1, 2), True, 0),
     (11, (23, 24), True, 1),
     (10, (3, 4), False, 0),
 ])
-def test_kronecker_batches(require_gpu, width, scale, seed, undirected, heur):
+def test_kronecker_batches(require_gpu, width, scale, seed, undirected, heur, pull):
     g = Graph.kronecker(scale, 16, seed=seed, undirected=undirected, heuristic=heur)
     deg = g.degrees()
     rng = np.random.default_rng(scale + width)
     cand = np.flatnonzero(deg > 0)
     roots = rng.choice(cand, size=min(len(cand), width + width // 2), replace=False)
-    check_batch(g, roots, width, f"kron s{scale} H{heur} U{int(not undirected)}", batches=2)
+    check_batch(g, roots, width, f"kron s{scale} H{heur} U{int(not undirected)}", batches=2, pull=pull)
 
 
-def test_split_vertices_and_long_paths(require_gpu):
+@pytest.mark.parametrize("pull", [0, 65536])
+def test_split_vertices_and_long_paths(require_gpu, pull):
     # a star centre with 5000 leaves plus a long path: split chunks in push,
-    # sigma and dependency passes, and ~2000 levels
+    # sigma (or bottom-up) and dependency passes, and ~2000 levels
     n = 7000
     e = [(0, i) for i in range(1, 5000)] + [(i, i + 1) for i in range(4999, n - 1)]
     g = Graph.from_edges(n, np.array(e))
     roots = np.array([0, 1, 2, 4999, 6999, 5500, 3], np.uint32)
-    check_batch(g, roots, 64, "star+path")
+    check_batch(g, roots, 64, "star+path", pull=pull)
+
+
+@pytest.mark.parametrize("width", [64, 128])
+def test_bottom_up_directed_hubs(require_gpu, width):
+    # directed Kronecker graph with in-hubs above the chunk size, every level
+    # after the first bottom-up: split pull chunks append through NX / mark
+    g = Graph.kronecker(12, 16, seed=(5, 6), undirected=False, heuristic=0)
+    deg = g.degrees()
+    rng = np.random.default_rng(7)
+    roots = rng.choice(np.flatnonzero(deg > 0), size=width, replace=False)
+    check_batch(g, roots, width, "kron s12 directed bottom-up", pull=65536)
 
 
 def test_partial_batch_and_single_level_roots(require_gpu):

@@ -59,3 +59,27 @@ def test_lcc_matches_oracle(require_gpu):
         total = L.oracle_fixed_weights(g.N, lcc_ref.astype(np.float64), W)
         np.testing.assert_array_equal(lw, W)
         assert t.value == total
+
+
+def test_autotuned_loop_matches_oracle(require_gpu):
+    """The autotuner leaves bc untouched, and the loop it configures (width,
+    split threshold, pull ratio) still gives the oracle's loop."""
+    from axbc_amd.autotune import tune
+    g = Graph.kronecker(11, 16, seed=(3, 4), heuristic=1)
+    nbfs = 200
+    pv, pr, bc_ref, rs = oracle.run(g, nbfs, c=1, x=4, z=1)
+    with Engine(g) as e:
+        e.pivot(int(np.argmax(g.degrees())), 0.5)
+        before = e.bc()
+        rng = np.random.default_rng(5)
+        sample = rng.choice(np.flatnonzero(g.degrees() > 0), 384, replace=False)
+        t = tune(e, sample, reps=1, widths=(32, 64, 128), chunks=(64, 256), pulls=(0, 32, 65536))
+        print(f"[autotune] {t['width']} {t['chunk']} {t['pull']}", flush=True)
+        assert t["width"] in (32, 64, 128) and t["chunk"] in (64, 256) and t["pull"] in (0, 32, 65536)
+        np.testing.assert_array_equal(e.bc(), before)
+        e.reset_bc()
+        res = PivotLoop(e, nbfs, c=1, x=4, z=1, batch=t["width"]).run()
+        assert res.pivots == [int(v) for v in pv]
+        assert res.nrounds == rs["nrounds"] and res.stopped == rs["stopped"]
+        ok, err = close(e.bc(), bc_ref)
+        assert ok, err
