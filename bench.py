@@ -85,7 +85,7 @@ def measured_traffic(kernel):
     k = t.get("kernels", {}).get(kernel)
     if not k or t.get("workload") != WORKLOAD["workload"]:
         return None
-    return int((k["fetch_kib"] + k["write_kib"]) * 1024 * t["copy_factor"])
+    return int(k["hbm_bytes_per_launch"])
 
 
 def copy_calibration(mib):
