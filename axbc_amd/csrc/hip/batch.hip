@@ -49,6 +49,7 @@ namespace axbc {
 constexpr uint32_t kChunkMin = 64;  // smallest edges-per-work-item for split vertices
 constexpr int kBurst = 8;           // row loads in flight per wave in the pull loops
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
+constexpr int kGraphSteps = 8;    // levels (depths) per captured graph
 
 struct BCtl {
   uint32_t tail;       // list length
@@ -62,6 +63,16 @@ struct BCtl {
   unsigned long long split_claim;  // (entry index << 40) | chunk base
   unsigned long long fe_edges;     // out-edges of the level's entries (next level's push work)
   unsigned long long cand_edges;   // in-edges of vertices not yet reached by every root
+  // device-resident loop state, so that every level (depth) runs the same
+  // launch sequence and a captured graph can replay it
+  int level;                       // forward level the next kernels run
+  uint32_t mark_base;              // level stamps of `mark`: mark_base + level
+  uint32_t stamp;                  // batch stamp of `seen`
+  int depth;                       // backward depth the next kernels run
+  int dmin;                        // last backward depth (0 only for single-level roots)
+  uint32_t pad2;
+  unsigned long long all[2];       // the batch's roots per word
+  unsigned long long rootmask[2];  // roots whose deepest level is 1 (depth 0)
 };
 
 struct BatchState {
@@ -98,6 +109,8 @@ struct BatchState {
   int lev_guess = 12;
   uint32_t chunk = 256;  // edges per work item of split vertices (tunable)
   uint32_t pull16 = 32;  // bottom-up when candidate in-edges <= pull16/16 x push edges (0: never)
+  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // captured level / depth steps
+  uint64_t gkey = 0;
   size_t bytes = 0;
 };
 
@@ -105,8 +118,12 @@ struct BatchState {
 
 namespace {
 
-__device__ __forceinline__ uint32_t ldg_acq(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Flags set by an earlier kernel of the stream (the done flag of the level
+// loop): that kernel's completion already orders them, so a relaxed load
+// suffices.  An acquire here would invalidate the vector cache in every
+// wave, which costs tens of microseconds per launch at these grid sizes.
+__device__ __forceinline__ uint32_t ld_flag(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool bit(uint64_t w, int i) { return (w >> i) & 1ull; }
@@ -277,7 +294,8 @@ __global__ __launch_bounds__(kBlock) void k_breset(uint32_t n, const uint32_t *_
 template <int NW>
 __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis, uint64_t *fr0, uint32_t *sig,
                          uint32_t *q, uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
-                         unsigned long long *cnt, uint32_t stamp, uint32_t *seen, unsigned long long in_edges) {
+                         unsigned long long *cnt, uint32_t stamp, uint32_t *seen, unsigned long long in_edges,
+                         uint32_t mark_base, uint64_t all0, uint64_t all1) {
   constexpr int B = 64 * NW;
   const int s = threadIdx.x + blockIdx.x * blockDim.x;
   if (s < nb) {
@@ -311,6 +329,13 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     ctl->bu = 0;
     ctl->fe_edges = 0;
     ctl->cand_edges = in_edges;
+    ctl->level = 1;
+    ctl->mark_base = mark_base;
+    ctl->stamp = stamp;
+    ctl->all[0] = all0;
+    ctl->all[1] = all1;
+    ctl->depth = -1;
+    ctl->dmin = 0;
   }
 }
 
@@ -357,12 +382,15 @@ __device__ __forceinline__ bool push_edge(uint32_t w, const Words<NW> &f, const 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                   const uint64_t *__restrict__ vis, uint64_t *nx, uint32_t *mark,
-                                                  uint64_t *fr_clear, uint32_t *q, const uint64_t *__restrict__ qw,
-                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
-                                                  uint64_t item_cap, uint32_t qcap, int L, uint32_t mark_id,
-                                                  unsigned long long *kb, uint32_t kChunk) {
-  if (ldg_acq(&ctl->done) || ctl->bu) return;
+                                                  uint64_t *fr0, uint64_t *fr1, uint32_t *q,
+                                                  const uint64_t *__restrict__ qw, const uint32_t *__restrict__ qoff,
+                                                  axbc::BCtl *ctl, uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
+                                                  uint64_t item_cap, uint32_t qcap, unsigned long long *kb,
+                                                  uint32_t kChunk) {
+  if (ld_flag(&ctl->done) || ctl->bu) return;
+  const int L = ctl->level;
+  const uint32_t mark_id = ctl->mark_base + (uint32_t)L;
+  uint64_t *fr_clear = (L & 1) ? fr1 : fr0;
   const int lane = lane_id();
   unsigned long long bytes = 0;
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -439,9 +467,10 @@ __global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restri
                                                         uint32_t *mark, uint32_t *q, const uint64_t *__restrict__ qw,
                                                         axbc::BCtl *ctl, const uint32_t *__restrict__ sp_e,
                                                         const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
-                                                        uint64_t item_cap, uint32_t qcap, uint32_t mark_id,
-                                                        unsigned long long *kb, uint32_t kChunk) {
-  if (ldg_acq(&ctl->done) || ctl->bu) return;
+                                                        uint64_t item_cap, uint32_t qcap, unsigned long long *kb,
+                                                        uint32_t kChunk) {
+  if (ld_flag(&ctl->done) || ctl->bu) return;
+  const uint32_t mark_id = ctl->mark_base + (uint32_t)ctl->level;
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
   if (r.lo >= r.hi) return;
@@ -541,16 +570,94 @@ __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t c
   }
 }
 
+// Path counts of up to 64 vertices at once (one per lane: st = first pull
+// edge, d = pull degree, 0 to skip, mask = roots to look for).  Their edges
+// are dealt 64 per step in vertex order; for an edge u -> owner with
+// m = FR[u] & mask(owner) != 0, lanes = roots add sigma_s[u].  Set edges are
+// taken in lane order, kBurst row loads in flight, so every owner's sums
+// are complete when the owner changes: flush(owner lane, acc, found) then
+// receives them (found = the roots that had a predecessor).
+template <int NW, class Flush>
+__device__ __forceinline__ void group_sigma(uint32_t st, uint32_t d, const Words<NW> &mask,
+                                            const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ frp,
+                                            const uint32_t *__restrict__ sig, uint32_t dag[NW], uint32_t *contrib,
+                                            Flush flush) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  const uint32_t incl = wave_incl_scan(d);
+  const uint32_t total = __shfl(incl, kWave - 1);
+  int cur = -1;
+  uint32_t acc[NW];
+  uint64_t found[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) acc[j] = 0, found[j] = 0;
+  for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+    const uint32_t e = e0 + lane;
+    const int own = wave_owner(incl, e) & (kWave - 1);
+    const uint32_t ost = __shfl(st, own);
+    const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+    const uint32_t oex = own ? pin : 0u;
+    const Words<NW> om = shflw<NW>(mask, own);
+    uint32_t u = 0;
+    Words<NW> m;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) m.w[j] = 0;
+    if (e < total) {
+      u = rrow[ost + (e - oex)];
+      const Words<NW> f = ldw<NW>(frp, u);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) m.w[j] = f.w[j] & om.w[j];
+    }
+    unsigned long long todo = __ballot(m.any());
+    while (todo) {
+      uint32_t got[kBurst][NW];
+      uint64_t mmt[kBurst][NW];
+      int ot[kBurst];
+#pragma unroll
+      for (int t = 0; t < kBurst; ++t) {
+        const int i = todo ? __ffsll((long long)todo) - 1 : 0;
+        const bool live = todo != 0;
+        todo &= todo - 1;
+        const uint32_t uu = __shfl(u, i);
+        ot[t] = live ? __shfl(own, i) : -1;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
+          mmt[t][j] = mm;
+          *contrib += (uint32_t)__popcll(mm);
+          got[t][j] = bit(mm, lane) ? sig[(size_t)uu * B + lane + 64 * j] : 0u;
+          dag[j] += bit(mm, lane) ? 1u : 0u;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kBurst; ++t) {
+        if (ot[t] < 0) continue;
+        if (ot[t] != cur) {
+          if (cur >= 0) flush(cur, acc, found);
+          cur = ot[t];
+#pragma unroll
+          for (int j = 0; j < NW; ++j) acc[j] = 0, found[j] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc[j] += got[t][j], found[j] |= mmt[t][j];
+      }
+    }
+  }
+  if (cur >= 0) flush(cur, acc, found);
+}
+
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
-                                                   const uint64_t *__restrict__ nx, const uint64_t *__restrict__ frp,
-                                                   uint32_t *sig, const uint32_t *__restrict__ q,
+                                                   const uint64_t *__restrict__ nx, const uint64_t *fr0,
+                                                   const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t *sp_e, uint64_t *sp_base,
-                                                   uint32_t sp_cap, uint64_t item_cap, int L, uint32_t qcap,
+                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
                                                    unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done) || ctl->bu) return;
+  if (ld_flag(&ctl->done) || ctl->bu) return;
+  const int L = ctl->level;
+  const uint64_t *frp = (L & 1) ? fr0 : fr1;  // level L-1
   unsigned long long bytes = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
@@ -558,31 +665,44 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
   uint32_t dag_tot[NW];
 #pragma unroll
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
-  for (uint32_t i = nb + gw; i < ne; i += nwv) {
-    const uint32_t w = q[i];
-    const uint32_t st = rcol[w], d = rcol[w + 1] - st;
-    if (d > kChunk) {
-      if (lane == 0) claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
-      bytes += 12;
-      continue;
-    }
-    const Words<NW> nw = ldw<NW>(nx, w);
-    uint32_t acc[NW], dag[NW];
+  // a wave takes 64 new entries of the level at a time
+  for (uint32_t base = nb + gw * kWave; base < ne; base += nwv * kWave) {
+    const uint32_t i = base + lane;
+    uint32_t w = 0, st = 0, d = 0;
+    Words<NW> nw;
 #pragma unroll
-    for (int j = 0; j < NW; ++j) acc[j] = dag[j] = 0;
+    for (int j = 0; j < NW; ++j) nw.w[j] = 0;
+    bool split = false;
+    if (i < ne) {
+      w = q[i];
+      st = rcol[w];
+      d = rcol[w + 1] - st;
+      if (d > kChunk) {
+        claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
+        split = true;
+        d = 0;
+      } else {
+        nw = ldw<NW>(nx, w);
+      }
+    }
+    bytes += 12ull * (unsigned long long)__popcll(__ballot(split));
     uint32_t contrib = 0;
-    uint64_t found[NW] = {};
-    sigma_chunk<NW>(st, 0, d, rrow, frp, nw, sig, acc, dag, &contrib, found);
+    group_sigma<NW>(st, d, nw, rrow, frp, sig, dag_tot, &contrib,
+                    [&](int o, const uint32_t *acc, const uint64_t *found) {
+                      const uint32_t wo = __shfl(w, o);
+#pragma unroll
+                      for (int j = 0; j < NW; ++j)
+                        if (bit(found[j], lane)) sig[(size_t)wo * B + lane + 64 * j] = acc[j];
+                    });
+    // vertex: id 4 + offsets 8 + word 8NW + sigma write 4 per root; per
+    // edge: id 4 + frontier word 8NW; per term: sigma 4
     uint32_t roots = 0;
 #pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      roots += (uint32_t)__popcll(nw.w[j]);
-      if (bit(nw.w[j], lane)) sig[(size_t)w * B + lane + 64 * j] = acc[j];
-      dag_tot[j] += dag[j];
-    }
-    // vertex: id 4 + offsets 8 + word 8NW; per edge: id 4 + frontier word
-    // 8NW; per term: sigma 4; per root reached: sigma write 4
-    bytes += 12 + 8 * NW + (unsigned long long)d * (4 + 8 * NW) + 4ull * contrib + 4ull * roots;
+    for (int j = 0; j < NW; ++j) roots += d ? (uint32_t)__popcll(nw.w[j]) : 0u;
+    const uint32_t dsum = (uint32_t)wave_sum_u64(d), rsum = (uint32_t)wave_sum_u64(roots);
+    const uint32_t nv = (uint32_t)__popcll(__ballot(i < ne && d > 0));
+    bytes += (unsigned long long)nv * (12 + 8 * NW) + (unsigned long long)dsum * (4 + 8 * NW) + 4ull * contrib +
+             4ull * rsum;
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 #pragma unroll
@@ -593,14 +713,15 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restrict__ rcol,
                                                          const uint32_t *__restrict__ rrow,
-                                                         const uint64_t *__restrict__ nx,
-                                                         const uint64_t *__restrict__ frp, uint32_t *sig,
+                                                         const uint64_t *__restrict__ nx, const uint64_t *fr0,
+                                                         const uint64_t *fr1, uint32_t *sig,
                                                          const uint32_t *__restrict__ q, axbc::BCtl *ctl,
                                                          unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
                                                          const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
                                                          uint64_t item_cap, unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done) || ctl->bu) return;
+  if (ld_flag(&ctl->done) || ctl->bu) return;
+  const uint64_t *frp = (ctl->level & 1) ? fr0 : fr1;
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
   if (r.lo >= r.hi) return;
@@ -661,15 +782,17 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bpull(uint32_t n, const uint32_t *__restrict__ rcol,
                                                   const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
-                                                  uint64_t *nx, const uint64_t *__restrict__ frp, uint64_t *fr_clear,
-                                                  uint32_t *sig, uint32_t *q, const uint32_t *__restrict__ qoff,
-                                                  axbc::BCtl *ctl, unsigned long long *cnt, uint32_t *sp_e,
-                                                  uint64_t *sp_base, uint32_t sp_cap, uint64_t item_cap, int L,
-                                                  uint32_t qcap, uint64_t all0, uint64_t all1, unsigned long long *kb,
-                                                  uint32_t kChunk) {
+                                                  uint64_t *nx, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
+                                                  uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                  unsigned long long *cnt, uint32_t *sp_e, uint64_t *sp_base,
+                                                  uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
+                                                  unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done) || !ctl->bu) return;
-  const uint64_t all[2] = {all0, all1};
+  if (ld_flag(&ctl->done) || !ctl->bu) return;
+  const int L = ctl->level;
+  const uint64_t *frp = (L & 1) ? fr0 : fr1;  // level L-1
+  uint64_t *fr_clear = (L & 1) ? fr1 : fr0;   // receives level L
+  const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
   // clear the dense frontier buffer of level L-2 (it receives level L)
   if (L >= 2) {
     const uint32_t cb = qoff[L - 2], ce = qoff[L - 1];
@@ -760,16 +883,17 @@ template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bpull_split(const uint32_t *__restrict__ rcol,
                                                         const uint32_t *__restrict__ rrow,
                                                         const uint64_t *__restrict__ vis, uint64_t *nx,
-                                                        const uint64_t *__restrict__ frp, uint32_t *sig, uint32_t *mark,
-                                                        uint32_t *q, axbc::BCtl *ctl, unsigned long long *cnt,
-                                                        const uint32_t *__restrict__ sp_e,
+                                                        const uint64_t *fr0, const uint64_t *fr1, uint32_t *sig,
+                                                        uint32_t *mark, uint32_t *q, axbc::BCtl *ctl,
+                                                        unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
                                                         const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
-                                                        uint64_t item_cap, uint32_t qcap, uint32_t mark_id,
-                                                        uint64_t all0, uint64_t all1, unsigned long long *kb,
+                                                        uint64_t item_cap, uint32_t qcap, unsigned long long *kb,
                                                         uint32_t kChunk) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done) || !ctl->bu) return;
-  const uint64_t all[2] = {all0, all1};
+  if (ld_flag(&ctl->done) || !ctl->bu) return;
+  const uint64_t *frp = (ctl->level & 1) ? fr0 : fr1;
+  const uint32_t mark_id = ctl->mark_base + (uint32_t)ctl->level;
+  const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
   if (r.lo >= r.hi) return;
@@ -837,15 +961,18 @@ __global__ __launch_bounds__(kBlock) void k_bpull_split(const uint32_t *__restri
 // Close level L: frontier words, VIS, list words, counters, x rows.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
-                                                   uint64_t *nx, uint64_t *vis, uint64_t *frc, const uint32_t *sig,
-                                                   double *xr, const uint32_t *__restrict__ q, uint64_t *qw,
-                                                   uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
-                                                   unsigned long long *cnt, uint32_t stamp, uint32_t *seen, int L,
-                                                   uint32_t qcap, const uint32_t *__restrict__ rcol, uint64_t all0,
-                                                   uint64_t all1, uint32_t pull16, unsigned long long *kb) {
+                                                   uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
+                                                   const uint32_t *sig, double *xr, const uint32_t *__restrict__ q,
+                                                   uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
+                                                   unsigned long long *cnt, uint32_t *seen, uint32_t qcap,
+                                                   const uint32_t *__restrict__ rcol, uint32_t pull16,
+                                                   unsigned long long *kb) {
   constexpr int B = 64 * NW;
-  if (ldg_acq(&ctl->done)) return;
-  const uint64_t all[2] = {all0, all1};
+  if (ld_flag(&ctl->done)) return;
+  const int L = ctl->level;
+  const uint32_t stamp = ctl->stamp;
+  uint64_t *frc = (L & 1) ? fr1 : fr0;
+  const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
   unsigned long long bytes = 0, fe = 0, full = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
@@ -854,47 +981,82 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   uint64_t a_or[NW];
 #pragma unroll
   for (int j = 0; j < NW; ++j) c_vis[j] = c_all[j] = c_er[j] = a_or[j] = 0;
-  for (uint32_t i = nb + gw; i < ne; i += nwv) {
-    const uint32_t w = q[i];
-    const Words<NW> nw = ldw<NW>(nx, w);
-    const Words<NW> vo = ldw<NW>(vis, w);
-    const uint32_t rw = reach[w];
-    const uint32_t dw = col[w + 1] - col[w];
-    // direction bookkeeping: out-edges of the level, and in-edges of the
-    // vertices every root has now reached
-    bool was_full = true, is_full = true;
+  // a wave takes 64 entries at a time: per-entry words with lane = entry,
+  // then the per-root rows entry by entry with lane = root
+  for (uint32_t base = nb + gw * kWave; base < ne; base += nwv * kWave) {
+    const uint32_t i = base + lane;
+    const bool valid = i < ne;
+    uint32_t w = 0, rw = 0, dw = 0;
+    Words<NW> nw;
 #pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      was_full &= (vo.w[j] & all[j]) == all[j];
-      is_full &= ((vo.w[j] | nw.w[j]) & all[j]) == all[j];
-    }
-    fe += dw;
-    if (is_full && !was_full) full += rcol[w + 1] - rcol[w];
+    for (int j = 0; j < NW; ++j) nw.w[j] = 0;
+    if (valid) {
+      w = q[i];
+      nw = ldw<NW>(nx, w);
+      const Words<NW> vo = ldw<NW>(vis, w);
+      rw = reach[w];
+      dw = col[w + 1] - col[w];
+      // direction bookkeeping: out-edges of the level, and in-edges of the
+      // vertices every root has now reached
+      bool was_full = true, is_full = true;
 #pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      if (bit(nw.w[j], lane)) {
-        const uint32_t s = sig[(size_t)w * B + lane + 64 * j];
-        xr[(size_t)w * B + lane + 64 * j] = ((double)rw + 1.0) / (double)s;
-        c_vis[j] += 1;
-        c_all[j] += rw;
-        c_er[j] += dw;
+      for (int j = 0; j < NW; ++j) {
+        was_full &= (vo.w[j] & all[j]) == all[j];
+        is_full &= ((vo.w[j] | nw.w[j]) & all[j]) == all[j];
       }
-      a_or[j] |= nw.w[j];
+      fe += dw;
+      if (is_full && !was_full) full += rcol[w + 1] - rcol[w];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        frc[(size_t)w * NW + j] = nw.w[j];
+        vis[(size_t)w * NW + j] = vo.w[j] | nw.w[j];
+        qw[(size_t)i * NW + j] = nw.w[j];
+        nx[(size_t)w * NW + j] = 0;
+        a_or[j] |= nw.w[j];
+      }
+      seen[w] = stamp;
     }
-    if (lane < NW) {
-      const uint64_t word = nw.w[lane < NW ? lane : 0];
-      frc[(size_t)w * NW + lane] = word;
-      vis[(size_t)w * NW + lane] |= word;
-      qw[(size_t)i * NW + lane] = word;
-      nx[(size_t)w * NW + lane] = 0;
+    const uint32_t cntv = (uint32_t)__popcll(__ballot(valid));
+    uint32_t roots = 0;
+    for (uint32_t k0 = 0; k0 < cntv; k0 += kBurst) {
+      uint32_t sg[kBurst][NW];
+#pragma unroll
+      for (int t = 0; t < kBurst; ++t) {
+        const int k = (int)(k0 + t) & (kWave - 1);
+        const uint32_t wk = __shfl(w, k);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const uint64_t mk = k0 + t < cntv ? __shfl(nw.w[j], k) : 0ull;
+          sg[t][j] = bit(mk, lane) ? sig[(size_t)wk * B + lane + 64 * j] : 1u;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kBurst; ++t) {
+        const int k = (int)(k0 + t) & (kWave - 1);
+        const uint32_t wk = __shfl(w, k), rk = __shfl(rw, k), dk = __shfl(dw, k);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const uint64_t mk = k0 + t < cntv ? __shfl(nw.w[j], k) : 0ull;
+          roots += (uint32_t)__popcll(mk);
+          if (bit(mk, lane)) {
+            xr[(size_t)wk * B + lane + 64 * j] = ((double)rk + 1.0) / (double)sg[t][j];
+            c_vis[j] += 1;
+            c_all[j] += rk;
+            c_er[j] += dk;
+          }
+        }
+      }
     }
-    if (lane == 0) seen[w] = stamp;
     // entry: id 4 + NX 8NW + reach 4 + offsets 8 + seen 4; writes FR, VIS
     // (read+write), list word, NX: 5 * 8NW; per root reached: sigma 4 + x 8
-    uint32_t roots = 0;
+    bytes += (unsigned long long)cntv * (20 + 48 * NW) + 12ull * roots;
+  }
+  fe = wave_sum_u64(fe);
+  full = wave_sum_u64(full);
 #pragma unroll
-    for (int j = 0; j < NW; ++j) roots += (uint32_t)__popcll(nw.w[j]);
-    bytes += 20 + 48 * NW + 12ull * roots;
+  for (int j = 0; j < NW; ++j) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) a_or[j] |= __shfl_xor(a_or[j], off);
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
   if (lane == 0 && fe) atomicAdd(&ctl->fe_edges, fe);
@@ -928,6 +1090,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
     ctl->fe_edges = 0;
     ctl->split_claim = 0;
     ctl->wg_done = 0;
+    if (!ctl->done) ctl->level = L + 1;
   }
 }
 
@@ -944,7 +1107,10 @@ __global__ void k_broot_deg(const uint32_t *__restrict__ col, const uint32_t *__
 
 template <int NW>
 __global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
-                         const uint32_t *__restrict__ qoff, int lev) {
+                         const uint32_t *__restrict__ qoff, const axbc::BCtl *ctl) {
+  const int depth = ctl->depth;
+  if (depth < ctl->dmin) return;
+  const int lev = depth + 1;
   const uint32_t b = qoff[lev], e = qoff[lev + 1];
   for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
     const uint32_t v = q[i];
@@ -1010,9 +1176,11 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
                                                  const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
-                                                 int depth, uint64_t root_mask0, uint64_t root_mask1,
                                                  unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
+  const int depth = ctl->depth;
+  if (depth < ctl->dmin) return;
+  const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
   const uint32_t b = qoff[depth], e = qoff[depth + 1];
   unsigned long long bytes = 0;
   const int lane = lane_id();
@@ -1064,10 +1232,12 @@ __global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restric
                                                        const uint64_t *__restrict__ qw, const axbc::BCtl *ctl,
                                                        const uint32_t *__restrict__ sp_e,
                                                        const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
-                                                       uint64_t part_rows, double *part, int depth,
-                                                       uint64_t root_mask0, uint64_t root_mask1,
-                                                       unsigned long long *kb, uint32_t kChunk) {
+                                                       uint64_t part_rows, double *part, unsigned long long *kb,
+                                                       uint32_t kChunk) {
   constexpr int B = 64 * NW;
+  const int depth = ctl->depth;
+  if (depth < ctl->dmin) return;
+  const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, part_rows);
   if (r.lo >= r.hi) return;
@@ -1116,10 +1286,12 @@ __global__ __launch_bounds__(kBlock) void k_bdep_combine(const uint32_t *__restr
                                                          const uint64_t *__restrict__ qw, const axbc::BCtl *ctl,
                                                          const uint32_t *__restrict__ sp_e,
                                                          const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
-                                                         const double *__restrict__ part, int depth,
-                                                         uint64_t root_mask0, uint64_t root_mask1,
-                                                         unsigned long long *kb, uint32_t kChunk) {
+                                                         const double *__restrict__ part, unsigned long long *kb,
+                                                         uint32_t kChunk) {
   constexpr int B = 64 * NW;
+  const int depth = ctl->depth;
+  if (depth < ctl->dmin) return;
+  const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
   const unsigned long long claim = ctl->split_claim;
   const uint32_t ns = min((uint32_t)(claim >> 40), sp_cap);
   const int lane = lane_id();
@@ -1157,6 +1329,25 @@ __global__ __launch_bounds__(kBlock) void k_bdep_combine(const uint32_t *__restr
 
 __global__ void k_bsplit_reset(axbc::BCtl *ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->split_claim = 0;
+  }
+}
+
+// End of a backward depth step: next depth.
+__global__ void k_bstep_down(axbc::BCtl *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->split_claim = 0;
+    if (ctl->depth >= ctl->dmin) ctl->depth -= 1;
+  }
+}
+
+// Backward sweep set-up: depths dmax-1 .. dmin.
+__global__ void k_bset_depth(axbc::BCtl *ctl, int first, int last, uint64_t rootmask0, uint64_t rootmask1) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->depth = first;
+    ctl->dmin = last;
+    ctl->rootmask[0] = rootmask0;
+    ctl->rootmask[1] = rootmask1;
     ctl->split_claim = 0;
   }
 }
@@ -1296,6 +1487,8 @@ namespace axbc {
 void batch_free(axbc_ctx *c) {
   BatchState *b = c->batch;
   if (!b) return;
+  if (b->gfwd) (void)hipGraphExecDestroy(b->gfwd);
+  if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
                b->check, b->ctl};
@@ -1423,14 +1616,140 @@ int pull_bctl(axbc_ctx *c) {
   return AXBC_OK;
 }
 
+// One forward level: the same launch sequence for every level (the kernels
+// read the level from the device), so it can be captured once and replayed.
+template <int NW>
+void enqueue_level(axbc_ctx *c, BatchState *b, bool split_out, bool split_in) {
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  kt_begin(c, KK_PUSH);
+  hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, b->fr[0],
+                     b->fr[1], b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                     kt_bytes(c, KK_PUSH), b->chunk);
+  kt_end(c, KK_PUSH);
+  if (split_out) {
+    kt_begin(c, KK_PUSH_SPLIT);
+    hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark,
+                       b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                       kt_bytes(c, KK_PUSH_SPLIT), b->chunk);
+    kt_end(c, KK_PUSH_SPLIT);
+    hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+  }
+  kt_begin(c, KK_SIGMA);
+  hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
+                     b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                     kt_bytes(c, KK_SIGMA), b->chunk);
+  kt_end(c, KK_SIGMA);
+  if (split_in) {
+    kt_begin(c, KK_SIGMA_SPLIT);
+    hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, b->fr[0],
+                       b->fr[1], b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                       kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
+    kt_end(c, KK_SIGMA_SPLIT);
+  }
+  if (b->pull16) {
+    kt_begin(c, KK_PULL);
+    hipLaunchKernelGGL(k_bpull<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, c->rcol, c->rrow, b->vis, b->nx, b->fr[0],
+                       b->fr[1], b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                       b->qcap, kt_bytes(c, KK_PULL), b->chunk);
+    kt_end(c, KK_PULL);
+    if (split_in) {
+      kt_begin(c, KK_PULL_SPLIT);
+      hipLaunchKernelGGL(k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->vis, b->nx,
+                         b->fr[0], b->fr[1], b->sig, b->mark, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap,
+                         b->part_rows, b->qcap, kt_bytes(c, KK_PULL_SPLIT), b->chunk);
+      kt_end(c, KK_PULL_SPLIT);
+    }
+  }
+  kt_begin(c, KK_CLOSE);
+  hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, b->fr[0],
+                     b->fr[1], b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->seen, b->qcap, c->rcol,
+                     b->pull16, kt_bytes(c, KK_CLOSE));
+  kt_end(c, KK_CLOSE);
+}
+
+// One backward depth step (the depth comes from the device; the last
+// kernel moves it down).  Before depth d the level d+1 bits leave VIS.
+template <int NW>
+void enqueue_depth(axbc_ctx *c, BatchState *b, bool split_out) {
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, b->vis, b->q, b->qw, b->qoff, b->ctl);
+  kt_begin(c, KK_DEP);
+  hipLaunchKernelGGL(k_bdep<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
+                     c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                     kt_bytes(c, KK_DEP), b->chunk);
+  kt_end(c, KK_DEP);
+  if (split_out) {
+    kt_begin(c, KK_DEP_SPLIT);
+    hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->xr,
+                       b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->part,
+                       kt_bytes(c, KK_DEP_SPLIT), b->chunk);
+    kt_end(c, KK_DEP_SPLIT);
+    kt_begin(c, KK_DEP_COMBINE);
+    hipLaunchKernelGGL(k_bdep_combine<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, b->sig, b->dlt, b->xr,
+                       c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part,
+                       kt_bytes(c, KK_DEP_COMBINE), b->chunk);
+    kt_end(c, KK_DEP_COMBINE);
+  }
+  hipLaunchKernelGGL(k_bstep_down, dim3(1), dim3(64), 0, s, b->ctl);
+}
+
+void graphs_free(BatchState *b) {
+  if (b->gfwd) (void)hipGraphExecDestroy(b->gfwd);
+  if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
+  b->gfwd = b->gbwd = nullptr;
+}
+
+// kGraphSteps levels (depths) per captured graph, re-captured when the
+// launch sequence changes (split kernels, pull ratio, chunk).
+template <int NW>
+int ensure_graphs(axbc_ctx *c, bool split_out, bool split_in) {
+  BatchState *b = c->batch;
+  const uint64_t key = ((uint64_t)b->chunk << 32) | ((uint64_t)b->pull16 << 2) | ((uint64_t)split_out << 1) |
+                       (uint64_t)split_in;
+  if (b->gfwd && b->gbwd && b->gkey == key) return AXBC_OK;
+  graphs_free(b);
+  hipStream_t s = c->stream;
+  for (int pass = 0; pass < 2; ++pass) {
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphSteps; ++k) {
+      if (pass == 0)
+        enqueue_level<NW>(c, b, split_out, split_in);
+      else
+        enqueue_depth<NW>(c, b, split_out);
+    }
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(s, &g);
+    if (le != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(AXBC_EHIP, "graph capture failed: %s (line %d)", hipGetErrorString(le != hipSuccess ? le : ee),
+                  __LINE__);
+    }
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail(AXBC_EHIP, "graph instantiation failed: %s (line %d)", hipGetErrorString(ie), __LINE__);
+    (pass == 0 ? b->gfwd : b->gbwd) = ge;
+  }
+  b->gkey = key;
+  return AXBC_OK;
+}
+
 template <int NW>
 int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
               axbc_pivot_stats *stats) {
   BatchState *b = c->batch;
   hipStream_t s = c->stream;
   const uint32_t n = c->n;
+  int rc;
   std::vector<uint32_t> r0(nb);
   for (int k = 0; k < nb; ++k) r0[k] = c->h_reach[v0s[k]];
+  // split kernels only where some vertex has more edges than a chunk
+  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
+  // the level loop runs from captured graphs unless per-kernel timing is on
+  const bool graphs = c->use_graphs && !c->ktimer.on;
+  if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
   HIP_TRY(hipMemcpyAsync(b->src, v0s, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(b->prob, ps, sizeof(double) * nb, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(b->r0, r0.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
@@ -1440,77 +1759,35 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
                        b->fr[0], b->fr[1], b->sig, b->dlt);
   b->have = true;
   b->stamp += 1;
-  HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)(b->lev_guess + 2) * NW, s));
-  hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
-                     b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, (unsigned long long)c->ned);
   // roots of this batch, per word (the "reached by every root" test)
   const uint64_t all0 = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
   const uint64_t all1 = NW == 1 || nb <= 64 ? 0ull : (nb >= 128 ? ~0ull : (1ull << (nb - 64)) - 1ull);
-  // split kernels only where some vertex has more edges than a chunk
-  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
+  int zeroed = std::min(b->lev_guess + 2, (int)n + 2);
+  HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)zeroed * NW, s));
+  hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
+                     b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, (unsigned long long)c->ned, b->epoch,
+                     all0, all1);
   hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
   HIP_TRY(hipGetLastError());
-  // forward levels, speculatively in chunks
+  // forward levels, speculatively in chunks; kernels past the last level
+  // exit on the device done flag
   int L = 1, chunk = b->lev_guess;
-  int zeroed = b->lev_guess + 2;
   for (;;) {
-    if (L + chunk + 2 > zeroed) {
-      const int upto = L + chunk + 2;
+    const int steps = graphs ? (chunk + kGraphSteps - 1) / kGraphSteps * kGraphSteps : chunk;
+    // per-level root words of the levels about to run (a BFS has at most
+    // n levels: the buffer holds n + 2 rows)
+    if (L + steps + 2 > zeroed && zeroed < (int)n + 2) {
+      const int upto = std::min(L + steps + 2, (int)n + 2);
       HIP_TRY(hipMemsetAsync(b->act + (size_t)zeroed * NW, 0, sizeof(uint64_t) * (size_t)(upto - zeroed) * NW, s));
       zeroed = upto;
     }
-    for (int j = 0; j < chunk; ++j) {
-      const int lv = L + j;
-      uint64_t *frp = b->fr[(lv - 1) & 1], *frc = b->fr[lv & 1];
-      const uint32_t mark_id = b->epoch + (uint32_t)lv;
-      kt_begin(c, KK_PUSH);
-      hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, frc,
-                         b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap, lv,
-                         mark_id, kt_bytes(c, KK_PUSH), b->chunk);
-      kt_end(c, KK_PUSH);
-      if (split_out) {
-        kt_begin(c, KK_PUSH_SPLIT);
-        hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx,
-                           b->mark, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
-                           mark_id, kt_bytes(c, KK_PUSH_SPLIT), b->chunk);
-        kt_end(c, KK_PUSH_SPLIT);
-        hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
-      }
-      kt_begin(c, KK_SIGMA);
-      hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp, b->sig, b->q,
-                         b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, lv, b->qcap,
-                         kt_bytes(c, KK_SIGMA), b->chunk);
-      kt_end(c, KK_SIGMA);
-      if (split_in) {
-        kt_begin(c, KK_SIGMA_SPLIT);
-        hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, frp,
-                           b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                           kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
-        kt_end(c, KK_SIGMA_SPLIT);
-      }
-      if (b->pull16) {
-        kt_begin(c, KK_PULL);
-        hipLaunchKernelGGL(k_bpull<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, c->rcol, c->rrow, b->vis, b->nx, frp,
-                           frc, b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                           lv, b->qcap, all0, all1, kt_bytes(c, KK_PULL), b->chunk);
-        kt_end(c, KK_PULL);
-        if (split_in) {
-          kt_begin(c, KK_PULL_SPLIT);
-          hipLaunchKernelGGL(k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->vis, b->nx,
-                             frp, b->sig, b->mark, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap,
-                             b->part_rows, b->qcap, mark_id, all0, all1, kt_bytes(c, KK_PULL_SPLIT), b->chunk);
-          kt_end(c, KK_PULL_SPLIT);
-        }
-      }
-      kt_begin(c, KK_CLOSE);
-      hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, frc,
-                         b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, lv, b->qcap,
-                         c->rcol, all0, all1, b->pull16, kt_bytes(c, KK_CLOSE));
-      kt_end(c, KK_CLOSE);
+    if (graphs) {
+      for (int j = 0; j < steps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gfwd, s));
+    } else {
+      for (int j = 0; j < steps; ++j) enqueue_level<NW>(c, b, split_out, split_in);
+      HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipGetLastError());
-    L += chunk;
-    int rc;
+    L += steps;
     if ((rc = pull_bctl(c))) return rc;
     if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
     if (b->h_ctl->done) break;
@@ -1532,28 +1809,18 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       if ((b->h_act[(size_t)lv * NW + k / 64] >> (k % 64)) & 1ull) nlev[k] = lv;
     if (nlev[k] == 1) rootmask[k / 64] |= 1ull << (k % 64);
   }
-  // backward: depth d = dmax-1 .. 0; before depth d the level d+1 bits
-  // leave VIS.  Depth 0 only for the roots with a single level.
-  for (int d = dmax - 1; d >= 0; --d) {
-    hipLaunchKernelGGL(k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, b->vis, b->q, b->qw, b->qoff, d + 1);
-    if (d == 0 && !(rootmask[0] | rootmask[1])) break;
-    kt_begin(c, KK_DEP);
-    hipLaunchKernelGGL(k_bdep<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
-                       c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, d,
-                       rootmask[0], NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP), b->chunk);
-    kt_end(c, KK_DEP);
-    if (!split_out) continue;
-    kt_begin(c, KK_DEP_SPLIT);
-    hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->xr,
-                       b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->part, d, rootmask[0],
-                       NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP_SPLIT), b->chunk);
-    kt_end(c, KK_DEP_SPLIT);
-    kt_begin(c, KK_DEP_COMBINE);
-    hipLaunchKernelGGL(k_bdep_combine<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, b->sig, b->dlt, b->xr,
-                       c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part, d, rootmask[0],
-                       NW > 1 ? rootmask[1] : 0ull, kt_bytes(c, KK_DEP_COMBINE), b->chunk);
-    kt_end(c, KK_DEP_COMBINE);
-    hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+  // backward: depths dmax-1 .. dmin, dmin = 0 only for the roots with a
+  // single level
+  const int dmin = (rootmask[0] | rootmask[1]) ? 0 : 1;
+  const int dsteps = dmax - dmin;
+  if (dsteps > 0) {
+    hipLaunchKernelGGL(k_bset_depth, dim3(1), dim3(64), 0, s, b->ctl, dmax - 1, dmin, rootmask[0],
+                       NW > 1 ? rootmask[1] : 0ull);
+    if (graphs) {
+      for (int j = 0; j < dsteps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gbwd, s));
+    } else {
+      for (int j = 0; j < dsteps; ++j) enqueue_depth<NW>(c, b, split_out);
+    }
   }
   HIP_TRY(hipGetLastError());
   // exact stop test per root, then the bc update for the applied roots
@@ -1620,6 +1887,12 @@ int axbc_set_batch_chunk(axbc_ctx *c, uint32_t chunk) {
   return AXBC_OK;
 }
 
+int axbc_set_batch_graphs(axbc_ctx *c, int enable) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  c->use_graphs = enable != 0;
+  return AXBC_OK;
+}
+
 int axbc_set_batch_pull(axbc_ctx *c, uint32_t ratio16) {
   if (!c || ratio16 > (1u << 16)) return fail(AXBC_EINVAL, "pull ratio must be in [0, 65536]%s (line %d)", "", __LINE__);
   c->batch_pull16 = ratio16;
@@ -1636,23 +1909,24 @@ int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint3
     if (v0s[k] >= c->n) return fail(AXBC_EINVAL, "root out of range%s (line %d)", "", __LINE__);
   HIP_TRY(hipSetDevice(c->device));
   int rc;
+  // list capacity: a vertex is listed once per distinct level among the
+  // roots, so at most width times; small-world graphs need a few times N,
+  // high-diameter ones (roots sharing no levels) up to width times N
+  const uint64_t bound = std::min<uint64_t>((uint64_t)c->n * (uint64_t)c->batch_width + 1024, 0xFFFFFFF0ull);
   if (!c->batch) {
-    // list capacity: every vertex once per distinct level among the roots;
-    // start at 4N and grow on overflow
-    uint64_t cap = (uint64_t)c->n * 4 + 1024;
-    if (cap > 0xFFFFFFF0ull) cap = 0xFFFFFFF0ull;
-    if ((rc = batch_create(c, c->batch_width, (uint32_t)cap))) return rc;
+    if ((rc = batch_create(c, c->batch_width, (uint32_t)std::min<uint64_t>((uint64_t)c->n * 8 + 1024, bound))))
+      return rc;
   }
-  for (int attempt = 0; attempt < 6; ++attempt) {
+  for (int attempt = 0; attempt < 8; ++attempt) {
     rc = c->batch->nw == 1 ? run_batch<1>(c, v0s, ps, nb, stop_c, applied, stats)
                            : run_batch<2>(c, v0s, ps, nb, stop_c, applied, stats);
     if (rc != AXBC_ENOMEM || !c->batch || !c->batch->h_ctl->overflow) return rc;
     // grow the level list and retry the batch from scratch (bc untouched:
     // the update runs only after a complete sweep)
-    const uint64_t cap = (uint64_t)c->batch->qcap * 2;
+    const uint64_t cap = std::min<uint64_t>((uint64_t)c->batch->qcap * 4, bound);
     const int w = c->batch->B;
     batch_free(c);
-    if ((rc = batch_create(c, w, (uint32_t)(cap > 0xFFFFFFF0ull ? 0xFFFFFFF0ull : cap)))) return rc;
+    if ((rc = batch_create(c, w, (uint32_t)cap))) return rc;
   }
   return fail(AXBC_ENOMEM, "batch list capacity%s (line %d)", "", __LINE__);
 }

@@ -107,8 +107,12 @@ int grid_for(uint64_t n, int cap) {
 
 namespace {
 
-__device__ __forceinline__ uint32_t ld_acq(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Flags set by an earlier kernel of the stream (the done flag of the level
+// loop): that kernel's completion already orders them, so a relaxed load
+// suffices.  An acquire here would invalidate the vector cache in every
+// wave, which costs tens of microseconds per launch at these grid sizes.
+__device__ __forceinline__ uint32_t ld_flag(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
                                                        const uint32_t *__restrict__ qoff,
                                                        DevCtl *ctl, uint32_t *hubs, uint64_t *hub_base,
                                                        uint32_t hub_cap, uint64_t item_cap, int L) {
-  if (ld_acq(&ctl->done)) return;
+  if (ld_flag(&ctl->done)) return;
   const uint32_t fb = qoff[L - 1], fe = qoff[L];
   const int lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
                                                      DevCtl *ctl, const uint32_t *__restrict__ hubs,
                                                      const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
                                                      uint64_t item_cap, int L) {
-  if (ld_acq(&ctl->done)) return;
+  if (ld_flag(&ctl->done)) return;
   const int lane = lane_id();
   unsigned long long dag = 0;
   HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);

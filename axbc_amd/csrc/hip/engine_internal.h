@@ -138,6 +138,7 @@ struct axbc_ctx {
   int batch_width = 64;
   uint32_t batch_chunk = 256;
   uint32_t batch_pull16 = 32;        // bottom-up ratio x16 for batched levels (0: top-down only)
+  bool use_graphs = true;            // batched level loop from captured HIP graphs
   uint32_t max_out = 0, max_in = 0;  // largest out- and in-degree (set with the transpose)
   size_t budget = 0;  // 0: no limit
   axbc::KernelTimer ktimer;

@@ -136,6 +136,8 @@ def main():
     ap.add_argument("--plain-rounds", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0, help="split threshold with --batch (default 256)")
+    ap.add_argument("--pull", type=int, default=-1, help="bottom-up ratio x16 with --batch (default 32)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the calibration copy kernel over MIB MiB (8 launches) and print its rate")
     args = ap.parse_args()
@@ -170,6 +172,13 @@ def main():
         log(f"autotune {time.perf_counter() - t:.1f}s: width {tuned['width']} chunk {tuned['chunk']} "
             f"pull {tuned['pull']} ({tuned['s_per_root'] * 1e3:.3f} ms/root)")
         e.reset_bc()
+    else:
+        from axbc_amd import _ffi
+        L = _ffi.hip()
+        if args.chunk and L.axbc_set_batch_chunk(e.ctx, args.chunk) != 0:
+            raise RuntimeError(L.axbc_last_error())
+        if args.pull >= 0 and L.axbc_set_batch_pull(e.ctx, args.pull) != 0:
+            raise RuntimeError(L.axbc_last_error())
 
     # ---- batched pivot loop ---------------------------------------------
     loop = PivotLoop(e, 4096, c=1, x=1024, z=1, batch=batch)

@@ -187,6 +187,12 @@ int axbc_set_batch_chunk(axbc_ctx *ctx, uint32_t chunk);
  * top-down push would scan; 0 keeps every level top-down.  Default 32.
  * Results do not depend on it. */
 int axbc_set_batch_pull(axbc_ctx *ctx, uint32_t ratio16);
+/* Level loop of the batched sweep: 1 (default) replays captured HIP graphs
+ * of 8 levels (depths) each, the kernels reading the level from device
+ * memory, so a level costs no host launches; 0 launches every kernel from
+ * the host.  Kernel timing (axbc_kernel_timing) always uses host launches.
+ * Results do not depend on it. */
+int axbc_set_batch_graphs(axbc_ctx *ctx, int enable);
 int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);

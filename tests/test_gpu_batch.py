@@ -127,3 +127,42 @@ def test_stop_inside_batch(require_gpu):
         assert applied == stop_k + 1
         ok, err = close(e.bc(), bc_ref)
         assert ok, err
+
+
+@pytest.mark.parametrize("width", [64, 128])
+def test_graph_replays_match_host_launches(require_gpu, width):
+    """The level loop replayed from captured graphs (the default) and the
+    host-launched loop give bit-identical bc and delta over several batches
+    (graph replays with eager work and syncs between them), on a
+    high-diameter graph (a long path plus a grid) and a Kronecker graph."""
+    n = 3000
+    e = [(i, i + 1) for i in range(n - 1)] + [(n + r * 30 + c, n + r * 30 + c + 1) for r in range(30) for c in range(29)]
+    e += [(n + r * 30 + c, n + (r + 1) * 30 + c) for r in range(29) for c in range(30)]
+    graphs = [Graph.from_edges(n + 900, np.array(e)), Graph.kronecker(11, 16, seed=(9, 10))]
+    for g in graphs:
+        deg = g.degrees()
+        rng = np.random.default_rng(width)
+        roots = rng.choice(np.flatnonzero(deg > 0), size=3 * width, replace=False).astype(np.uint32)
+        probs = np.full(width, 1.0 / g.N)
+        out = []
+        for use_graphs in (1, 0):
+            with Engine(g) as eng:
+                L = _ffi.hip()
+                assert L.axbc_set_batch_width(eng.ctx, width) == 0
+                assert L.axbc_set_batch_graphs(eng.ctx, use_graphs) == 0
+                deltas = []
+                for b0 in range(0, len(roots), width):
+                    run_batch(eng, roots[b0:b0 + width], probs)
+                    eng.sync()
+                    deltas.append(delta_of(eng, width - 1))
+                out.append((eng.bc(), deltas))
+        print(f"[graphs] N={g.N} width={width}", flush=True)
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        for a, b in zip(out[0][1], out[1][1]):
+            np.testing.assert_array_equal(a, b)
+        # and against the oracle
+        bc_ref = np.zeros(g.N)
+        for v0 in roots:
+            _, _, _, bc_ref, _ = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
+        ok, err = close(out[0][0], bc_ref)
+        assert ok, err
