@@ -155,19 +155,6 @@ __device__ __forceinline__ Words<NW> shflw(const Words<NW> &x, int src) {
   return r;
 }
 
-// Last workgroup of a kernel (counter reset by the caller of the result).
-__device__ __forceinline__ bool last_wg(uint32_t *counter) {
-  __shared__ bool is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    if (is_last) __threadfence();
-  }
-  __syncthreads();
-  return is_last;
-}
-
 // Split vertices.  An entry with more than `ch` edges is claimed with one
 // 64-bit atomic that returns both its slot k and the first of its
 // ceil(d/ch) chunk items (split_claim = entries << 40 | items), so sp_base[]
@@ -965,8 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
                                                    const uint32_t *sig, double *xr, const uint32_t *__restrict__ q,
                                                    uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t *seen, uint32_t qcap,
-                                                   const uint32_t *__restrict__ rcol, uint32_t pull16,
-                                                   unsigned long long *kb) {
+                                                   const uint32_t *__restrict__ rcol, unsigned long long *kb) {
   constexpr int B = 64 * NW;
   if (ld_flag(&ctl->done)) return;
   const int L = ctl->level;
@@ -1071,27 +1057,32 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
     }
     if (lane == 0 && a_or[j]) atomicOr((unsigned long long *)&act[(size_t)L * NW + j], (unsigned long long)a_or[j]);
   }
-  if (last_wg(&ctl->wg_done) && threadIdx.x == 0) {
-    uint32_t tail = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (tail > qcap) {  // list overflow: stop the sweep, the host retries larger
-      tail = qcap;
-      ctl->overflow = 1;
-    }
-    qoff[L + 1] = tail;
-    if (tail == qoff[L] || ctl->overflow) {
-      ctl->empty_lev = (uint32_t)L;
-      __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    // next level bottom-up when the vertices still open to some root have
-    // at most pull16/16 times the in-edges that a push would scan
-    const unsigned long long fe_all = __hip_atomic_load(&ctl->fe_edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long cand = __hip_atomic_load(&ctl->cand_edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe_all ? 1u : 0u;
-    ctl->fe_edges = 0;
-    ctl->split_claim = 0;
-    ctl->wg_done = 0;
-    if (!ctl->done) ctl->level = L + 1;
+}
+
+
+// End of level L, one thread, after k_bclose in stream order (so every
+// count of the level is complete without fences): level offsets, the done
+// flag, and the direction of the next level.
+__global__ void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->done) return;
+  const int L = ctl->level;
+  uint32_t tail = ctl->tail;
+  if (tail > qcap) {  // list overflow: stop the sweep, the host retries larger
+    tail = qcap;
+    ctl->overflow = 1;
   }
+  qoff[L + 1] = tail;
+  if (tail == qoff[L] || ctl->overflow) {
+    ctl->empty_lev = (uint32_t)L;
+    ctl->done = 1u;
+  }
+  // next level bottom-up when the vertices still open to some root have
+  // at most pull16/16 times the in-edges that a push would scan
+  const unsigned long long fe_all = ctl->fe_edges, cand = ctl->cand_edges;
+  ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe_all ? 1u : 0u;
+  ctl->fe_edges = 0;
+  ctl->split_claim = 0;
+  if (!ctl->done) ctl->level = L + 1;
 }
 
 // e_r of the roots (degree of the root itself).
@@ -1664,8 +1655,9 @@ void enqueue_level(axbc_ctx *c, BatchState *b, bool split_out, bool split_in) {
   kt_begin(c, KK_CLOSE);
   hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, b->fr[0],
                      b->fr[1], b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->seen, b->qcap, c->rcol,
-                     b->pull16, kt_bytes(c, KK_CLOSE));
+                     kt_bytes(c, KK_CLOSE));
   kt_end(c, KK_CLOSE);
+  hipLaunchKernelGGL(k_bclose_end, dim3(1), dim3(64), 0, s, b->ctl, b->qoff, b->qcap, b->pull16);
 }
 
 // One backward depth step (the depth comes from the device; the last

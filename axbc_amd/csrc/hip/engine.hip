@@ -305,20 +305,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
   }
 }
 
-// Last workgroup of a level's second kernel closes the level.
-__device__ __forceinline__ bool last_workgroup(DevCtl *ctl) {
-  __shared__ bool is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t prev = atomicAdd(&ctl->wg_done, 1u);
-    is_last = prev == gridDim.x - 1;
-    if (is_last) __threadfence();
-  }
-  __syncthreads();
-  return is_last;
-}
-
 __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict__ col,
                                                      const uint32_t *__restrict__ row, int *lvl,
                                                      uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
@@ -347,16 +333,24 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
   }
   dag = wave_sum_u64(dag);
   if (lane == 0 && dag) atomicAdd(&ctl->e_dag, dag);
-  if (last_workgroup(ctl) && threadIdx.x == 0) {
-    const uint32_t tail = __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    qoff[L + 1] = tail;
-    if (tail == qoff[L]) {
-      ctl->empty_lev = (uint32_t)L;
-      __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    ctl->hub_claim = 0;
-    ctl->wg_done = 0;
+}
+
+// End of forward level L, one thread, after the level's kernels in stream
+// order (their counts are complete without fences): level offset and done.
+__global__ void k_fwd_end(uint32_t *qoff, DevCtl *ctl, int L) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->done) return;
+  const uint32_t tail = ctl->tail;
+  qoff[L + 1] = tail;
+  if (tail == qoff[L]) {
+    ctl->empty_lev = (uint32_t)L;
+    ctl->done = 1u;
   }
+  ctl->hub_claim = 0;
+}
+
+// Clears the hub claims of a backward depth.
+__global__ void k_hub_reset(DevCtl *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->hub_claim = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -479,10 +473,6 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hub_combine(const uint32_t *__re
     for (uint32_t c = 0; c < nch; ++c) tot += p[c];
     delta[v] = (double)reach[v] + tot;
   }
-  if (last_workgroup(ctl) && threadIdx.x == 0) {
-    ctl->hub_claim = 0;
-    ctl->wg_done = 0;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -578,6 +568,7 @@ int enqueue_forward(axbc_ctx *c, int L) {
   hipLaunchKernelGGL(k_fwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
                      c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
   kt_end(c, KK_FWD_HUBS);
+  hipLaunchKernelGGL(k_fwd_end, dim3(1), dim3(64), 0, c->stream, c->qoff, c->ctl, L);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }
@@ -595,6 +586,7 @@ int enqueue_backward(axbc_ctx *c, int depth) {
   kt_end(c, KK_BWD_HUBS);
   hipLaunchKernelGGL(k_bwd_hub_combine, dim3(64), dim3(kBlock), 0, c->stream, c->col, c->delta, c->reach,
                      c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_part);
+  hipLaunchKernelGGL(k_hub_reset, dim3(1), dim3(64), 0, c->stream, c->ctl);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }

@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--plain-rounds", type=int, default=48)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every level kernel from the host (the timed batches always do, for the per-kernel "
+                         "timers; this also covers the autotuning trials)")
     ap.add_argument("--chunk", type=int, default=0, help="split threshold with --batch (default 256)")
     ap.add_argument("--pull", type=int, default=-1, help="bottom-up ratio x16 with --batch (default 32)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -158,6 +161,10 @@ def main():
     log(f"graph: N={g.N} ned={g.ned} built in {time.perf_counter() - t0:.1f}s")
     e = Engine(g)
     log(f"device bytes {e.device_bytes >> 20} MB")
+    if args.no_graphs:
+        from axbc_amd import _ffi
+        if _ffi.hip().axbc_set_batch_graphs(e.ctx, 0) != 0:
+            raise RuntimeError(_ffi.hip().axbc_last_error())
 
     # ---- autotuning (untimed): batch width, split threshold, pull ratio ----
     tuned = None
