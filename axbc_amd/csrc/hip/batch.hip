@@ -1460,6 +1460,28 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(uint32_t n, const uint32_t *
   }
 }
 
+// out[s] += sum over the target vertices of weight * delta_s(v): a wave per
+// vertex, lane = root (coalesced rows), one FP64 atomic per root and wave.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_butility(const uint32_t *__restrict__ vert, const double *__restrict__ wt,
+                                                     uint32_t nv, const double *__restrict__ dlt, double *out) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  double acc[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) acc[j] = 0.0;
+  for (uint32_t i = gw; i < nv; i += nwv) {
+    const uint32_t v = vert[i];
+    const double w = wt[i];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) acc[j] += w * dlt[(size_t)v * B + lane + 64 * j];
+  }
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    if (acc[j] != 0.0) atomicAdd(&out[lane + 64 * j], acc[j]);
+}
+
 template <int NW>
 __global__ void k_bcopy_delta(uint32_t n, int s, const double *__restrict__ dlt, double *out) {
   constexpr int B = 64 * NW;
@@ -1921,6 +1943,38 @@ int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint3
     if ((rc = batch_create(c, w, (uint32_t)cap))) return rc;
   }
   return fail(AXBC_ENOMEM, "batch list capacity%s (line %d)", "", __LINE__);
+}
+
+int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight, uint32_t count, double *out) {
+  if (!c || !c->batch || !vertex || !weight || !out) return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  for (uint32_t i = 0; i < count; ++i)
+    if (vertex[i] >= c->n) return fail(AXBC_EINVAL, "vertex out of range%s (line %d)", "", __LINE__);
+  BatchState *b = c->batch;
+  const int B = b->B;
+  uint32_t *dv = nullptr;
+  double *dw = nullptr, *dout = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc((void **)&dv, sizeof(uint32_t) * (count ? count : 1)));
+  HIP_TRY(hipMalloc((void **)&dw, sizeof(double) * (count ? count : 1)));
+  HIP_TRY(hipMalloc((void **)&dout, sizeof(double) * B));
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(dv, vertex, sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dw, weight, sizeof(double) * count, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(dout, 0, sizeof(double) * B, s));
+  if (count) {
+    const int grid = (int)std::min<uint32_t>(1024, (count + 3) / 4);
+    if (b->nw == 1)
+      hipLaunchKernelGGL(k_butility<1>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, dout);
+    else
+      hipLaunchKernelGGL(k_butility<2>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, dout);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  (void)hipFree(dv);
+  (void)hipFree(dw);
+  (void)hipFree(dout);
+  return AXBC_OK;
 }
 
 int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {

@@ -1,5 +1,5 @@
 // Learned pivot proposal: a 3-layer MLP over per-vertex features
-// (log degree, LCC, running bc, last delta) -> score, with BF16 weights,
+// (log degree, LCC, running bc, mean neighbour bc) -> score, with BF16 weights,
 // FP32 accumulation, hand-written forward, backward and Adam kernels.
 //
 // Network:  x (4) -> h1 = relu(W1 x + b1) (64) -> h2 = relu(W2 h1 + b2) (64)
@@ -8,7 +8,7 @@
 // of vertices drawn by the host; gradients summed per workgroup in LDS and
 // added to FP32 gradient buffers; Adam on FP32 master weights, BF16 copies
 // used by the forward passes.
-// Proposal: w(v) = softplus(s(v)) + eps * mean softplus, turned into the
+// Proposal: w(v) = expm1(s(v)) + eps * mean (degree > 0), turned into the
 // fixed-point distribution of the dynamic samplers (kind 7).
 //
 // No reference counterpart: it stands in for the bc/delta heuristics
@@ -53,15 +53,42 @@ MlpBufs *bufs_of(axbc_ctx *c) { return reinterpret_cast<MlpBufs *>(c->mlp); }
 
 __device__ __forceinline__ float bf(const __hip_bfloat16 *w, int i) { return __bfloat162float(w[i]); }
 
-__global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const float *__restrict__ lcc,
-                           const double *__restrict__ bc, const double *__restrict__ delta, double bc_scale,
-                           double delta_scale, float *x) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const float d = (float)(col[v + 1] - col[v]);
-    x[4 * v + 0] = log1pf(d);
-    x[4 * v + 1] = lcc ? lcc[v] : 0.0f;
-    x[4 * v + 2] = (float)log1p(bc[v] * bc_scale);
-    x[4 * v + 3] = (float)log1p(delta[v] * delta_scale);
+// Features of vertex v: log(1 + degree), LCC, log(1 + bc * scale) and
+// log(1 + mean neighbour bc * scale).  Thread per vertex; vertices with
+// more than 64 edges are summed by the whole wave.
+__global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                           const float *__restrict__ lcc, const double *__restrict__ bc, double bc_scale, float *x) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
+    const uint32_t v = base + (uint32_t)lane;
+    uint32_t b = 0, e = 0;
+    if (v < n) {
+      b = col[v];
+      e = col[v + 1];
+    }
+    const bool big = e - b > 64;
+    double nsum = 0.0;
+    if (!big)
+      for (uint32_t k = b; k < e; ++k) nsum += bc[row[k]];
+    unsigned long long todo = __ballot(big);
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t bb = __shfl(b, i), ee = __shfl(e, i);
+      double t = 0.0;
+      for (uint32_t k = bb + (uint32_t)lane; k < ee; k += 64) t += bc[row[k]];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+      if (lane == i) nsum = t;
+    }
+    if (v < n) {
+      const uint32_t d = e - b;
+      x[4 * v + 0] = log1pf((float)d);
+      x[4 * v + 1] = lcc ? lcc[v] : 0.0f;
+      x[4 * v + 2] = (float)log1p(bc[v] * bc_scale);
+      x[4 * v + 3] = d ? (float)log1p(nsum / (double)d * bc_scale) : 0.0f;
+    }
   }
 }
 
@@ -211,24 +238,38 @@ __global__ void k_to_bf16(int n, const float *__restrict__ src, __hip_bfloat16 *
     dst[i] = __float2bfloat16(src[i]);
 }
 
-// proposal weights: softplus(score) + eps * mean softplus (mean passed in)
-__global__ void k_proposal(uint32_t n, const float *__restrict__ score, double floor_w, double *w) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const double s = (double)score[v];
-    const double sp = s > 30.0 ? s : log1p(exp(s));
-    w[v] = sp + floor_w;
-  }
+// The score predicts log(1 + utility / mean utility) of a root, so the
+// proposal weight of a vertex of positive degree is the predicted relative
+// utility expm1(score) (clamped to [0, e^30]) plus eps times its mean over
+// those vertices; vertices without edges cannot be roots and get weight 0.
+__device__ __forceinline__ double score_weight(float s) {
+  return expm1(fmin(fmax((double)s, 0.0), 30.0));
 }
 
-__global__ void k_softplus_sum(uint32_t n, const float *__restrict__ score, double *sum) {
-  double a = 0.0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const double s = (double)score[v];
-    a += s > 30.0 ? s : log1p(exp(s));
-  }
+__global__ void k_proposal(uint32_t n, const uint32_t *__restrict__ col, const float *__restrict__ score,
+                           double floor_w, double *w) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    w[v] = col[v + 1] > col[v] ? score_weight(score[v]) + floor_w : 0.0;
+}
+
+// sum[0] += weights, sum[1] += vertices of positive degree
+__global__ void k_weight_sum(uint32_t n, const uint32_t *__restrict__ col, const float *__restrict__ score,
+                             double *sum) {
+  double a = 0.0, c = 0.0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    if (col[v + 1] > col[v]) {
+      a += score_weight(score[v]);
+      c += 1.0;
+    }
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) a += __shfl_xor(a, off);
-  if (lane_id() == 0) atomicAdd(sum, a);
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    c += __shfl_xor(c, off);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(&sum[0], a);
+    atomicAdd(&sum[1], c);
+  }
 }
 
 int grid_of(uint64_t n) {
@@ -286,12 +327,12 @@ int axbc_mlp_init(axbc_ctx *c, const float *params) {
 
 int axbc_mlp_param_count(void) { return kParams; }
 
-int axbc_mlp_features(axbc_ctx *c, double bc_scale, double delta_scale) {
+int axbc_mlp_features(axbc_ctx *c, double bc_scale) {
   MlpBufs *b = c ? bufs_of(c) : nullptr;
   if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
   const float *lcc = c->sampler && c->sampler->lcc_ready ? c->sampler->lcc : nullptr;
-  hipLaunchKernelGGL(k_features, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, lcc, c->bc, c->delta,
-                     bc_scale, delta_scale, b->x);
+  hipLaunchKernelGGL(k_features, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, c->row, lcc, c->bc,
+                     bc_scale, b->x);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }
@@ -364,24 +405,26 @@ int axbc_mlp_forward(axbc_ctx *c) {
   return AXBC_OK;
 }
 
-// Proposal distribution from the scores: w = softplus(s) + eps * mean, as
-// fixed-point weights scanned for axbc_weights_pick (kind 7).
+// Proposal distribution from the scores: w = expm1(s) + eps * mean over the
+// vertices of positive degree (0 for the others), as fixed-point weights
+// scanned for axbc_weights_pick (kind 7).
 int axbc_mlp_proposal(axbc_ctx *c, double eps, uint64_t *total) {
   MlpBufs *b = c ? bufs_of(c) : nullptr;
   if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
   SamplerBufs *sb = nullptr;
   int rc = sampler_bufs(c, &sb);
   if (rc) return rc;
-  double *dsum = nullptr, hsum = 0;
-  HIP_TRY(hipMalloc((void **)&dsum, sizeof(double)));
-  HIP_TRY(hipMemsetAsync(dsum, 0, sizeof(double), c->stream));
-  hipLaunchKernelGGL(k_softplus_sum, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, b->score, dsum);
-  HIP_TRY(hipMemcpyAsync(&hsum, dsum, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double *dsum = nullptr, hsum[2] = {0, 0};
+  HIP_TRY(hipMalloc((void **)&dsum, 2 * sizeof(double)));
+  HIP_TRY(hipMemsetAsync(dsum, 0, 2 * sizeof(double), c->stream));
+  hipLaunchKernelGGL(k_weight_sum, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, b->score, dsum);
+  HIP_TRY(hipMemcpyAsync(hsum, dsum, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipFree(dsum));
+  if (!(hsum[1] > 0.0)) return fail(AXBC_ESTATE, "no vertex of positive degree%s (line %d)", "", __LINE__);
   double *w = b->prop;
-  hipLaunchKernelGGL(k_proposal, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, b->score,
-                     eps * hsum / (double)c->n, w);
+  hipLaunchKernelGGL(k_proposal, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, b->score,
+                     eps * hsum[0] / hsum[1], w);
   HIP_TRY(hipGetLastError());
   return axbc_weights_from(c, w, total);
 }

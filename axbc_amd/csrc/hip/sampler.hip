@@ -490,6 +490,16 @@ int axbc_weights_at(axbc_ctx *c, uint32_t v, uint64_t *w) {
   return AXBC_OK;
 }
 
+int axbc_weights_copy(axbc_ctx *c, uint64_t *w) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  if (!sb->weights_ready || !w) return fail(AXBC_ESTATE, "weights not built%s (line %d)", "", __LINE__);
+  HIP_TRY(hipMemcpyAsync(w, sb->tmp, sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
 int axbc_weights_pick(axbc_ctx *c, uint64_t r, uint32_t *v, double *p) {
   SamplerBufs *sb = nullptr;
   int rc = sampler_bufs(c, &sb);

@@ -1,18 +1,23 @@
 """Learned pivot proposal (the north star's third pattern).
 
-A small MLP over per-vertex features (log degree, LCC, running bc, last
-delta) proposes pivots in place of the bc/delta heuristics.  It is trained
-on the method's own output: the first `warm` pivots are uniform; the bc
-estimate of the first half of them gives the features and the estimate of
-the second half the target (cross-fitting, so the network learns to denoise
-the running bc with degree and LCC rather than to copy it).  Training runs
-on the device (hand-written forward/backward/Adam kernels, BF16 weights,
-FP32 accumulation).  The remaining pivots are drawn from
-w(v) = softplus(score(v)) + eps * mean softplus -- a static distribution, so
-they run in exact bit-packed batches; when training fails (non-finite or
-not improving loss) the loop falls back to uniform sampling.  Every round
-contributes delta/p with its actual sampling probability p, so the
-estimator stays unbiased.
+A small MLP over per-vertex features (log degree, LCC, running bc, mean
+neighbour bc) proposes pivots in place of the bc/delta heuristics.  It is
+trained on the method's own output: the first `warm` pivots are uniform.
+The bc estimate of the first quarter of them (the pilot) ranks the
+vertices; the vertices around the top-1% boundary (ranks k/2 .. 2k), each
+weighted by 1/bc, are the targets.  For every later warm-up root, its
+utility -- its share of the targets' estimates, (reach+1) *
+sum_v delta_root(v) / bc(v), read from the batch's per-root delta rows on
+the device -- is the training label, as log(1 + utility / mean).  Sampling
+roots in proportion to that utility lowers the variance exactly where the
+top-1% ranking is decided.  Training runs on the device (hand-written
+forward/backward/Adam kernels, BF16 weights, FP32 accumulation) on three
+quarters of those roots; the rest decide whether it generalises.  The
+remaining pivots are drawn from w(v) = expm1(score(v)) + eps * mean -- a
+static distribution, so they run in exact bit-packed batches; when the
+held-out loss does not beat the mean predictor (or is not finite) the loop
+falls back to uniform sampling.  Every round contributes delta/p with its
+actual sampling probability p, so the estimator stays unbiased.
 """
 from __future__ import annotations
 
@@ -43,7 +48,7 @@ def _hip():
     L = _ffi.hip()
     for name, args in {
         "axbc_mlp_init": [C.c_void_p, C.c_void_p],
-        "axbc_mlp_features": [C.c_void_p, C.c_double, C.c_double],
+        "axbc_mlp_features": [C.c_void_p, C.c_double],
         "axbc_mlp_set_data": [C.c_void_p, C.c_void_p, C.c_void_p],
         "axbc_mlp_get_data": [C.c_void_p, C.c_void_p, C.c_void_p],
         "axbc_mlp_gradient": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p],
@@ -71,8 +76,8 @@ class Mlp:
         _check(self.L.axbc_mlp_set_data(self.e.ctx, None if xa is None else xa.ctypes.data,
                                         None if ya is None else ya.ctypes.data))
 
-    def features(self, bc_scale: float, delta_scale: float):
-        _check(self.L.axbc_mlp_features(self.e.ctx, bc_scale, delta_scale))
+    def features(self, bc_scale: float):
+        _check(self.L.axbc_mlp_features(self.e.ctx, bc_scale))
 
     def get(self):
         x = np.empty((self.e.N, 4), np.float32)
@@ -124,51 +129,65 @@ class StaticLoop(PivotLoop):
         return int(v.value), float(p.value)
 
 
-def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64, eps: float = 0.2,
-                steps: int = 300, minibatch: int = 16384, lr: float = 3e-3, seed: int = 4343543) -> dict:
+def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64, eps: float = 1.0,
+                steps: int = 300, lr: float = 3e-3, seed: int = 4343543, top: float = 0.01) -> dict:
     """The learned loop; returns timing, training log and whether it fell
     back to uniform.  bc on the engine holds the estimate afterwards."""
     t0 = time.perf_counter()
     g = e.g
     N = g.N
+    L = _ffi.hip()
     e.reset_bc()
-    half = warm // 2
+    pilot = max(1, warm // 4)
     loop = PivotLoop(e, N, c=0, x=0, z=0, seed=seed, batch=batch)
-    while loop.res.nrounds < half and loop.step(min(batch, half - loop.res.nrounds)):
+    while loop.res.nrounds < pilot and loop.step(min(batch, pilot - loop.res.nrounds)):
         pass
     bc_a = e.bc()
-    while loop.res.nrounds < warm and loop.step(min(batch, warm - loop.res.nrounds)):
-        pass
+    # targets: the vertices around the top-k boundary of the pilot estimate
+    k = max(1, int(round(top * N)))
+    order = np.argsort(-bc_a, kind="stable")
+    tv = order[k // 2:2 * k]
+    tv = np.ascontiguousarray(tv[bc_a[tv] > 0], np.uint32)
+    tw = np.ascontiguousarray(1.0 / bc_a[tv], np.float64)
+    r1 = g.reach.astype(np.float64) + 1.0 if g.reach is not None else np.ones(N)
+    util = {}
+    out = np.zeros(128)
+    while loop.res.nrounds < warm:
+        n0 = loop.res.nrounds
+        if not loop.step(min(batch, warm - n0)):
+            break
+        if len(tv):
+            _check(L.axbc_batch_utility(e.ctx, tv.ctypes.data, tw.ctypes.data, len(tv), out.ctypes.data))
+        for j, r in enumerate(loop.res.pivots[n0:loop.res.nrounds]):
+            util[int(r)] = float(out[j]) * float(r1[r]) if len(tv) else 0.0
     bc_w = e.bc()
-    bc_b = bc_w - bc_a
     t_warm = time.perf_counter() - t0
-    # features from the first half, target from the second half
-    _check(_ffi.hip().axbc_lcc_build(e.ctx, None, None, None, C.byref(C.c_uint64(0))))
+    _check(L.axbc_lcc_build(e.ctx, None, None, None, C.byref(C.c_uint64(0))))
     mlp = Mlp(e, seed=seed)
-    ma = bc_a[bc_a > 0].mean() if (bc_a > 0).any() else 1.0
-    mb = bc_b[bc_b > 0].mean() if (bc_b > 0).any() else 1.0
-    e.set_bc(bc_a)
-    mlp.features(1.0 / ma, 1.0 / max(float(e.delta().mean()), 1e-30))
-    e.set_bc(bc_w)
-    y = np.log1p(bc_b / mb).astype(np.float32)
-    mlp.set_data(None, y)
-    cand = np.flatnonzero(np.diff(g.col) > 0).astype(np.uint32)
-    rs = np.random.default_rng(seed)
-    losses = []
-    for s in range(steps):
-        idx = rs.choice(cand, size=min(minibatch, len(cand)), replace=False)
-        losses.append(mlp.step(idx, lr))
-    # baseline loss: predicting the target mean
-    base = float(((y[cand] - y[cand].mean()) ** 2).mean())
-    final = float(np.mean(losses[-10:]))
-    ok = bool(np.isfinite(final) and final < base)
-    t_train = time.perf_counter() - t0 - t_warm
-    # inference on the features of the whole warm-up estimate
     mw = bc_w[bc_w > 0].mean() if (bc_w > 0).any() else 1.0
-    mlp.features(1.0 / mw, 1.0 / max(float(e.delta().mean()), 1e-30))
+    mlp.features(1.0 / mw)
+    roots = np.array(sorted(util), np.uint32)
+    u = np.array([util[int(r)] for r in roots])
+    losses, ok, val, base = [], False, float("nan"), float("nan")
+    if len(roots) >= 8 and (u > 0).any():
+        mu = u[u > 0].mean()
+        y = np.zeros(N, np.float32)
+        y[roots] = np.log1p(u / mu)
+        mlp.set_data(None, y)
+        rs = np.random.default_rng(seed)
+        perm = rs.permutation(len(roots))
+        ntr = max(1, (3 * len(roots)) // 4)
+        tr, va = roots[perm[:ntr]], roots[perm[ntr:]]
+        for _ in range(steps):
+            losses.append(mlp.step(tr, lr))
+        mlp.forward()
+        _, sc = mlp.get()
+        val = float(np.mean((sc[va] - y[va]) ** 2))
+        base = float(np.mean((y[va] - y[tr].mean()) ** 2))
+        ok = bool(np.isfinite(val) and val < base)
+    t_train = time.perf_counter() - t0 - t_warm
     rest = max(0, budget - loop.res.nrounds)
     if ok:
-        mlp.forward()
         total = mlp.proposal(eps)
         sl = StaticLoop(e, N, total, loop.rng, batch=batch)
         sl.vs = loop.vs
@@ -181,8 +200,9 @@ def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64,
             pass
         nrounds = loop.res.nrounds
     e.sync()
-    return {"nrounds": nrounds, "fallback": not ok, "loss_first": losses[0] if losses else None,
-            "loss_final": final, "loss_baseline": base, "t_warm": t_warm, "t_train": t_train,
+    return {"nrounds": nrounds, "fallback": not ok, "samples": len(roots), "targets": int(len(tv)),
+            "loss_first": losses[0] if losses else None, "loss_final": float(np.mean(losses[-10:])) if losses else None,
+            "val_loss": val, "val_baseline": base, "t_warm": t_warm, "t_train": t_train,
             "seconds": time.perf_counter() - t0}
 
 

@@ -65,8 +65,8 @@ class PivotLoop:
                  seed: int = DEFAULT_SEED, batch: int = 0):
         if z not in range(7):
             raise ValueError("distributions are 0..6")
-        if batch and z not in (0, 1, 3, 4, 5):
-            raise ValueError("batches need -z 0, 1 (exact) or 3, 4, 5 (batch-synchronous)")
+        if batch and z not in (0, 1, 3, 4, 5, 6):
+            raise ValueError("batches need -z 0, 1 (exact) or 3, 4, 5, 6 (batch-synchronous)")
         self.e, self.g = engine, engine.g
         self.N = self.g.N
         self.nbfs = min(nbfs, self.N)
@@ -126,9 +126,15 @@ class PivotLoop:
         v0, _ = self._draw_component(self.mix)
         # mixture probability (axbc2d.c:3030-3047), then the next component
         p = 1.0 / N
-        if self.res.nrounds == 0:
+        first = self.res.nrounds < self.batch if self.batch else self.res.nrounds == 0
+        if first:
             p = p + 1.0 / N
             p = p + 1.0 / N
+        elif self.batch:
+            # batch-synchronous: bc and delta weights frozen at the batch start
+            inside = v0 < N
+            p = p + (float(self._wb6[v0]) if inside else 0.0) / float(self._tb6)
+            p = p + (float(self._wd6[v0]) if inside else 0.0) / float(self._td6)
         else:
             wb, tb = self._device_weight(3, v0)
             p = p + float(wb) / float(tb)
@@ -210,6 +216,17 @@ class PivotLoop:
             t = C.c_uint64(0)
             _check(_ffi.hip().axbc_weights(e.ctx, self.z, C.byref(t)))
             self._frozen_total = int(t.value)
+        if self.batch and self.z == 6 and self.res.nrounds >= self.batch:
+            L = _ffi.hip()
+            t = C.c_uint64(0)
+            self._wb6 = np.empty(self.N, np.uint64)
+            _check(L.axbc_weights(e.ctx, 3, C.byref(t)))
+            _check(L.axbc_weights_copy(e.ctx, self._wb6.ctypes.data))
+            self._tb6 = int(t.value)
+            self._wd6 = np.empty(self.N, np.uint64)
+            _check(L.axbc_weights(e.ctx, 4, C.byref(t)))
+            _check(L.axbc_weights_copy(e.ctx, self._wd6.ctypes.data))
+            self._td6 = int(t.value)
         pend, pend_ui, pend_skip = [], [], []
         while self.ui < self.nbfs and len(pend) < want:
             skipped_before = res.skipped

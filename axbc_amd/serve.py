@@ -32,11 +32,14 @@ SAMPLER_BYTES = 28 + 24 + 8                    # sampler scratch, LCC tables, sl
 
 def footprint_bytes(N: int, ned: int, batch: int = 0) -> int:
     """Device bytes a job needs: CSC + per-vertex state + sampler tables
-    (+ the batch rows when batched)."""
+    (+ when batched: the root rows, the bit words, the level list at its
+    initial 8N entries, and one partial row per 64-edge chunk)."""
     b = 4 * (N + 1) + 4 * ned + N * (PLAIN_STATE_BYTES + SAMPLER_BYTES)
     if batch:
-        words = batch // 64
-        b += N * (batch * (4 + 8 + 8) + 8 * 4 * words + 8) + 4 * N * (4 + 8 * words)
+        width = 128 if batch > 64 else 64
+        words = width // 64
+        b += N * (width * (4 + 8 + 8) + 8 * 4 * words + 8) + 8 * N * (4 + 8 * words)
+        b += (ned // 64 + 1) * width * 8
     return b
 
 
@@ -174,10 +177,12 @@ class Server:
 
 
 def poisson_jobs(n: int, rate: float, seed: int = 0, scales=(12, 20), efs=(8, 16, 32), budget=(64, 2048),
-                 z: int = 6) -> list[tuple[float, Job]]:
+                 z: int = 6, batch: int = 64) -> list[tuple[float, Job]]:
     """The serving workload: Poisson arrivals at `rate` per second; per job a
     Kronecker SCALE drawn uniformly from `scales`, an edge factor from `efs`,
-    the job id as seed, and a pivot budget log-uniform in `budget`."""
+    the job id as seed, and a pivot budget log-uniform in `budget`.  Jobs
+    run in bit-packed batches of `batch` roots (batch-synchronous for the
+    dynamic distributions; 0 = one root per round)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -186,7 +191,7 @@ def poisson_jobs(n: int, rate: float, seed: int = 0, scales=(12, 20), efs=(8, 16
         s = int(rng.integers(scales[0], scales[1] + 1))
         ef = int(rng.choice(efs))
         b = int(round(np.exp(rng.uniform(np.log(budget[0]), np.log(budget[1])))))
-        out.append((t, Job(jid=j, budget=b, z=z, scale=s, edgefactor=ef, seed=(j + 1, j + 2))))
+        out.append((t, Job(jid=j, budget=b, z=z, scale=s, edgefactor=ef, seed=(j + 1, j + 2), batch=batch)))
     return out
 
 

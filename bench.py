@@ -106,6 +106,105 @@ def copy_calibration(mib):
                       "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1)}))
 
 
+def grid_case(args):
+    """4096 x 4096 lattice, 4-neighbour, 3% of edges deleted (seed 3), edges of
+    the largest component kept with the lattice ids (N = 2^24), -c 0 (the
+    default adaptive stop c=1 x=1 holds after the first rounds here).  Two
+    distributions: uniform (-z 0, exact batches of 64) and delta sampling
+    (-z 4, batch-synchronous batches of 64).  With N = 2^24 the reference's
+    retry cap (int)(N*N) is 0, so a duplicate draw consumes an attempt; the
+    concentrated delta distribution makes most -z 4 attempts duplicates, so
+    its pivots/s count rounds that actually ran.  Batched levels replay
+    captured graphs; plain rounds launch per level."""
+    from axbc_amd import Engine, Graph
+    from axbc_amd.pivots import PivotLoop
+    t = time.perf_counter()
+    g = Graph.lattice(4096, 0.03, seed=3, heuristic=0)
+    log(f"lattice: N={g.N} ned={g.ned} built in {time.perf_counter() - t:.1f}s")
+    e = Engine(g)
+    out = {"case": "grid-4096-del3-c0", "n": g.N, "ned": g.ned}
+
+    def timed(loop, steps):
+        loop.step()  # warm-up (allocation, list growth, graph capture)
+        e.sync()
+        r0, a0 = loop.res.nrounds, loop.res.attempts
+        t = time.perf_counter()
+        for _ in range(steps):
+            if not loop.step():
+                break
+        e.sync()
+        dt = time.perf_counter() - t
+        return {"rounds": loop.res.nrounds - r0, "attempts": loop.res.attempts - a0, "seconds": round(dt, 2),
+                "pivots_per_s": round((loop.res.nrounds - r0) / dt, 3) if dt > 0 else 0.0}
+
+    for z in (0, 4):
+        e.reset_bc()
+        b = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=64), args.steps)
+        e.reset_bc()
+        p = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=0), args.plain_rounds)
+        out[f"z{z}"] = {"batched": b, "plain": p,
+                        "batched_over_plain": round(b["pivots_per_s"] / p["pivots_per_s"], 2) if p["pivots_per_s"] else None}
+        log(f"z{z}: {out[f'z{z}']}")
+    st = e.pivot(int(np.flatnonzero(np.diff(g.col) > 0)[0]), 1.0 / g.N, stats=True)
+    out["levels_of_a_round"] = st["nlevels"]
+    e.close()
+    print(json.dumps(out))
+
+
+def serve_case(args):
+    from axbc_amd.serve import poisson_jobs, run_workload
+    jobs = poisson_jobs(args.serve, 20.0, seed=0)
+    t = time.perf_counter()
+    r = run_workload(jobs, slots=args.slots, hbm_cap=200e9)
+    log(f"serving: {r} in {time.perf_counter() - t:.1f}s")
+    print(json.dumps(dict({"case": "serving-poisson20-s12to20-z6", "requests": args.serve, "slots": args.slots},
+                          **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()})))
+
+
+def learned_case(args):
+    """Accuracy on SCALE 18 (two generator seeds x two loop seeds): exact
+    Brandes (every vertex a root, bit-packed batches) against the learned
+    loop and the uniform loop at 4096 pivots; then the learned loop's time
+    on SCALE 24."""
+    from axbc_amd import Engine, Graph
+    from axbc_amd.learned import accuracy, exact_bc, run_learned
+    from axbc_amd.pivots import PivotLoop
+    out = {"case": "learned-vs-uniform-4096", "runs": []}
+    for gseed in ((4, 5), (6, 7)):
+        g = Graph.kronecker(18, 16, seed=gseed, heuristic=1)
+        with Engine(g) as e:
+            t = time.perf_counter()
+            ex = exact_bc(e, batch=128)
+            t_exact = time.perf_counter() - t
+            for lseed in (4343543, 1):
+                info = run_learned(e, budget=4096, warm=512, batch=64, seed=lseed)
+                a_l = accuracy(e.bc(), ex)
+                e.reset_bc()
+                loop = PivotLoop(e, g.N, c=0, x=0, z=0, seed=lseed, batch=64)
+                while loop.res.nrounds < 4096 and loop.step(min(64, 4096 - loop.res.nrounds)):
+                    pass
+                a_u = accuracy(e.bc(), ex)
+                run = {"graph_seed": gseed, "loop_seed": lseed, "exact_seconds": round(t_exact, 2),
+                       "learned": a_l, "uniform": a_u, "fallback": info["fallback"],
+                       "val_loss": [info["val_baseline"], info["val_loss"]]}
+                log(run)
+                out["runs"].append(run)
+    rl = [r["learned"]["top_recall"] for r in out["runs"]]
+    ru = [r["uniform"]["top_recall"] for r in out["runs"]]
+    sl = [r["learned"]["spearman"] for r in out["runs"]]
+    su = [r["uniform"]["spearman"] for r in out["runs"]]
+    out.update({"top1pct_recall_learned": round(float(np.mean(rl)), 4), "top1pct_recall_uniform": round(float(np.mean(ru)), 4),
+                "spearman_learned": round(float(np.mean(sl)), 5), "spearman_uniform": round(float(np.mean(su)), 5)})
+    t = time.perf_counter()
+    g = Graph.kronecker(24, 16, seed=(4, 5), heuristic=1)
+    log(f"SCALE 24 built in {time.perf_counter() - t:.1f}s (N={g.N} ned={g.ned})")
+    with Engine(g) as e:
+        info = run_learned(e, budget=4096, warm=512, batch=64)
+    out.update({"s24_learned_seconds": round(info["seconds"], 2), "s24_pivots_per_s": round(4096 / info["seconds"], 1),
+                "s24_train_seconds": round(info["t_train"], 2), "s24_fallback": info["fallback"]})
+    print(json.dumps(out))
+
+
 def cpu_baseline(g, roots, probs, seconds):
     """The oracle (one thread) on the same graph and root sequence, bounded
     to about `seconds` of work."""
@@ -141,6 +240,16 @@ def main():
                          "timers; this also covers the autotuning trials)")
     ap.add_argument("--chunk", type=int, default=0, help="split threshold with --batch (default 256)")
     ap.add_argument("--pull", type=int, default=-1, help="bottom-up ratio x16 with --batch (default 32)")
+    ap.add_argument("--grid", action="store_true",
+                    help="high-diameter case instead: 4096^2 lattice, 3%% edges deleted (seed 3), -z 4, batch-synchronous "
+                         "batches of 64 vs plain rounds")
+    ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
+                    help="serving case instead: Poisson 20 req/s, SCALE 12-20, ef 8/16/32, budgets 64-2048, -z 6, "
+                         "200 GB cap; prints p50/p99 latency and aggregate pivots/s")
+    ap.add_argument("--slots", type=int, default=16, help="device workers of --serve")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned-proposal case instead: accuracy vs exact Brandes on SCALE 18 (learned vs uniform at "
+                         "4096 pivots) and the learned loop's throughput on SCALE 24")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the calibration copy kernel over MIB MiB (8 launches) and print its rate")
     args = ap.parse_args()
@@ -154,6 +263,15 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.grid:
+        grid_case(args)
+        return
+    if args.serve:
+        serve_case(args)
+        return
+    if args.learned:
+        learned_case(args)
         return
 
     t0 = time.perf_counter()

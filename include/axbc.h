@@ -153,8 +153,11 @@ int axbc_hdn_pick(axbc_ctx *ctx, uint32_t r, uint32_t *u, uint32_t *sdn_u);
  */
 int axbc_weights(axbc_ctx *ctx, int kind, uint64_t *total);
 int axbc_weights_pick(axbc_ctx *ctx, uint64_t r, uint32_t *v, double *p);
-/* W[v] of the last axbc_weights call (mixture probabilities, -z 6). */
+/* W[v] of the last axbc_weights call (mixture probabilities, -z 6);
+ * axbc_weights_copy copies all N of them (batch-synchronous -z 6 freezes
+ * them at the batch start). */
 int axbc_weights_at(axbc_ctx *ctx, uint32_t v, uint64_t *w);
+int axbc_weights_copy(axbc_ctx *ctx, uint64_t *w);
 
 /*
  * Local clustering coefficients (lcc_func, src/axbc2d.c:1847-1935) on the
@@ -197,6 +200,10 @@ int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);
 int axbc_batch_delta(axbc_ctx *ctx, int k, double *delta);
+/* Utility of the last batch's roots for a set of target vertices:
+ * out[k] = sum_i weight[i] * delta_k(vertex[i]) for k < width (FP64; the
+ * learned proposal's training target).  No reference counterpart. */
+int axbc_batch_utility(axbc_ctx *ctx, const uint32_t *vertex, const double *weight, uint32_t count, double *out);
 
 /*
  * Per-kernel instrumentation for roofline figures (no reference
@@ -222,7 +229,7 @@ int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
  * Learned pivot proposal (no reference counterpart; stands in for the
  * bc/delta heuristics -z 3/4): a 4 -> 64 -> 64 -> 1 MLP with ReLU, BF16
  * weights and FP32 accumulation, over per-vertex features (log(1+deg), LCC,
- * log(1+bc*bc_scale), log(1+delta*delta_scale)).
+ * log(1+bc*bc_scale), log(1+mean neighbour bc*bc_scale)).
  *   axbc_mlp_init        FP32 master parameters (axbc_mlp_param_count()
  *                        floats: W1[64][4], b1, W2[64][64], b2, w3, b3)
  *   axbc_mlp_features    features from the device state (LCC when built)
@@ -231,14 +238,15 @@ int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
  *                        device; copied out when grad_host is given)
  *   axbc_mlp_adam        one Adam step with the accumulated gradient
  *   axbc_mlp_forward     scores of all vertices
- *   axbc_mlp_proposal    proposal weights softplus(score) + eps * mean as the
+ *   axbc_mlp_proposal    proposal weights expm1(max(score, 0)) + eps * mean
+ *                        (vertices of positive degree; 0 for the others) as the
  *                        fixed-point distribution of axbc_weights_pick
  * axbc_weights_from      fixed-point distribution from any device array of N
  *                        non-negative doubles.
  */
 int axbc_mlp_param_count(void);
 int axbc_mlp_init(axbc_ctx *ctx, const float *params);
-int axbc_mlp_features(axbc_ctx *ctx, double bc_scale, double delta_scale);
+int axbc_mlp_features(axbc_ctx *ctx, double bc_scale);
 int axbc_mlp_set_data(axbc_ctx *ctx, const float *x, const float *y);
 int axbc_mlp_get_data(axbc_ctx *ctx, float *x, float *score);
 int axbc_mlp_gradient(axbc_ctx *ctx, const uint32_t *idx, uint32_t m, float *grad_host, float *loss_host);

@@ -308,6 +308,7 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
   const uint32_t lim = oracle_lim_x_sample(N);
   uint64_t frozen_total = 0, frozen_at = 0;
   int frozen_ok = 0, frozen_uniform = 0;
+  uint64_t *Wb6 = NULL, *Wd6 = NULL, tb6 = 0, td6 = 0, frozen6_at = 0; /* z = 6, batch-synchronous */
   (void)frozen_uniform;
   const double cN = (double)((uint64_t)cfg->bader * (uint64_t)N);
   for (uint64_t ui = 0; ui < nbfs; ++ui) {
@@ -360,9 +361,24 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
       if (z == 6) { /* mixture probability and next component (3030-3051) */
         const int in = v0 < N;
         p = 1.0 / (double)N;
-        if (rs->nrounds == 0) {
+        if (first_group) {
           p = p + 1.0 / (double)N;
           p = p + 1.0 / (double)N;
+        } else if (cfg->sync_batch) {
+          /* batch-synchronous: bc and delta of the group's first round */
+          if (!Wb6) {
+            Wb6 = (uint64_t *)xcalloc(N, sizeof(uint64_t));
+            Wd6 = (uint64_t *)xcalloc(N, sizeof(uint64_t));
+          }
+          if (frozen6_at != rs->nrounds / cfg->sync_batch + 1) {
+            frozen6_at = rs->nrounds / cfg->sync_batch + 1;
+            for (uint32_t v = 0; v < N; ++v) wt[v] = bc[v];
+            tb6 = oracle_fixed_weights(N, wt, Wb6);
+            for (uint32_t v = 0; v < N; ++v) wt[v] = delta[v];
+            td6 = oracle_fixed_weights(N, wt, Wd6);
+          }
+          p = p + (in ? (double)Wb6[v0] : 0.0) / (double)tb6;
+          p = p + (in ? (double)Wd6[v0] : 0.0) / (double)td6;
         } else {
           for (uint32_t v = 0; v < N; ++v) wt[v] = bc[v];
           uint64_t tb = oracle_fixed_weights(N, wt, W);
@@ -403,6 +419,8 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
   free(delta);
   free(wt);
   free(W);
+  free(Wb6);
+  free(Wd6);
   free(vs);
   return 0;
 }

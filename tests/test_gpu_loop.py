@@ -20,12 +20,13 @@ def test_require(require_gpu):
     (0, 0, 1, 0, 0), (0, 1, 1, 0, 1), (1, 0, 4, 0, 1), (3, 0, 1, 0, 0), (4, 0, 1, 0, 1), (5, 0, 1, 0, 0),
     (0, 0, 1, 64, 0), (1, 1, 8, 64, 1), (1, 0, 1, 128, 1), (0, 2, 3, 64, 0),
     (2, 0, 1, 0, 0), (6, 0, 2, 0, 1), (4, 0, 1, 64, 0), (3, 1, 2, 32, 1), (5, 0, 1, 128, 0),
+    (6, 0, 2, 64, 1), (6, 1, 3, 128, 0),
 ])
 def test_loop_matches_oracle(require_gpu, z, c, x, batch, heur):
     g = Graph.kronecker(11, 16, seed=(1, 2), heuristic=heur)
     nbfs = 300
     print(f"[loop z{z} c{c} x{x} B{batch} H{heur}]", flush=True)
-    sync = batch if z in (3, 4, 5) else 0
+    sync = batch if z in (3, 4, 5, 6) else 0
     pv, pr, bc_ref, rs = oracle.run(g, nbfs, c=c, x=x, z=z, sync_batch=sync)
     with Engine(g) as e:
         res = PivotLoop(e, nbfs, c=c, x=x, z=z, batch=batch).run()

@@ -50,19 +50,31 @@ def test_training_reduces_loss(require_gpu):
 
 
 def test_learned_vs_uniform(require_gpu):
+    """What the trained proposal gains over the method without it (uniform
+    roots) at the same budget, judged against exact Brandes: the learned
+    loop must train (not fall back in every run), keep the round count, and
+    not lose top-1% recall beyond the noise of four runs; the gains are
+    printed (on this graph family they are within noise of zero)."""
     g = Graph.kronecker(13, 16, seed=(4, 5), heuristic=1)
+    from axbc_amd.pivots import PivotLoop
+    rows = []
     with Engine(g) as e:
         ex = exact_bc(e)
-        info = run_learned(e, budget=512, warm=128, batch=64, steps=150, minibatch=4096)
-        est_l = e.bc()
-        e.reset_bc()
-        from axbc_amd.pivots import PivotLoop
-        loop = PivotLoop(e, g.N, c=0, x=0, z=0, batch=64)
-        while loop.res.nrounds < 512 and loop.step(min(64, 512 - loop.res.nrounds)):
-            pass
-        est_u = e.bc()
-    a_l, a_u = accuracy(est_l, ex), accuracy(est_u, ex)
-    print("learned", info, a_l, "uniform", a_u, flush=True)
-    assert info["nrounds"] == 512
-    assert a_l["spearman"] > 0.5 and a_u["spearman"] > 0.5
-    assert a_l["top_recall"] >= a_u["top_recall"] - 0.1
+        for seed in (4343543, 1, 2, 3):
+            info = run_learned(e, budget=1024, warm=384, batch=64, steps=200, seed=seed)
+            a_l = accuracy(e.bc(), ex)
+            e.reset_bc()
+            loop = PivotLoop(e, g.N, c=0, x=0, z=0, batch=64, seed=seed)
+            while loop.res.nrounds < 1024 and loop.step(min(64, 1024 - loop.res.nrounds)):
+                pass
+            a_u = accuracy(e.bc(), ex)
+            print("learned", {k: info[k] for k in ("fallback", "samples", "val_loss", "val_baseline")}, a_l,
+                  "uniform", a_u, flush=True)
+            assert info["nrounds"] == 1024
+            rows.append((a_l["top_recall"], a_u["top_recall"], a_l["spearman"], a_u["spearman"], info["fallback"]))
+    r = np.array(rows, dtype=float)
+    print(f"[learned] recall gain {r[:, 0].mean() - r[:, 1].mean():+.4f}  spearman gain "
+          f"{r[:, 2].mean() - r[:, 3].mean():+.5f}", flush=True)
+    assert not r[:, 4].all()  # the network trained (held-out loss below the mean predictor) at least once
+    assert (r[:, 2] > 0.5).all() and (r[:, 3] > 0.5).all()
+    assert r[:, 0].mean() >= r[:, 1].mean() - 0.03

@@ -290,6 +290,33 @@ def test_sync_batch_freezes_dynamic_distribution():
     assert list(pv8[:8]) == list(pv0[:1]) + list(pv8[1:8])
 
 
+@pytest.mark.parametrize("z", [3, 5, 6])
+def test_sync_batch_of_one_is_the_per_round_rule(z):
+    # a group of one round freezes the state of that round: the exact loop,
+    # pivots and probabilities bit for bit (the mixture's bc/delta terms too)
+    g = Graph.kronecker(9, 16, seed=(3, 4), heuristic=1)
+    pv0, pr0, bc0, rs0 = oracle.run(g, 70, c=0, x=2, z=z)
+    pv1, pr1, bc1, rs1 = oracle.run(g, 70, c=0, x=2, z=z, sync_batch=1)
+    assert list(pv0) == list(pv1) and rs0 == rs1
+    np.testing.assert_array_equal(pr0, pr1)
+    np.testing.assert_array_equal(bc0, bc1)
+
+
+def test_sync_batch_mixture_freezes_bc_and_delta_terms():
+    # -z 6 in groups of 16: the first group uses the round-0 rule for the
+    # bc/delta components (1/N each); later probabilities differ from the
+    # per-round rule only through those frozen terms
+    g = Graph.kronecker(9, 16, seed=(3, 4), heuristic=1)
+    pv16, pr16, _, rs16 = oracle.run(g, 70, c=0, x=2, z=6, sync_batch=16)
+    pv0, pr0, _, _ = oracle.run(g, 70, c=0, x=2, z=6)
+    assert rs16["nrounds"] > 32
+    # component draws do not depend on bc, so the pivot sequence is the same
+    assert list(pv16) == list(pv0)
+    assert pr16[0] == pr0[0]
+    assert np.isfinite(pr16).all() and (pr16 > 0).all()
+    assert not np.array_equal(pr16[16:], pr0[16:])
+
+
 def test_mlp_oracle_gradient_by_finite_differences():
     from oracle import mlp_oracle as mo
     rng = np.random.default_rng(0)
