@@ -87,6 +87,9 @@ def hip() -> C.CDLL:
         "axbc_set_batch_pull": (C.c_int, [ctxp, C.c_uint32]),
         "axbc_batch_utility": (C.c_int, [ctxp, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
         "axbc_weights_copy": (C.c_int, [ctxp, C.c_void_p]),
+        "axbc_ingest": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint32)]),
         "axbc_set_batch_graphs": (C.c_int, [ctxp, C.c_int]),
         "axbc_batch_delta": (C.c_int, [ctxp, C.c_int, f64p]),
         "axbc_hdn_build": (C.c_int, [ctxp, u32p, u32p, C.POINTER(C.c_uint32)]),

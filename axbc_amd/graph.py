@@ -95,6 +95,34 @@ class Graph:
         return out
 
     @staticmethod
+    def from_keys_device(N: int, keys: np.ndarray, undirected: bool = True, heuristic: int = 0, device: int = 0,
+                         name: str = "edges") -> "Graph":
+        """The same preprocessing (mirror, normalise, 1-degree reduction,
+        CSC) run on the device (axbc_ingest); keys are packed (u << 32 | v)."""
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        m = int(k.shape[0])
+        col = np.empty(N + 1, np.uint32)
+        row = np.empty(max(2 * m, 1), np.uint32)
+        reach = np.empty(N, np.uint32)
+        ned, rem = C.c_uint64(0), C.c_uint32(0)
+        L = _ffi.hip()
+        rc = L.axbc_ingest(device, k.ctypes.data, m, N, int(undirected), heuristic, col.ctypes.data, row.ctypes.data,
+                           row.shape[0], reach.ctypes.data, C.byref(ned), C.byref(rem))
+        if rc != 0:
+            raise RuntimeError(L.axbc_last_error().decode())
+        return Graph(N, col, row[:ned.value].copy(), reach, heuristic, m, int(rem.value), name)
+
+    @staticmethod
+    def kronecker_device(scale: int, edgefactor: int = 16, seed: tuple[int, int] = (1, 2),
+                         undirected: bool = True, heuristic: int = 0, device: int = 0) -> "Graph":
+        """Graph.kronecker with the preprocessing on the device: the host
+        generator's edges, then axbc_ingest."""
+        e = kronecker_edge_list(scale, (1 << scale) * edgefactor, seed[0], seed[1]).astype(np.uint64)
+        keys = (e[:, 0] << np.uint64(32)) | e[:, 1]
+        return Graph.from_keys_device(1 << scale, keys, undirected, heuristic, device,
+                                      f"kron-s{scale}-ef{edgefactor}-seed{seed[0]}")
+
+    @staticmethod
     def read(path: str, N: int, undirected: bool = True, heuristic: int = 0) -> "Graph":
         """Text edge list ("u v" lines, '#' comments; src/axbc2d.c:356-445)."""
         L = _ffi.host()

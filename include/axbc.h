@@ -226,6 +226,21 @@ int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, u
 int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 
 /*
+ * Graph construction on the device (no reference device counterpart; the
+ * host steps are src/axbc2d.c:333-351 mirror, 818-841 norm_graph, 695-807
+ * and 866-876 the 1-degree reduction, 883-907 build_csc).  edges: ned
+ * packed keys (u << 32 | v), endpoints < N.  undirected: add the reversed
+ * non-loop edges; heuristic 1: normalise + 1-degree reduction (reach[]
+ * counts), 0: none; then normalise and build the CSC.  col: N + 1
+ * entries, row: row_cap entries (at least the final edge count; 2 * ned
+ * always suffices), reach: N entries.  Results equal the host pipeline's
+ * bit for bit.
+ */
+int axbc_ingest(int device, const uint64_t *edges, uint64_t ned, uint64_t N, int undirected, int heuristic,
+                uint32_t *col, uint32_t *row, uint64_t row_cap, uint32_t *reach, uint64_t *ned_out,
+                uint32_t *rem_ed);
+
+/*
  * Learned pivot proposal (no reference counterpart; stands in for the
  * bc/delta heuristics -z 3/4): a 4 -> 64 -> 64 -> 1 MLP with ReLU, BF16
  * weights and FP32 accumulation, over per-vertex features (log(1+deg), LCC,

@@ -47,3 +47,25 @@ def test_driver_matches_oracle(require_gpu, tmp_path, flags):
     assert res.shape == (g.N, 2) and (res[:, 0] == np.arange(g.N)).all()
     exp = bc / 2.0 / max(rounds, 1)
     assert np.allclose(res[:, 1], exp, rtol=1e-9, atol=0.0051)
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_plain_config_scale16(require_gpu, tmp_path, directed):
+    """The CPU-runnable plain configuration: SCALE 16, edge factor 16, seed
+    pair (1, 2), -z 0 -N 256 with the default adaptive stop (c=1, x=1); once
+    symmetrised and once with -U (which turns mirroring off)."""
+    out = tmp_path / "res"
+    cmd = [DRIVER, "-p", "1x1", "-S", "16", "-E", "16", "-s", "1", "-z", "0", "-N", "256", "-q", "-o", str(out)]
+    if directed:
+        cmd.append("-U")
+    print("[driver]", " ".join(cmd[1:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rounds = int(re.search(r"task 0 BC rounds: (\d+)", r.stdout).group(1))
+    skipped = int(re.search(r"task 0 BC skipped: (\d+)", r.stdout).group(1))
+    g = Graph.kronecker(16, 16, seed=(1, 2), undirected=not directed, heuristic=0)
+    pv, pr, bc, rs = oracle.run(g, 256, c=1, x=1, z=0)
+    assert rounds == rs["nrounds"] and skipped == rs["skipped"]
+    res = np.loadtxt(str(out) + "_1X1_0.log")
+    exp = bc / 2.0 / max(rounds, 1)
+    assert np.allclose(res[:, 1], exp, rtol=1e-9, atol=0.0051)
