@@ -47,7 +47,10 @@ using namespace axbc;
 namespace axbc {
 
 constexpr uint32_t kChunkMin = 64;  // smallest edges-per-work-item for split vertices
-constexpr int kBurst = 8;           // row loads in flight per wave in the pull loops
+#ifndef AXBC_BURST
+#define AXBC_BURST 8
+#endif
+constexpr int kBurst = AXBC_BURST;  // row loads in flight per wave in the pull loops
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
 constexpr int kGraphSteps = 8;    // levels (depths) per captured graph
 

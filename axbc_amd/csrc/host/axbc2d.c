@@ -68,9 +68,28 @@ static void usage(const char *p) {
           "\t\t -H <0|1>       1-degree reduction\n"
           "\t\t -U             DO NOT make graph Undirected\n"
           "\t\t -R <0|1>       fixed / time-based seed\n"
-          "\t\t -B <0|64|128>  roots per GPU batch\n",
+          "\t\t -B <0|64|128>  roots per GPU batch\n"
+          "\t\t -g             graph preprocessing on the GPU\n",
           p);
   exit(EXIT_FAILURE);
+}
+
+/* Preprocessing on the device (axbc_ingest): the same CSC, reach[] and
+ * removed-edge count as axbc_graph_from_edges; frees `ed`. */
+static int device_ingest(uint64_t *ed, uint64_t ned, uint64_t N, int undirected, int heuristic, axbc_graph *g) {
+  memset(g, 0, sizeof(*g));
+  g->N = N;
+  g->col = (uint32_t *)malloc(sizeof(uint32_t) * (N + 1));
+  g->row = (uint32_t *)malloc(sizeof(uint32_t) * (2 * ned + 1));
+  g->reach = (uint32_t *)malloc(sizeof(uint32_t) * (N ? N : 1));
+  if (!g->col || !g->row || !g->reach) die("%s", "host allocation failed");
+  uint64_t out = 0;
+  const int rc = axbc_ingest(0, ed, ned, N, undirected, heuristic, g->col, g->row, 2 * ned + 1, g->reach, &out,
+                             &g->rem_ed);
+  free(ed);
+  g->ned = out;
+  g->generated = ned;
+  return rc;
 }
 
 /* uniform_int(max, min) = lrand48() % (max - min + 1) + min in uint32
@@ -115,13 +134,14 @@ int main(int argc, char **argv) {
   uint32_t bader = 1, approx_n = 1;
   int distribution = 0, rootset = 0, have_seed = 0;
   char *gfile = NULL, *ofile = NULL, *gout = NULL;
+  int device_build = 0;
   int c;
   if (argc == 1) usage(argv[0]);
   setvbuf(stdout, NULL, _IOLBF, 0);
   printf("MPI Tasks 1\n");
   for (int i = 0; i < argc; ++i) printf(" %s", argv[i]);
   printf("\n\n");
-  while ((c = getopt(argc, argv, "o:p:amhDR:dUf:n:r:S:E:N:H:c:x:z:B:s:T:qG:")) != -1) {
+  while ((c = getopt(argc, argv, "o:p:amhDR:dUf:n:r:S:E:N:H:c:x:z:B:s:T:qG:g")) != -1) {
     switch (c) {
       case 'c': bader = (uint32_t)strtoul(optarg, NULL, 10); break;
       case 'x': approx_n = (uint32_t)strtoul(optarg, NULL, 10); break;
@@ -176,6 +196,7 @@ int main(int argc, char **argv) {
         break;
       case 'T': nthreads = atoi(optarg); break;
       case 'q': quiet = 1; break;
+      case 'g': device_build = 1; break;
       case 'G': gout = strdup(optarg); break;
       default: usage(argv[0]);
     }
@@ -225,7 +246,21 @@ int main(int argc, char **argv) {
     uint64_t *ed = axbc_read_edge_file(gfile, N, undirected, &ned, err, sizeof(err));
     if (!ed) die("%s", err);
     generated = ned;
-    if (axbc_graph_from_edges(ed, ned, N, heuristic, nthreads, &g)) die("%s", "graph construction failed");
+    if (device_build) {
+      if (device_ingest(ed, ned, N, 0, heuristic, &g)) die("device graph construction failed: %s", axbc_last_error());
+    } else if (axbc_graph_from_edges(ed, ned, N, heuristic, nthreads, &g)) {
+      die("%s", "graph construction failed");
+    }
+  } else if (device_build) {
+    const double ini[4] = {.57, .19, .19, .05};
+    const int64_t m = ((int64_t)1 << scale) * edgef;
+    uint64_t *pairs = axbc_kronecker_edges(scale, m, s1, s2, ini, nthreads);
+    if (!pairs) die("%s", "generator allocation failed");
+    uint64_t *ed = axbc_pack_pairs(pairs, (uint64_t)m);
+    free(pairs);
+    if (device_ingest(ed, (uint64_t)m, N, undirected, heuristic, &g))
+      die("device graph construction failed: %s", axbc_last_error());
+    generated = (uint64_t)m;
   } else {
     if (axbc_graph_kronecker(scale, edgef, s1, s2, undirected, heuristic, nthreads, &g))
       die("%s", "graph construction failed");

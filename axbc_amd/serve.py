@@ -114,9 +114,9 @@ class Server:
     # -- internals ----------------------------------------------------------
     def _build(self, job: Job) -> None:
         try:
-            if job.graph is None:
-                job.graph = Graph.kronecker(job.scale, job.edgefactor, seed=job.seed, heuristic=job.heuristic,
-                                            threads=2)
+            if job.graph is None:  # host generator, preprocessing on the device
+                job.graph = Graph.kronecker_device(job.scale, job.edgefactor, seed=job.seed,
+                                                   heuristic=job.heuristic, device=self.device)
         except Exception as ex:  # the job fails, the server goes on
             job.error = f"graph build failed: {ex}"
         with self._lock:

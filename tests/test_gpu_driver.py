@@ -26,6 +26,7 @@ def test_require(require_gpu):
     ["-z", "1", "-H", "1", "-c", "0", "-B", "64"],
     ["-z", "4", "-c", "0"],
     ["-z", "6", "-H", "1", "-c", "0"],
+    ["-z", "1", "-H", "1", "-c", "0", "-B", "64", "-g"],  # preprocessing on the device
 ])
 def test_driver_matches_oracle(require_gpu, tmp_path, flags):
     scale, nbfs = 10, 200
