@@ -118,7 +118,7 @@ class Server:
                 job.graph = Graph.kronecker_device(job.scale, job.edgefactor, seed=job.seed,
                                                    heuristic=job.heuristic, device=self.device)
         except Exception as ex:  # the job fails, the server goes on
-            job.error = f"graph build failed: {ex}"
+            job.error = f"graph build failed: {type(ex).__name__}: {ex}"
         with self._lock:
             self._ready.append(job)
             self._lock.notify_all()
@@ -155,7 +155,7 @@ class Server:
                 if not job.error:
                     self._run(job)
             except Exception as ex:
-                job.error = str(ex)
+                job.error = f"{type(ex).__name__}: {ex}"
             job.finished = time.perf_counter()
             with self._lock:
                 self._used -= getattr(job, "_need", 0)
@@ -213,8 +213,12 @@ def run_workload(jobs: list[tuple[float, Job]], slots: int = 4, hbm_cap: float =
     ok = [j for j in done if not j.error]
     lat = np.array([j.latency for j in ok]) if ok else np.zeros(1)
     rounds = sum(j.nrounds for j in ok)
+    errors: dict[str, int] = {}
+    for j in done:
+        if j.error:
+            errors[j.error] = errors.get(j.error, 0) + 1
     return {
-        "jobs": len(done), "failed": len(done) - len(ok),
+        "jobs": len(done), "failed": len(done) - len(ok), "errors": errors,
         "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
         "mean_s": float(lat.mean()), "pivots": rounds, "wall_s": t1 - t0,
         "pivots_per_s": rounds / (t1 - t0) if t1 > t0 else 0.0,
