@@ -102,6 +102,10 @@ struct BatchState {
   uint32_t *src = nullptr;
   double *prob = nullptr;
   uint32_t *r0 = nullptr;            // reach of each root
+  // src/prob/r0/cnt and the root rows are in lane order; lane_of[k] is the
+  // lane of the batch's k-th root (device copy for the stop test and update)
+  uint32_t *lane_of = nullptr;
+  std::vector<uint32_t> h_lane_of;
   uint32_t *check = nullptr;
   BCtl *ctl = nullptr, *h_ctl = nullptr;
   unsigned long long *h_cnt = nullptr;
@@ -1374,10 +1378,12 @@ __device__ __forceinline__ double bc_term(uint32_t v, int s, const uint32_t *__r
 
 template <int NW>
 __global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t n, const double *__restrict__ bc,
-                        double cn, int nb, const uint32_t *__restrict__ src, const double *__restrict__ prob,
-                        const uint32_t *__restrict__ r0, const double *__restrict__ dlt,
-                        const unsigned long long *__restrict__ cnt, unsigned long long *ok_bits) {
-  // ok_bits[j] bit s: every check vertex is satisfied after round s
+                        double cn, int nb, const uint32_t *__restrict__ lane_of, const uint32_t *__restrict__ src,
+                        const double *__restrict__ prob, const uint32_t *__restrict__ r0,
+                        const double *__restrict__ dlt, const unsigned long long *__restrict__ cnt,
+                        unsigned long long *ok_bits) {
+  // ok_bits[j] bit s: every check vertex is satisfied after root s (draw
+  // order; root s sits in lane lane_of[s])
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < x; i += gridDim.x * blockDim.x) {
     const uint32_t a = check[i];
     double b = a < n ? bc[a] : 0.0;
@@ -1387,7 +1393,7 @@ __global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t
     for (int s = 0; s < nb; ++s) {
       if (a < n) {
         bool has;
-        const double t = bc_term<NW>(a, s, src, prob, r0, dlt, cnt, &has);
+        const double t = bc_term<NW>(a, (int)lane_of[s], src, prob, r0, dlt, cnt, &has);
         if (has) b += t;
       }
       if (!(b / 2.0 < cn)) sat[s / 64] |= 1ull << (s % 64);
@@ -1397,65 +1403,70 @@ __global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t
   }
 }
 
-// bc += the batch's terms in root order, roots [0, nr).  A workgroup takes
-// kBlock vertices at a time.  The terms of kUpdRoots roots at a time are
-// computed with lanes over (vertex, root), so every wave reads whole
-// 128-byte segments of the delta rows, and staged in LDS; then each thread
-// folds its own vertex's terms in root order (the sequence of one-root
-// updates, bit for bit).
-constexpr int kUpdRoots = 16;
+// bc += the batch's terms in draw order, roots [0, nr).  A workgroup takes
+// kUpdVerts vertices at a time: the terms of all lanes are computed with
+// threads over (vertex, lane), so every wave reads whole 512-byte segments
+// of the delta rows, and staged in LDS; then one thread per vertex folds its
+// row in draw order through lane_of (the sequence of one-root updates, bit
+// for bit).
+constexpr int kUpdVerts = 32;
 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bupdate(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
                                                     const uint32_t *__restrict__ reach, int nr,
+                                                    const uint32_t *__restrict__ lane_of,
                                                     const uint32_t *__restrict__ src, const double *__restrict__ prob,
                                                     const uint32_t *__restrict__ r0, const double *__restrict__ dlt,
                                                     const unsigned long long *__restrict__ cnt, double *bc,
                                                     unsigned long long *kb) {
   constexpr int B = 64 * NW;
-  constexpr int G = kUpdRoots;
-  __shared__ double tile[kBlock][G + 1];
+  __shared__ double tile[kUpdVerts][B + 1];
   __shared__ double s_r1[B], s_p[B], s_root[B];
-  __shared__ uint32_t s_src[B];
-  __shared__ uint32_t s_want[kBlock];
-  for (int s = threadIdx.x; s < B; s += blockDim.x) {
-    if (s < nr) {
-      const uint32_t rr = r0[s];
-      s_src[s] = src[s];
-      s_r1[s] = (double)(uint32_t)(rr + 1u);
-      s_p[s] = prob[s];
+  __shared__ uint32_t s_src[B], s_lane[B];
+  __shared__ uint8_t s_use[B];
+  __shared__ uint8_t s_want[kUpdVerts];
+  for (int l = threadIdx.x; l < B; l += blockDim.x) s_use[l] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < B; k += blockDim.x) {
+    if (k < nr) {
+      const uint32_t l = lane_of[k];
+      const uint32_t rr = r0[l];
+      s_lane[k] = l;
+      s_use[l] = 1;
+      s_src[l] = src[l];
+      s_r1[l] = (double)(uint32_t)(rr + 1u);
+      s_p[l] = prob[l];
       // the root's own term (update_bc's v == v0 branch); absent when rr == 0
-      const uint32_t all = (uint32_t)cnt[4 * s + 1] + rr;
-      const unsigned long long nvt = cnt[4 * s + 0] + (unsigned long long)all;
-      s_root[s] = rr ? (double)((unsigned long long)rr * (nvt - 2ull)) : 0.0;
+      const uint32_t all = (uint32_t)cnt[4 * l + 1] + rr;
+      const unsigned long long nvt = cnt[4 * l + 0] + (unsigned long long)all;
+      s_root[l] = rr ? (double)((unsigned long long)rr * (nvt - 2ull)) : 0.0;
     }
   }
+  __syncthreads();
   unsigned long long bytes = 0;
-  for (uint32_t vb = blockIdx.x * kBlock; vb < n; vb += gridDim.x * kBlock) {
-    const uint32_t v = vb + threadIdx.x;
-    const bool want = v < n && (seen[v] == stamp || reach[v] != 0);
-    s_want[threadIdx.x] = want;
-    double acc = want ? bc[v] : 0.0;
-    bytes += v < n ? 8 + (want ? 16 + 8ull * (unsigned)nr : 0) : 0;
-    if (!__syncthreads_or(want)) continue;
-    for (int g = 0; g < nr; g += G) {
-#pragma unroll 4
-      for (int k = 0; k < G; ++k) {
-        const int flat = k * kBlock + threadIdx.x;
-        const int vi = flat / G, sl = flat % G, s = g + sl;
-        const uint32_t vv = vb + (uint32_t)vi;
-        double t = 0.0;
-        if (s < nr && s_want[vi]) t = vv == s_src[s] ? s_root[s] : dlt[(size_t)vv * B + s] * s_r1[s] / s_p[s];
-        tile[vi][sl] = t;
-      }
-      __syncthreads();
-      if (want) {
-        const int m = min(G, nr - g);
-        for (int sl = 0; sl < m; ++sl) acc += tile[threadIdx.x][sl];
-      }
-      __syncthreads();
+  for (uint32_t vb = blockIdx.x * kUpdVerts; vb < n; vb += gridDim.x * kUpdVerts) {
+    if (threadIdx.x < kUpdVerts) {
+      const uint32_t v = vb + threadIdx.x;
+      const bool want = v < n && (seen[v] == stamp || reach[v] != 0);
+      s_want[threadIdx.x] = want;
+      bytes += v < n ? 8 + (want ? 16 + 8ull * (unsigned)nr : 0) : 0;
     }
-    if (want) bc[v] = acc;
+    __syncthreads();
+    for (int flat = threadIdx.x; flat < kUpdVerts * B; flat += blockDim.x) {
+      const int vi = flat / B, l = flat % B;
+      const uint32_t vv = vb + (uint32_t)vi;
+      double t = 0.0;
+      if (s_want[vi] && s_use[l]) t = vv == s_src[l] ? s_root[l] : dlt[(size_t)vv * B + l] * s_r1[l] / s_p[l];
+      tile[vi][l] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < kUpdVerts && s_want[threadIdx.x]) {
+      const uint32_t v = vb + threadIdx.x;
+      double acc = bc[v];
+      for (int k = 0; k < nr; ++k) acc += tile[threadIdx.x][s_lane[k]];
+      bc[v] = acc;
+    }
+    __syncthreads();
   }
   if (kb) {
     bytes = wave_sum_u64(bytes);
@@ -1507,7 +1518,7 @@ void batch_free(axbc_ctx *c) {
   if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
-               b->check, b->ctl};
+               b->lane_of, b->check, b->ctl};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1607,6 +1618,7 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   BA(b->src, B);
   BA(b->prob, B);
   BA(b->r0, B);
+  BA(b->lane_of, B);
   BA(b->ctl, 1);
 #undef BA
   HIP_TRY(hipHostMalloc((void **)&b->h_ctl, sizeof(BCtl), hipHostMallocDefault));
@@ -1762,14 +1774,35 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   int rc;
   std::vector<uint32_t> r0(nb);
   for (int k = 0; k < nb; ++k) r0[k] = c->h_reach[v0s[k]];
+  // lanes: the roots by degree, highest first (ties in draw order).  Roots
+  // of similar degree have similar level structure, so the roots that meet
+  // at an edge of the sweep sit in fewer cache lines of the root-minor rows.
+  std::vector<uint32_t> root_at(nb);
+  for (int k = 0; k < nb; ++k) root_at[k] = (uint32_t)k;
+  std::stable_sort(root_at.begin(), root_at.end(), [&](uint32_t a, uint32_t b2) {
+    return c->h_col[v0s[a] + 1] - c->h_col[v0s[a]] > c->h_col[v0s[b2] + 1] - c->h_col[v0s[b2]];
+  });
+  std::vector<uint32_t> &lane_of = b->h_lane_of;
+  lane_of.resize(b->B);
+  for (int l = 0; l < b->B; ++l) lane_of[l] = (uint32_t)l;  // lanes past nb are unused
+  std::vector<uint32_t> src_l(nb), r0_l(nb);
+  std::vector<double> p_l(nb);
+  for (int l = 0; l < nb; ++l) {
+    const uint32_t k = root_at[l];
+    lane_of[k] = (uint32_t)l;
+    src_l[l] = v0s[k];
+    r0_l[l] = r0[k];
+    p_l[l] = ps[k];
+  }
   // split kernels only where some vertex has more edges than a chunk
   const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
   // the level loop runs from captured graphs unless per-kernel timing is on
   const bool graphs = c->use_graphs && !c->ktimer.on;
   if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
-  HIP_TRY(hipMemcpyAsync(b->src, v0s, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->prob, ps, sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->r0, r0.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->src, src_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->prob, p_l.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->r0, r0_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->lane_of, lane_of.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
   // reset the previous batch, then level 0
   if (b->have)
     hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, b->vis,
@@ -1820,11 +1853,11 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   HIP_TRY(hipMemcpyAsync(b->h_act, b->act, sizeof(uint64_t) * (size_t)(empty + 1) * NW, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   uint64_t rootmask[2] = {0, 0};
-  std::vector<int> nlev(nb, 0);
-  for (int k = 0; k < nb; ++k) {
+  std::vector<int> nlev(nb, 0);  // per lane
+  for (int l = 0; l < nb; ++l) {
     for (int lv = 1; lv <= dmax; ++lv)
-      if ((b->h_act[(size_t)lv * NW + k / 64] >> (k % 64)) & 1ull) nlev[k] = lv;
-    if (nlev[k] == 1) rootmask[k / 64] |= 1ull << (k % 64);
+      if ((b->h_act[(size_t)lv * NW + l / 64] >> (l % 64)) & 1ull) nlev[l] = lv;
+    if (nlev[l] == 1) rootmask[l / 64] |= 1ull << (l % 64);
   }
   // backward: depths dmax-1 .. dmin, dmin = 0 only for the roots with a
   // single level
@@ -1847,7 +1880,8 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     std::vector<unsigned long long> ones(NW, ~0ull);
     HIP_TRY(hipMemcpyAsync(okb, ones.data(), sizeof(unsigned long long) * NW, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_bstop<NW>, dim3(grid_for(c->ncheck, 64)), dim3(kBlock), 0, s, c->check, c->ncheck, n, c->bc,
-                       (double)((uint64_t)stop_c * (uint64_t)n), nb, b->src, b->prob, b->r0, b->dlt, b->cnt, okb);
+                       (double)((uint64_t)stop_c * (uint64_t)n), nb, b->lane_of, b->src, b->prob, b->r0, b->dlt,
+                       b->cnt, okb);
     unsigned long long okh[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(okh, okb, sizeof(unsigned long long) * NW, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1858,14 +1892,15 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       }
   }
   kt_begin(c, KK_UPDATE);
-  hipLaunchKernelGGL(k_bupdate<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, nr, b->src,
-                     b->prob, b->r0, b->dlt, b->cnt, c->bc, kt_bytes(c, KK_UPDATE));
+  hipLaunchKernelGGL(k_bupdate<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, nr,
+                     b->lane_of, b->src, b->prob, b->r0, b->dlt, b->cnt, c->bc, kt_bytes(c, KK_UPDATE));
   kt_end(c, KK_UPDATE);
   HIP_TRY(hipGetLastError());
   // the context's delta becomes the last applied root's (the -z 4
   // distribution of the next draw); plain rounds re-initialise it
   if (nr > 0) {
-    hipLaunchKernelGGL(k_bcopy_delta<NW>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, nr - 1, b->dlt, c->delta);
+    hipLaunchKernelGGL(k_bcopy_delta<NW>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (int)lane_of[nr - 1], b->dlt,
+                       c->delta);
     c->dirty_delta = true;
   }
   if (applied) *applied = nr;
@@ -1873,11 +1908,12 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     HIP_TRY(hipMemcpyAsync(b->h_cnt, b->cnt, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int k = 0; k < nb; ++k) {
-      stats[k].nvisited = b->h_cnt[4 * k + 0];
-      stats[k].nlevels = (uint64_t)nlev[k];
-      stats[k].e_r = b->h_cnt[4 * k + 2];
-      stats[k].e_dag = b->h_cnt[4 * k + 3];
-      stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * k + 1] * (c->heuristic == 1 || c->heuristic == 3) + r0[k]);
+      const uint32_t l = lane_of[k];
+      stats[k].nvisited = b->h_cnt[4 * l + 0];
+      stats[k].nlevels = (uint64_t)nlev[l];
+      stats[k].e_r = b->h_cnt[4 * l + 2];
+      stats[k].e_dag = b->h_cnt[4 * l + 3];
+      stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * l + 1] * (c->heuristic == 1 || c->heuristic == 3) + r0[k]);
     }
   }
   return AXBC_OK;
@@ -1972,8 +2008,10 @@ int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight
       hipLaunchKernelGGL(k_butility<2>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, dout);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+  std::vector<double> per_lane(B);
+  HIP_TRY(hipMemcpyAsync(per_lane.data(), dout, sizeof(double) * B, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  for (int k = 0; k < B; ++k) out[k] = per_lane[b->h_lane_of.empty() ? k : b->h_lane_of[k]];
   (void)hipFree(dv);
   (void)hipFree(dw);
   (void)hipFree(dout);
@@ -1983,12 +2021,15 @@ int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight
 int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
   if (!c || !c->batch || !delta || k < 0 || k >= c->batch->B)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  const int lane = c->batch->h_lane_of.empty() ? k : (int)c->batch->h_lane_of[k];
   double *tmp = nullptr;
   HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * c->n));
   if (c->batch->nw == 1)
-    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, k, c->batch->dlt, tmp);
+    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, c->batch->dlt,
+                       tmp);
   else
-    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, k, c->batch->dlt, tmp);
+    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, c->batch->dlt,
+                       tmp);
   HIP_TRY(hipMemcpyAsync(delta, tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipFree(tmp));
