@@ -48,9 +48,14 @@ namespace axbc {
 
 constexpr uint32_t kChunkMin = 64;  // smallest edges-per-work-item for split vertices
 #ifndef AXBC_BURST
-#define AXBC_BURST 8
+#define AXBC_BURST 2
 #endif
 constexpr int kBurst = AXBC_BURST;  // row loads in flight per wave in the pull loops
+// Waves per SIMD asked of the compiler for the frontier kernels (a 64-VGPR
+// budget): these loops are bound by memory latency, and 8 resident waves
+// with 2 row loads each beat 4 waves with 8 (measured on SCALE 22, 128
+// roots); k_bclose keeps the compiler's choice.
+constexpr int kWavesPerSimd = 8;
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
 constexpr int kGraphSteps = 8;    // levels (depths) per captured graph
 
@@ -254,7 +259,7 @@ __global__ void k_bfill(uint32_t n, const uint32_t *__restrict__ reach, uint64_t
 // Undo the previous batch on the vertices it reached (seen == its stamp),
 // each once: coalesced row stores, a wave per 64 vertices.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_breset(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_breset(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
                                                    const uint32_t *__restrict__ reach, uint64_t *vis,
                                                    uint64_t *fr0, uint64_t *fr1, uint32_t *sig, double *dlt) {
   constexpr int B = 64 * NW;
@@ -374,7 +379,7 @@ __device__ __forceinline__ bool push_edge(uint32_t w, const Words<NW> &f, const 
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                   const uint64_t *__restrict__ vis, uint64_t *nx, uint32_t *mark,
                                                   uint64_t *fr0, uint64_t *fr1, uint32_t *q,
                                                   const uint64_t *__restrict__ qw, const uint32_t *__restrict__ qoff,
@@ -455,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_bpush(const uint32_t *__restrict__ c
 
 // Frontier entries with more than kChunk edges, kChunk edges per item.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bpush_split(const uint32_t *__restrict__ col,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush_split(const uint32_t *__restrict__ col,
                                                         const uint32_t *__restrict__ row,
                                                         const uint64_t *__restrict__ vis, uint64_t *nx,
                                                         uint32_t *mark, uint32_t *q, const uint64_t *__restrict__ qw,
@@ -641,7 +646,7 @@ __device__ __forceinline__ void group_sigma(uint32_t st, uint32_t d, const Words
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
                                                    const uint64_t *__restrict__ nx, const uint64_t *fr0,
                                                    const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
@@ -705,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma(const uint32_t *__restrict__ 
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restrict__ rcol,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const uint32_t *__restrict__ rcol,
                                                          const uint32_t *__restrict__ rrow,
                                                          const uint64_t *__restrict__ nx, const uint64_t *fr0,
                                                          const uint64_t *fr1, uint32_t *sig,
@@ -774,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_split(const uint32_t *__restr
 // appends the new ones with one atomic.  Vertices with more than kChunk
 // in-edges go to k_bpull_split.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bpull(uint32_t n, const uint32_t *__restrict__ rcol,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, const uint32_t *__restrict__ rcol,
                                                   const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
                                                   uint64_t *nx, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
                                                   uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
@@ -874,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_bpull(uint32_t n, const uint32_t *__
 // and roots found are added atomically; the chunk that first makes NX[w]
 // non-empty appends w.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bpull_split(const uint32_t *__restrict__ rcol,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uint32_t *__restrict__ rcol,
                                                         const uint32_t *__restrict__ rrow,
                                                         const uint64_t *__restrict__ vis, uint64_t *nx,
                                                         const uint64_t *fr0, const uint64_t *fr1, uint32_t *sig,
@@ -1168,7 +1173,7 @@ __device__ __forceinline__ uint32_t dep_chunk(uint32_t st, uint32_t cb, uint32_t
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                  const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
                                                  double *dlt, double *xr, const uint32_t *__restrict__ reach,
                                                  const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
@@ -1224,7 +1229,7 @@ __global__ __launch_bounds__(kBlock) void k_bdep(const uint32_t *__restrict__ co
 // Chunks of the split entries: per item the partial sums of its kChunk
 // edges (left fold from 0.0) go to its row of `part`.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                        const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
                                                        const double *__restrict__ xr, const uint32_t *__restrict__ q,
                                                        const uint64_t *__restrict__ qw, const axbc::BCtl *ctl,
@@ -1277,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) void k_bdep_split(const uint32_t *__restric
 
 // One wave per split entry: delta = reach + partials in chunk order.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bdep_combine(const uint32_t *__restrict__ col,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_combine(const uint32_t *__restrict__ col,
                                                          const uint32_t *__restrict__ sig, double *dlt, double *xr,
                                                          const uint32_t *__restrict__ reach,
                                                          const uint32_t *__restrict__ q,
@@ -1412,7 +1417,7 @@ __global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t
 constexpr int kUpdVerts = 32;
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bupdate(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bupdate(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
                                                     const uint32_t *__restrict__ reach, int nr,
                                                     const uint32_t *__restrict__ lane_of,
                                                     const uint32_t *__restrict__ src, const double *__restrict__ prob,
@@ -1477,7 +1482,7 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(uint32_t n, const uint32_t *
 // out[s] += sum over the target vertices of weight * delta_s(v): a wave per
 // vertex, lane = root (coalesced rows), one FP64 atomic per root and wave.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_butility(const uint32_t *__restrict__ vert, const double *__restrict__ wt,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_butility(const uint32_t *__restrict__ vert, const double *__restrict__ wt,
                                                      uint32_t nv, const double *__restrict__ dlt, double *out) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
