@@ -90,6 +90,12 @@ def hip() -> C.CDLL:
         "axbc_ingest": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_void_p,
                                  C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint32)]),
+        "axbc_kronecker_generate": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
+                                              C.POINTER(C.c_double), C.c_void_p]),
+        "axbc_kronecker_ingest": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
+                                            C.POINTER(C.c_double), C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint32)]),
         "axbc_set_batch_graphs": (C.c_int, [ctxp, C.c_int]),
         "axbc_batch_delta": (C.c_int, [ctxp, C.c_int, f64p]),
         "axbc_hdn_build": (C.c_int, [ctxp, u32p, u32p, C.POINTER(C.c_uint32)]),

@@ -150,6 +150,10 @@ void sampler_free(axbc_ctx *c);
 void batch_free(axbc_ctx *c);
 void mlp_free(axbc_ctx *c);
 int grid_for(uint64_t n, int cap = 2048);
+// Kronecker generator on the device (kronecker.hip): nedges packed
+// (min << 32 | max) edges of 2^scale vertices into edges on stream s.
+int kronecker_device_edges(hipStream_t s, int scale, uint64_t nedges, uint64_t seed1, uint64_t seed2,
+                           const double initiator[4], uint64_t *edges);
 // Event pair around one launch of kind k when instrumentation is on.
 void kt_begin(axbc_ctx *c, int kind);
 void kt_end(axbc_ctx *c, int kind);

@@ -241,40 +241,39 @@ int normalise(Ingest &in, DevBufs &m, uint64_t *&buf, uint64_t *&spare, uint64_t
   return in.select(spare, m.f1, buf, n, &n, m.dnum);
 }
 
-}  // namespace
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  ~StreamGuard() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
 
-extern "C" {
-
-int axbc_ingest(int device, const uint64_t *edges, uint64_t ned, uint64_t N, int undirected, int heuristic,
-                uint32_t *col, uint32_t *row, uint64_t row_cap, uint32_t *reach, uint64_t *ned_out, uint32_t *rem_ed) {
-  if (!col || !reach || !ned_out || (ned && !edges) || N == 0 || N > 0xFFFFFFFFull)
-    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
-  if (heuristic < 0 || heuristic > 1) return fail(AXBC_EINVAL, "heuristic must be 0 or 1%s (line %d)", "", __LINE__);
+// Stream, key width and buffers for ned input edges of N vertices.
+int ingest_setup(int device, uint64_t ned, uint64_t N, int undirected, Ingest &in, StreamGuard &guard, DevBufs &m) {
   const uint64_t cap = undirected ? 2 * ned : ned;
   if (cap > 0x7FFFFFFFull) return fail(AXBC_EINVAL, "too many edges for one pass%s (line %d)", "", __LINE__);
-  for (uint64_t i = 0; i < ned; ++i)
-    if ((edges[i] >> 32) >= N || (edges[i] & 0xFFFFFFFFull) >= N)
-      return fail(AXBC_EINVAL, "edge endpoint out of range%s (line %d)", "", __LINE__);
   HIP_TRY(hipSetDevice(device));
-  Ingest in;
   HIP_TRY(hipStreamCreateWithFlags(&in.s, hipStreamNonBlocking));
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
-  } guard{in.s};
+  guard.s = in.s;
   int bits = 0;
   while (bits < 32 && (N - 1) >> bits) ++bits;
   in.end_bit = 32 + std::max(bits, 1);  // keys are below N << 32
-  DevBufs m;
   int rc;
   if ((rc = dev_alloc(&m.a, cap)) || (rc = dev_alloc(&m.b, cap)) || (rc = dev_alloc(&m.c, cap)) ||
       (rc = dev_alloc(&m.d, cap)) || (rc = dev_alloc(&m.dnum, 1)) || (rc = dev_alloc(&m.f1, cap)) ||
       (rc = dev_alloc(&m.f2, cap)) || (rc = dev_alloc(&m.reach, N)))
     return rc;
+  HIP_TRY(hipMemsetAsync(m.reach, 0, sizeof(uint32_t) * N, in.s));
+  return AXBC_OK;
+}
+
+// The preprocessing of the ned edges in m.a (which holds room for their
+// mirror); results into the host arrays.
+int ingest_run(Ingest &in, DevBufs &m, uint64_t ned, uint64_t N, int undirected, int heuristic, uint32_t *col,
+               uint32_t *row, uint64_t row_cap, uint32_t *reach, uint64_t *ned_out, uint32_t *rem_ed) {
+  int rc;
   hipStream_t s = in.s;
-  HIP_TRY(hipMemsetAsync(m.reach, 0, sizeof(uint32_t) * N, s));
   uint64_t n = ned;
-  if (ned) HIP_TRY(hipMemcpyAsync(m.a, edges, sizeof(uint64_t) * ned, hipMemcpyHostToDevice, s));
   uint64_t *buf = m.a, *spare = m.b;
   // mirror: the reversed non-loop edges after the originals
   if (undirected && ned) {
@@ -337,6 +336,44 @@ int axbc_ingest(int device, const uint64_t *edges, uint64_t ned, uint64_t N, int
   *ned_out = n;
   if (rem_ed) *rem_ed = (uint32_t)pnod;
   return AXBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int axbc_ingest(int device, const uint64_t *edges, uint64_t ned, uint64_t N, int undirected, int heuristic,
+                uint32_t *col, uint32_t *row, uint64_t row_cap, uint32_t *reach, uint64_t *ned_out, uint32_t *rem_ed) {
+  if (!col || !reach || !ned_out || (ned && !edges) || N == 0 || N > 0xFFFFFFFFull)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  if (heuristic < 0 || heuristic > 1) return fail(AXBC_EINVAL, "heuristic must be 0 or 1%s (line %d)", "", __LINE__);
+  for (uint64_t i = 0; i < ned; ++i)
+    if ((edges[i] >> 32) >= N || (edges[i] & 0xFFFFFFFFull) >= N)
+      return fail(AXBC_EINVAL, "edge endpoint out of range%s (line %d)", "", __LINE__);
+  Ingest in;
+  StreamGuard guard;
+  DevBufs m;
+  int rc;
+  if ((rc = ingest_setup(device, ned, N, undirected, in, guard, m))) return rc;
+  if (ned) HIP_TRY(hipMemcpyAsync(m.a, edges, sizeof(uint64_t) * ned, hipMemcpyHostToDevice, in.s));
+  return ingest_run(in, m, ned, N, undirected, heuristic, col, row, row_cap, reach, ned_out, rem_ed);
+}
+
+int axbc_kronecker_ingest(int device, int scale, uint64_t nedges, uint64_t seed1, uint64_t seed2,
+                          const double initiator[4], int undirected, int heuristic, uint32_t *col, uint32_t *row,
+                          uint64_t row_cap, uint32_t *reach, uint64_t *ned_out, uint32_t *rem_ed) {
+  if (!initiator || !col || !reach || !ned_out || scale < 1 || scale > 30)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  if (heuristic < 0 || heuristic > 1) return fail(AXBC_EINVAL, "heuristic must be 0 or 1%s (line %d)", "", __LINE__);
+  const uint64_t N = 1ull << scale;
+  Ingest in;
+  StreamGuard guard;
+  DevBufs m;
+  int rc;
+  if ((rc = ingest_setup(device, nedges, N, undirected, in, guard, m))) return rc;
+  // the generated edges go straight into the first buffer
+  if ((rc = kronecker_device_edges(in.s, scale, nedges, seed1, seed2, initiator, m.a))) return rc;
+  return ingest_run(in, m, nedges, N, undirected, heuristic, col, row, row_cap, reach, ned_out, rem_ed);
 }
 
 }  // extern "C"

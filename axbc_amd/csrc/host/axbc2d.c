@@ -69,7 +69,7 @@ static void usage(const char *p) {
           "\t\t -U             DO NOT make graph Undirected\n"
           "\t\t -R <0|1>       fixed / time-based seed\n"
           "\t\t -B <0|64|128>  roots per GPU batch\n"
-          "\t\t -g             graph preprocessing on the GPU\n",
+          "\t\t -g             graph generation and preprocessing on the GPU\n",
           p);
   exit(EXIT_FAILURE);
 }
@@ -89,6 +89,26 @@ static int device_ingest(uint64_t *ed, uint64_t ned, uint64_t N, int undirected,
   free(ed);
   g->ned = out;
   g->generated = ned;
+  return rc;
+}
+
+/* Generation and preprocessing on the device (axbc_kronecker_ingest): the
+ * graph of axbc_graph_kronecker without the edges leaving the GPU. */
+static int device_generate(int scale, int edgef, uint64_t s1, uint64_t s2, int undirected, int heuristic,
+                           axbc_graph *g) {
+  const double ini[4] = {.57, .19, .19, .05};
+  const uint64_t N = (uint64_t)1 << scale, m = N * (uint64_t)edgef;
+  memset(g, 0, sizeof(*g));
+  g->N = N;
+  g->col = (uint32_t *)malloc(sizeof(uint32_t) * (N + 1));
+  g->row = (uint32_t *)malloc(sizeof(uint32_t) * (2 * m + 1));
+  g->reach = (uint32_t *)malloc(sizeof(uint32_t) * N);
+  if (!g->col || !g->row || !g->reach) die("%s", "host allocation failed");
+  uint64_t out = 0;
+  const int rc = axbc_kronecker_ingest(0, scale, m, s1, s2, ini, undirected, heuristic, g->col, g->row, 2 * m + 1,
+                                       g->reach, &out, &g->rem_ed);
+  g->ned = out;
+  g->generated = m;
   return rc;
 }
 
@@ -252,15 +272,9 @@ int main(int argc, char **argv) {
       die("%s", "graph construction failed");
     }
   } else if (device_build) {
-    const double ini[4] = {.57, .19, .19, .05};
-    const int64_t m = ((int64_t)1 << scale) * edgef;
-    uint64_t *pairs = axbc_kronecker_edges(scale, m, s1, s2, ini, nthreads);
-    if (!pairs) die("%s", "generator allocation failed");
-    uint64_t *ed = axbc_pack_pairs(pairs, (uint64_t)m);
-    free(pairs);
-    if (device_ingest(ed, (uint64_t)m, N, undirected, heuristic, &g))
+    if (device_generate(scale, edgef, s1, s2, undirected, heuristic, &g))
       die("device graph construction failed: %s", axbc_last_error());
-    generated = (uint64_t)m;
+    generated = g.generated;
   } else {
     if (axbc_graph_kronecker(scale, edgef, s1, s2, undirected, heuristic, nthreads, &g))
       die("%s", "graph construction failed");

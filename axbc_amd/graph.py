@@ -115,12 +115,22 @@ class Graph:
     @staticmethod
     def kronecker_device(scale: int, edgefactor: int = 16, seed: tuple[int, int] = (1, 2),
                          undirected: bool = True, heuristic: int = 0, device: int = 0) -> "Graph":
-        """Graph.kronecker with the preprocessing on the device: the host
-        generator's edges, then axbc_ingest."""
-        e = kronecker_edge_list(scale, (1 << scale) * edgefactor, seed[0], seed[1]).astype(np.uint64)
-        keys = (e[:, 0] << np.uint64(32)) | e[:, 1]
-        return Graph.from_keys_device(1 << scale, keys, undirected, heuristic, device,
-                                      f"kron-s{scale}-ef{edgefactor}-seed{seed[0]}")
+        """Graph.kronecker built on the device: generation and preprocessing
+        (axbc_kronecker_ingest), the same graph as the host pipeline's."""
+        N, m = 1 << scale, (1 << scale) * edgefactor
+        col = np.empty(N + 1, np.uint32)
+        row = np.empty(max(2 * m, 1), np.uint32)
+        reach = np.empty(N, np.uint32)
+        ned, rem = C.c_uint64(0), C.c_uint32(0)
+        ini = (C.c_double * 4)(*INITIATOR)
+        L = _ffi.hip()
+        rc = L.axbc_kronecker_ingest(device, scale, m, seed[0], seed[1], ini, int(undirected), heuristic,
+                                     col.ctypes.data, row.ctypes.data, row.shape[0], reach.ctypes.data,
+                                     C.byref(ned), C.byref(rem))
+        if rc != 0:
+            raise RuntimeError(L.axbc_last_error().decode())
+        return Graph(N, col, row[:ned.value].copy(), reach, heuristic, m, int(rem.value),
+                     f"kron-s{scale}-ef{edgefactor}-seed{seed[0]}")
 
     @staticmethod
     def read(path: str, N: int, undirected: bool = True, heuristic: int = 0) -> "Graph":
@@ -152,6 +162,17 @@ class Graph:
         if rc != 0:
             raise RuntimeError("graph construction failed")
         return Graph._from_c(g, heuristic, f"lattice-{side}")
+
+
+def kronecker_keys_device(scale: int, nedges: int, seed1: int, seed2: int, device: int = 0) -> np.ndarray:
+    """The generator run on the device: packed keys (min << 32 | max) in the
+    host generator's order (axbc_kronecker_generate)."""
+    out = np.empty(max(nedges, 1), np.uint64)
+    ini = (C.c_double * 4)(*INITIATOR)
+    L = _ffi.hip()
+    if L.axbc_kronecker_generate(device, scale, nedges, seed1, seed2, ini, out.ctypes.data) != 0:
+        raise RuntimeError(L.axbc_last_error().decode())
+    return out[:nedges]
 
 
 def kronecker_edge_list(scale: int, nedges: int, seed1: int, seed2: int,

@@ -241,6 +241,22 @@ int axbc_ingest(int device, const uint64_t *edges, uint64_t ned, uint64_t N, int
                 uint32_t *rem_ed);
 
 /*
+ * Kronecker generation on the device (no reference device counterpart; the
+ * host generator is generator/make_graph.c:271-353 as called from
+ * src/axbc2d.c:518-527).  axbc_kronecker_generate writes the nedges edges
+ * of 2^scale vertices (scale 1..30) for the seed pair and the initiator
+ * (A, B, C, D) to edges[] as packed keys (min << 32 | max), in the same
+ * positions as the host generator's output.  axbc_kronecker_ingest
+ * generates on the device and runs axbc_ingest's preprocessing on the
+ * result without a host round trip (same outputs as axbc_ingest).
+ */
+int axbc_kronecker_generate(int device, int scale, uint64_t nedges, uint64_t seed1, uint64_t seed2,
+                            const double initiator[4], uint64_t *edges);
+int axbc_kronecker_ingest(int device, int scale, uint64_t nedges, uint64_t seed1, uint64_t seed2,
+                          const double initiator[4], int undirected, int heuristic, uint32_t *col, uint32_t *row,
+                          uint64_t row_cap, uint32_t *reach, uint64_t *ned_out, uint32_t *rem_ed);
+
+/*
  * Learned pivot proposal (no reference counterpart; stands in for the
  * bc/delta heuristics -z 3/4): a 4 -> 64 -> 64 -> 1 MLP with ReLU, BF16
  * weights and FP32 accumulation, over per-vertex features (log(1+deg), LCC,
