@@ -171,7 +171,7 @@ def learned_case(args):
     from axbc_amd.pivots import PivotLoop
     out = {"case": "learned-vs-uniform-4096", "runs": []}
     for gseed in ((4, 5), (6, 7)):
-        g = Graph.kronecker(18, 16, seed=gseed, heuristic=1)
+        g = Graph.kronecker_device(18, 16, seed=gseed, heuristic=1)
         with Engine(g) as e:
             t = time.perf_counter()
             ex = exact_bc(e, batch=128)
@@ -196,7 +196,7 @@ def learned_case(args):
     out.update({"top1pct_recall_learned": round(float(np.mean(rl)), 4), "top1pct_recall_uniform": round(float(np.mean(ru)), 4),
                 "spearman_learned": round(float(np.mean(sl)), 5), "spearman_uniform": round(float(np.mean(su)), 5)})
     t = time.perf_counter()
-    g = Graph.kronecker(24, 16, seed=(4, 5), heuristic=1)
+    g = Graph.kronecker_device(24, 16, seed=(4, 5), heuristic=1)
     log(f"SCALE 24 built in {time.perf_counter() - t:.1f}s (N={g.N} ned={g.ned})")
     with Engine(g) as e:
         info = run_learned(e, budget=4096, warm=512, batch=64)
@@ -275,7 +275,7 @@ def main():
         return
 
     t0 = time.perf_counter()
-    g = Graph.kronecker(args.scale, 16, seed=(2, 3), heuristic=1)
+    g = Graph.kronecker_device(args.scale, 16, seed=(2, 3), heuristic=1)
     log(f"graph: N={g.N} ned={g.ned} built in {time.perf_counter() - t0:.1f}s")
     e = Engine(g)
     log(f"device bytes {e.device_bytes >> 20} MB")
