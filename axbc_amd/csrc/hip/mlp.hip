@@ -43,6 +43,9 @@ struct MlpBufs {
   float *grad = nullptr, *m = nullptr, *v = nullptr;
   float *loss = nullptr;
   double *prop = nullptr;    // N proposal weights
+  double *tw = nullptr;      // N target weights (feature 1 while targets are set)
+  double *tred = nullptr;    // per-block (sum, count) of the target feature, then its mean
+  bool targets = false;
   uint32_t *idx = nullptr;
   uint32_t idx_cap = 0;
   int step = 0;
@@ -53,11 +56,13 @@ MlpBufs *bufs_of(axbc_ctx *c) { return reinterpret_cast<MlpBufs *>(c->mlp); }
 
 __device__ __forceinline__ float bf(const __hip_bfloat16 *w, int i) { return __bfloat162float(w[i]); }
 
-// Features of vertex v: log(1 + degree), LCC, log(1 + bc * scale) and
-// log(1 + mean neighbour bc * scale).  Thread per vertex; vertices with
-// more than 64 edges are summed by the whole wave.
+// Features of vertex v: log(1 + degree), LCC (or, while targets are set,
+// the target sum t(v) = sum of tw over v's neighbours, normalised below),
+// log(1 + bc * scale) and log(1 + mean neighbour bc * scale).  Thread per
+// vertex; vertices with more than 64 edges are summed by the whole wave.
 __global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
-                           const float *__restrict__ lcc, const double *__restrict__ bc, double bc_scale, float *x) {
+                           const float *__restrict__ lcc, const double *__restrict__ bc, double bc_scale,
+                           const double *__restrict__ tw, float *x) {
   const int lane = threadIdx.x & 63;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
@@ -68,28 +73,93 @@ __global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const u
       e = col[v + 1];
     }
     const bool big = e - b > 64;
-    double nsum = 0.0;
+    double nsum = 0.0, tsum = 0.0;
     if (!big)
-      for (uint32_t k = b; k < e; ++k) nsum += bc[row[k]];
+      for (uint32_t k = b; k < e; ++k) {
+        nsum += bc[row[k]];
+        if (tw) tsum += tw[row[k]];
+      }
     unsigned long long todo = __ballot(big);
     while (todo) {
       const int i = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
       const uint32_t bb = __shfl(b, i), ee = __shfl(e, i);
-      double t = 0.0;
-      for (uint32_t k = bb + (uint32_t)lane; k < ee; k += 64) t += bc[row[k]];
+      double t = 0.0, u = 0.0;
+      for (uint32_t k = bb + (uint32_t)lane; k < ee; k += 64) {
+        t += bc[row[k]];
+        if (tw) u += tw[row[k]];
+      }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
-      if (lane == i) nsum = t;
+      for (int off = 32; off > 0; off >>= 1) {
+        t += __shfl_xor(t, off);
+        u += __shfl_xor(u, off);
+      }
+      if (lane == i) {
+        nsum = t;
+        tsum = u;
+      }
     }
     if (v < n) {
       const uint32_t d = e - b;
       x[4 * v + 0] = log1pf((float)d);
-      x[4 * v + 1] = lcc ? lcc[v] : 0.0f;
+      x[4 * v + 1] = tw ? (float)tsum : lcc ? lcc[v] : 0.0f;
       x[4 * v + 2] = (float)log1p(bc[v] * bc_scale);
       x[4 * v + 3] = d ? (float)log1p(nsum / (double)d * bc_scale) : 0.0f;
     }
   }
+}
+
+constexpr int kRedBlocks = 256;
+
+// Per block, the sum and count of the positive target features, reduced in
+// a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_target_partials(uint32_t n, const float *__restrict__ x, double *part) {
+  __shared__ double ss[256], sc[256];
+  double a = 0.0, c = 0.0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const float t = x[4 * v + 1];
+    if (t > 0.f) {
+      a += (double)t;
+      c += 1.0;
+    }
+  }
+  ss[threadIdx.x] = a;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      ss[threadIdx.x] += ss[threadIdx.x + h];
+      sc[threadIdx.x] += sc[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ss[0];
+    part[2 * blockIdx.x + 1] = sc[0];
+  }
+}
+
+// part[2 kRedBlocks] = mean of the positive target features (1 if none).
+__global__ void k_target_mean(double *part) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0.0, c = 0.0;
+  for (int i = 0; i < kRedBlocks; ++i) {
+    a += part[2 * i];
+    c += part[2 * i + 1];
+  }
+  part[2 * kRedBlocks] = c > 0.0 ? a / c : 1.0;
+}
+
+// Target feature: log(1 + t / mean positive t).
+__global__ void k_target_norm(uint32_t n, const double *__restrict__ part, float *x) {
+  const double mean = part[2 * kRedBlocks];
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    x[4 * v + 1] = (float)log1p((double)x[4 * v + 1] / mean);
+}
+
+__global__ void k_scatter_targets(uint32_t m, const uint32_t *__restrict__ vert, const double *__restrict__ w,
+                                  double *tw) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) tw[vert[i]] = w[i];
 }
 
 // weights of the workgroup in LDS as FP32
@@ -283,7 +353,7 @@ namespace axbc {
 void mlp_free(axbc_ctx *c) {
   MlpBufs *b = bufs_of(c);
   if (!b) return;
-  void *d[] = {b->x, b->y, b->score, b->master, b->wb, b->grad, b->m, b->v, b->loss, b->idx, b->prop};
+  void *d[] = {b->x, b->y, b->score, b->master, b->wb, b->grad, b->m, b->v, b->loss, b->idx, b->prop, b->tw, b->tred};
   for (void *p : d)
     if (p) (void)hipFree(p);
   delete b;
@@ -331,9 +401,44 @@ int axbc_mlp_features(axbc_ctx *c, double bc_scale) {
   MlpBufs *b = c ? bufs_of(c) : nullptr;
   if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
   const float *lcc = c->sampler && c->sampler->lcc_ready ? c->sampler->lcc : nullptr;
+  const double *tw = b->targets ? b->tw : nullptr;
   hipLaunchKernelGGL(k_features, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, c->row, lcc, c->bc,
-                     bc_scale, b->x);
+                     bc_scale, tw, b->x);
+  if (tw) {
+    hipLaunchKernelGGL(k_target_partials, dim3(kRedBlocks), dim3(256), 0, c->stream, c->n, b->x, b->tred);
+    hipLaunchKernelGGL(k_target_mean, dim3(1), dim3(64), 0, c->stream, b->tred);
+    hipLaunchKernelGGL(k_target_norm, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, b->tred, b->x);
+  }
   HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+int axbc_mlp_targets(axbc_ctx *c, const uint32_t *vertex, const double *weight, uint32_t count) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  if (count && (!vertex || !weight)) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  for (uint32_t i = 0; i < count; ++i)
+    if (vertex[i] >= c->n) return fail(AXBC_EINVAL, "vertex out of range%s (line %d)", "", __LINE__);
+  b->targets = count > 0;
+  if (!count) return AXBC_OK;
+  hipStream_t s = c->stream;
+  if (!b->tw) {
+    HIP_TRY(hipMalloc((void **)&b->tw, sizeof(double) * c->n));
+    HIP_TRY(hipMalloc((void **)&b->tred, sizeof(double) * (2 * kRedBlocks + 1)));
+    c->bytes += 8ull * c->n;
+  }
+  uint32_t *dv = nullptr;
+  double *dw = nullptr;
+  HIP_TRY(hipMalloc((void **)&dv, sizeof(uint32_t) * count));
+  HIP_TRY(hipMalloc((void **)&dw, sizeof(double) * count));
+  HIP_TRY(hipMemsetAsync(b->tw, 0, sizeof(double) * c->n, s));
+  HIP_TRY(hipMemcpyAsync(dv, vertex, sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dw, weight, sizeof(double) * count, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_scatter_targets, dim3(grid_of(count)), dim3(256), 0, s, count, dv, dw, b->tw);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  (void)hipFree(dv);
+  (void)hipFree(dw);
   return AXBC_OK;
 }
 

@@ -1,11 +1,14 @@
 """Learned pivot proposal (the north star's third pattern).
 
-A small MLP over per-vertex features (log degree, LCC, running bc, mean
-neighbour bc) proposes pivots in place of the bc/delta heuristics.  It is
-trained on the method's own output: the first `warm` pivots are uniform.
-The bc estimate of the first quarter of them (the pilot) ranks the
+A small MLP over per-vertex features (log degree, target adjacency, running
+bc, mean neighbour bc) proposes pivots in place of the bc/delta heuristics.
+It is trained on the method's own output: the first `warm` pivots are
+uniform.  The bc estimate of the first quarter of them (the pilot) ranks the
 vertices; the vertices around the top-1% boundary (ranks k/2 .. 2k), each
-weighted by 1/bc, are the targets.  For every later warm-up root, its
+weighted by 1/bc, are the targets.  Target adjacency is the sum of those
+weights over a vertex's neighbours (normalised by its mean): a root next to
+boundary vertices sends many of its shortest paths through them (the LCC
+takes its place when there are no targets).  For every later warm-up root, its
 utility -- its share of the targets' estimates, (reach+1) *
 sum_v delta_root(v) / bc(v), read from the batch's per-root delta rows on
 the device -- is the training label, as log(1 + utility / mean).  Sampling
@@ -49,6 +52,7 @@ def _hip():
     for name, args in {
         "axbc_mlp_init": [C.c_void_p, C.c_void_p],
         "axbc_mlp_features": [C.c_void_p, C.c_double],
+        "axbc_mlp_targets": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32],
         "axbc_mlp_set_data": [C.c_void_p, C.c_void_p, C.c_void_p],
         "axbc_mlp_get_data": [C.c_void_p, C.c_void_p, C.c_void_p],
         "axbc_mlp_gradient": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p],
@@ -78,6 +82,11 @@ class Mlp:
 
     def features(self, bc_scale: float):
         _check(self.L.axbc_mlp_features(self.e.ctx, bc_scale))
+
+    def targets(self, vertex: np.ndarray, weight: np.ndarray):
+        v = np.ascontiguousarray(vertex, np.uint32)
+        w = np.ascontiguousarray(weight, np.float64)
+        _check(self.L.axbc_mlp_targets(self.e.ctx, v.ctypes.data, w.ctypes.data, len(v)))
 
     def get(self):
         x = np.empty((self.e.N, 4), np.float32)
@@ -130,7 +139,8 @@ class StaticLoop(PivotLoop):
 
 
 def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64, eps: float = 1.0,
-                steps: int = 300, lr: float = 3e-3, seed: int = 4343543, top: float = 0.01) -> dict:
+                steps: int = 300, lr: float = 3e-3, seed: int = 4343543, top: float = 0.01,
+                target_feature: bool = True) -> dict:
     """The learned loop; returns timing, training log and whether it fell
     back to uniform.  bc on the engine holds the estimate afterwards."""
     t0 = time.perf_counter()
@@ -162,9 +172,13 @@ def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64,
             util[int(r)] = float(out[j]) * float(r1[r]) if len(tv) else 0.0
     bc_w = e.bc()
     t_warm = time.perf_counter() - t0
-    _check(L.axbc_lcc_build(e.ctx, None, None, None, C.byref(C.c_uint64(0))))
+    use_targets = target_feature and len(tv) > 0
+    if not use_targets:
+        _check(L.axbc_lcc_build(e.ctx, None, None, None, C.byref(C.c_uint64(0))))
     mlp = Mlp(e, seed=seed)
     mw = bc_w[bc_w > 0].mean() if (bc_w > 0).any() else 1.0
+    if use_targets:
+        mlp.targets(tv, tw)
     mlp.features(1.0 / mw)
     roots = np.array(sorted(util), np.uint32)
     u = np.array([util[int(r)] for r in roots])

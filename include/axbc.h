@@ -259,11 +259,16 @@ int axbc_kronecker_ingest(int device, int scale, uint64_t nedges, uint64_t seed1
 /*
  * Learned pivot proposal (no reference counterpart; stands in for the
  * bc/delta heuristics -z 3/4): a 4 -> 64 -> 64 -> 1 MLP with ReLU, BF16
- * weights and FP32 accumulation, over per-vertex features (log(1+deg), LCC,
- * log(1+bc*bc_scale), log(1+mean neighbour bc*bc_scale)).
+ * weights and FP32 accumulation, over per-vertex features (log(1+deg), LCC
+ * or target adjacency, log(1+bc*bc_scale), log(1+mean neighbour
+ * bc*bc_scale)).
  *   axbc_mlp_init        FP32 master parameters (axbc_mlp_param_count()
  *                        floats: W1[64][4], b1, W2[64][64], b2, w3, b3)
  *   axbc_mlp_features    features from the device state (LCC when built)
+ *   axbc_mlp_targets     target vertices and weights: while set (count > 0),
+ *                        feature 1 is log(1 + t(v) / mean positive t) with
+ *                        t(v) the sum of the weights of v's neighbours
+ *                        instead of the LCC; count 0 clears them
  *   axbc_mlp_set_data    host features / targets (either may be NULL)
  *   axbc_mlp_gradient    MSE gradient over the sample ids (accumulated on the
  *                        device; copied out when grad_host is given)
@@ -278,6 +283,7 @@ int axbc_kronecker_ingest(int device, int scale, uint64_t nedges, uint64_t seed1
 int axbc_mlp_param_count(void);
 int axbc_mlp_init(axbc_ctx *ctx, const float *params);
 int axbc_mlp_features(axbc_ctx *ctx, double bc_scale);
+int axbc_mlp_targets(axbc_ctx *ctx, const uint32_t *vertex, const double *weight, uint32_t count);
 int axbc_mlp_set_data(axbc_ctx *ctx, const float *x, const float *y);
 int axbc_mlp_get_data(axbc_ctx *ctx, float *x, float *score);
 int axbc_mlp_gradient(axbc_ctx *ctx, const uint32_t *idx, uint32_t m, float *grad_host, float *loss_host);

@@ -166,3 +166,29 @@ def test_graph_replays_match_host_launches(require_gpu, width):
             _, _, _, bc_ref, _ = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
         ok, err = close(out[0][0], bc_ref)
         assert ok, err
+
+
+def test_utility_per_root_in_draw_order(require_gpu):
+    # roots of very different degrees sit in lanes by degree; the utility of
+    # root k must still be sum_v weight(v) * delta_k(v) of the k-th root drawn
+    g = Graph.kronecker(11, 16, seed=(3, 4), heuristic=0)
+    deg = g.degrees()
+    order = np.argsort(deg, kind="stable")
+    cand = order[deg[order] > 0]
+    roots = np.concatenate([cand[:20], cand[-20:], cand[len(cand) // 2:len(cand) // 2 + 24]])
+    rng = np.random.default_rng(7)
+    rng.shuffle(roots)
+    verts = rng.choice(g.N, size=300, replace=False).astype(np.uint32)
+    wts = rng.uniform(0.1, 2.0, size=300)
+    with Engine(g) as e:
+        assert _ffi.hip().axbc_set_batch_width(e.ctx, 64) == 0
+        applied, _ = run_batch(e, roots, np.full(len(roots), 1.0 / g.N))
+        assert applied == len(roots)
+        out = np.zeros(64, np.float64)
+        assert _ffi.hip().axbc_batch_utility(e.ctx, verts.ctypes.data, wts.ctypes.data, len(verts),
+                                             out.ctypes.data) == 0
+        bc_ref = np.zeros(g.N)
+        for k, v0 in enumerate(roots):
+            _, _, dl, bc_ref, _ = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
+            want = float(np.dot(wts, dl[verts]))
+            assert abs(out[k] - want) <= 1e-9 * max(abs(want), 1.0), (k, out[k], want)

@@ -37,6 +37,36 @@ def test_gradient_matches_oracle(require_gpu):
     assert np.abs(score - s_ref).max() <= 1e-4 * (np.abs(s_ref).max() + 1)
 
 
+def test_features_with_targets(require_gpu):
+    """The four device features against NumPy: log(1+deg), target adjacency
+    log(1 + t/mean positive t) with t(v) the sum of the target weights over
+    v's neighbours, log(1+bc*s), log(1+mean neighbour bc*s)."""
+    g = Graph.kronecker(12, 16, seed=(1, 2))
+    deg = np.diff(g.col).astype(np.int64)
+    rng = np.random.default_rng(5)
+    bc = rng.uniform(0, 100, g.N) * (deg > 0)
+    tv = rng.choice(g.N, 200, replace=False).astype(np.uint32)
+    tw = rng.uniform(0.5, 2.0, 200)
+    scale = 0.01
+    with Engine(g) as e:
+        e.set_bc(bc)
+        m = Mlp(e, init_params(0))
+        m.targets(tv, tw)
+        m.features(scale)
+        x, _ = m.get()
+    src = np.repeat(np.arange(g.N), deg)
+    twv = np.zeros(g.N)
+    twv[tv] = tw
+    t = np.bincount(src, weights=twv[g.row], minlength=g.N).astype(np.float32).astype(np.float64)
+    nb = np.bincount(src, weights=bc[g.row], minlength=g.N)
+    want = np.stack([np.log1p(deg.astype(np.float32)),
+                     np.log1p(t / t[t > 0].mean()),
+                     np.log1p(bc * scale),
+                     np.where(deg > 0, np.log1p(nb / np.maximum(deg, 1) * scale), 0.0)], axis=1)
+    assert (t > 0).sum() > 100
+    np.testing.assert_allclose(x, want, rtol=1e-5, atol=1e-6)
+
+
 def test_training_reduces_loss(require_gpu):
     g = Graph.kronecker(12, 16, seed=(1, 2))
     rng = np.random.default_rng(1)
