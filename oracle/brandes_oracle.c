@@ -134,6 +134,37 @@ void oracle_pivot(uint32_t N, const uint32_t *col, const uint32_t *row, const ui
   free(off);
 }
 
+/* Throughput form for the CPU baseline: the rounds of `count` given roots
+ * (a static distribution, no stop test, so the rounds are independent)
+ * spread over `nthreads` OpenMP threads, each with its own round state and
+ * bc accumulator, added into bc at the end.  Returns the summed E_r. */
+uint64_t oracle_pivots_omp(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
+                           int heuristic, const uint32_t *roots, const double *probs, uint32_t count,
+                           int nthreads, double *bc) {
+  uint64_t er = 0;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(+ : er)
+  {
+    int32_t *lvl = (int32_t *)xcalloc(N, sizeof(int32_t));
+    uint32_t *sigma = (uint32_t *)xcalloc(N, sizeof(uint32_t));
+    double *delta = (double *)xcalloc(N, sizeof(double));
+    double *mine = (double *)xcalloc(N, sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t i = 0; i < (int64_t)count; ++i) {
+      oracle_pivot_stats st;
+      oracle_pivot(N, col, row, reach, heuristic, roots[i], probs[i], lvl, sigma, delta, mine, &st);
+      er += st.e_r;
+    }
+#pragma omp critical
+    for (uint32_t v = 0; v < N; ++v) bc[v] += mine[v];
+    free(lvl);
+    free(sigma);
+    free(delta);
+    free(mine);
+  }
+  return er;
+}
+
 /* ---- samplers ------------------------------------------------------------ */
 
 /* int LIMxSAMPLE = N*N (axbc2d.c:2835), compared against the uint32 BSM

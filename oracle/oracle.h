@@ -35,6 +35,13 @@ void oracle_pivot(uint32_t N, const uint32_t *col, const uint32_t *row, const ui
                   int heuristic, uint32_t v0, double p, int32_t *lvl, uint32_t *sigma, double *delta,
                   double *bc, oracle_pivot_stats *st);
 
+/* The rounds of `count` roots over `nthreads` OpenMP threads (independent
+ * rounds of a static distribution, no stop test); bc += their sum.
+ * Returns the summed E_r.  Throughput baseline only. */
+uint64_t oracle_pivots_omp(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
+                           int heuristic, const uint32_t *roots, const double *probs, uint32_t count,
+                           int nthreads, double *bc);
+
 /* Pivot-loop configuration (the CLI flags of the 1x1 path). */
 typedef struct {
   uint64_t nbfs;        /* -N */

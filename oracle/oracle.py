@@ -49,6 +49,9 @@ def lib() -> C.CDLL:
         L.oracle_pivot.argtypes = [C.c_uint32, u32p, u32p, u32p, C.c_int, C.c_uint32, C.c_double, i32p,
                                    u32p, f64p, f64p, C.POINTER(Stats)]
         L.oracle_run.restype = C.c_int
+        L.oracle_pivots_omp.restype = C.c_uint64
+        L.oracle_pivots_omp.argtypes = [C.c_uint32, u32p, u32p, u32p, C.c_int, u32p, f64p, C.c_uint32, C.c_int,
+                                        f64p]
         L.oracle_run.argtypes = [C.c_uint32, u32p, u32p, u32p, C.POINTER(Cfg), u32p, f64p, f64p,
                                  C.POINTER(RunStats)]
         L.oracle_lim_x_sample.restype = C.c_uint32
@@ -82,6 +85,20 @@ def pivot(g, v0: int, p: float, bc: np.ndarray | None = None):
     lib().oracle_pivot(N, _arr(g.col, np.uint32), row, _arr(g.reach, np.uint32), int(g.heuristic),
                        int(v0), float(p), lvl, sig, dl, bc, C.byref(st))
     return lvl, sig, dl, bc, {k: int(getattr(st, k)) for k, _ in Stats._fields_}
+
+
+def pivots_parallel(g, roots, probs, threads: int, bc: np.ndarray | None = None):
+    """The rounds of the given roots over `threads` OpenMP threads (a
+    throughput baseline for static distributions); returns (bc, summed E_r)."""
+    N = g.N
+    if bc is None:
+        bc = np.zeros(N, np.float64)
+    r = np.ascontiguousarray(roots, np.uint32)
+    p = np.ascontiguousarray(probs, np.float64)
+    row = _arr(g.row, np.uint32) if len(g.row) else np.zeros(1, np.uint32)
+    er = lib().oracle_pivots_omp(N, _arr(g.col, np.uint32), row, _arr(g.reach, np.uint32), int(g.heuristic),
+                                 r, p, len(r), int(threads), bc)
+    return bc, int(er)
 
 
 def run(g, nbfs: int, c: int = 1, x: int = 1, z: int = 0, seed: int = 4343543, sync_batch: int = 0):

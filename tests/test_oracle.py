@@ -341,3 +341,18 @@ def test_bf16_rounding():
     assert b[0] == 1.0 and b[1] == 1.0  # tie to even
     assert b[2] == np.float32(1.0 + 2 ** -7)
     assert abs(b[3] + 3.140625) < 1e-6
+
+
+def test_parallel_rounds_equal_sequential():
+    # the OpenMP throughput form (CPU baseline) sums the same rounds
+    g = Graph.kronecker(10, 16, seed=(1, 2), heuristic=1)
+    deg = np.diff(g.col)
+    roots = np.flatnonzero(deg > 0)[:40].astype(np.uint32)
+    probs = np.full(len(roots), 1.0 / g.N)
+    bc, er = np.zeros(g.N), 0
+    for v, p in zip(roots, probs):
+        _, _, _, bc, st = oracle.pivot(g, int(v), float(p), bc)
+        er += st["e_r"]
+    bp, erp = oracle.pivots_parallel(g, roots, probs, 4)
+    assert erp == er
+    assert np.max(np.abs(bp - bc) / np.maximum(np.abs(bc), 1e-300)) < 1e-12
