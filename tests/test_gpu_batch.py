@@ -192,3 +192,13 @@ def test_utility_per_root_in_draw_order(require_gpu):
             _, _, dl, bc_ref, _ = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
             want = float(np.dot(wts, dl[verts]))
             assert abs(out[k] - want) <= 1e-9 * max(abs(want), 1.0), (k, out[k], want)
+
+
+@pytest.mark.parametrize("width", [64, 128])
+def test_sigma_wraps_to_zero_batched(require_gpu, width):
+    # sigma = 2^32 = 0 mod 2^32 on layer 4: inf/NaN dependencies must come
+    # out at the same positions as the one-root oracle rounds
+    from tests.test_gpu_plain import layered
+    g = layered(256, 6)
+    roots = np.array([0, 1 + 5 * 256 + 7, 1 + 2 * 256 + 3, 1 + 4 * 256, 5], np.uint32)
+    check_batch(g, roots, width, "sigma wraps to 0")

@@ -15,10 +15,12 @@ RTOL = 1e-9
 
 
 def close(a, b, rtol=RTOL):
+    """Within rtol relative; infinities equal and NaNs at the same positions
+    (sigma can wrap to 0 mod 2^32, and (delta+1)/0 then propagates)."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     scale = np.maximum(np.abs(a), np.abs(b))
-    ok = (a == b) | (np.abs(a - b) <= rtol * scale)
+    ok = (a == b) | (np.isnan(a) & np.isnan(b)) | (np.abs(a - b) <= rtol * scale)
     return bool(ok.all()), (np.abs(a - b) / np.where(scale > 0, scale, 1)).max() if a.size else 0.0
 
 
@@ -71,3 +73,23 @@ def test_path_depths(require_gpu):
     n = 3000
     g = Graph.from_edges(n, np.array([(i, i + 1) for i in range(n - 1)]), undirected=True)
     check_rounds(g, [0, n // 2, n - 1], "long path")
+
+
+def layered(a: int, layers: int) -> Graph:
+    """Root 0, then `layers` layers of a vertices, consecutive layers
+    completely connected: sigma of layer j (from 0) is a^j mod 2^32."""
+    e = [(0, 1 + x) for x in range(a)]
+    for j in range(layers - 1):
+        xs = np.arange(a)
+        u = np.repeat(1 + j * a + xs, a)
+        w = np.tile(1 + (j + 1) * a + xs, a)
+        e.extend(zip(u.tolist(), w.tolist()))
+    return Graph.from_edges(1 + layers * a, np.array(e))
+
+
+def test_sigma_wraps_to_zero(require_gpu):
+    # a = 256: sigma of layer 4 is 2^32 = 0 mod 2^32, so x = (delta+1)/0 is
+    # infinite and the dependencies above it become inf/NaN, as in the
+    # reference's uint32 arithmetic; both sides must agree position by position
+    g = layered(256, 6)
+    check_rounds(g, [0, 1 + 5 * 256 + 7, 1 + 2 * 256 + 3], "sigma wraps to 0")

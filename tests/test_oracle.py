@@ -356,3 +356,20 @@ def test_parallel_rounds_equal_sequential():
     bp, erp = oracle.pivots_parallel(g, roots, probs, 4)
     assert erp == er
     assert np.max(np.abs(bp - bc) / np.maximum(np.abs(bc), 1e-300)) < 1e-12
+
+
+def test_sigma_wrapping_to_zero_gives_infinite_dependencies():
+    # layers of 256: sigma of layers 4 and 5 is 2^32, 2^40 = 0 (uint32); the
+    # last layer's x = (0+1)/0 is infinite, layer 4 adds inf * sigma = inf * 0
+    # = NaN, and NaN reaches every layer above
+    a, layers = 256, 6
+    e = [(0, 1 + x) for x in range(a)]
+    for j in range(layers - 1):
+        for x in range(a):
+            for y in range(a):
+                e.append((1 + j * a + x, 1 + (j + 1) * a + y))
+    g = Graph.from_edges(1 + layers * a, np.array(e))
+    lvl, sig, dl, _, _ = oracle.pivot(g, 0, 1.0)
+    assert (sig[1 + 4 * a:] == 0).all()
+    assert (dl[1 + 5 * a:] == 0).all()
+    assert np.isnan(dl[1:1 + 5 * a]).all()
