@@ -49,6 +49,7 @@ struct SamplerBufs {
   uint64_t *lw = nullptr, *lp = nullptr;  // fixed-point weights and their prefix
   uint64_t ltotal = 0;
   int lcc_ready = 0;
+  int lcc_sym = -1;  // CSC symmetric, loop-free, strictly sorted: 1 yes, 0 no, -1 not checked
 };
 
 struct BatchState;  // batch.hip

@@ -239,6 +239,73 @@ __global__ __launch_bounds__(256) void k_lcc_hubs(const uint32_t *__restrict__ c
   }
 }
 
+// Whether every list is strictly increasing, loop-free and mirrored (j in
+// adj(i) <=> i in adj(j)); bit 0 of *bad set otherwise.  A wave per vertex.
+__global__ __launch_bounds__(256) void k_lcc_check(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                   uint32_t n, unsigned int *bad) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t i = gw; i < n; i += nwv) {
+    const uint32_t b = col[i], d = col[i + 1] - b;
+    bool ok = true;
+    for (uint32_t k = lane_id(); k < d; k += kWave) {
+      const uint32_t j = row[b + k];
+      ok &= j != i && j < n && (k == 0 || row[b + k - 1] < j) && in_sorted(row + col[j], col[j + 1] - col[j], i);
+    }
+    if (__ballot(!ok) && lane_id() == 0) atomicOr(bad, 1u);
+  }
+}
+
+// Symmetric, loop-free, strictly sorted lists: |adj(i) cap adj(j)| is the
+// same for (i, j) and (j, i), so every undirected edge is intersected once
+// (from its smaller endpoint) and added to both counters; uint32 atomics
+// wrap mod 2^32 like the reference's counter.
+__global__ __launch_bounds__(256) void k_lcc_small_half(const uint32_t *__restrict__ col,
+                                                        const uint32_t *__restrict__ row, uint32_t n, uint32_t hub_deg,
+                                                        uint32_t *counter) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t i = gw; i < n; i += nwv) {
+    const uint32_t b = col[i], d = col[i + 1] - b;
+    if (d < 2 || d > hub_deg) continue;
+    uint32_t ci = 0;
+    for (uint32_t k = 0; k < d; ++k) {
+      const uint32_t j = row[b + k];
+      if (j < i) continue;
+      const uint32_t c = (uint32_t)wave_sum_u64(pair_count(col, row, i, j));
+      ci += c;
+      if (lane_id() == 0 && c) atomicAdd(&counter[j], c);
+    }
+    if (lane_id() == 0 && ci) atomicAdd(&counter[i], ci);
+  }
+}
+
+// The hubs' edges (i, j) with j > i, one wave per item.
+__global__ __launch_bounds__(256) void k_lcc_hubs_half(const uint32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ row,
+                                                       const uint32_t *__restrict__ hubs,
+                                                       const uint64_t *__restrict__ hoff, uint32_t nh,
+                                                       uint32_t *counter) {
+  const uint64_t total = hoff[nh];
+  const uint64_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint64_t t = gw; t < total; t += nwv) {
+    uint32_t lo = 0, hi = nh;  // hoff[lo] <= t < hoff[lo+1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (hoff[mid] <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t i = hubs[lo];
+    const uint32_t j = row[col[i] + (uint32_t)(t - hoff[lo])];
+    if (j < i) continue;
+    const uint32_t c = (uint32_t)wave_sum_u64(pair_count(col, row, i, j));
+    if (lane_id() == 0 && c) {
+      atomicAdd(&counter[i], c);
+      atomicAdd(&counter[j], c);
+    }
+  }
+}
+
 __global__ void k_lcc_value(const uint32_t *__restrict__ col, uint32_t n, const uint32_t *__restrict__ counter,
                             float *lcc) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -451,11 +518,26 @@ int axbc_lcc_build(axbc_ctx *c, float *lcc_host, uint64_t *lw_host, uint64_t *lp
   HIP_TRY(hipMalloc((void **)&doff, sizeof(uint64_t) * hoff.size()));
   if (!hubs.empty()) HIP_TRY(hipMemcpyAsync(dh, hubs.data(), sizeof(uint32_t) * hubs.size(), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(doff, hoff.data(), sizeof(uint64_t) * hoff.size(), hipMemcpyHostToDevice, s));
+  if (sb->lcc_sym < 0) {  // once per context: may every edge be intersected once?
+    HIP_TRY(hipMemsetAsync(sb->lcc_cnt, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_lcc_check, dim3(4096), dim3(256), 0, s, c->col, c->row, n, sb->lcc_cnt);
+    uint32_t bad = 1;
+    HIP_TRY(hipMemcpyAsync(&bad, sb->lcc_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    sb->lcc_sym = bad ? 0 : 1;
+  }
   HIP_TRY(hipMemsetAsync(sb->lcc_cnt, 0, sizeof(uint32_t) * n, s));
-  hipLaunchKernelGGL(k_lcc_small, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccHub, sb->lcc_cnt);
-  if (!hubs.empty())
-    hipLaunchKernelGGL(k_lcc_hubs, dim3(4096), dim3(256), 0, s, c->col, c->row, dh, doff, (uint32_t)hubs.size(),
-                       sb->lcc_cnt);
+  if (sb->lcc_sym == 1) {
+    hipLaunchKernelGGL(k_lcc_small_half, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccHub, sb->lcc_cnt);
+    if (!hubs.empty())
+      hipLaunchKernelGGL(k_lcc_hubs_half, dim3(4096), dim3(256), 0, s, c->col, c->row, dh, doff,
+                         (uint32_t)hubs.size(), sb->lcc_cnt);
+  } else {
+    hipLaunchKernelGGL(k_lcc_small, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccHub, sb->lcc_cnt);
+    if (!hubs.empty())
+      hipLaunchKernelGGL(k_lcc_hubs, dim3(4096), dim3(256), 0, s, c->col, c->row, dh, doff, (uint32_t)hubs.size(),
+                         sb->lcc_cnt);
+  }
   hipLaunchKernelGGL(k_lcc_value, dim3(grid_n(n)), dim3(256), 0, s, c->col, n, sb->lcc_cnt, sb->lcc);
   // fixed-point weights of the LCC values and their prefix
   HIP_TRY(hipMemsetAsync(sb->wmax, 0, sizeof(unsigned long long), s));

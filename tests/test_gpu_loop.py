@@ -41,10 +41,27 @@ def test_loop_matches_oracle(require_gpu, z, c, x, batch, heur):
         assert ok, err
 
 
+def _looped():
+    # the first sorted edge survives normalisation even as a loop (0, 0):
+    # the LCC then takes the general (per-direction) path
+    rng = np.random.default_rng(3)
+    e = rng.integers(0, 300, size=(3000, 2))
+    e = np.concatenate([[[0, 0]], e])
+    return Graph.from_edges(300, e, heuristic=0)
+
+
 def test_lcc_matches_oracle(require_gpu):
+    """Symmetric loop-free graphs intersect every undirected edge once (with
+    hub items above 4096 edges at SCALE 16); directed graphs and a graph
+    with a loop take the per-direction kernels."""
     import ctypes as C
     from axbc_amd import _ffi
-    for g in (Graph.kronecker(12, 16, seed=(1, 2)), Graph.kronecker(11, 32, seed=(7, 8), undirected=False)):
+    graphs = (Graph.kronecker(12, 16, seed=(1, 2)), Graph.kronecker(11, 32, seed=(7, 8), undirected=False),
+              Graph.kronecker(16, 16, seed=(2, 3), heuristic=1), _looped())
+    assert int(np.diff(graphs[2].col).max()) > 4096
+    assert graphs[3].row[graphs[3].col[0]] == 0
+    for g in graphs:
+        print(f"[lcc] N={g.N} ned={g.ned}", flush=True)
         lcc_ref = np.zeros(g.N, np.float32)
         L = oracle.lib()
         L.oracle_lcc.argtypes = [C.c_uint32, oracle.u32p, oracle.u32p,
