@@ -306,6 +306,90 @@ __global__ __launch_bounds__(256) void k_lcc_hubs_half(const uint32_t *__restric
   }
 }
 
+// Owner of an undirected edge: the endpoint of larger (degree, id).
+__device__ __forceinline__ bool ranks_below(const uint32_t *__restrict__ col, uint32_t s, uint32_t o) {
+  const uint32_t ds = col[s + 1] - col[s], d_o = col[o + 1] - col[o];
+  return ds < d_o || (ds == d_o && s < o);
+}
+
+// Owners of degree below kLccBit: for each owned edge (o, s) lanes walk
+// adj(s) and binary-search the short adj(o); both counters get the count.
+__global__ __launch_bounds__(256) void k_lcc_light(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                   uint32_t n, uint32_t bit_deg, uint32_t *counter) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t o = gw; o < n; o += nwv) {
+    const uint32_t b = col[o], d = col[o + 1] - b;
+    if (d < 2 || d >= bit_deg) continue;
+    uint32_t co = 0;
+    for (uint32_t k = 0; k < d; ++k) {
+      const uint32_t sv = row[b + k];
+      if (!ranks_below(col, sv, o)) continue;
+      const uint32_t *as = row + col[sv];
+      const uint32_t ds = col[sv + 1] - col[sv];
+      uint32_t c = 0;
+      for (uint32_t l = lane_id(); l < ds; l += kWave) c += in_sorted(row + b, d, as[l]) ? 1u : 0u;
+      c = (uint32_t)wave_sum_u64(c);
+      co += c;
+      if (lane_id() == 0 && c) atomicAdd(&counter[sv], c);
+    }
+    if (lane_id() == 0 && co) atomicAdd(&counter[o], co);
+  }
+}
+
+// Bitmaps of the adjacency of a batch of owners (one N-bit row per slot):
+// set (on = 1) or clear (on = 0) the words their neighbours touch.
+__global__ __launch_bounds__(256) void k_lcc_bits(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ owner, uint32_t nslots, uint64_t words,
+                                                  uint32_t *bits, int on) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t sl = gw; sl < nslots; sl += nwv) {
+    const uint32_t o = owner[sl];
+    uint32_t *bm = bits + (size_t)sl * words;
+    for (uint32_t k = col[o] + lane_id(); k < col[o + 1]; k += kWave) {
+      const uint32_t x = row[k];
+      if (on)
+        atomicOr(&bm[x >> 5], 1u << (x & 31));
+      else
+        bm[x >> 5] = 0u;
+    }
+  }
+}
+
+// Owned edges of a batch of bitmap owners, one wave per (slot, neighbour)
+// item: lanes walk adj(s) and test the owner's bits.
+__global__ __launch_bounds__(256) void k_lcc_heavy(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                   const uint32_t *__restrict__ owner,
+                                                   const uint64_t *__restrict__ ioff, uint32_t nslots, uint64_t words,
+                                                   const uint32_t *__restrict__ bits, uint32_t *counter) {
+  const uint64_t total = ioff[nslots] - ioff[0];
+  const uint64_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint64_t t = gw; t < total; t += nwv) {
+    const uint64_t tt = t + ioff[0];
+    uint32_t lo = 0, hi = nslots;  // ioff[lo] <= tt < ioff[lo+1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ioff[mid] <= tt)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t o = owner[lo];
+    const uint32_t sv = row[col[o] + (uint32_t)(tt - ioff[lo])];
+    if (!ranks_below(col, sv, o)) continue;
+    const uint32_t *bm = bits + (size_t)lo * words;
+    uint32_t c = 0;
+    for (uint32_t k = col[sv] + lane_id(); k < col[sv + 1]; k += kWave) {
+      const uint32_t x = row[k];
+      c += (bm[x >> 5] >> (x & 31)) & 1u;
+    }
+    c = (uint32_t)wave_sum_u64(c);
+    if (lane_id() == 0 && c) {
+      atomicAdd(&counter[o], c);
+      atomicAdd(&counter[sv], c);
+    }
+  }
+}
+
 __global__ void k_lcc_value(const uint32_t *__restrict__ col, uint32_t n, const uint32_t *__restrict__ counter,
                             float *lcc) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -528,10 +612,56 @@ int axbc_lcc_build(axbc_ctx *c, float *lcc_host, uint64_t *lw_host, uint64_t *lp
   }
   HIP_TRY(hipMemsetAsync(sb->lcc_cnt, 0, sizeof(uint32_t) * n, s));
   if (sb->lcc_sym == 1) {
-    hipLaunchKernelGGL(k_lcc_small_half, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccHub, sb->lcc_cnt);
-    if (!hubs.empty())
-      hipLaunchKernelGGL(k_lcc_hubs_half, dim3(4096), dim3(256), 0, s, c->col, c->row, dh, doff,
-                         (uint32_t)hubs.size(), sb->lcc_cnt);
+    // every undirected edge once, from its owner (larger degree, id):
+    // short owners binary-search, owners of kLccBit or more edges test a
+    // bitmap of their adjacency, built for a batch of them at a time
+    constexpr uint32_t kLccBit = 256;
+    constexpr uint64_t kBitBytes = 512ull << 20;
+    hipLaunchKernelGGL(k_lcc_light, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccBit, sb->lcc_cnt);
+    std::vector<uint32_t> heavy;
+    std::vector<uint64_t> ioff(1, 0);
+    for (uint32_t v = 0; v < n; ++v) {
+      const uint32_t d = c->h_col[v + 1] - c->h_col[v];
+      if (d >= kLccBit) {
+        heavy.push_back(v);
+        ioff.push_back(ioff.back() + d);
+      }
+    }
+    if (!heavy.empty()) {
+      const uint64_t words = ((uint64_t)n + 31) / 32;
+      const uint32_t slots = (uint32_t)std::max<uint64_t>(
+          1, std::min<uint64_t>(heavy.size(), kBitBytes / (words * sizeof(uint32_t))));
+      uint32_t *dhv = nullptr, *bits = nullptr;
+      uint64_t *dio = nullptr;
+      hipError_t e1 = hipMalloc((void **)&dhv, sizeof(uint32_t) * heavy.size());
+      hipError_t e2 = hipMalloc((void **)&dio, sizeof(uint64_t) * ioff.size());
+      hipError_t e3 = hipMalloc((void **)&bits, sizeof(uint32_t) * words * slots);
+      if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (dhv) (void)hipFree(dhv);
+        if (dio) (void)hipFree(dio);
+        if (bits) (void)hipFree(bits);
+        (void)hipFree(dh);
+        (void)hipFree(doff);
+        return fail(AXBC_ENOMEM, "LCC bitmaps%s (line %d)", "", __LINE__);
+      }
+      HIP_TRY(hipMemcpyAsync(dhv, heavy.data(), sizeof(uint32_t) * heavy.size(), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(dio, ioff.data(), sizeof(uint64_t) * ioff.size(), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemsetAsync(bits, 0, sizeof(uint32_t) * words * slots, s));
+      for (size_t b0 = 0; b0 < heavy.size(); b0 += slots) {
+        const uint32_t nb = (uint32_t)std::min<size_t>(slots, heavy.size() - b0);
+        hipLaunchKernelGGL(k_lcc_bits, dim3(grid_n((uint64_t)nb * kWave)), dim3(256), 0, s, c->col, c->row,
+                           dhv + b0, nb, words, bits, 1);
+        hipLaunchKernelGGL(k_lcc_heavy, dim3(4096), dim3(256), 0, s, c->col, c->row, dhv + b0, dio + b0, nb, words,
+                           bits, sb->lcc_cnt);
+        hipLaunchKernelGGL(k_lcc_bits, dim3(grid_n((uint64_t)nb * kWave)), dim3(256), 0, s, c->col, c->row,
+                           dhv + b0, nb, words, bits, 0);
+      }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(s));
+      (void)hipFree(dhv);
+      (void)hipFree(dio);
+      (void)hipFree(bits);
+    }
   } else {
     hipLaunchKernelGGL(k_lcc_small, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccHub, sb->lcc_cnt);
     if (!hubs.empty())
