@@ -36,6 +36,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../../include/axbc.h"
@@ -57,7 +59,7 @@ constexpr int kBurst = AXBC_BURST;  // row loads in flight per wave in the pull 
 // roots); k_bclose keeps the compiler's choice.
 constexpr int kWavesPerSimd = 8;
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
-constexpr int kGraphSteps = 8;    // levels (depths) per captured graph
+constexpr int kGraphSteps = 8;    // levels (depths) per graph
 
 struct BCtl {
   uint32_t tail;       // list length
@@ -72,7 +74,7 @@ struct BCtl {
   unsigned long long fe_edges;     // out-edges of the level's entries (next level's push work)
   unsigned long long cand_edges;   // in-edges of vertices not yet reached by every root
   // device-resident loop state, so that every level (depth) runs the same
-  // launch sequence and a captured graph can replay it
+  // launch sequence and a prebuilt graph can replay it
   int level;                       // forward level the next kernels run
   uint32_t mark_base;              // level stamps of `mark`: mark_base + level
   uint32_t stamp;                  // batch stamp of `seen`
@@ -121,7 +123,7 @@ struct BatchState {
   int lev_guess = 12;
   uint32_t chunk = 256;  // edges per work item of split vertices (tunable)
   uint32_t pull16 = 32;  // bottom-up when candidate in-edges <= pull16/16 x push edges (0: never)
-  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // captured level / depth steps
+  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // prebuilt level / depth steps
   uint64_t gkey = 0;
   size_t bytes = 0;
 };
@@ -1551,8 +1553,10 @@ int balloc(BatchState *b, T **p, size_t count) {
 int ensure_transpose(axbc_ctx *c) {
   if (c->rcol) return AXBC_OK;
   std::vector<uint32_t> col(c->n + 1), row(c->ned);
-  HIP_TRY(hipMemcpy(col.data(), c->col, sizeof(uint32_t) * (c->n + 1), hipMemcpyDeviceToHost));
-  if (c->ned) HIP_TRY(hipMemcpy(row.data(), c->row, sizeof(uint32_t) * c->ned, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(col.data(), c->col, sizeof(uint32_t) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
+  if (c->ned)
+    HIP_TRY(hipMemcpyAsync(row.data(), c->row, sizeof(uint32_t) * c->ned, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   std::vector<uint32_t> rcol(c->n + 1, 0), rrow(c->ned);
   for (uint64_t k = 0; k < c->ned; ++k) rcol[row[k] + 1]++;
   for (uint32_t v = 0; v < c->n; ++v) rcol[v + 1] += rcol[v];
@@ -1573,8 +1577,10 @@ int ensure_transpose(axbc_ctx *c) {
   c->symmetric = false;
   HIP_TRY(hipMalloc((void **)&c->rcol, sizeof(uint32_t) * (c->n + 1)));
   HIP_TRY(hipMalloc((void **)&c->rrow, sizeof(uint32_t) * (c->ned ? c->ned : 1)));
-  HIP_TRY(hipMemcpy(c->rcol, rcol.data(), sizeof(uint32_t) * (c->n + 1), hipMemcpyHostToDevice));
-  if (c->ned) HIP_TRY(hipMemcpy(c->rrow, rrow.data(), sizeof(uint32_t) * c->ned, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(c->rcol, rcol.data(), sizeof(uint32_t) * (c->n + 1), hipMemcpyHostToDevice, c->stream));
+  if (c->ned)
+    HIP_TRY(hipMemcpyAsync(c->rrow, rrow.data(), sizeof(uint32_t) * c->ned, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->bytes += sizeof(uint32_t) * (c->n + 1 + c->ned);
   return AXBC_OK;
 }
@@ -1649,83 +1655,125 @@ int pull_bctl(axbc_ctx *c) {
   return AXBC_OK;
 }
 
+// Where a level's launches go: the context stream, or kernel nodes appended
+// in order to a graph under construction.  The graphs are built node by node
+// rather than by stream capture: capture is process-wide state in HIP, and
+// other threads' launches and copies fail while one is under way (several
+// contexts run concurrently when serving).
+struct Sink {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+};
+
+template <typename Tuple, size_t... I>
+void param_ptrs(Tuple &t, void **out, std::index_sequence<I...>) {
+  ((out[I] = (void *)&std::get<I>(t)), ...);
+}
+
+// Launch f<<<g, b>>>(a...) on the sink: on its stream, or as the next node
+// of its graph (the arguments are copied when the node is created).
+template <typename... P, typename... A>
+void launch(Sink &k, void (*f)(P...), dim3 g, dim3 b, A... a) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  std::tuple<P...> t(a...);
+  void *args[sizeof...(P) ? sizeof...(P) : 1];
+  param_ptrs(t, args, std::index_sequence_for<P...>{});
+  hipError_t e;
+  if (!k.graph) {
+    e = hipLaunchKernel((const void *)f, g, b, args, 0, k.stream);
+  } else {
+    hipKernelNodeParams np{};
+    np.func = (void *)f;
+    np.gridDim = g;
+    np.blockDim = b;
+    np.sharedMemBytes = 0;
+    np.kernelParams = args;
+    np.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    e = hipGraphAddKernelNode(&node, k.graph, k.last ? &k.last : nullptr, k.last ? 1 : 0, &np);
+    if (e == hipSuccess) k.last = node;
+  }
+  if (e != hipSuccess && k.err == hipSuccess) k.err = e;
+}
+
 // One forward level: the same launch sequence for every level (the kernels
-// read the level from the device), so it can be captured once and replayed.
+// read the level from the device), so it can be built into a graph once and
+// replayed.
 template <int NW>
-void enqueue_level(axbc_ctx *c, BatchState *b, bool split_out, bool split_in) {
-  hipStream_t s = c->stream;
+void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool split_in) {
   const uint32_t n = c->n;
   kt_begin(c, KK_PUSH);
-  hipLaunchKernelGGL(k_bpush<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark, b->fr[0],
+  launch(k, k_bpush<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->nx, b->mark, b->fr[0],
                      b->fr[1], b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
                      kt_bytes(c, KK_PUSH), b->chunk);
   kt_end(c, KK_PUSH);
   if (split_out) {
     kt_begin(c, KK_PUSH_SPLIT);
-    hipLaunchKernelGGL(k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->nx, b->mark,
+    launch(k, k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->nx, b->mark,
                        b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
                        kt_bytes(c, KK_PUSH_SPLIT), b->chunk);
     kt_end(c, KK_PUSH_SPLIT);
-    hipLaunchKernelGGL(k_bsplit_reset, dim3(1), dim3(64), 0, s, b->ctl);
+    launch(k, k_bsplit_reset, dim3(1), dim3(64), b->ctl);
   }
   kt_begin(c, KK_SIGMA);
-  hipLaunchKernelGGL(k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
+  launch(k, k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
                      b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
                      kt_bytes(c, KK_SIGMA), b->chunk);
   kt_end(c, KK_SIGMA);
   if (split_in) {
     kt_begin(c, KK_SIGMA_SPLIT);
-    hipLaunchKernelGGL(k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->nx, b->fr[0],
+    launch(k, k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0],
                        b->fr[1], b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
                        kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
     kt_end(c, KK_SIGMA_SPLIT);
   }
   if (b->pull16) {
     kt_begin(c, KK_PULL);
-    hipLaunchKernelGGL(k_bpull<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, c->rcol, c->rrow, b->vis, b->nx, b->fr[0],
+    launch(k, k_bpull<NW>, dim3(kBGrid), dim3(kBlock), n, c->rcol, c->rrow, b->vis, b->nx, b->fr[0],
                        b->fr[1], b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
                        b->qcap, kt_bytes(c, KK_PULL), b->chunk);
     kt_end(c, KK_PULL);
     if (split_in) {
       kt_begin(c, KK_PULL_SPLIT);
-      hipLaunchKernelGGL(k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->rcol, c->rrow, b->vis, b->nx,
+      launch(k, k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->vis, b->nx,
                          b->fr[0], b->fr[1], b->sig, b->mark, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap,
                          b->part_rows, b->qcap, kt_bytes(c, KK_PULL_SPLIT), b->chunk);
       kt_end(c, KK_PULL_SPLIT);
     }
   }
   kt_begin(c, KK_CLOSE);
-  hipLaunchKernelGGL(k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, c->col, c->reach, b->nx, b->vis, b->fr[0],
+  launch(k, k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), c->col, c->reach, b->nx, b->vis, b->fr[0],
                      b->fr[1], b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->seen, b->qcap, c->rcol,
                      kt_bytes(c, KK_CLOSE));
   kt_end(c, KK_CLOSE);
-  hipLaunchKernelGGL(k_bclose_end, dim3(1), dim3(64), 0, s, b->ctl, b->qoff, b->qcap, b->pull16);
+  launch(k, k_bclose_end, dim3(1), dim3(64), b->ctl, b->qoff, b->qcap, b->pull16);
 }
 
 // One backward depth step (the depth comes from the device; the last
 // kernel moves it down).  Before depth d the level d+1 bits leave VIS.
 template <int NW>
-void enqueue_depth(axbc_ctx *c, BatchState *b, bool split_out) {
-  hipStream_t s = c->stream;
-  hipLaunchKernelGGL(k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), 0, s, b->vis, b->q, b->qw, b->qoff, b->ctl);
+void enqueue_depth(Sink &k, axbc_ctx *c, BatchState *b, bool split_out) {
+  launch(k, k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), b->vis, b->q, b->qw, b->qoff, b->ctl);
   kt_begin(c, KK_DEP);
-  hipLaunchKernelGGL(k_bdep<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
+  launch(k, k_bdep<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
                      c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
                      kt_bytes(c, KK_DEP), b->chunk);
   kt_end(c, KK_DEP);
   if (split_out) {
     kt_begin(c, KK_DEP_SPLIT);
-    hipLaunchKernelGGL(k_bdep_split<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->col, c->row, b->vis, b->sig, b->xr,
+    launch(k, k_bdep_split<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->sig, b->xr,
                        b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->part,
                        kt_bytes(c, KK_DEP_SPLIT), b->chunk);
     kt_end(c, KK_DEP_SPLIT);
     kt_begin(c, KK_DEP_COMBINE);
-    hipLaunchKernelGGL(k_bdep_combine<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, c->col, b->sig, b->dlt, b->xr,
+    launch(k, k_bdep_combine<NW>, dim3(kBGrid / 4), dim3(kBlock), c->col, b->sig, b->dlt, b->xr,
                        c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part,
                        kt_bytes(c, KK_DEP_COMBINE), b->chunk);
     kt_end(c, KK_DEP_COMBINE);
   }
-  hipLaunchKernelGGL(k_bstep_down, dim3(1), dim3(64), 0, s, b->ctl);
+  launch(k, k_bstep_down, dim3(1), dim3(64), b->ctl);
 }
 
 void graphs_free(BatchState *b) {
@@ -1734,7 +1782,7 @@ void graphs_free(BatchState *b) {
   b->gfwd = b->gbwd = nullptr;
 }
 
-// kGraphSteps levels (depths) per captured graph, re-captured when the
+// kGraphSteps levels (depths) per graph, rebuilt when the
 // launch sequence changes (split kernels, pull ratio, chunk).
 template <int NW>
 int ensure_graphs(axbc_ctx *c, bool split_out, bool split_in) {
@@ -1743,22 +1791,20 @@ int ensure_graphs(axbc_ctx *c, bool split_out, bool split_in) {
                        (uint64_t)split_in;
   if (b->gfwd && b->gbwd && b->gkey == key) return AXBC_OK;
   graphs_free(b);
-  hipStream_t s = c->stream;
   for (int pass = 0; pass < 2; ++pass) {
     hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < kGraphSteps; ++k) {
+    HIP_TRY(hipGraphCreate(&g, 0));
+    Sink k;
+    k.graph = g;
+    for (int step = 0; step < kGraphSteps; ++step) {
       if (pass == 0)
-        enqueue_level<NW>(c, b, split_out, split_in);
+        enqueue_level<NW>(k, c, b, split_out, split_in);
       else
-        enqueue_depth<NW>(c, b, split_out);
+        enqueue_depth<NW>(k, c, b, split_out);
     }
-    const hipError_t le = hipGetLastError();
-    const hipError_t ee = hipStreamEndCapture(s, &g);
-    if (le != hipSuccess || ee != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(AXBC_EHIP, "graph capture failed: %s (line %d)", hipGetErrorString(le != hipSuccess ? le : ee),
-                  __LINE__);
+    if (k.err != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return fail(AXBC_EHIP, "graph construction failed: %s (line %d)", hipGetErrorString(k.err), __LINE__);
     }
     hipGraphExec_t ge = nullptr;
     const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -1801,7 +1847,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   }
   // split kernels only where some vertex has more edges than a chunk
   const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
-  // the level loop runs from captured graphs unless per-kernel timing is on
+  // the level loop runs from prebuilt graphs unless per-kernel timing is on
   const bool graphs = c->use_graphs && !c->ktimer.on;
   if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
   HIP_TRY(hipMemcpyAsync(b->src, src_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
@@ -1839,8 +1885,10 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     if (graphs) {
       for (int j = 0; j < steps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gfwd, s));
     } else {
-      for (int j = 0; j < steps; ++j) enqueue_level<NW>(c, b, split_out, split_in);
-      HIP_TRY(hipGetLastError());
+      Sink k;
+      k.stream = s;
+      for (int j = 0; j < steps; ++j) enqueue_level<NW>(k, c, b, split_out, split_in);
+      HIP_TRY(k.err);
     }
     L += steps;
     if ((rc = pull_bctl(c))) return rc;
@@ -1874,7 +1922,10 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     if (graphs) {
       for (int j = 0; j < dsteps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gbwd, s));
     } else {
-      for (int j = 0; j < dsteps; ++j) enqueue_depth<NW>(c, b, split_out);
+      Sink k;
+      k.stream = s;
+      for (int j = 0; j < dsteps; ++j) enqueue_depth<NW>(k, c, b, split_out);
+      HIP_TRY(k.err);
     }
   }
   HIP_TRY(hipGetLastError());

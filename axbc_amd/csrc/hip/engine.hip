@@ -832,7 +832,8 @@ int axbc_kernel_stats(axbc_ctx *c, int kind, double *ms, uint64_t *launches, uin
   if (rc) return rc;
   unsigned long long b = 0;
   if (c->ktimer.dbytes)
-    HIP_TRY(hipMemcpy(&b, c->ktimer.dbytes + kind, sizeof(b), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(&b, c->ktimer.dbytes + kind, sizeof(b), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   if (ms) *ms = c->ktimer.ms[kind];
   if (launches) *launches = c->ktimer.launches[kind];
   if (bytes) *bytes = b;
@@ -999,7 +1000,8 @@ int axbc_forward_level(axbc_ctx *c, int level, uint32_t *nfrt) {
   if ((rc = enqueue_forward(c, level))) return rc;
   if ((rc = pull_ctl(c))) return rc;
   uint32_t q[2] = {0, 0};
-  HIP_TRY(hipMemcpy(q, c->qoff + level, sizeof(q), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(q, c->qoff + level, sizeof(q), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   if (nfrt) *nfrt = c->h_ctl->done && (int)c->h_ctl->empty_lev <= level ? 0u : q[1] - q[0];
   return AXBC_OK;
 }
@@ -1019,8 +1021,9 @@ int axbc_reach_sum(axbc_ctx *c, uint32_t *all) {
   if ((rc = pull_ctl(c))) return rc;
   std::vector<uint32_t> q(c->h_ctl->tail);
   std::vector<int> l(c->n);
-  HIP_TRY(hipMemcpy(q.data(), c->queue, sizeof(uint32_t) * q.size(), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(l.data(), c->lvl, sizeof(int) * c->n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(q.data(), c->queue, sizeof(uint32_t) * q.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(l.data(), c->lvl, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   uint32_t s = 0;
   for (uint32_t v : q)
     if (l[v] > 0) s += c->h_reach[v];

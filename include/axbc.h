@@ -190,7 +190,7 @@ int axbc_set_batch_chunk(axbc_ctx *ctx, uint32_t chunk);
  * top-down push would scan; 0 keeps every level top-down.  Default 32.
  * Results do not depend on it. */
 int axbc_set_batch_pull(axbc_ctx *ctx, uint32_t ratio16);
-/* Level loop of the batched sweep: 1 (default) replays captured HIP graphs
+/* Level loop of the batched sweep: 1 (default) replays prebuilt HIP graphs
  * of 8 levels (depths) each, the kernels reading the level from device
  * memory, so a level costs no host launches; 0 launches every kernel from
  * the host.  Kernel timing (axbc_kernel_timing) always uses host launches.
