@@ -80,3 +80,39 @@ def test_admission_by_footprint(require_gpu):
     (d,) = srv.drain(timeout=120)
     srv.close()
     assert d.error and "exceeds" in d.error
+
+
+def test_concurrent_graph_builds(require_gpu):
+    """Contexts on several threads rebuild their level-loop graphs (chunk
+    changes) while the others launch, copy and build graphs on the device;
+    every thread's batches must succeed and match its own single-thread
+    result (graph construction touches no stream)."""
+    import threading
+    from axbc_amd import Engine, Graph, _ffi
+    from axbc_amd.pivots import PivotLoop
+
+    def work(seed, out):
+        try:
+            g = Graph.kronecker_device(12 + seed % 3, 16, seed=(seed + 1, seed + 2), heuristic=1)
+            with Engine(g) as e:
+                L = _ffi.hip()
+                for chunk in (128, 256, 512, 1024, 128):
+                    assert L.axbc_set_batch_chunk(e.ctx, chunk) == 0
+                    loop = PivotLoop(e, 256, c=0, x=1, z=1, batch=64)
+                    loop.run()
+                out[seed] = (e.bc(), None)
+        except Exception as ex:  # reported below
+            out[seed] = (None, repr(ex))
+
+    out = {}
+    threads = [threading.Thread(target=work, args=(s, out)) for s in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    errors = {s: err for s, (_, err) in out.items() if err}
+    assert not errors, errors
+    alone = {}
+    work(3, alone)
+    ok, err = close(out[3][0], alone[3][0])
+    assert ok, err
