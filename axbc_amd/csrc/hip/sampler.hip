@@ -255,57 +255,6 @@ __global__ __launch_bounds__(256) void k_lcc_check(const uint32_t *__restrict__ 
   }
 }
 
-// Symmetric, loop-free, strictly sorted lists: |adj(i) cap adj(j)| is the
-// same for (i, j) and (j, i), so every undirected edge is intersected once
-// (from its smaller endpoint) and added to both counters; uint32 atomics
-// wrap mod 2^32 like the reference's counter.
-__global__ __launch_bounds__(256) void k_lcc_small_half(const uint32_t *__restrict__ col,
-                                                        const uint32_t *__restrict__ row, uint32_t n, uint32_t hub_deg,
-                                                        uint32_t *counter) {
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  for (uint32_t i = gw; i < n; i += nwv) {
-    const uint32_t b = col[i], d = col[i + 1] - b;
-    if (d < 2 || d > hub_deg) continue;
-    uint32_t ci = 0;
-    for (uint32_t k = 0; k < d; ++k) {
-      const uint32_t j = row[b + k];
-      if (j < i) continue;
-      const uint32_t c = (uint32_t)wave_sum_u64(pair_count(col, row, i, j));
-      ci += c;
-      if (lane_id() == 0 && c) atomicAdd(&counter[j], c);
-    }
-    if (lane_id() == 0 && ci) atomicAdd(&counter[i], ci);
-  }
-}
-
-// The hubs' edges (i, j) with j > i, one wave per item.
-__global__ __launch_bounds__(256) void k_lcc_hubs_half(const uint32_t *__restrict__ col,
-                                                       const uint32_t *__restrict__ row,
-                                                       const uint32_t *__restrict__ hubs,
-                                                       const uint64_t *__restrict__ hoff, uint32_t nh,
-                                                       uint32_t *counter) {
-  const uint64_t total = hoff[nh];
-  const uint64_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  for (uint64_t t = gw; t < total; t += nwv) {
-    uint32_t lo = 0, hi = nh;  // hoff[lo] <= t < hoff[lo+1]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (hoff[mid] <= t)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const uint32_t i = hubs[lo];
-    const uint32_t j = row[col[i] + (uint32_t)(t - hoff[lo])];
-    if (j < i) continue;
-    const uint32_t c = (uint32_t)wave_sum_u64(pair_count(col, row, i, j));
-    if (lane_id() == 0 && c) {
-      atomicAdd(&counter[i], c);
-      atomicAdd(&counter[j], c);
-    }
-  }
-}
-
 // Owner of an undirected edge: the endpoint of larger (degree, id).
 __device__ __forceinline__ bool ranks_below(const uint32_t *__restrict__ col, uint32_t s, uint32_t o) {
   const uint32_t ds = col[s + 1] - col[s], d_o = col[o + 1] - col[o];
