@@ -1510,6 +1510,36 @@ __global__ void k_bcopy_delta(uint32_t n, int s, const double *__restrict__ dlt,
     out[v] = dlt[(size_t)v * B + s];
 }
 
+// Whether every adjacency list is strictly increasing and every edge has its
+// reverse (then the transpose equals the CSC); bit 0 of *bad set otherwise.
+// A wave per vertex, the reverse found by binary search.
+__global__ __launch_bounds__(kBlock) void k_mirror_check(const uint32_t *__restrict__ col,
+                                                         const uint32_t *__restrict__ row, uint32_t n,
+                                                         unsigned int *bad) {
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t i = gw; i < n; i += nwv) {
+    const uint32_t b = col[i], d = col[i + 1] - b;
+    bool ok = true;
+    for (uint32_t k = lane_id(); k < d; k += kWave) {
+      const uint32_t j = row[b + k];
+      ok &= j < n && (k == 0 || row[b + k - 1] < j);
+      if (ok) {
+        const uint32_t *a = row + col[j];
+        uint32_t lo = 0, hi = col[j + 1] - col[j];
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (a[mid] < i)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        ok = lo < col[j + 1] - col[j] && a[lo] == i;
+      }
+    }
+    if (__ballot(!ok) && lane_id() == 0) atomicOr(bad, 1u);
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1552,6 +1582,31 @@ int balloc(BatchState *b, T **p, size_t count) {
 // CSC; otherwise the transpose is built on the host once.
 int ensure_transpose(axbc_ctx *c) {
   if (c->rcol) return AXBC_OK;
+  // strictly increasing lists in which every edge has its reverse: the
+  // transpose is the CSC itself, decided on the device without copying it
+  {
+    unsigned int *bad = nullptr;
+    HIP_TRY(hipMalloc((void **)&bad, sizeof(unsigned int)));
+    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(unsigned int), c->stream));
+    hipLaunchKernelGGL(k_mirror_check, dim3(4096), dim3(kBlock), 0, c->stream, c->col, c->row, c->n, bad);
+    unsigned int hb = 1;
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e3 = hipStreamSynchronize(c->stream);
+    (void)hipFree(bad);
+    HIP_TRY(e1);
+    HIP_TRY(e2);
+    HIP_TRY(e3);
+    if (!hb) {
+      uint32_t mx = 0;
+      for (uint32_t v = 0; v < c->n; ++v) mx = std::max(mx, c->h_col[v + 1] - c->h_col[v]);
+      c->max_out = c->max_in = mx;
+      c->symmetric = true;
+      c->rcol = c->col;
+      c->rrow = c->row;
+      return AXBC_OK;
+    }
+  }
   std::vector<uint32_t> col(c->n + 1), row(c->ned);
   HIP_TRY(hipMemcpyAsync(col.data(), c->col, sizeof(uint32_t) * (c->n + 1), hipMemcpyDeviceToHost, c->stream));
   if (c->ned)
