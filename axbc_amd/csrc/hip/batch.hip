@@ -963,11 +963,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
                                                    uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
-                                                   const uint32_t *sig, double *xr, const uint32_t *__restrict__ q,
+                                                   const uint32_t *__restrict__ q,
                                                    uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t *seen, uint32_t qcap,
                                                    const uint32_t *__restrict__ rcol, unsigned long long *kb) {
-  constexpr int B = 64 * NW;
   if (ld_flag(&ctl->done)) return;
   const int L = ctl->level;
   const uint32_t stamp = ctl->stamp;
@@ -1016,40 +1015,24 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
       }
       seen[w] = stamp;
     }
+    // per-root counters of the level, entry by entry with lane = root (x is
+    // set by the dependency pass, or by k_bleaf on the deepest level)
     const uint32_t cntv = (uint32_t)__popcll(__ballot(valid));
-    uint32_t roots = 0;
-    for (uint32_t k0 = 0; k0 < cntv; k0 += kBurst) {
-      uint32_t sg[kBurst][NW];
+    for (uint32_t k = 0; k < cntv; ++k) {
+      const uint32_t rk = __shfl(rw, (int)k), dk = __shfl(dw, (int)k);
 #pragma unroll
-      for (int t = 0; t < kBurst; ++t) {
-        const int k = (int)(k0 + t) & (kWave - 1);
-        const uint32_t wk = __shfl(w, k);
-#pragma unroll
-        for (int j = 0; j < NW; ++j) {
-          const uint64_t mk = k0 + t < cntv ? __shfl(nw.w[j], k) : 0ull;
-          sg[t][j] = bit(mk, lane) ? sig[(size_t)wk * B + lane + 64 * j] : 1u;
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < kBurst; ++t) {
-        const int k = (int)(k0 + t) & (kWave - 1);
-        const uint32_t wk = __shfl(w, k), rk = __shfl(rw, k), dk = __shfl(dw, k);
-#pragma unroll
-        for (int j = 0; j < NW; ++j) {
-          const uint64_t mk = k0 + t < cntv ? __shfl(nw.w[j], k) : 0ull;
-          roots += (uint32_t)__popcll(mk);
-          if (bit(mk, lane)) {
-            xr[(size_t)wk * B + lane + 64 * j] = ((double)rk + 1.0) / (double)sg[t][j];
-            c_vis[j] += 1;
-            c_all[j] += rk;
-            c_er[j] += dk;
-          }
+      for (int j = 0; j < NW; ++j) {
+        const uint64_t mk = __shfl(nw.w[j], (int)k);
+        if (bit(mk, lane)) {
+          c_vis[j] += 1;
+          c_all[j] += rk;
+          c_er[j] += dk;
         }
       }
     }
     // entry: id 4 + NX 8NW + reach 4 + offsets 8 + seen 4; writes FR, VIS
-    // (read+write), list word, NX: 5 * 8NW; per root reached: sigma 4 + x 8
-    bytes += (unsigned long long)cntv * (20 + 48 * NW) + 12ull * roots;
+    // (read+write), list word, NX: 5 * 8NW
+    bytes += (unsigned long long)cntv * (20 + 48 * NW);
   }
   fe = wave_sum_u64(fe);
   full = wave_sum_u64(full);
@@ -1097,6 +1080,44 @@ __global__ void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uin
   ctl->fe_edges = 0;
   ctl->split_claim = 0;
   if (!ctl->done) ctl->level = L + 1;
+}
+
+// x = (reach + 1) / sigma for the entries of the deepest level L of the
+// sweep: they have no dependencies, and the dependency pass (which writes x
+// of every shallower entry) never visits them.  A wave per 64 entries, then
+// entry by entry with lane = root.
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bleaf(const uint32_t *__restrict__ q,
+                                                  const uint64_t *__restrict__ qw,
+                                                  const uint32_t *__restrict__ qoff, int L,
+                                                  const uint32_t *__restrict__ reach,
+                                                  const uint32_t *__restrict__ sig, double *xr) {
+  constexpr int B = 64 * NW;
+  const uint32_t nb = qoff[L], ne = qoff[L + 1];
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t base = nb + gw * kWave; base < ne; base += nwv * kWave) {
+    const uint32_t i = base + lane;
+    uint32_t w = 0, rw = 0;
+    Words<NW> f;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) f.w[j] = 0;
+    if (i < ne) {
+      w = q[i];
+      f = ldw<NW>(qw, i);
+      rw = reach[w];
+    }
+    const uint32_t cntv = (uint32_t)__popcll(__ballot(i < ne));
+    for (uint32_t k = 0; k < cntv; ++k) {
+      const uint32_t wk = __shfl(w, (int)k), rk = __shfl(rw, (int)k);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const uint64_t mk = __shfl(f.w[j], (int)k);
+        if (bit(mk, lane))
+          xr[(size_t)wk * B + lane + 64 * j] = ((double)rk + 1.0) / (double)sig[(size_t)wk * B + lane + 64 * j];
+      }
+    }
+  }
 }
 
 // e_r of the roots (degree of the root itself).
@@ -1800,7 +1821,7 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
   }
   kt_begin(c, KK_CLOSE);
   launch(k, k_bclose<NW>, dim3(kBGrid / 2), dim3(kBlock), c->col, c->reach, b->nx, b->vis, b->fr[0],
-                     b->fr[1], b->sig, b->xr, b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->seen, b->qcap, c->rcol,
+                     b->fr[1], b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->seen, b->qcap, c->rcol,
                      kt_bytes(c, KK_CLOSE));
   kt_end(c, KK_CLOSE);
   launch(k, k_bclose_end, dim3(1), dim3(64), b->ctl, b->qoff, b->qcap, b->pull16);
@@ -1967,6 +1988,10 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       if ((b->h_act[(size_t)lv * NW + l / 64] >> (l % 64)) & 1ull) nlev[l] = lv;
     if (nlev[l] == 1) rootmask[l / 64] |= 1ull << (l % 64);
   }
+  // x of the deepest level's entries (the dependency pass sets the others)
+  if (dmax >= 1)
+    hipLaunchKernelGGL(k_bleaf<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, dmax, c->reach, b->sig,
+                       b->xr);
   // backward: depths dmax-1 .. dmin, dmin = 0 only for the roots with a
   // single level
   const int dmin = (rootmask[0] | rootmask[1]) ? 0 : 1;
