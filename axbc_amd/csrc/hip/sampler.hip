@@ -285,6 +285,56 @@ __global__ __launch_bounds__(256) void k_lcc_light(const uint32_t *__restrict__ 
   }
 }
 
+// Owners of kLccBit .. kLccHash edges: a workgroup per owner keeps an
+// open-addressing hash of adj(o) in LDS (load factor <= 1/2, linear
+// probing; the lists hold distinct ids), and its waves take the owned
+// neighbours s in turn, lanes walking adj(s) and probing the table.
+constexpr uint32_t kLccHash = 4096;
+constexpr uint32_t kHashSlots = 2 * kLccHash;
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t x, uint32_t mask) { return (x * 2654435761u) & mask; }
+
+__global__ __launch_bounds__(256) void k_lcc_hash(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ owner, uint32_t nown,
+                                                  uint32_t *counter) {
+  __shared__ uint32_t tab[kHashSlots];
+  const uint32_t wid = threadIdx.x / kWave, nwg = blockDim.x / kWave;
+  for (uint32_t oi = blockIdx.x; oi < nown; oi += gridDim.x) {
+    const uint32_t o = owner[oi];
+    const uint32_t b = col[o], d = col[o + 1] - b;
+    uint32_t size = 64;
+    while (size < 2 * d) size <<= 1;
+    const uint32_t mask = size - 1;
+    for (uint32_t t = threadIdx.x; t < size; t += blockDim.x) tab[t] = kHashEmpty;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < d; k += blockDim.x) {
+      const uint32_t x = row[b + k];
+      uint32_t h = hash_slot(x, mask);
+      while (atomicCAS(&tab[h], kHashEmpty, x) != kHashEmpty) h = (h + 1) & mask;
+    }
+    __syncthreads();
+    uint32_t co = 0;
+    for (uint32_t k = wid; k < d; k += nwg) {
+      const uint32_t sv = row[b + k];
+      if (!ranks_below(col, sv, o)) continue;
+      uint32_t c = 0;
+      for (uint32_t l = col[sv] + lane_id(); l < col[sv + 1]; l += kWave) {
+        const uint32_t x = row[l];
+        uint32_t h = hash_slot(x, mask);
+        uint32_t y;
+        while ((y = tab[h]) != kHashEmpty && y != x) h = (h + 1) & mask;
+        c += y == x ? 1u : 0u;
+      }
+      c = (uint32_t)wave_sum_u64(c);
+      co += c;
+      if (lane_id() == 0 && c) atomicAdd(&counter[sv], c);
+    }
+    if (lane_id() == 0 && co) atomicAdd(&counter[o], co);
+    __syncthreads();
+  }
+}
+
 // Bitmaps of the adjacency of a batch of owners (one N-bit row per slot):
 // set (on = 1) or clear (on = 0) the words their neighbours touch.
 __global__ __launch_bounds__(256) void k_lcc_bits(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
@@ -562,19 +612,39 @@ int axbc_lcc_build(axbc_ctx *c, float *lcc_host, uint64_t *lw_host, uint64_t *lp
   HIP_TRY(hipMemsetAsync(sb->lcc_cnt, 0, sizeof(uint32_t) * n, s));
   if (sb->lcc_sym == 1) {
     // every undirected edge once, from its owner (larger degree, id):
-    // short owners binary-search, owners of kLccBit or more edges test a
-    // bitmap of their adjacency, built for a batch of them at a time
+    // short owners binary-search, owners of kLccBit .. kLccHash edges probe
+    // an LDS hash of their adjacency, larger ones test a bitmap of it, built
+    // for a batch of them at a time
     constexpr uint32_t kLccBit = 256;
     constexpr uint64_t kBitBytes = 512ull << 20;
     hipLaunchKernelGGL(k_lcc_light, dim3(4096), dim3(256), 0, s, c->col, c->row, n, kLccBit, sb->lcc_cnt);
-    std::vector<uint32_t> heavy;
+    // the LDS hash pays once an owner's bitmap (n bits) no longer stays in
+    // cache: from 2^21 vertices (256 KB bitmaps) on
+    const uint32_t hash_max = n >= (1u << 21) ? kLccHash : kLccBit - 1;
+    std::vector<uint32_t> heavy, mid;
     std::vector<uint64_t> ioff(1, 0);
     for (uint32_t v = 0; v < n; ++v) {
       const uint32_t d = c->h_col[v + 1] - c->h_col[v];
-      if (d >= kLccBit) {
+      if (d > hash_max) {
         heavy.push_back(v);
         ioff.push_back(ioff.back() + d);
+      } else if (d >= kLccBit) {
+        mid.push_back(v);
       }
+    }
+    if (!mid.empty()) {
+      uint32_t *dm = nullptr;
+      if (hipMalloc((void **)&dm, sizeof(uint32_t) * mid.size()) != hipSuccess) {
+        (void)hipFree(dh);
+        (void)hipFree(doff);
+        return fail(AXBC_ENOMEM, "LCC owner list%s (line %d)", "", __LINE__);
+      }
+      HIP_TRY(hipMemcpyAsync(dm, mid.data(), sizeof(uint32_t) * mid.size(), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_lcc_hash, dim3((uint32_t)std::min<size_t>(mid.size(), 2048)), dim3(256), 0, s, c->col,
+                         c->row, dm, (uint32_t)mid.size(), sb->lcc_cnt);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(s));
+      (void)hipFree(dm);
     }
     if (!heavy.empty()) {
       const uint64_t words = ((uint64_t)n + 31) / 32;

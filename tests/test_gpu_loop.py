@@ -56,10 +56,16 @@ def test_lcc_matches_oracle(require_gpu):
     with a loop take the per-direction kernels."""
     import ctypes as C
     from axbc_amd import _ffi
+    from axbc_amd import kronecker_edge_list
+    # a SCALE-12 edge list among 2^21 ids: from 2^21 vertices on, owners of
+    # 256..4096 edges probe an LDS hash instead of a bitmap
+    sparse = Graph.from_edges(1 << 21, kronecker_edge_list(12, 1 << 16, 5, 6), heuristic=1)
     graphs = (Graph.kronecker(12, 16, seed=(1, 2)), Graph.kronecker(11, 32, seed=(7, 8), undirected=False),
-              Graph.kronecker(16, 16, seed=(2, 3), heuristic=1), _looped())
+              Graph.kronecker(16, 16, seed=(2, 3), heuristic=1), _looped(), sparse)
     assert int(np.diff(graphs[2].col).max()) > 4096
     assert graphs[3].row[graphs[3].col[0]] == 0
+    sd = np.diff(sparse.col)
+    assert ((sd >= 256) & (sd <= 4096)).any()
     for g in graphs:
         print(f"[lcc] N={g.N} ned={g.ned}", flush=True)
         lcc_ref = np.zeros(g.N, np.float32)
