@@ -50,7 +50,7 @@ namespace axbc {
 
 constexpr uint32_t kChunkMin = 64;  // smallest edges-per-work-item for split vertices
 #ifndef AXBC_BURST
-#define AXBC_BURST 2
+#define AXBC_BURST 3
 #endif
 constexpr int kBurst = AXBC_BURST;  // row loads in flight per wave in the pull loops
 // Waves per SIMD asked of the compiler for the frontier kernels (a 64-VGPR
@@ -169,6 +169,27 @@ __device__ __forceinline__ Words<NW> shflw(const Words<NW> &x, int src) {
   return r;
 }
 
+// Value of x in lane i for a wave-uniform i (from a ballot, a loop counter or
+// a constant): one scalar read of that lane (the result lives in scalar
+// registers) instead of a cross-lane permute through the LDS unit.
+__device__ __forceinline__ uint32_t rdl(uint32_t x, int i) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, i);
+}
+__device__ __forceinline__ uint64_t rdl(uint64_t x, int i) {
+  return ((uint64_t)rdl((uint32_t)(x >> 32), i) << 32) | rdl((uint32_t)x, i);
+}
+__device__ __forceinline__ double rdl(double x, int i) {
+  return __builtin_bit_cast(double, rdl(__builtin_bit_cast(uint64_t, x), i));
+}
+
+template <int NW>
+__device__ __forceinline__ Words<NW> rdlw(const Words<NW> &x, int i) {
+  Words<NW> r;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) r.w[j] = rdl(x.w[j], i);
+  return r;
+}
+
 // Split vertices.  An entry with more than `ch` edges is claimed with one
 // 64-bit atomic that returns both its slot k and the first of its
 // ceil(d/ch) chunk items (split_claim = entries << 40 | items), so sp_base[]
@@ -276,7 +297,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_breset(uint32_t n, co
       const int i = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
       const uint32_t vv = vb + (uint32_t)i;
-      const double rr = __shfl(r, i);
+      const double rr = rdl(r, i);
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         sig[(size_t)vv * B + lane + 64 * j] = 0;
@@ -422,7 +443,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
       d = 0;
     }
     const uint32_t incl = wave_incl_scan(d);
-    const uint32_t total = __shfl(incl, kWave - 1);
+    const uint32_t total = rdl(incl, kWave - 1);
     // entries: id 4 + offsets 8 + word 8NW; per edge: id 4 + VIS 8NW
     bytes += (unsigned long long)__popcll(__ballot(i < fe)) * (12 + 8 * NW) +
              (unsigned long long)total * (4 + 8 * NW);
@@ -445,7 +466,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
         const int leader = __ffsll((long long)m) - 1;
         uint32_t b = 0;
         if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
-        b = __shfl(b, leader);
+        b = rdl(b, leader);
         const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (first) {
           if (slot < qcap)
@@ -506,7 +527,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush_split(const uin
         const int leader = __ffsll((long long)m) - 1;
         uint32_t b = 0;
         if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
-        b = __shfl(b, leader);
+        b = rdl(b, leader);
         const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (first) {
           if (slot < qcap)
@@ -553,10 +574,10 @@ __device__ __forceinline__ void sigma_chunk(uint32_t st, uint32_t cb, uint32_t c
         const int i = todo ? __ffsll((long long)todo) - 1 : 0;
         const bool live = todo != 0;
         todo &= todo - 1;
-        const uint32_t uu = __shfl(u, i);
+        const uint32_t uu = rdl(u, i);
 #pragma unroll
         for (int j = 0; j < NW; ++j) {
-          const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
+          const uint64_t mm = live ? rdl(m.w[j], i) : 0ull;
           *contrib += (uint32_t)__popcll(mm);
           found[j] |= mm;
           got[t][j] = bit(mm, lane) ? sig[(size_t)uu * B + lane + 64 * j] : 0u;
@@ -586,7 +607,7 @@ __device__ __forceinline__ void group_sigma(uint32_t st, uint32_t d, const Words
   constexpr int B = 64 * NW;
   const int lane = lane_id();
   const uint32_t incl = wave_incl_scan(d);
-  const uint32_t total = __shfl(incl, kWave - 1);
+  const uint32_t total = rdl(incl, kWave - 1);
   int cur = -1;
   uint32_t acc[NW];
   uint64_t found[NW];
@@ -619,11 +640,11 @@ __device__ __forceinline__ void group_sigma(uint32_t st, uint32_t d, const Words
         const int i = todo ? __ffsll((long long)todo) - 1 : 0;
         const bool live = todo != 0;
         todo &= todo - 1;
-        const uint32_t uu = __shfl(u, i);
-        ot[t] = live ? __shfl(own, i) : -1;
+        const uint32_t uu = rdl(u, i);
+        ot[t] = live ? (int)rdl((uint32_t)own, i) : -1;
 #pragma unroll
         for (int j = 0; j < NW; ++j) {
-          const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
+          const uint64_t mm = live ? rdl(m.w[j], i) : 0ull;
           mmt[t][j] = mm;
           *contrib += (uint32_t)__popcll(mm);
           got[t][j] = bit(mm, lane) ? sig[(size_t)uu * B + lane + 64 * j] : 0u;
@@ -690,7 +711,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
     uint32_t contrib = 0;
     group_sigma<NW>(st, d, nw, rrow, frp, sig, dag_tot, &contrib,
                     [&](int o, const uint32_t *acc, const uint64_t *found) {
-                      const uint32_t wo = __shfl(w, o);
+                      const uint32_t wo = rdl(w, o);
 #pragma unroll
                       for (int j = 0; j < NW; ++j)
                         if (bit(found[j], lane)) sig[(size_t)wo * B + lane + 64 * j] = acc[j];
@@ -834,8 +855,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
       const int i = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
       const uint32_t w = vb + (uint32_t)i;
-      const uint32_t sw = __shfl(st, i), dw = __shfl(d, i);
-      const Words<NW> ow = shflw<NW>(open, i);
+      const uint32_t sw = rdl(st, i), dw = rdl(d, i);
+      const Words<NW> ow = rdlw<NW>(open, i);
       uint32_t acc[NW], dag[NW];
       uint64_t found[NW] = {};
 #pragma unroll
@@ -860,7 +881,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
     if (fresh) {
       uint32_t b = 0;
       if (lane == 0) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(fresh));
-      b = __shfl(b, 0);
+      b = rdl(b, 0);
       if (bit(fresh, lane)) {
         const uint32_t slot = b + (uint32_t)__popcll(fresh & ((1ull << lane) - 1ull));
         if (slot < qcap)
@@ -1019,10 +1040,10 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
     // set by the dependency pass, or by k_bleaf on the deepest level)
     const uint32_t cntv = (uint32_t)__popcll(__ballot(valid));
     for (uint32_t k = 0; k < cntv; ++k) {
-      const uint32_t rk = __shfl(rw, (int)k), dk = __shfl(dw, (int)k);
+      const uint32_t rk = rdl(rw, (int)k), dk = rdl(dw, (int)k);
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
-        const uint64_t mk = __shfl(nw.w[j], (int)k);
+        const uint64_t mk = rdl(nw.w[j], (int)k);
         if (bit(mk, lane)) {
           c_vis[j] += 1;
           c_all[j] += rk;
@@ -1109,10 +1130,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bleaf(const uint32_t 
     }
     const uint32_t cntv = (uint32_t)__popcll(__ballot(i < ne));
     for (uint32_t k = 0; k < cntv; ++k) {
-      const uint32_t wk = __shfl(w, (int)k), rk = __shfl(rw, (int)k);
+      const uint32_t wk = rdl(w, (int)k), rk = rdl(rw, (int)k);
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
-        const uint64_t mk = __shfl(f.w[j], (int)k);
+        const uint64_t mk = rdl(f.w[j], (int)k);
         if (bit(mk, lane))
           xr[(size_t)wk * B + lane + 64 * j] = ((double)rk + 1.0) / (double)sig[(size_t)wk * B + lane + 64 * j];
       }
@@ -1176,10 +1197,10 @@ __device__ __forceinline__ uint32_t dep_chunk(uint32_t st, uint32_t cb, uint32_t
         const int i = todo ? __ffsll((long long)todo) - 1 : 0;
         const bool live = todo != 0;
         todo &= todo - 1;
-        const uint32_t ww = __shfl(w, i);
+        const uint32_t ww = rdl(w, i);
 #pragma unroll
         for (int j = 0; j < NW; ++j) {
-          const uint64_t mm = live ? __shfl(m.w[j], i) : 0ull;
+          const uint64_t mm = live ? rdl(m.w[j], i) : 0ull;
           contrib += (uint32_t)__popcll(mm);
           use[t][j] = bit(mm, lane);
           got[t][j] = use[t][j] ? xr[(size_t)ww * B + lane + 64 * j] : 0.0;

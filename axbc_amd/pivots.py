@@ -12,6 +12,7 @@ every root inside the batch, so the result equals the one-root loop.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -61,6 +62,9 @@ class LoopResult:
 class PivotLoop:
     """Stateful pivot loop: draws follow libc's lrand48 stream."""
 
+    # batched -z 0/1 loops draw the next batch's roots while a batch runs
+    draw_ahead = True
+
     def __init__(self, engine: Engine, nbfs: int, c: int = 1, x: int = 1, z: int = 0,
                  seed: int = DEFAULT_SEED, batch: int = 0):
         if z not in range(7):
@@ -104,6 +108,8 @@ class PivotLoop:
         self.vs = np.zeros(self.N, np.uint8)
         self.ui = 0
         self.res = LoopResult()
+        self._marks = None  # vertices marked in vs while drawing ahead
+        self._ahead = None  # roots drawn during the previous batch
 
     # -- sampling ---------------------------------------------------------
     def _fixed_draw(self, total: int) -> int:
@@ -193,6 +199,8 @@ class PivotLoop:
                 bsm += 1
                 continue
             break
+        if v0 < self.N and self._marks is not None:
+            self._marks.append(v0)
         if v0 >= self.N or self.deg[v0] == 0:
             self.res.skipped += 1
             if v0 < self.N:
@@ -201,17 +209,96 @@ class PivotLoop:
         self.vs[v0] = 1
         return v0, p
 
+    def _collect(self, want: int):
+        """Draw the next roots, up to `want` for a batch and one otherwise:
+        (roots, attempt index after each, skipped count before each)."""
+        res = self.res
+        pend, pend_ui, pend_skip = [], [], []
+        while self.ui < self.nbfs and len(pend) < want:
+            skipped_before = res.skipped
+            r = self._next_root()
+            self.ui += 1
+            if r is None:
+                continue
+            pend.append(r)
+            pend_ui.append(self.ui)
+            pend_skip.append(skipped_before)
+            if not self.batch:
+                break
+        return pend, pend_ui, pend_skip
+
+    # -- drawing ahead ------------------------------------------------------
+    # Static exact distributions (-z 0, 1) do not depend on the rounds run,
+    # so the next batch's roots are drawn on a host thread while the device
+    # runs the current batch (the foreign call releases the interpreter
+    # lock).  Between steps the attempt and skip counts show the rounds
+    # taken so far, as without drawing ahead; the draws are undone
+    # (generator state, sampled marks) when they are not used as drawn: the
+    # loop stops, or the next step asks for another count.
+    def _draw_ahead(self, want: int):
+        """Start drawing the next `want` roots: (thread, box, want, state
+        before).  The thread leaves the roots, the counts after them and the
+        marks it set in box."""
+        snap = (tuple(self.rng.x), self.ui, self.res.skipped)
+        box = {}
+
+        def work():
+            self._marks = []
+            try:
+                box["drawn"] = self._collect(want)
+                box["after"] = (self.ui, self.res.skipped)
+            except BaseException as ex:  # raised again by the joiner
+                box["error"] = ex
+            box["marks"] = self._marks
+            self._marks = None
+
+        t = threading.Thread(target=work, daemon=True)
+        t.start()
+        return t, box, want, snap
+
+    def _join_ahead(self):
+        """Wait for the draws; the counts go back to the state before them."""
+        t, box, _, (_, ui, skipped) = self._ahead
+        t.join()
+        self.ui, self.res.skipped = ui, skipped
+        if "error" in box:
+            self._undo_ahead()
+            raise box["error"]
+
+    def _take_ahead(self):
+        """The roots drawn ahead, with the counts moved past them."""
+        _, box, _, _ = self._ahead
+        self._ahead = None
+        self.ui, self.res.skipped = box["after"]
+        return box["drawn"]
+
+    def _undo_ahead(self):
+        """Discard the roots drawn ahead and restore the sampler state."""
+        if self._ahead is None:
+            return
+        t, box, _, (x, ui, skipped) = self._ahead
+        t.join()
+        self._ahead = None
+        for j, v in enumerate(x):
+            self.rng.x[j] = v
+        self.ui, self.res.skipped = ui, skipped
+        for v in box.get("marks", ()):
+            self.vs[v] = 0
+
     # -- running ----------------------------------------------------------
     def step(self, max_rounds: int | None = None) -> int:
         """Run rounds until max_rounds (a batch for batched loops) or the
         end; returns the rounds applied (0 when the loop is over)."""
         res, e = self.res, self.e
-        if res.stopped or self.ui >= self.nbfs:
+        want = max_rounds or (self.batch or 1)
+        if self._ahead is not None and (self._ahead[2] != want or res.stopped):
+            self._undo_ahead()
+        if self._ahead is None and (res.stopped or self.ui >= self.nbfs):
             return 0
         if self.c > 0 and e.stop_test(self.c):
+            self._undo_ahead()
             res.stopped = True
             return 0
-        want = max_rounds or (self.batch or 1)
         if self.batch and self.z in (3, 4, 5) and self.res.nrounds >= self.batch:
             t = C.c_uint64(0)
             _check(_ffi.hip().axbc_weights(e.ctx, self.z, C.byref(t)))
@@ -227,18 +314,10 @@ class PivotLoop:
             _check(L.axbc_weights(e.ctx, 4, C.byref(t)))
             _check(L.axbc_weights_copy(e.ctx, self._wd6.ctypes.data))
             self._td6 = int(t.value)
-        pend, pend_ui, pend_skip = [], [], []
-        while self.ui < self.nbfs and len(pend) < want:
-            skipped_before = res.skipped
-            r = self._next_root()
-            self.ui += 1
-            if r is None:
-                continue
-            pend.append(r)
-            pend_ui.append(self.ui)
-            pend_skip.append(skipped_before)
-            if not self.batch:
-                break
+        if self._ahead is not None:
+            pend, pend_ui, pend_skip = self._take_ahead()
+        else:
+            pend, pend_ui, pend_skip = self._collect(want)
         if not pend:
             return 0
         t = time.perf_counter()
@@ -253,10 +332,20 @@ class PivotLoop:
             pp = np.array([b for _, b in pend], np.float64)
             st = (_ffi.PivotStats * len(v))()
             ap = C.c_int(0)
-            _check(L.axbc_batch(e.ctx, v, pp, len(v), self.c, C.byref(ap), C.cast(st, C.c_void_p)))
+            if self.draw_ahead and self.z in (0, 1) and self.ui < self.nbfs:
+                self._ahead = self._draw_ahead(want)
+            try:
+                rc = L.axbc_batch(e.ctx, v, pp, len(v), self.c, C.byref(ap), C.cast(st, C.c_void_p))
+            finally:
+                if self._ahead is not None:
+                    self._join_ahead()
+            if rc != 0:
+                self._undo_ahead()
+                _check(rc)
             applied = ap.value
             res.e_r += sum(int(st[k].e_r) for k in range(applied))
             if applied < len(pend):  # stopped inside the batch
+                self._undo_ahead()
                 res.stopped = True
                 res.skipped = pend_skip[applied - 1]
                 self.ui = pend_ui[applied - 1]

@@ -230,7 +230,20 @@ def cpu_baseline(g, roots, probs, seconds):
     return {"value": round(m / dtm, 4), "unit": "pivots/s", "cores": threads, "kind": "port",
             "sample": f"oracle/brandes_oracle.c on the same graph and roots: {m} pivots over {threads} OpenMP "
                       f"threads in {dtm:.1f} s (GTEPS {2 * erm / dtm / 1e9:.4f}); one thread: {done} pivots in "
-                      f"{dt1:.1f} s ({rate1:.4f} pivots/s, GTEPS {2 * er / dt1 / 1e9:.4f})"}
+                      f"{dt1:.1f} s ({rate1:.4f} pivots/s, GTEPS {2 * er / dt1 / 1e9:.4f}); host: {cpu_model()}, "
+                      f"nproc {os.cpu_count()}"}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name from /proc/cpuinfo (empty if unreadable)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def main():
