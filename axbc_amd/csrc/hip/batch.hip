@@ -122,6 +122,9 @@ struct BatchState {
   bool have = false;
   int lev_guess = 12;
   uint32_t chunk = 256;  // edges per work item of split vertices (tunable)
+  // sigma rows of the vertices with more than this many pull edges are zero
+  // at a batch start (all rows are after the fill)
+  uint32_t sig_zero_above = 0;
   uint32_t pull16 = 32;  // bottom-up when candidate in-edges <= pull16/16 x push edges (0: never)
   hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // prebuilt level / depth steps
   uint64_t gkey = 0;
@@ -279,35 +282,102 @@ __global__ void k_bfill(uint32_t n, const uint32_t *__restrict__ reach, uint64_t
   }
 }
 
-// Undo the previous batch on the vertices it reached (seen == its stamp),
-// each once: coalesced row stores, a wave per 64 vertices.
+// Undo the previous batch on the vertices it reached (seen == its stamp):
+// their VIS and frontier words, and the path-count rows of the vertices
+// with more than `chunk` pull edges, whose split items add into the row
+// atomically.  Every other row entry is stored before it is read in a batch
+// (sigma when its root reaches the vertex, delta by the dependency pass,
+// k_bleaf, k_broots or k_bfix), so those rows are left as they are.
 template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_breset(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
-                                                   const uint32_t *__restrict__ reach, uint64_t *vis,
-                                                   uint64_t *fr0, uint64_t *fr1, uint32_t *sig, double *dlt) {
+                                                   const uint32_t *__restrict__ rcol, uint32_t chunk,
+                                                   uint64_t *vis, uint64_t *fr0, uint64_t *fr1, uint32_t *sig) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
   for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
     const uint32_t v = vb + (uint32_t)lane;
     const bool hit = v < n && seen[v] == stamp;
-    const double r = hit ? (double)reach[v] : 0.0;
-    unsigned long long todo = __ballot(hit);
+    if (hit) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        vis[(size_t)v * NW + j] = 0;
+        fr0[(size_t)v * NW + j] = 0;
+        fr1[(size_t)v * NW + j] = 0;
+      }
+    }
+    unsigned long long todo = __ballot(hit && rcol[v + 1] - rcol[v] > chunk);
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t vv = vb + (uint32_t)i;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) sig[(size_t)vv * B + lane + 64 * j] = 0;
+    }
+  }
+}
+
+// Zero the path-count rows of every vertex with more than `chunk` pull
+// edges (when the split threshold drops below the one the rows were last
+// cleared for).
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsig_clear(uint32_t n, const uint32_t *__restrict__ rcol,
+                                                       uint32_t chunk, uint32_t *sig) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
+    const uint32_t v = vb + (uint32_t)lane;
+    unsigned long long todo = __ballot(v < n && rcol[v + 1] - rcol[v] > chunk);
+    while (todo) {
+      const int i = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t vv = vb + (uint32_t)i;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) sig[(size_t)vv * B + lane + 64 * j] = 0;
+    }
+  }
+}
+
+// After the forward sweep, before the dependency pass strips VIS: delta =
+// reach for every (vertex, root) the batch did not reach -- the lanes
+// outside VIS of the vertices it reached, and whole rows of the vertices
+// the previous batch reached (seen == prev) and this one did not.  Vertices
+// neither batch reached keep rows equal to reach, so every row entry a
+// reader sees is the batch's delta or, unreached, reach (the reference's
+// init_delta).
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bfix(uint32_t n, const uint32_t *__restrict__ seen, uint32_t stamp,
+                                                 uint32_t prev, int has_prev, const uint64_t *__restrict__ vis,
+                                                 const uint32_t *__restrict__ reach, double *dlt) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
+    const uint32_t v = vb + (uint32_t)lane;
+    const uint32_t st = v < n ? seen[v] : 0u;
+    Words<NW> open;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) open.w[j] = 0;
+    if (v < n && st == stamp) {
+      const Words<NW> vw = ldw<NW>(vis, v);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) open.w[j] = ~vw.w[j];
+    } else if (v < n && has_prev && st == prev) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j) open.w[j] = ~0ull;
+    }
+    const bool any = open.any();
+    const double r = any ? (double)reach[v] : 0.0;
+    unsigned long long todo = __ballot(any);
     while (todo) {
       const int i = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
       const uint32_t vv = vb + (uint32_t)i;
       const double rr = rdl(r, i);
 #pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        sig[(size_t)vv * B + lane + 64 * j] = 0;
-        dlt[(size_t)vv * B + lane + 64 * j] = rr;
-      }
-      if (lane < NW) {
-        vis[(size_t)vv * NW + lane] = 0;
-        fr0[(size_t)vv * NW + lane] = 0;
-        fr1[(size_t)vv * NW + lane] = 0;
-      }
+      for (int j = 0; j < NW; ++j)
+        if (bit(rdl(open.w[j], i), lane)) dlt[(size_t)vv * B + lane + 64 * j] = rr;
     }
   }
 }
@@ -317,7 +387,8 @@ template <int NW>
 __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis, uint64_t *fr0, uint32_t *sig,
                          uint32_t *q, uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
                          unsigned long long *cnt, uint32_t stamp, uint32_t *seen, unsigned long long in_edges,
-                         uint32_t mark_base, uint64_t all0, uint64_t all1) {
+                         uint32_t mark_base, uint64_t all0, uint64_t all1, const uint32_t *__restrict__ reach,
+                         double *dlt) {
   constexpr int B = 64 * NW;
   const int s = threadIdx.x + blockIdx.x * blockDim.x;
   if (s < nb) {
@@ -325,6 +396,8 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     atomicOr((unsigned long long *)&vis[(size_t)v * NW + s / 64], 1ull << (s % 64));
     atomicOr((unsigned long long *)&fr0[(size_t)v * NW + s / 64], 1ull << (s % 64));
     sig[(size_t)v * B + s] = 1;
+    // the root's own delta stays reach unless the depth-0 pass runs
+    dlt[(size_t)v * B + s] = (double)reach[v];
     q[s] = v;
     for (int j = 0; j < NW; ++j) qw[(size_t)s * NW + j] = j == s / 64 ? 1ull << (s % 64) : 0ull;
     seen[v] = stamp;
@@ -1112,7 +1185,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bleaf(const uint32_t 
                                                   const uint64_t *__restrict__ qw,
                                                   const uint32_t *__restrict__ qoff, int L,
                                                   const uint32_t *__restrict__ reach,
-                                                  const uint32_t *__restrict__ sig, double *xr) {
+                                                  const uint32_t *__restrict__ sig, double *xr, double *dlt) {
   constexpr int B = 64 * NW;
   const uint32_t nb = qoff[L], ne = qoff[L + 1];
   const int lane = lane_id();
@@ -1134,8 +1207,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bleaf(const uint32_t 
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         const uint64_t mk = rdl(f.w[j], (int)k);
-        if (bit(mk, lane))
+        if (bit(mk, lane)) {
+          dlt[(size_t)wk * B + lane + 64 * j] = (double)rk;
           xr[(size_t)wk * B + lane + 64 * j] = ((double)rk + 1.0) / (double)sig[(size_t)wk * B + lane + 64 * j];
+        }
       }
     }
   }
@@ -1953,8 +2028,14 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   HIP_TRY(hipMemcpyAsync(b->lane_of, lane_of.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
   // reset the previous batch, then level 0
   if (b->have)
-    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->reach, b->vis,
-                       b->fr[0], b->fr[1], b->sig, b->dlt);
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
+                       b->vis, b->fr[0], b->fr[1], b->sig);
+  // rows of vertices above the split threshold start at zero: clear them
+  // all when the threshold dropped below the one they were cleared for
+  if (b->chunk < b->sig_zero_above)
+    hipLaunchKernelGGL(k_bsig_clear<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, c->rcol, b->chunk, b->sig);
+  b->sig_zero_above = b->chunk;
+  const bool had_prev = b->have;
   b->have = true;
   b->stamp += 1;
   // roots of this batch, per word (the "reached by every root" test)
@@ -1964,7 +2045,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)zeroed * NW, s));
   hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
                      b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, (unsigned long long)c->ned, b->epoch,
-                     all0, all1);
+                     all0, all1, c->reach, b->dlt);
   hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
   HIP_TRY(hipGetLastError());
   // forward levels, speculatively in chunks; kernels past the last level
@@ -2009,10 +2090,14 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
       if ((b->h_act[(size_t)lv * NW + l / 64] >> (l % 64)) & 1ull) nlev[l] = lv;
     if (nlev[l] == 1) rootmask[l / 64] |= 1ull << (l % 64);
   }
-  // x of the deepest level's entries (the dependency pass sets the others)
+  // delta = reach where the batch did not reach (VIS is still whole)
+  hipLaunchKernelGGL(k_bfix<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, b->stamp - 1,
+                     had_prev ? 1 : 0, b->vis, c->reach, b->dlt);
+  // delta and x of the deepest level's entries (the dependency pass sets
+  // the others)
   if (dmax >= 1)
     hipLaunchKernelGGL(k_bleaf<NW>, dim3(kBGrid / 4), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, dmax, c->reach, b->sig,
-                       b->xr);
+                       b->xr, b->dlt);
   // backward: depths dmax-1 .. dmin, dmin = 0 only for the roots with a
   // single level
   const int dmin = (rootmask[0] | rootmask[1]) ? 0 : 1;

@@ -31,8 +31,8 @@ def delta_of(e, k):
     return out
 
 
-def check_batch(g, roots, width, label, batches=1, pull=None):
-    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)} pull={pull}", flush=True)
+def check_batch(g, roots, width, label, batches=1, pull=None, chunks=None):
+    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)} pull={pull} chunks={chunks}", flush=True)
     bc_ref = np.zeros(g.N)
     probs = np.full(len(roots), 1.0 / g.N)
     with Engine(g) as e:
@@ -40,7 +40,9 @@ def check_batch(g, roots, width, label, batches=1, pull=None):
         if pull is not None:
             assert _ffi.hip().axbc_set_batch_pull(e.ctx, pull) == 0
         step = (len(roots) + batches - 1) // batches
-        for b0 in range(0, len(roots), step):
+        for bi, b0 in enumerate(range(0, len(roots), step)):
+            if chunks is not None:
+                assert _ffi.hip().axbc_set_batch_chunk(e.ctx, chunks[bi]) == 0
             rb = roots[b0:b0 + step]
             applied, stats = run_batch(e, rb, probs[b0:b0 + step])
             assert applied == len(rb)
@@ -96,6 +98,31 @@ def test_bottom_up_directed_hubs(require_gpu, width):
     rng = np.random.default_rng(7)
     roots = rng.choice(np.flatnonzero(deg > 0), size=width, replace=False)
     check_batch(g, roots, width, "kron s12 directed bottom-up", pull=65536)
+
+
+@pytest.mark.parametrize("pull", [0, 65536])
+def test_split_threshold_changes_between_batches(require_gpu, pull):
+    """Split items add into path-count rows that start at zero, and a batch
+    clears only the rows of the vertices the previous batch reached and that
+    are above the threshold it runs at.  Two disjoint Kronecker graphs A and
+    B; batches alternate between roots in A and roots in B at thresholds
+    1024, 2048, 64, 128, 64: at the third batch, A's vertices of 65..1024
+    edges become split while their rows still hold the first batch's counts
+    (B's batch in between did not reach them), and A's delta rows must read
+    reach while B's batch runs.  Top-down and bottom-up, every delta and the
+    bc against the oracle."""
+    from axbc_amd import kronecker_edge_list
+    ea = kronecker_edge_list(11, 16 << 11, 5, 6)
+    eb = kronecker_edge_list(11, 16 << 11, 7, 8) + (1 << 11)
+    g = Graph.from_edges(1 << 12, np.concatenate([ea, eb]), heuristic=1)
+    deg = np.diff(g.col)
+    assert ((deg[: 1 << 11] > 64) & (deg[: 1 << 11] <= 1024)).sum() > 20
+    rng = np.random.default_rng(11)
+    ra = rng.choice(np.flatnonzero(deg[: 1 << 11] > 0), size=3 * 48, replace=False)
+    rb = rng.choice(np.flatnonzero(deg[1 << 11:] > 0), size=2 * 48, replace=False) + (1 << 11)
+    roots = np.concatenate([ra[:48], rb[:48], ra[48:96], rb[48:], ra[96:]])
+    check_batch(g, roots, 64, "two kron s11, chunk changes", batches=5, pull=pull,
+                chunks=[1024, 2048, 64, 128, 64])
 
 
 def test_partial_batch_and_single_level_roots(require_gpu):
