@@ -1530,9 +1530,10 @@ __global__ void k_bstop(const uint32_t *__restrict__ check, uint32_t x, uint32_t
 // bc += the batch's terms in draw order, roots [0, nr).  A workgroup takes
 // kUpdVerts vertices at a time: the terms of all lanes are computed with
 // threads over (vertex, lane), so every wave reads whole 512-byte segments
-// of the delta rows, and staged in LDS; then one thread per vertex folds its
-// row in draw order through lane_of (the sequence of one-root updates, bit
-// for bit).
+// of the delta rows, and stored in LDS at their draw position (the inverse
+// of lane_of); then one thread per vertex folds its row left to right, the
+// sequence of one-root updates bit for bit, with independent LDS reads the
+// compiler can issue ahead of the dependent adds.
 constexpr int kUpdVerts = 32;
 
 template <int NW>
@@ -1544,9 +1545,11 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bupdate(uint32_t n, c
                                                     const unsigned long long *__restrict__ cnt, double *bc,
                                                     unsigned long long *kb) {
   constexpr int B = 64 * NW;
+  constexpr int kUpdPer = kUpdVerts * B / kBlock;  // (vertex, lane) terms per thread
+  static_assert(kUpdVerts * B % kBlock == 0, "tile must split evenly over the workgroup");
   __shared__ double tile[kUpdVerts][B + 1];
   __shared__ double s_r1[B], s_p[B], s_root[B];
-  __shared__ uint32_t s_src[B], s_lane[B];
+  __shared__ uint32_t s_src[B], s_draw[B];
   __shared__ uint8_t s_use[B];
   __shared__ uint8_t s_want[kUpdVerts];
   for (int l = threadIdx.x; l < B; l += blockDim.x) s_use[l] = 0;
@@ -1555,7 +1558,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bupdate(uint32_t n, c
     if (k < nr) {
       const uint32_t l = lane_of[k];
       const uint32_t rr = r0[l];
-      s_lane[k] = l;
+      s_draw[l] = (uint32_t)k;
       s_use[l] = 1;
       s_src[l] = src[l];
       s_r1[l] = (double)(uint32_t)(rr + 1u);
@@ -1576,18 +1579,28 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bupdate(uint32_t n, c
       bytes += v < n ? 8 + (want ? 16 + 8ull * (unsigned)nr : 0) : 0;
     }
     __syncthreads();
-    for (int flat = threadIdx.x; flat < kUpdVerts * B; flat += blockDim.x) {
+    // all of a thread's row loads first, then the terms
+    double got[kUpdPer];
+#pragma unroll
+    for (int r = 0; r < kUpdPer; ++r) {
+      const int flat = threadIdx.x + r * kBlock;
       const int vi = flat / B, l = flat % B;
       const uint32_t vv = vb + (uint32_t)vi;
-      double t = 0.0;
-      if (s_want[vi] && s_use[l]) t = vv == s_src[l] ? s_root[l] : dlt[(size_t)vv * B + l] * s_r1[l] / s_p[l];
-      tile[vi][l] = t;
+      got[r] = s_want[vi] && s_use[l] && vv != s_src[l] ? dlt[(size_t)vv * B + l] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kUpdPer; ++r) {
+      const int flat = threadIdx.x + r * kBlock;
+      const int vi = flat / B, l = flat % B;
+      const uint32_t vv = vb + (uint32_t)vi;
+      if (s_want[vi] && s_use[l]) tile[vi][s_draw[l]] = vv == s_src[l] ? s_root[l] : got[r] * s_r1[l] / s_p[l];
     }
     __syncthreads();
     if (threadIdx.x < kUpdVerts && s_want[threadIdx.x]) {
       const uint32_t v = vb + threadIdx.x;
       double acc = bc[v];
-      for (int k = 0; k < nr; ++k) acc += tile[threadIdx.x][s_lane[k]];
+#pragma unroll 8
+      for (int k = 0; k < nr; ++k) acc += tile[threadIdx.x][k];
       bc[v] = acc;
     }
     __syncthreads();
