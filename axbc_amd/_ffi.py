@@ -106,6 +106,7 @@ def hip() -> C.CDLL:
         "axbc_lcc_build": (C.c_int, [ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
         "axbc_set_reach": (C.c_int, [ctxp, C.c_void_p]),
         "axbc_kernel_timing": (C.c_int, [ctxp, C.c_int]),
+        "axbc_kernel_timing_kinds": (C.c_int, [ctxp, C.c_uint32]),
         "axbc_kernel_stats": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),
         "axbc_copy_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),

@@ -61,22 +61,24 @@ static hipEvent_t kt_event(axbc_ctx *c) {
   return e;
 }
 
+static bool kt_timed(const axbc_ctx *c, int kind) { return c->ktimer.on && ((c->ktimer.mask >> kind) & 1u); }
+
 void kt_begin(axbc_ctx *c, int kind) {
-  if (!c->ktimer.on) return;
+  if (!kt_timed(c, kind)) return;
   hipEvent_t a = kt_event(c), b = kt_event(c);
   (void)hipEventRecord(a, c->stream);
   c->ktimer.pending.push_back({kind, {a, b}});
 }
 
 void kt_end(axbc_ctx *c, int kind) {
-  if (!c->ktimer.on || c->ktimer.pending.empty()) return;
+  if (!kt_timed(c, kind) || c->ktimer.pending.empty()) return;
   auto &pr = c->ktimer.pending.back();
   if (pr.first != kind) return;
   (void)hipEventRecord(pr.second.second, c->stream);
 }
 
 unsigned long long *kt_bytes(axbc_ctx *c, int kind) {
-  return c->ktimer.on && c->ktimer.dbytes ? c->ktimer.dbytes + kind : nullptr;
+  return kt_timed(c, kind) && c->ktimer.dbytes ? c->ktimer.dbytes + kind : nullptr;
 }
 
 // Fold finished event pairs into the per-kind totals.
@@ -806,8 +808,7 @@ int axbc_last_timing(axbc_ctx *c, float *f, float *b, float *u) {
   return AXBC_OK;
 }
 
-int axbc_kernel_timing(axbc_ctx *c, int enable) {
-  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+static int kt_enable(axbc_ctx *c, bool enable, uint32_t mask) {
   KernelTimer &t = c->ktimer;
   if (enable) {
     if (!t.dbytes) HIP_TRY(hipMalloc((void **)&t.dbytes, sizeof(unsigned long long) * KK_COUNT));
@@ -822,8 +823,20 @@ int axbc_kernel_timing(axbc_ctx *c, int enable) {
     int rc = kt_collect(c);
     if (rc) return rc;
   }
-  t.on = enable != 0;
+  t.on = enable;
+  t.mask = mask;
   return AXBC_OK;
+}
+
+int axbc_kernel_timing(axbc_ctx *c, int enable) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  return kt_enable(c, enable != 0, ~0u);
+}
+
+int axbc_kernel_timing_kinds(axbc_ctx *c, uint32_t mask) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  if (mask >> KK_COUNT) return fail(AXBC_EINVAL, "unknown kernel kind in the mask%s (line %d)", "", __LINE__);
+  return kt_enable(c, mask != 0, mask ? mask : ~0u);
 }
 
 int axbc_kernel_stats(axbc_ctx *c, int kind, double *ms, uint64_t *launches, uint64_t *bytes) {

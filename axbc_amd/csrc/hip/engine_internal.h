@@ -79,6 +79,7 @@ enum KernelKind {
 
 struct KernelTimer {
   bool on = false;
+  uint32_t mask = ~0u;  // kinds timed while on (bit k = kind k)
   unsigned long long *dbytes = nullptr;  // device: KK_COUNT counters
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> pool;

@@ -218,10 +218,14 @@ int axbc_batch_utility(axbc_ctx *ctx, const uint32_t *vertex, const double *weig
  * dependency pull hub chunks, 14 batched bottom-up level, 15 its split
  * vertices.  axbc_kernel_stats returns the summed milliseconds,
  * the launch count and the byte count of one kind since enabling.
+ * axbc_kernel_timing_kinds enables the same for the kinds whose bit is set
+ * in `mask` only (0 disables), so that a timed run carries the events of
+ * the kernel it measures and no others.
  * axbc_copy_kernel runs a plain device copy of `bytes` bytes (kind 7) to
  * calibrate hardware counters against a known traffic.
  */
 int axbc_kernel_timing(axbc_ctx *ctx, int enable);
+int axbc_kernel_timing_kinds(axbc_ctx *ctx, uint32_t mask);
 int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, uint64_t *bytes);
 int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 
