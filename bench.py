@@ -79,10 +79,11 @@ def set_timing_kinds(e, kinds):
 
 
 def log_kernels(kstats):
+    """One line per kernel kind; the plain kernels count no bytes."""
     for k, (ms, n, b) in kstats.items():
         if n:
-            log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us, "
-                f"{b / 1e9:.3f} GB -> {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s")
+            rate = f", {b / 1e9:.3f} GB -> {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s" if b else ""
+            log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us{rate}")
 
 
 def measured_traffic(kernel):
