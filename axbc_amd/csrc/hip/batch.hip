@@ -1400,6 +1400,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint
 }
 
 // One wave per split entry: delta = reach + partials in chunk order.
+constexpr int kCombineRun = 8;  // partial rows loaded ahead of their adds
+
 template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_combine(const uint32_t *__restrict__ col,
                                                          const uint32_t *__restrict__ sig, double *dlt, double *xr,
@@ -1431,16 +1433,34 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_combine(const ui
       if (NW > 1) f.w[NW - 1] &= root_mask1;
     }
     const double r = (double)reach[v];
+    // t = reach + p_0 + p_1 + ... in item order; the loads of 8 items are
+    // issued before their adds
+    const double *pp = part + (size_t)base * B + lane;
+    double t[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) t[j] = r;
+    uint32_t cc = 0;
+    for (; cc + kCombineRun <= nch; cc += kCombineRun) {
+      double p[kCombineRun][NW];
+#pragma unroll
+      for (int u = 0; u < kCombineRun; ++u)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) p[u][j] = pp[(size_t)(cc + u) * B + 64 * j];
+#pragma unroll
+      for (int u = 0; u < kCombineRun; ++u)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) t[j] += p[u][j];
+    }
+    for (; cc < nch; ++cc)
+#pragma unroll
+      for (int j = 0; j < NW; ++j) t[j] += pp[(size_t)cc * B + 64 * j];
     uint32_t roots = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-      double t = r;
-      const double *pp = part + (size_t)base * B + lane + 64 * j;
-      for (uint32_t cc = 0; cc < nch; ++cc) t += pp[(size_t)cc * B];
       roots += (uint32_t)__popcll(f.w[j]);
       if (bit(f.w[j], lane)) {
-        dlt[(size_t)v * B + lane + 64 * j] = t;
-        xr[(size_t)v * B + lane + 64 * j] = (t + 1.0) / (double)sig[(size_t)v * B + lane + 64 * j];
+        dlt[(size_t)v * B + lane + 64 * j] = t[j];
+        xr[(size_t)v * B + lane + 64 * j] = (t[j] + 1.0) / (double)sig[(size_t)v * B + lane + 64 * j];
       }
     }
     // per root of the entry: sigma 4 + delta 8 + x 8
