@@ -748,7 +748,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                    unsigned long long *cnt, uint32_t *sp_e, uint64_t *sp_base,
                                                    uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
-                                                   unsigned long long *kb, uint32_t kChunk) {
+                                                   unsigned long long *kb, uint32_t kChunk, int strict) {
   constexpr int B = 64 * NW;
   if (ld_flag(&ctl->done) || ctl->bu) return;
   const int L = ctl->level;
@@ -760,8 +760,41 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
   uint32_t dag_tot[NW];
 #pragma unroll
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
+  // Level 1 over strictly increasing lists: the frontier is the roots
+  // (sigma 1 each), and a new vertex w has exactly one edge from every root
+  // in NX[w], so sigma_s(w) = 1 there without scanning w's in-edges (a hub
+  // next to the roots would be scanned whole).  The same DAG edges count.
+  const bool level1 = strict && L == 1;
+  for (uint32_t base = nb + gw * kWave; level1 && base < ne; base += nwv * kWave) {
+    const uint32_t i = base + lane;
+    uint32_t w = 0;
+    Words<NW> nw;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) nw.w[j] = 0;
+    if (i < ne) {
+      w = q[i];
+      nw = ldw<NW>(nx, w);
+    }
+    unsigned long long todo = __ballot(nw.any());
+    // per entry: id 4 + NX 8NW; per root reached: sigma write 4
+    bytes += (unsigned long long)__popcll(todo) * (4 + 8 * NW);
+    while (todo) {
+      const int k = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t wk = rdl(w, k);
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const uint64_t mk = rdl(nw.w[j], k);
+        bytes += 4ull * (unsigned long long)__popcll(mk);
+        if (bit(mk, lane)) {
+          sig[(size_t)wk * B + lane + 64 * j] = 1u;
+          dag_tot[j] += 1;
+        }
+      }
+    }
+  }
   // a wave takes 64 new entries of the level at a time
-  for (uint32_t base = nb + gw * kWave; base < ne; base += nwv * kWave) {
+  for (uint32_t base = nb + gw * kWave; !level1 && base < ne; base += nwv * kWave) {
     const uint32_t i = base + lane;
     uint32_t w = 0, st = 0, d = 0;
     Words<NW> nw;
@@ -1752,6 +1785,7 @@ int ensure_transpose(axbc_ctx *c) {
       for (uint32_t v = 0; v < c->n; ++v) mx = std::max(mx, c->h_col[v + 1] - c->h_col[v]);
       c->max_out = c->max_in = mx;
       c->symmetric = true;
+      c->pull_strict = true;
       c->rcol = c->col;
       c->rrow = c->row;
       return AXBC_OK;
@@ -1768,6 +1802,14 @@ int ensure_transpose(axbc_ctx *c) {
   std::vector<uint32_t> fill(rcol.begin(), rcol.end() - 1);
   for (uint32_t u = 0; u < c->n; ++u)
     for (uint32_t k = col[u]; k < col[u + 1]; ++k) rrow[fill[row[k]]++] = u;
+  // the lists are filled in increasing u: strict unless an edge repeats
+  c->pull_strict = true;
+  for (uint32_t v = 0; v < c->n && c->pull_strict; ++v)
+    for (uint32_t k = rcol[v] + 1; k < rcol[v + 1]; ++k)
+      if (rrow[k] <= rrow[k - 1]) {
+        c->pull_strict = false;
+        break;
+      }
   c->max_out = c->max_in = 0;
   for (uint32_t v = 0; v < c->n; ++v) {
     c->max_out = std::max(c->max_out, col[v + 1] - col[v]);
@@ -1925,7 +1967,7 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
   kt_begin(c, KK_SIGMA);
   launch(k, k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
                      b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
-                     kt_bytes(c, KK_SIGMA), b->chunk);
+                     kt_bytes(c, KK_SIGMA), b->chunk, c->pull_strict ? 1 : 0);
   kt_end(c, KK_SIGMA);
   if (split_in) {
     kt_begin(c, KK_SIGMA_SPLIT);

@@ -111,6 +111,9 @@ struct axbc_ctx {
   uint32_t *col = nullptr, *row = nullptr, *reach = nullptr, *rlist = nullptr;
   uint32_t *rcol = nullptr, *rrow = nullptr;  // transpose (pull direction); alias when symmetric
   bool symmetric = true;
+  // every pull list strictly increasing (no repeated edge): a level-1 vertex
+  // has exactly one edge from each root that reaches it
+  bool pull_strict = false;
   uint32_t nr = 0;
   std::vector<uint32_t> h_reach;
   std::vector<uint32_t> h_col;

@@ -125,6 +125,32 @@ def test_split_threshold_changes_between_batches(require_gpu, pull):
                 chunks=[1024, 2048, 64, 128, 64])
 
 
+@pytest.mark.parametrize("mirrored", [True, False])
+def test_repeated_edges(require_gpu, mirrored):
+    """A CSC handed over through the ABI may repeat an edge, and the path
+    counts then count it twice (per-edge sums, as scan_col_mono2 adds them).
+    Level 1 takes sigma = 1 per root only for lists without repeats, so
+    these graphs must take the scan.  Every 7th vertex repeats its first
+    neighbour (mirrored: the neighbour repeats it back)."""
+    base = Graph.kronecker(9, 8, seed=(1, 2))
+    lists = [list(base.row[base.col[v]:base.col[v + 1]]) for v in range(base.N)]
+    for v in range(0, base.N, 7):
+        if lists[v]:
+            w = lists[v][0]
+            lists[v].insert(0, w)
+            if mirrored:
+                lists[w].append(v)
+                lists[w].sort()
+    col = np.zeros(base.N + 1, np.uint32)
+    col[1:] = np.cumsum([len(x) for x in lists])
+    row = np.array([u for x in lists for u in x], np.uint32)
+    g = Graph(base.N, col, row, np.zeros(base.N, np.uint32))
+    deg = np.diff(col)
+    rng = np.random.default_rng(4)
+    roots = rng.choice(np.flatnonzero(deg > 0), size=96, replace=False)
+    check_batch(g, roots, 64, f"repeated edges mirrored={mirrored}", batches=2)
+
+
 def test_partial_batch_and_single_level_roots(require_gpu):
     n = 40  # star: the centre's round has a single level
     g = Graph.from_edges(n, np.array([(0, i) for i in range(1, n)]))
