@@ -1,4 +1,4 @@
-"""Learned pivot proposal (the north star's third pattern).
+"""Learned pivot proposal: pivots drawn from an MLP's predicted betweenness.
 
 A small MLP over per-vertex features (log degree, target adjacency, running
 bc, mean neighbour bc) proposes pivots in place of the bc/delta heuristics.

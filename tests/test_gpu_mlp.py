@@ -1,6 +1,6 @@
 """Learned proposal: the device MLP's loss and gradient against the float64
 CPU oracle (oracle/mlp_oracle.py), training progress, and the learned loop
-against uniform sampling at the same pivot budget, judged by exact Brandes."""
+against uniform sampling at the same pivot budget, scored against exact Brandes."""
 import numpy as np
 import pytest
 
@@ -81,7 +81,7 @@ def test_training_reduces_loss(require_gpu):
 
 def test_learned_vs_uniform(require_gpu):
     """What the trained proposal gains over the method without it (uniform
-    roots) at the same budget, judged against exact Brandes: the learned
+    roots) at the same budget, scored against exact Brandes: the learned
     loop must train (not fall back in every run), keep the round count, and
     not lose top-1% recall beyond the noise of four runs; the gains are
     printed (on this graph family they are within noise of zero)."""

@@ -1,5 +1,5 @@
 """Pins of the CPU oracle (oracle/brandes_oracle.c) by brute force, closed
-forms and invariants, so that the oracle can judge the HIP path."""
+forms and invariants, so that the oracle can check the HIP path."""
 import math
 
 import numpy as np
