@@ -289,7 +289,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs (pivot sharding with one "
-                                   "all-reduce of bc) are not part of this build", "n_gpus": args.gpus}))
+                                   "all-reduce of bc) are not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from axbc_amd import Engine, Graph
