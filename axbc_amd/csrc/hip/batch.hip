@@ -1358,7 +1358,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
     const double r = (double)reach[v];
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
-      sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
+      // only the entry's roots are used: the other lanes skip the load
+      sv[j] = bit(f.w[j], lane) ? (double)sig[(size_t)v * B + lane + 64 * j] : 1.0;
       acc[j] = r;
     }
     const uint32_t contrib = dep_chunk<NW>(st, 0, d, row, vis, f, xr, sv, acc);
@@ -1415,7 +1416,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint
         if (NW > 1) f.w[NW - 1] &= root_mask1;
       }
 #pragma unroll
-      for (int j = 0; j < NW; ++j) sv[j] = (double)sig[(size_t)v * B + lane + 64 * j];
+      for (int j = 0; j < NW; ++j) sv[j] = bit(f.w[j], lane) ? (double)sig[(size_t)v * B + lane + 64 * j] : 1.0;
       fresh = false;
     }
     const uint32_t cb = (uint32_t)(item - base) * kChunk, ce = min(d, cb + kChunk);
