@@ -1,0 +1,45 @@
+"""The benchmark workload at its full size: Kronecker SCALE 22, edgefactor
+16, seed pair (2, 3), -H 1, the first 128 high-degree-neighbourhood roots of
+srand48(4343543) in one batch of width 128 at chunk 512.  The delta of
+sampled roots (first, middle and last in draw order) against the oracle,
+and the bc of the batch against the same roots run one per round on the
+device, within 1e-9 relative (bc and delta are FP64)."""
+import numpy as np
+import pytest
+
+from axbc_amd import Engine, Graph, _ffi
+from axbc_amd.pivots import PivotLoop
+from oracle import oracle
+from tests.test_gpu_batch import delta_of
+from tests.test_gpu_plain import close
+
+pytestmark = pytest.mark.gpu
+
+
+def test_require(require_gpu):
+    pass
+
+
+def test_scale22_batch_against_oracle_and_plain(require_gpu):
+    print("[fullsize] building SCALE 22 on the device", flush=True)
+    g = Graph.kronecker_device(22, 16, seed=(2, 3), heuristic=1)
+    L = _ffi.hip()
+    with Engine(g) as e:
+        assert L.axbc_set_batch_width(e.ctx, 128) == 0
+        assert L.axbc_set_batch_chunk(e.ctx, 512) == 0
+        loop = PivotLoop(e, 4096, c=0, x=1, z=1, batch=128)
+        assert loop.step() == 128
+        roots, probs = loop.res.pivots, loop.res.probs
+        bc_batch = e.bc()
+        print("[fullsize] batch done; oracle rounds", flush=True)
+        for k in (0, 64, 127):
+            _, _, dl, _, ost = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            ok, err = close(delta_of(e, k), dl)
+            assert ok, f"root #{k} ({roots[k]}): delta rel err {err}"
+            assert ost["nvisited"] > 1_000_000  # a full-size round
+        print("[fullsize] plain rounds", flush=True)
+        e.reset_bc()
+        for v, p in zip(roots, probs):
+            e.pivot(int(v), float(p))
+        ok, err = close(bc_batch, e.bc())
+        assert ok, f"batch vs plain bc rel err {err}"
