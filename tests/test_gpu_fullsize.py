@@ -43,3 +43,32 @@ def test_scale22_batch_against_oracle_and_plain(require_gpu):
             e.pivot(int(v), float(p))
         ok, err = close(bc_batch, e.bc())
         assert ok, f"batch vs plain bc rel err {err}"
+
+
+def test_lattice_batch_against_oracle_and_plain(require_gpu):
+    """The high-diameter case at full size: the 4096 x 4096 lattice with 3 %
+    of its edges deleted (seed 3), ~8000 levels per round, 32 uniform roots
+    in one batch of width 64.  Path counts there are binomial coefficients
+    and wrap mod 2^32 (to 0 at times), so deltas hold infinities and NaNs:
+    the comparison takes them at equal positions."""
+    print("[fullsize] building the 4096^2 lattice", flush=True)
+    g = Graph.lattice(4096, 0.03, seed=3, heuristic=0)
+    L = _ffi.hip()
+    with Engine(g) as e:
+        assert L.axbc_set_batch_width(e.ctx, 64) == 0
+        loop = PivotLoop(e, 32, c=0, x=1, z=0, batch=64)
+        assert loop.step() == 32
+        roots, probs = loop.res.pivots, loop.res.probs
+        bc_batch = e.bc()
+        print("[fullsize] lattice batch done; oracle rounds", flush=True)
+        for k in (0, 31):
+            _, _, dl, _, ost = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            ok, err = close(delta_of(e, k), dl)
+            assert ok, f"root #{k} ({roots[k]}): delta rel err {err}"
+            assert ost["nlevels"] > 4000  # a high-diameter round
+        print("[fullsize] lattice plain rounds", flush=True)
+        e.reset_bc()
+        for v, p in zip(roots, probs):
+            e.pivot(int(v), float(p))
+        ok, err = close(bc_batch, e.bc())
+        assert ok, f"batch vs plain bc rel err {err}"
