@@ -72,3 +72,30 @@ def test_lattice_batch_against_oracle_and_plain(require_gpu):
             e.pivot(int(v), float(p))
         ok, err = close(bc_batch, e.bc())
         assert ok, f"batch vs plain bc rel err {err}"
+
+
+def test_scale24_batch_against_oracle_and_plain(require_gpu):
+    """The learned-proposal graph at full size: Kronecker SCALE 24 (16.8 M
+    vertices, ~0.5 G directed entries), seed pair (4, 5), -H 1, 64 uniform
+    roots in one batch of width 64: two roots' delta against the oracle and
+    the batch's bc against one-root rounds on the device."""
+    print("[fullsize] building SCALE 24 on the device", flush=True)
+    g = Graph.kronecker_device(24, 16, seed=(4, 5), heuristic=1)
+    L = _ffi.hip()
+    with Engine(g) as e:
+        assert L.axbc_set_batch_width(e.ctx, 64) == 0
+        loop = PivotLoop(e, 512, c=0, x=1, z=0, batch=64)  # degree-0 draws consume attempts
+        assert loop.step() == 64
+        roots, probs = loop.res.pivots, loop.res.probs
+        bc_batch = e.bc()
+        print("[fullsize] SCALE 24 batch done; oracle rounds", flush=True)
+        for k in (0, len(roots) - 1):
+            _, _, dl, _, _ = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            ok, err = close(delta_of(e, k), dl)
+            assert ok, f"root #{k} ({roots[k]}): delta rel err {err}"
+        print("[fullsize] SCALE 24 plain rounds", flush=True)
+        e.reset_bc()
+        for v, p in zip(roots, probs):
+            e.pivot(int(v), float(p))
+        ok, err = close(bc_batch, e.bc())
+        assert ok, f"batch vs plain bc rel err {err}"
