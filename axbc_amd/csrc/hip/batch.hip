@@ -129,6 +129,12 @@ struct BatchState {
   hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // prebuilt level / depth steps
   uint64_t gkey = 0;
   size_t bytes = 0;
+  int levels = 0;            // levels of the last batch's list (qoff[0 .. levels])
+  // scratch of the getters and the utility, allocated on first use and kept
+  void *row_out = nullptr;   // N x 8 bytes: one root's delta, or its lvl + sigma
+  uint32_t *util_v = nullptr;
+  double *util_w = nullptr, *util_part = nullptr, *util_out = nullptr;
+  uint32_t util_cap = 0;
 };
 
 }  // namespace axbc
@@ -1665,11 +1671,15 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bupdate(uint32_t n, c
   }
 }
 
-// out[s] += sum over the target vertices of weight * delta_s(v): a wave per
-// vertex, lane = root (coalesced rows), one FP64 atomic per root and wave.
+// Utility of the batch's roots for a set of target vertices, in a fixed
+// order: wave gw sums weight * delta_s(v) over the targets gw, gw + W, ...
+// into its own partial row (lane = root, coalesced rows, no atomics), then
+// k_butility_sum adds the W partial rows in wave order.  The same count
+// gives the same grid, so the result does not depend on the run.
+
 template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_butility(const uint32_t *__restrict__ vert, const double *__restrict__ wt,
-                                                     uint32_t nv, const double *__restrict__ dlt, double *out) {
+                                                     uint32_t nv, const double *__restrict__ dlt, double *part) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -1683,8 +1693,50 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_butility(const uint32
     for (int j = 0; j < NW; ++j) acc[j] += w * dlt[(size_t)v * B + lane + 64 * j];
   }
 #pragma unroll
-  for (int j = 0; j < NW; ++j)
-    if (acc[j] != 0.0) atomicAdd(&out[lane + 64 * j], acc[j]);
+  for (int j = 0; j < NW; ++j) part[(size_t)gw * B + lane + 64 * j] = acc[j];
+}
+
+// out[s] = sum of the partial rows in wave order; one thread per root.
+__global__ void k_butility_sum(const double *__restrict__ part, uint32_t rows, int B, double *out) {
+  const int s = (int)threadIdx.x;
+  if (s >= B) return;
+  double t = 0.0;
+  for (uint32_t r = 0; r < rows; ++r) t += part[(size_t)r * B + s];
+  out[s] = t;
+}
+
+// One root's levels and path counts from the batch's level list: lvl = -1
+// and sigma = 0 first (k_blevels_init), then every list entry of level L
+// whose word holds the root's lane sets lvl = L and copies the root's path
+// count from the vertex's row.
+__global__ void k_blevels_init(uint32_t n, int32_t *lvl, uint32_t *sigma) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    lvl[v] = -1;
+    sigma[v] = 0;
+  }
+}
+
+template <int NW>
+__global__ void k_blevels(const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                          const uint32_t *__restrict__ qoff, int levels, int lane,
+                          const uint32_t *__restrict__ sig, int32_t *lvl, uint32_t *sigma) {
+  constexpr int B = 64 * NW;
+  const uint32_t ne = qoff[levels];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += gridDim.x * blockDim.x) {
+    if (!((qw[(size_t)i * NW + lane / 64] >> (lane % 64)) & 1ull)) continue;
+    // level of entry i: the last L with qoff[L] <= i
+    int lo = 0, hi = levels;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (qoff[mid] <= i)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t v = q[i];
+    lvl[v] = lo;
+    sigma[v] = sig[(size_t)v * B + lane];
+  }
 }
 
 template <int NW>
@@ -1739,7 +1791,7 @@ void batch_free(axbc_ctx *c) {
   if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
-               b->lane_of, b->check, b->ctl};
+               b->lane_of, b->check, b->ctl, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -2156,6 +2208,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   }
   const int empty = (int)b->h_ctl->empty_lev;
   const int dmax = empty - 1;  // deepest non-empty level over the batch
+  b->levels = empty;
   b->lev_guess = dmax + 2;
   b->epoch += (uint32_t)(L + 1);
   // per-root deepest level from the active words: roots whose deepest
@@ -2242,6 +2295,36 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
 
 }  // namespace
 
+namespace axbc {
+// Utility on the given device buffers: stage the targets, partial rows per
+// wave, their sum in wave order, and the per-root values in draw order.
+int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
+                     const double *weight, uint32_t count, double *out) {
+  BatchState *b = c->batch;
+  const int B = b->B;
+  hipStream_t s = c->stream;
+  uint32_t rows = 0;
+  if (count) {
+    HIP_TRY(hipMemcpyAsync(dv, vertex, sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dw, weight, sizeof(double) * count, hipMemcpyHostToDevice, s));
+    const int grid = (int)std::min<uint32_t>(kUtilWaves / 4, (count + 3) / 4);
+    rows = (uint32_t)grid * 4;
+    if (b->nw == 1)
+      hipLaunchKernelGGL(k_butility<1>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, part);
+    else
+      hipLaunchKernelGGL(k_butility<2>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, part);
+  }
+  // every out entry is written (zero rows give zero), so nothing is cleared
+  hipLaunchKernelGGL(k_butility_sum, dim3(1), dim3(128), 0, s, part, rows, B, dout);
+  HIP_TRY(hipGetLastError());
+  double per_lane[128];
+  HIP_TRY(hipMemcpyAsync(per_lane, dout, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int k = 0; k < B; ++k) out[k] = per_lane[b->h_lane_of.empty() ? k : b->h_lane_of[k]];
+  return AXBC_OK;
+}
+}  // namespace axbc
+
 extern "C" {
 
 int axbc_set_batch_width(axbc_ctx *c, int width) {
@@ -2310,50 +2393,76 @@ int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight
   for (uint32_t i = 0; i < count; ++i)
     if (vertex[i] >= c->n) return fail(AXBC_EINVAL, "vertex out of range%s (line %d)", "", __LINE__);
   BatchState *b = c->batch;
-  const int B = b->B;
-  uint32_t *dv = nullptr;
-  double *dw = nullptr, *dout = nullptr;
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipMalloc((void **)&dv, sizeof(uint32_t) * (count ? count : 1)));
-  HIP_TRY(hipMalloc((void **)&dw, sizeof(double) * (count ? count : 1)));
-  HIP_TRY(hipMalloc((void **)&dout, sizeof(double) * B));
-  hipStream_t s = c->stream;
-  HIP_TRY(hipMemcpyAsync(dv, vertex, sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dw, weight, sizeof(double) * count, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemsetAsync(dout, 0, sizeof(double) * B, s));
-  if (count) {
-    const int grid = (int)std::min<uint32_t>(1024, (count + 3) / 4);
-    if (b->nw == 1)
-      hipLaunchKernelGGL(k_butility<1>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, dout);
-    else
-      hipLaunchKernelGGL(k_butility<2>, dim3(grid), dim3(kBlock), 0, s, dv, dw, count, b->dlt, dout);
-    HIP_TRY(hipGetLastError());
+  // staging and partial rows kept in the batch state, grown as needed
+  if (!b->util_part) {
+    int rc;
+    if ((rc = balloc(b, &b->util_part, (size_t)kUtilWaves * b->B)) || (rc = balloc(b, &b->util_out, (size_t)b->B)))
+      return rc;
+    c->bytes += sizeof(double) * ((size_t)kUtilWaves + 1) * b->B;
   }
-  std::vector<double> per_lane(B);
-  HIP_TRY(hipMemcpyAsync(per_lane.data(), dout, sizeof(double) * B, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  for (int k = 0; k < B; ++k) out[k] = per_lane[b->h_lane_of.empty() ? k : b->h_lane_of[k]];
-  (void)hipFree(dv);
-  (void)hipFree(dw);
-  (void)hipFree(dout);
-  return AXBC_OK;
+  if (count > b->util_cap) {
+    if (b->util_v) (void)hipFree(b->util_v);
+    if (b->util_w) (void)hipFree(b->util_w);
+    b->util_v = nullptr;
+    b->util_w = nullptr;
+    b->util_cap = 0;
+    const uint32_t cap = std::max<uint32_t>(count, 4096);
+    int rc;
+    if ((rc = balloc(b, &b->util_v, cap)) || (rc = balloc(b, &b->util_w, cap))) return rc;
+    b->util_cap = cap;
+  }
+  return batch_utility_on(c, b->util_v, b->util_w, b->util_part, b->util_out, vertex, weight, count, out);
 }
 
 int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
   if (!c || !c->batch || !delta || k < 0 || k >= c->batch->B)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
-  const int lane = c->batch->h_lane_of.empty() ? k : (int)c->batch->h_lane_of[k];
-  double *tmp = nullptr;
-  HIP_TRY(hipMalloc((void **)&tmp, sizeof(double) * c->n));
-  if (c->batch->nw == 1)
-    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, c->batch->dlt,
-                       tmp);
+  BatchState *b = c->batch;
+  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
+  int rc;
+  HIP_TRY(hipSetDevice(c->device));
+  if (!b->row_out) {
+    if ((rc = balloc(b, (double **)&b->row_out, c->n))) return rc;
+    c->bytes += sizeof(double) * c->n;
+  }
+  double *tmp = (double *)b->row_out;
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
   else
-    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, c->batch->dlt,
-                       tmp);
+    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(delta, tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipFree(tmp));
+  return AXBC_OK;
+}
+
+int axbc_batch_levels(axbc_ctx *c, int k, int32_t *lvl, uint32_t *sigma) {
+  if (!c || !c->batch || !lvl || !sigma || k < 0 || k >= c->batch->B)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  BatchState *b = c->batch;
+  if (!b->have || b->levels <= 0) return fail(AXBC_ESTATE, "no batch has run%s (line %d)", "", __LINE__);
+  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
+  int rc;
+  HIP_TRY(hipSetDevice(c->device));
+  if (!b->row_out) {
+    if ((rc = balloc(b, (double **)&b->row_out, c->n))) return rc;
+    c->bytes += sizeof(double) * c->n;
+  }
+  int32_t *dl = (int32_t *)b->row_out;
+  uint32_t *ds = (uint32_t *)(dl + c->n);
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_blevels_init, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->n, dl, ds);
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_blevels<1>, dim3(2048), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, b->levels, lane, b->sig,
+                       dl, ds);
+  else
+    hipLaunchKernelGGL(k_blevels<2>, dim3(2048), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, b->levels, lane, b->sig,
+                       dl, ds);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(lvl, dl, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(sigma, ds, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return AXBC_OK;
 }
 

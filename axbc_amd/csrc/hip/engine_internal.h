@@ -159,6 +159,13 @@ int grid_for(uint64_t n, int cap = 2048);
 // (min << 32 | max) edges of 2^scale vertices into edges on stream s.
 int kronecker_device_edges(hipStream_t s, int scale, uint64_t nedges, uint64_t seed1, uint64_t seed2,
                            const double initiator[4], uint64_t *edges);
+// Partial rows of the batch utility (256 workgroups of 4 waves at most).
+constexpr int kUtilWaves = 1024;
+// Utility of the last batch's roots (axbc_batch_utility) on the given
+// device buffers: dv/dw hold `count` staged targets, part kUtilWaves x B
+// partial rows, dout B values.
+int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
+                     const double *weight, uint32_t count, double *out);
 // Event pair around one launch of kind k when instrumentation is on.
 void kt_begin(axbc_ctx *c, int kind);
 void kt_end(axbc_ctx *c, int kind);

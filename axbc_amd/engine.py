@@ -91,6 +91,13 @@ class Engine:
         _check(_ffi.hip().axbc_batch_delta(self.ctx, int(k), out))
         return out
 
+    def batch_levels(self, k: int) -> tuple[np.ndarray, np.ndarray]:
+        """Root k's levels and path counts of the last batch."""
+        lvl = np.empty(self.N, np.int32)
+        sig = np.empty(self.N, np.uint32)
+        _check(_ffi.hip().axbc_batch_levels(self.ctx, int(k), lvl, sig))
+        return lvl, sig
+
     def sync(self) -> None:
         _check(_ffi.hip().axbc_sync(self.ctx))
 

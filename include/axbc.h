@@ -200,9 +200,15 @@ int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);
 int axbc_batch_delta(axbc_ctx *ctx, int k, double *delta);
+/* Root k's BFS levels (-1 unreached) and path counts (uint32, wrapping) of
+ * the last batch, from its level list: what axbc_get_lvl / axbc_get_sigma
+ * give after a one-root round (the per-pivot state of scan_col_mono2,
+ * src/cuda_kernels.cu:588-658).  For parity checks of the batched sweep. */
+int axbc_batch_levels(axbc_ctx *ctx, int k, int32_t *lvl, uint32_t *sigma);
 /* Utility of the last batch's roots for a set of target vertices:
  * out[k] = sum_i weight[i] * delta_k(vertex[i]) for k < width (FP64; the
- * learned proposal's training target).  No reference counterpart. */
+ * learned proposal's training target), summed in a fixed order (the same
+ * inputs give the same bits).  No reference counterpart. */
 int axbc_batch_utility(axbc_ctx *ctx, const uint32_t *vertex, const double *weight, uint32_t count, double *out);
 
 /*

@@ -302,11 +302,14 @@ void oracle_lcc(uint32_t N, const uint32_t *col, const uint32_t *row, float *lcc
   }
 }
 
-int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
-               const oracle_cfg *cfg, uint32_t *pivots, double *probs, double *bc,
-               oracle_run_stats *rs) {
+static int run_loop(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
+                    const oracle_cfg *cfg, uint32_t *pivots, double *probs, double *bc,
+                    oracle_run_stats *rs, int rounds) {
   const int z = cfg->distribution;
   if (z < 0 || z > 6) return -1;
+  /* without rounds only the static distributions draw the same sequence,
+   * and only without the stop test (which reads bc) */
+  if (!rounds && (z > 2 || cfg->bader > 0)) return -1;
   memset(rs, 0, sizeof(*rs));
   memset(bc, 0, sizeof(double) * N);
   uint64_t nbfs = cfg->nbfs < N ? cfg->nbfs : N;
@@ -434,7 +437,7 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
       if (v0 < N) vs[v0] = 1;
       continue;
     }
-    oracle_pivot(N, col, row, reach, cfg->heuristic, v0, p, lvl, sigma, delta, bc, NULL);
+    if (rounds) oracle_pivot(N, col, row, reach, cfg->heuristic, v0, p, lvl, sigma, delta, bc, NULL);
     vs[v0] = 1;
     pivots[rs->nrounds] = v0;
     probs[rs->nrounds] = p;
@@ -454,4 +457,18 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
   free(Wd6);
   free(vs);
   return 0;
+}
+
+int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint32_t *reach,
+               const oracle_cfg *cfg, uint32_t *pivots, double *probs, double *bc,
+               oracle_run_stats *rs) {
+  return run_loop(N, col, row, reach, cfg, pivots, probs, bc, rs, 1);
+}
+
+int oracle_draws(uint32_t N, const uint32_t *col, const uint32_t *row, const oracle_cfg *cfg,
+                 uint32_t *pivots, double *probs, oracle_run_stats *rs) {
+  double *bc = (double *)xcalloc(N, sizeof(double));
+  const int rc = run_loop(N, col, row, NULL, cfg, pivots, probs, bc, rs, 0);
+  free(bc);
+  return rc;
 }

@@ -70,6 +70,13 @@ int oracle_run(uint32_t N, const uint32_t *col, const uint32_t *row, const uint3
                const oracle_cfg *cfg, uint32_t *pivots, double *probs, double *bc,
                oracle_run_stats *rs);
 
+/* The draws of oracle_run without the rounds: the pivot sequence, the
+ * probabilities and the skip count of a static distribution (-z 0, 1, 2)
+ * with the stop test off (cfg->bader == 0), which do not depend on bc.
+ * Returns -1 for any other cfg. */
+int oracle_draws(uint32_t N, const uint32_t *col, const uint32_t *row, const oracle_cfg *cfg,
+                 uint32_t *pivots, double *probs, oracle_run_stats *rs);
+
 /* Local clustering coefficient of every vertex as the reference's
  * lcc_func computes it (float). */
 void oracle_lcc(uint32_t N, const uint32_t *col, const uint32_t *row, float *lcc);

@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
                                         f64p]
         L.oracle_run.argtypes = [C.c_uint32, u32p, u32p, u32p, C.POINTER(Cfg), u32p, f64p, f64p,
                                  C.POINTER(RunStats)]
+        L.oracle_draws.restype = C.c_int
+        L.oracle_draws.argtypes = [C.c_uint32, u32p, u32p, C.POINTER(Cfg), u32p, f64p, C.POINTER(RunStats)]
         L.oracle_lim_x_sample.restype = C.c_uint32
         L.oracle_lim_x_sample.argtypes = [C.c_uint64]
         L.oracle_hdn_sample.restype = C.c_uint32
@@ -118,3 +120,21 @@ def run(g, nbfs: int, c: int = 1, x: int = 1, z: int = 0, seed: int = 4343543, s
     n = rs.nrounds
     return pv[:n].copy(), pr[:n].copy(), bc, {"nrounds": n, "skipped": rs.skipped,
                                                "stopped": bool(rs.stopped), "attempts": rs.attempts}
+
+
+def draws(g, nbfs: int, x: int = 1, z: int = 1, seed: int = 4343543):
+    """The pivot sequence of run() for a static distribution (z = 0, 1, 2)
+    with the stop test off, without running the rounds; returns (pivots,
+    probs, stats dict)."""
+    N = g.N
+    cfg = Cfg(nbfs, 0, x, z, int(g.heuristic), 0, seed, 0)
+    pv = np.zeros(max(1, min(nbfs, N)), np.uint32)
+    pr = np.zeros(max(1, min(nbfs, N)), np.float64)
+    rs = RunStats()
+    row = _arr(g.row, np.uint32) if len(g.row) else np.zeros(1, np.uint32)
+    rc = lib().oracle_draws(N, _arr(g.col, np.uint32), row, C.byref(cfg), pv, pr, C.byref(rs))
+    if rc != 0:
+        raise ValueError("draws need a static distribution (z = 0, 1, 2)")
+    n = rs.nrounds
+    return pv[:n].copy(), pr[:n].copy(), {"nrounds": n, "skipped": rs.skipped, "attempts": rs.attempts}
+

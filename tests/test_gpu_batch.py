@@ -9,7 +9,7 @@ import pytest
 
 from axbc_amd import Engine, Graph, _ffi
 from oracle import oracle
-from tests.test_gpu_plain import close
+from tests.parity import close
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +50,9 @@ def check_batch(g, roots, width, label, batches=1, pull=None, chunks=None):
                 lvl, sig, dl, bc_ref, ost = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
                 ok, err = close(delta_of(e, k), dl)
                 assert ok, f"{label} root #{b0 + k} ({v0}): delta rel err {err}"
+                blvl, bsig = e.batch_levels(k)
+                assert np.array_equal(blvl, lvl), f"{label} root #{b0 + k} ({v0}): lvl differs"
+                assert np.array_equal(bsig, sig), f"{label} root #{b0 + k} ({v0}): sigma differs"
                 for key in ("nvisited", "nlevels", "e_r", "e_dag", "all"):
                     assert stats[k][key] == ost[key], f"{label} root {v0}: {key} {stats[k][key]} != {ost[key]}"
             ok, err = close(e.bc(), bc_ref)

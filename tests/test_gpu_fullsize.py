@@ -1,9 +1,10 @@
-"""The benchmark workload at its full size: Kronecker SCALE 22, edgefactor
-16, seed pair (2, 3), -H 1, the first 128 high-degree-neighbourhood roots of
-srand48(4343543) in one batch of width 128 at chunk 512.  The delta of
-sampled roots (first, middle and last in draw order) against the oracle,
-and the bc of the batch against the same roots run one per round on the
-device, within 1e-9 relative (bc and delta are FP64)."""
+"""The benchmark workload at its full size (Kronecker SCALE 22, edgefactor
+16, seed pair (2, 3), -H 1, high-degree-neighbourhood roots of
+srand48(4343543), batches of 128 at chunk 512), the 4096^2 lattice and
+Kronecker SCALE 24: the root sequence against the oracle's draws, sampled
+roots' levels and path counts bit-exact and delta within 1e-9 relative
+against the oracle, and each batch's bc against the same roots run one per
+round on the device (bc and delta are FP64)."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ from axbc_amd import Engine, Graph, _ffi
 from axbc_amd.pivots import PivotLoop
 from oracle import oracle
 from tests.test_gpu_batch import delta_of
-from tests.test_gpu_plain import close
+from tests.parity import close
 
 pytestmark = pytest.mark.gpu
 
@@ -21,21 +22,43 @@ def test_require(require_gpu):
 
 
 def test_scale22_batch_against_oracle_and_plain(require_gpu):
+    """The benchmark loop itself (-z 1, c = 1, x = 1024, batches of 128 at
+    chunk 512) for its first 1024 rounds: the root sequence, probabilities
+    and skip count equal the oracle's draws (the stop test never holds here:
+    check vertices removed by the 1-degree reduction keep bc = 0).  Then the
+    last batch's 128 roots again with counters: three roots' levels, path
+    counts (bit-exact), delta (1e-9) and counters against the oracle, and
+    the batch's bc against the same roots run one per round."""
     print("[fullsize] building SCALE 22 on the device", flush=True)
     g = Graph.kronecker_device(22, 16, seed=(2, 3), heuristic=1)
     L = _ffi.hip()
     with Engine(g) as e:
         assert L.axbc_set_batch_width(e.ctx, 128) == 0
         assert L.axbc_set_batch_chunk(e.ctx, 512) == 0
-        loop = PivotLoop(e, 4096, c=0, x=1, z=1, batch=128)
-        assert loop.step() == 128
-        roots, probs = loop.res.pivots, loop.res.probs
+        loop = PivotLoop(e, 4096, c=1, x=1024, z=1, batch=128)
+        while loop.res.nrounds < 1024:
+            assert loop.step() == 128
+        res = loop.res
+        assert not res.stopped
+        print(f"[fullsize] 1024 rounds in {res.attempts} attempts; oracle draws", flush=True)
+        dv, dp, ds = oracle.draws(g, res.attempts, x=1024, z=1)
+        assert list(dv) == list(res.pivots), "root sequence differs from the oracle's"
+        assert list(dp) == list(res.probs), "probabilities differ from the oracle's"
+        assert ds["skipped"] == res.skipped and ds["nrounds"] == res.nrounds
+        roots, probs = res.pivots[-128:], res.probs[-128:]
+        e.reset_bc()
+        applied, stats = e.batch(roots, probs, stats=True)
+        assert applied == 128
         bc_batch = e.bc()
         print("[fullsize] batch done; oracle rounds", flush=True)
         for k in (0, 64, 127):
-            _, _, dl, _, ost = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            lvl, sig, dl, _, ost = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            blvl, bsig = e.batch_levels(k)
+            assert np.array_equal(blvl, lvl), f"root #{k} ({roots[k]}): lvl differs"
+            assert np.array_equal(bsig, sig), f"root #{k} ({roots[k]}): sigma differs"
             ok, err = close(delta_of(e, k), dl)
             assert ok, f"root #{k} ({roots[k]}): delta rel err {err}"
+            assert stats[k] == ost, f"root #{k} ({roots[k]}): counters {stats[k]} != {ost}"
             assert ost["nvisited"] > 1_000_000  # a full-size round
         print("[fullsize] plain rounds", flush=True)
         e.reset_bc()
@@ -62,7 +85,10 @@ def test_lattice_batch_against_oracle_and_plain(require_gpu):
         bc_batch = e.bc()
         print("[fullsize] lattice batch done; oracle rounds", flush=True)
         for k in (0, 31):
-            _, _, dl, _, ost = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            lvl, sig, dl, _, ost = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            blvl, bsig = e.batch_levels(k)
+            assert np.array_equal(blvl, lvl), f"root #{k} ({roots[k]}): lvl differs"
+            assert np.array_equal(bsig, sig), f"root #{k} ({roots[k]}): sigma differs"
             ok, err = close(delta_of(e, k), dl)
             assert ok, f"root #{k} ({roots[k]}): delta rel err {err}"
             assert ost["nlevels"] > 4000  # a high-diameter round
@@ -90,7 +116,10 @@ def test_scale24_batch_against_oracle_and_plain(require_gpu):
         bc_batch = e.bc()
         print("[fullsize] SCALE 24 batch done; oracle rounds", flush=True)
         for k in (0, len(roots) - 1):
-            _, _, dl, _, _ = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            lvl, sig, dl, _, _ = oracle.pivot(g, int(roots[k]), float(probs[k]), np.zeros(g.N))
+            blvl, bsig = e.batch_levels(k)
+            assert np.array_equal(blvl, lvl), f"root #{k} ({roots[k]}): lvl differs"
+            assert np.array_equal(bsig, sig), f"root #{k} ({roots[k]}): sigma differs"
             ok, err = close(delta_of(e, k), dl)
             assert ok, f"root #{k} ({roots[k]}): delta rel err {err}"
         print("[fullsize] SCALE 24 plain rounds", flush=True)

@@ -68,7 +68,7 @@ def test_edge_cases(require_gpu, heuristic):
 def test_lattice_and_engine(require_gpu):
     from axbc_amd import Engine
     from oracle import oracle
-    from tests.test_gpu_plain import close
+    from tests.parity import close
     rng = np.random.default_rng(3)
     side = 64
     e = [(r * side + c, r * side + c + 1) for r in range(side) for c in range(side - 1)]

@@ -8,7 +8,7 @@ import pytest
 
 from axbc_amd import Graph
 from oracle import oracle
-from tests.test_gpu_plain import close
+from tests.parity import close
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))

@@ -7,7 +7,7 @@ import pytest
 from axbc_amd import Engine, Graph
 from axbc_amd.pivots import PivotLoop
 from oracle import oracle
-from tests.test_gpu_plain import close
+from tests.parity import close
 
 pytestmark = pytest.mark.gpu
 
@@ -107,3 +107,38 @@ def test_autotuned_loop_matches_oracle(require_gpu):
         assert res.nrounds == rs["nrounds"] and res.stopped == rs["stopped"]
         ok, err = close(e.bc(), bc_ref)
         assert ok, err
+
+
+def test_hdn_pick_over_wrapped_prefix(require_gpu):
+    """The device's high-degree-neighbourhood search on a graph whose uint32
+    (LOCINT)pow(deg, 2) prefix wraps (hubs of degree 50000, 40000, 70000 and
+    62000): for draws across the wrap the first vertex whose prefix reaches
+    r, against Python integers mod 2^32 and a linear first match
+    (src/axbc2d.c:2188-2198, 2895-2928)."""
+    import ctypes as C
+    from axbc_amd import _ffi
+    from tests.test_oracle_pins import hub_graph, hdn_expect
+    g = hub_graph()
+    deg, prefix = hdn_expect(g)
+    pre = np.array(prefix, np.uint64)
+    S = prefix[-1]
+    runmax = np.maximum.accumulate(pre)
+    L = _ffi.hip()
+    with Engine(g) as e:
+        pmax = np.empty(g.N, np.uint32)
+        sdn = np.empty(g.N, np.uint32)
+        s = C.c_uint32(0)
+        assert L.axbc_hdn_build(e.ctx, pmax, sdn, C.byref(s)) == 0
+        assert s.value == S
+        assert np.array_equal(pmax.astype(np.uint64), runmax)
+        rng = np.random.default_rng(9)
+        rs = [0, 1, 2500000000, 2500000001, 4100000000, 4100000001, 4254065408, 4254065409, S, S + 1]
+        rs += [int(x) for x in rng.integers(0, S + 1, size=200)]
+        for r in rs:
+            hits = np.flatnonzero(pre >= r)
+            want = int(hits[0]) if hits.size else g.N - 1
+            u, su = C.c_uint32(0), C.c_uint32(0)
+            assert L.axbc_hdn_pick(e.ctx, r, C.byref(u), C.byref(su)) == 0
+            assert u.value == want, f"r = {r}: device {u.value}, first match {want}"
+            want_sdn = sum(deg[w] for w in g.row[g.col[want]:g.col[want + 1]]) & 0xFFFFFFFF
+            assert su.value == want_sdn

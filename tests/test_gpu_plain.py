@@ -9,21 +9,9 @@ import pytest
 
 from axbc_amd import Engine, Graph
 from oracle import oracle
+from tests.parity import close
 
 pytestmark = pytest.mark.gpu
-RTOL = 1e-9
-
-
-def close(a, b, rtol=RTOL):
-    """Within rtol relative; infinities equal and NaNs at the same positions
-    (sigma can wrap to 0 mod 2^32, and (delta+1)/0 then propagates)."""
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    scale = np.maximum(np.abs(a), np.abs(b))
-    ok = (a == b) | (np.isnan(a) & np.isnan(b)) | (np.abs(a - b) <= rtol * scale)
-    return bool(ok.all()), (np.abs(a - b) / np.where(scale > 0, scale, 1)).max() if a.size else 0.0
-
-
 def check_rounds(g, roots, label):
     print(f"[{label}] N={g.N} ned={g.ned} roots={len(roots)}", flush=True)
     bc_ref = np.zeros(g.N)

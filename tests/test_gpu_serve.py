@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from axbc_amd.serve import Job, Server, footprint_bytes
-from tests.test_gpu_plain import close
+from tests.parity import close
 
 pytestmark = pytest.mark.gpu
 
