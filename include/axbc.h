@@ -210,6 +210,17 @@ int axbc_batch_levels(axbc_ctx *ctx, int k, int32_t *lvl, uint32_t *sigma);
  * learned proposal's training target), summed in a fixed order (the same
  * inputs give the same bits).  No reference counterpart. */
 int axbc_batch_utility(axbc_ctx *ctx, const uint32_t *vertex, const double *weight, uint32_t count, double *out);
+/* axbc_batch_utility without any device allocation, for callers that
+ * cannot allocate (code under stream capture, tight serving loops, hosts
+ * that forbid hipMalloc).  No counterpart in the reference.  Same result,
+ * bit for bit, as axbc_batch_utility on the same inputs.  Its staging and
+ * partial rows live in a fixed device array per GPU (about 1.8 MB, held
+ * until the process ends), so count is at most AXBC_NOALLOC_UTILITY_MAX
+ * (AXBC_EINVAL beyond).  Calls that overlap on one GPU share that array and
+ * race: serialise them. */
+#define AXBC_NOALLOC_UTILITY_MAX 65536u
+int axbc_batch_utility_noalloc(axbc_ctx *ctx, const uint32_t *vertex, const double *weight, uint32_t count,
+                               double *out);
 
 /*
  * Per-kernel instrumentation for roofline figures (no reference
