@@ -94,6 +94,69 @@ void get_bc(float *bc) { to_float(get_bc_f64, bc); }
 
 void get_delta(float *delta) { to_float(get_delta_f64, delta); }
 
+/* Per-vertex state of the last round (get_lvl cu:1431, get_sigma cu:1462
+ * copy the same arrays off the device): levels and path counts of the
+ * default context. */
+void get_lvl(int *lvl) { legacy_check(axbc_get_lvl(g_ctx, lvl), "get_lvl"); }
+
+void get_sigma(LOCINT_T *sigma) { legacy_check(axbc_get_sigma(g_ctx, sigma), "get_sigma"); }
+
+/* The 2D process grid (-p RxC with R*C > 1, bc_func in src/axbc2d.c) is not
+ * part of this engine: its frontier exchanges and state overwrites are
+ * linked so that AxBC's host code builds unchanged, and stop the program
+ * with a message if a 2D run reaches them (bc_func_mono never does). */
+static void two_d_only(const char *what) {
+  fprintf(stderr, "axbc error in %s: the 2D process grid is not supported by this engine; run with -p 1x1.\n", what);
+  exit(EXIT_FAILURE);
+}
+
+LOCINT_T scan_col_csc_cuda(LOCINT_T *rbuf, LOCINT_T ld, int *rnum, int np, LOCINT_T *sbuf, int *snum, LOCINT_T *frt,
+                           LOCINT_T *frt_sigma, int level) {
+  (void)rbuf, (void)ld, (void)rnum, (void)np, (void)sbuf, (void)snum, (void)frt, (void)frt_sigma, (void)level;
+  two_d_only("scan_col_csc_cuda");
+  return 0;
+}
+
+LOCINT_T scan_frt_csc_cuda(const LOCINT_T *frt, int ncol, int depth, float *hSRbuf) {
+  (void)frt, (void)ncol, (void)depth, (void)hSRbuf;
+  two_d_only("scan_frt_csc_cuda");
+  return 0;
+}
+
+LOCINT_T write_delta_cuda(LOCINT_T ncol, float *hRFbuf, float *hSFbuf) {
+  (void)ncol, (void)hRFbuf, (void)hSFbuf;
+  two_d_only("write_delta_cuda");
+  return 0;
+}
+
+LOCINT_T append_rows_cuda(LOCINT_T *rbuf, LOCINT_T ld, int *rnum, int np, LOCINT_T *frt, LOCINT_T *frt_sigma,
+                          LOCINT_T nfrt, int level) {
+  (void)rbuf, (void)ld, (void)rnum, (void)np, (void)frt, (void)frt_sigma, (void)nfrt, (void)level;
+  two_d_only("append_rows_cuda");
+  return 0;
+}
+
+void set_get_overlap(LOCINT_T *sigma, int *lvl) {
+  (void)sigma, (void)lvl;
+  two_d_only("set_get_overlap");
+}
+
+void set_sigma(LOCINT_T *sigma) {
+  (void)sigma;
+  two_d_only("set_sigma");
+}
+
+void set_lvl(int *lvl) {
+  (void)lvl;
+  two_d_only("set_lvl");
+}
+
+int set_delta_cuda(float *hSRbuf, int nrow) {
+  (void)hSRbuf, (void)nrow;
+  two_d_only("set_delta_cuda");
+  return 0;
+}
+
 void fincuda(void) {
   legacy_check(axbc_ctx_destroy(g_ctx), "fincuda");
   g_ctx = NULL;

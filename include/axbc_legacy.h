@@ -44,6 +44,23 @@ void get_bc(float *bc);
 void get_delta(float *delta);
 void get_bc_f64(double *bc);
 void get_delta_f64(double *delta);
+/* per-vertex levels / path counts of the last round: cu:1431, 1462 */
+void get_lvl(int *lvl);
+void get_sigma(LOCINT_T *sigma);
+/* Entry points of the 2D process grid (bc_func, src/axbc2d.c:1239-1429;
+ * declared at src/bc2d.h:142-176).  Exported so that AxBC's host code links
+ * unchanged; a 2D run that reaches one prints a message and exits (the 1x1
+ * path, bc_func_mono, never calls them). */
+LOCINT_T scan_col_csc_cuda(LOCINT_T *rbuf, LOCINT_T ld, int *rnum, int np, LOCINT_T *sbuf, int *snum,
+                           LOCINT_T *frt, LOCINT_T *frt_sigma, int level);
+LOCINT_T scan_frt_csc_cuda(const LOCINT_T *frt, int ncol, int depth, float *hSRbuf);
+LOCINT_T write_delta_cuda(LOCINT_T ncol, float *hRFbuf, float *hSFbuf);
+LOCINT_T append_rows_cuda(LOCINT_T *rbuf, LOCINT_T ld, int *rnum, int np, LOCINT_T *frt, LOCINT_T *frt_sigma,
+                          LOCINT_T nfrt, int level);
+void set_get_overlap(LOCINT_T *sigma, int *lvl);
+void set_sigma(LOCINT_T *sigma);
+void set_lvl(int *lvl);
+int set_delta_cuda(float *hSRbuf, int nrow);
 /* fincuda: cu:1659 */
 void fincuda(void);
 /* pinned host buffers: cu:825-846 */
