@@ -60,6 +60,18 @@ constexpr int kBurst = AXBC_BURST;  // row loads in flight per wave in the pull 
 constexpr int kWavesPerSimd = 8;
 constexpr int kBGrid = 2048;      // workgroups of the batch level kernels
 constexpr int kGraphSteps = 8;    // levels (depths) per graph
+// Per-wave counter slots.  Counters that many waves would add to the same
+// few addresses (per-root visits, reach sums, edges, DAG edges; the level's
+// edge sums for the direction test) are kept per wave instead: a wave owns
+// slot gw (its index in the grid) and adds into it with plain loads and
+// stores, and one reduction sums the slots (integers, so the order does
+// not matter).  Device-scope atomics on one address from thousands of waves
+// serialise, which made every level cost tens of microseconds whatever its
+// size.
+constexpr int kSlotWaves = kBGrid * kBlock / kWave;  // waves of the largest level grid
+constexpr int kCloseGrid = kBGrid / 4;
+constexpr int kCloseWaves = kCloseGrid * kBlock / kWave;
+constexpr int kAppendBuf = 256;   // list ids buffered per wave before one tail atomic
 
 struct BCtl {
   uint32_t tail;       // list length
@@ -71,7 +83,6 @@ struct BCtl {
   uint32_t stop_round; // first round after which the stop test holds (nb if none)
   uint32_t pad;
   unsigned long long split_claim;  // (entry index << 40) | chunk base
-  unsigned long long fe_edges;     // out-edges of the level's entries (next level's push work)
   unsigned long long cand_edges;   // in-edges of vertices not yet reached by every root
   // device-resident loop state, so that every level (depth) runs the same
   // launch sequence and a prebuilt graph can replay it
@@ -106,6 +117,8 @@ struct BatchState {
   double *part = nullptr;
   uint64_t part_rows = 0;
   unsigned long long *cnt = nullptr;  // per root: nvisited, all, e_r, e_dag
+  unsigned long long *wslot = nullptr;  // 4 x kSlotWaves x B per-wave partials of cnt
+  unsigned long long *lslot = nullptr;  // kCloseWaves x 4: per-wave level sums of k_bclose
   uint32_t *src = nullptr;
   double *prob = nullptr;
   uint32_t *r0 = nullptr;            // reach of each root
@@ -197,6 +210,68 @@ __device__ __forceinline__ Words<NW> rdlw(const Words<NW> &x, int i) {
 #pragma unroll
   for (int j = 0; j < NW; ++j) r.w[j] = rdl(x.w[j], i);
   return r;
+}
+
+// Adds this wave's per-root partials v[j] (lane = root) to its own slot of
+// counter kind k (0 visits, 1 reach sum, 2 edges, 3 DAG edges).  Only the
+// wave with index gw touches slot gw, and kernels of one stream run one
+// after another, so plain loads and stores suffice.
+template <int NW>
+__device__ __forceinline__ void slot_add(unsigned long long *wslot, int k, uint32_t gw, const uint32_t v[NW]) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) any |= v[j] != 0;
+  if (!__ballot(any)) return;
+  unsigned long long *row = wslot + ((size_t)k * kSlotWaves + gw) * B;
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    if (v[j]) row[lane + 64 * j] += v[j];
+}
+
+// A wave's buffer of ids to append to the level list, in LDS: ids are
+// collected here and written with one tail atomic per kAppendBuf ids (or at
+// the end) instead of one atomic per step.  The order of ids in the list
+// does not matter to any result.  n is wave-uniform.
+struct Appender {
+  uint32_t *buf;
+  uint32_t n;
+};
+
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void app_flush(Appender &a, uint32_t *q, axbc::BCtl *ctl, uint32_t qcap) {
+  if (a.n == 0) return;
+  const int lane = lane_id();
+  uint32_t b = 0;
+  if (lane == 0) b = atomicAdd(&ctl->tail, a.n);
+  b = rdl(b, 0);
+  lds_wave_sync();
+  for (uint32_t k = (uint32_t)lane; k < a.n; k += kWave) {
+    const uint32_t slot = b + k;
+    if (slot < qcap)
+      q[slot] = a.buf[k];
+    else
+      atomicOr(&ctl->overflow, 1u);
+  }
+  lds_wave_sync();
+  a.n = 0;
+}
+
+// Appends v from every lane with pred (all lanes active).
+__device__ __forceinline__ void app_push(Appender &a, bool pred, uint32_t v, uint32_t *q, axbc::BCtl *ctl,
+                                         uint32_t qcap) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  const uint32_t c = (uint32_t)__popcll(m);
+  if (a.n + c > (uint32_t)kAppendBuf) app_flush(a, q, ctl, qcap);
+  const int lane = lane_id();
+  if (pred) a.buf[a.n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+  a.n += c;
 }
 
 // Split vertices.  An entry with more than `ch` edges is claimed with one
@@ -428,7 +503,6 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     ctl->wg_done = 0;
     ctl->split_claim = 0;
     ctl->bu = 0;
-    ctl->fe_edges = 0;
     ctl->cand_edges = in_edges;
     ctl->level = 1;
     ctl->mark_base = mark_base;
@@ -504,6 +578,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
       for (int j = 0; j < NW; ++j) fr_clear[(size_t)v * NW + j] = 0;
     }
   }
+  __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
+  Appender app{s_app[threadIdx.x / kWave], 0};
   const uint32_t fb = qoff[L - 1], fe = qoff[L];
   for (uint32_t base = fb + gw * kWave; base < fe; base += nwv * kWave) {
     const uint32_t i = base + lane;
@@ -540,23 +616,11 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
         first = push_edge<NW>(w, of, vis, nx, mark, mark_id);
       }
       // append to the list (bounded)
-      const unsigned long long m = __ballot(first);
-      if (m) {
-        const int leader = __ffsll((long long)m) - 1;
-        uint32_t b = 0;
-        if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
-        b = rdl(b, leader);
-        const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (first) {
-          if (slot < qcap)
-            q[slot] = w;
-          else
-            atomicOr(&ctl->overflow, 1u);
-        }
-        bytes += 4ull * (unsigned long long)__popcll(m);  // appended ids
-      }
+      bytes += 4ull * (unsigned long long)__popcll(__ballot(first));  // appended ids
+      app_push(app, first, w, q, ctl, qcap);
     }
   }
+  app_flush(app, q, ctl, qcap);
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
@@ -752,7 +816,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
                                                    const uint64_t *__restrict__ nx, const uint64_t *fr0,
                                                    const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                   unsigned long long *cnt, uint32_t *sp_e, uint64_t *sp_base,
+                                                   unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
                                                    uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
                                                    unsigned long long *kb, uint32_t kChunk, int strict) {
   constexpr int B = 64 * NW;
@@ -839,9 +903,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
              4ull * rsum;
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-#pragma unroll
-  for (int j = 0; j < NW; ++j)
-    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+  slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
 template <int NW>
@@ -850,7 +912,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const ui
                                                          const uint64_t *__restrict__ nx, const uint64_t *fr0,
                                                          const uint64_t *fr1, uint32_t *sig,
                                                          const uint32_t *__restrict__ q, axbc::BCtl *ctl,
-                                                         unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
+                                                         unsigned long long *wslot, const uint32_t *__restrict__ sp_e,
                                                          const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
                                                          uint64_t item_cap, unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
@@ -900,9 +962,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const ui
     first_item = false;
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-#pragma unroll
-  for (int j = 0; j < NW; ++j)
-    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+  slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
 // Bottom-up level L (direction-optimising BFS, one pass for discovery and
@@ -918,7 +978,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
                                                   const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
                                                   uint64_t *nx, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
                                                   uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                  unsigned long long *cnt, uint32_t *sp_e, uint64_t *sp_base,
+                                                  unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
                                                   unsigned long long *kb, uint32_t kChunk) {
   constexpr int B = 64 * NW;
@@ -942,6 +1002,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
   uint32_t dag_tot[NW];
 #pragma unroll
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
+  __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
+  Appender app{s_app[threadIdx.x / kWave], 0};
   for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
     const uint32_t v = vb + (uint32_t)lane;
     uint32_t st = 0, d = 0;
@@ -990,24 +1052,12 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
       // reached: sigma write 4; NX words written
       bytes += (unsigned long long)dw * (4 + 8 * NW) + 4ull * contrib + 4ull * roots + (roots ? 8 * NW : 0);
     }
-    if (fresh) {
-      uint32_t b = 0;
-      if (lane == 0) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(fresh));
-      b = rdl(b, 0);
-      if (bit(fresh, lane)) {
-        const uint32_t slot = b + (uint32_t)__popcll(fresh & ((1ull << lane) - 1ull));
-        if (slot < qcap)
-          q[slot] = v;
-        else
-          atomicOr(&ctl->overflow, 1u);
-      }
-      bytes += 4ull * (unsigned long long)__popcll(fresh);
-    }
+    bytes += 4ull * (unsigned long long)__popcll(fresh);
+    app_push(app, bit(fresh, lane), v, q, ctl, qcap);
   }
+  app_flush(app, q, ctl, qcap);
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-#pragma unroll
-  for (int j = 0; j < NW; ++j)
-    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+  slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
 // Bottom-up chunks of the vertices with more than kChunk in-edges: counts
@@ -1019,7 +1069,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
                                                         const uint64_t *__restrict__ vis, uint64_t *nx,
                                                         const uint64_t *fr0, const uint64_t *fr1, uint32_t *sig,
                                                         uint32_t *mark, uint32_t *q, axbc::BCtl *ctl,
-                                                        unsigned long long *cnt, const uint32_t *__restrict__ sp_e,
+                                                        unsigned long long *wslot, const uint32_t *__restrict__ sp_e,
                                                         const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
                                                         uint64_t item_cap, uint32_t qcap, unsigned long long *kb,
                                                         uint32_t kChunk) {
@@ -1087,9 +1137,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
     bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW) + 4ull * contrib;
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-#pragma unroll
-  for (int j = 0; j < NW; ++j)
-    if (dag_tot[j]) atomicAdd(&cnt[4 * (lane + 64 * j) + 3], (unsigned long long)dag_tot[j]);
+  slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
 // Close level L: frontier words, VIS, list words, counters, x rows.
@@ -1097,8 +1145,9 @@ template <int NW>
 __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
                                                    uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
                                                    const uint32_t *__restrict__ q,
-                                                   uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
-                                                   unsigned long long *cnt, uint32_t *seen, uint32_t qcap,
+                                                   uint64_t *qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                   unsigned long long *wslot, unsigned long long *lslot,
+                                                   uint32_t *seen, uint32_t qcap,
                                                    const uint32_t *__restrict__ rcol, unsigned long long *kb) {
   if (ld_flag(&ctl->done)) return;
   const int L = ctl->level;
@@ -1109,12 +1158,13 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  unsigned long long c_vis[NW], c_all[NW], c_er[NW];
+  uint32_t c_vis[NW], c_all[NW], c_er[NW];
   uint64_t a_or[NW];
 #pragma unroll
-  for (int j = 0; j < NW; ++j) c_vis[j] = c_all[j] = c_er[j] = a_or[j] = 0;
+  for (int j = 0; j < NW; ++j) c_vis[j] = c_all[j] = c_er[j] = 0, a_or[j] = 0;
   // a wave takes 64 entries at a time: per-entry words with lane = entry,
-  // then the per-root rows entry by entry with lane = root
+  // then the per-root counts entry by entry with lane = root (x is set by
+  // the dependency pass, or by k_bleaf on the deepest level)
   for (uint32_t base = nb + gw * kWave; base < ne; base += nwv * kWave) {
     const uint32_t i = base + lane;
     const bool valid = i < ne;
@@ -1148,8 +1198,6 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
       }
       seen[w] = stamp;
     }
-    // per-root counters of the level, entry by entry with lane = root (x is
-    // set by the dependency pass, or by k_bleaf on the deepest level)
     const uint32_t cntv = (uint32_t)__popcll(__ballot(valid));
     for (uint32_t k = 0; k < cntv; ++k) {
       const uint32_t rk = rdl(rw, (int)k), dk = rdl(dw, (int)k);
@@ -1175,27 +1223,64 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
     for (int off = kWave / 2; off > 0; off >>= 1) a_or[j] |= __shfl_xor(a_or[j], off);
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-  if (lane == 0 && fe) atomicAdd(&ctl->fe_edges, fe);
-  if (lane == 0 && full) atomicAdd(&ctl->cand_edges, 0ull - full);
-#pragma unroll
-  for (int j = 0; j < NW; ++j) {
-    const int s = lane + 64 * j;
-    if (c_vis[j]) {
-      atomicAdd(&cnt[4 * s + 0], c_vis[j]);
-      atomicAdd(&cnt[4 * s + 1], c_all[j]);
-      atomicAdd(&cnt[4 * s + 2], c_er[j]);
-    }
-    if (lane == 0 && a_or[j]) atomicOr((unsigned long long *)&act[(size_t)L * NW + j], (unsigned long long)a_or[j]);
+  // per-root counts into this wave's slots; level sums into its level slot
+  // (every wave writes its level slot, zero or not; k_bclose_end adds them)
+  slot_add<NW>(wslot, 0, gw, c_vis);
+  slot_add<NW>(wslot, 1, gw, c_all);
+  slot_add<NW>(wslot, 2, gw, c_er);
+  if (lane == 0) {
+    unsigned long long *ls = lslot + 4 * (size_t)gw;
+    ls[0] = fe;
+    ls[1] = full;
+    ls[2] = a_or[0];
+    ls[3] = NW > 1 ? a_or[NW - 1] : 0ull;
   }
 }
 
 
-// End of level L, one thread, after k_bclose in stream order (so every
-// count of the level is complete without fences): level offsets, the done
-// flag, and the direction of the next level.
-__global__ void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->done) return;
+// End of level L, one workgroup, after k_bclose in stream order (so every
+// slot of the level is written without fences): the level's sums from the
+// k_bclose wave slots, the level offsets, the done flag, and the direction
+// of the next level.
+__global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
+                                                       const unsigned long long *__restrict__ lslot, uint64_t *act,
+                                                       int nw) {
+  if (ld_flag(&ctl->done)) return;
+  __shared__ unsigned long long s_red[kBlock / kWave][4];
+  unsigned long long fe = 0, full = 0, a0 = 0, a1 = 0;
+  for (int g = (int)threadIdx.x; g < kCloseWaves; g += (int)blockDim.x) {
+    const unsigned long long *ls = lslot + 4 * (size_t)g;
+    fe += ls[0];
+    full += ls[1];
+    a0 |= ls[2];
+    a1 |= ls[3];
+  }
+  fe = wave_sum_u64(fe);
+  full = wave_sum_u64(full);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    a0 |= __shfl_xor(a0, off);
+    a1 |= __shfl_xor(a1, off);
+  }
+  const int wv = (int)threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    s_red[wv][0] = fe;
+    s_red[wv][1] = full;
+    s_red[wv][2] = a0;
+    s_red[wv][3] = a1;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < (int)(blockDim.x / kWave); ++k) {
+    fe += s_red[k][0];
+    full += s_red[k][1];
+    a0 |= s_red[k][2];
+    a1 |= s_red[k][3];
+  }
   const int L = ctl->level;
+  act[(size_t)L * nw] = a0;
+  if (nw > 1) act[(size_t)L * nw + 1] = a1;
+  ctl->cand_edges -= full;
   uint32_t tail = ctl->tail;
   if (tail > qcap) {  // list overflow: stop the sweep, the host retries larger
     tail = qcap;
@@ -1208,11 +1293,32 @@ __global__ void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uin
   }
   // next level bottom-up when the vertices still open to some root have
   // at most pull16/16 times the in-edges that a push would scan
-  const unsigned long long fe_all = ctl->fe_edges, cand = ctl->cand_edges;
-  ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe_all ? 1u : 0u;
-  ctl->fe_edges = 0;
+  const unsigned long long cand = ctl->cand_edges;
+  ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe ? 1u : 0u;
   ctl->split_claim = 0;
   if (!ctl->done) ctl->level = L + 1;
+}
+
+// Per-root counters of the batch: cnt[4s + k] += the sum of the wave slots
+// of kind k.  Workgroup (k, chunk) adds the slots of its chunk of waves for
+// every root (thread = root), so the slot rows are read whole.
+template <int NW>
+__global__ void k_bcnt_reduce(const unsigned long long *__restrict__ wslot, unsigned long long *cnt) {
+  constexpr int B = 64 * NW;
+  const int s = (int)threadIdx.x;
+  const int k = (int)blockIdx.y;
+  const int chunks = (int)gridDim.x;
+  const int per = (kSlotWaves + chunks - 1) / chunks;
+  const int g0 = (int)blockIdx.x * per, g1 = min(kSlotWaves, g0 + per);
+  if (s >= B) return;
+  unsigned long long t = 0;
+  for (int g = g0; g < g1; ++g) t += wslot[((size_t)k * kSlotWaves + g) * B + s];
+  if (t) atomicAdd(&cnt[4 * s + k], t);
+}
+
+// Zeroes n 64-bit words.
+__global__ void k_bzero(unsigned long long *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // x = (reach + 1) / sigma for the entries of the deepest level L of the
@@ -1791,7 +1897,7 @@ void batch_free(axbc_ctx *c) {
   if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
-               b->lane_of, b->check, b->ctl, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
+               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -1926,6 +2032,8 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   BA(b->sp_base, b->sp_cap);
   BA(b->part, b->part_rows * B);
   BA(b->cnt, 4 * B);
+  BA(b->wslot, 4ull * kSlotWaves * B);
+  BA(b->lslot, 4ull * kCloseWaves);
   BA(b->src, B);
   BA(b->prob, B);
   BA(b->r0, B);
@@ -2019,39 +2127,39 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
   }
   kt_begin(c, KK_SIGMA);
   launch(k, k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
-                     b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                     b->sig, b->q, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
                      kt_bytes(c, KK_SIGMA), b->chunk, c->pull_strict ? 1 : 0);
   kt_end(c, KK_SIGMA);
   if (split_in) {
     kt_begin(c, KK_SIGMA_SPLIT);
     launch(k, k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0],
-                       b->fr[1], b->sig, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                       b->fr[1], b->sig, b->q, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
                        kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
     kt_end(c, KK_SIGMA_SPLIT);
   }
   if (b->pull16) {
     kt_begin(c, KK_PULL);
     launch(k, k_bpull<NW>, dim3(kBGrid), dim3(kBlock), n, c->rcol, c->rrow, b->vis, b->nx, b->fr[0],
-                       b->fr[1], b->sig, b->q, b->qoff, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                       b->fr[1], b->sig, b->q, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
                        b->qcap, kt_bytes(c, KK_PULL), b->chunk);
     kt_end(c, KK_PULL);
     if (split_in) {
       kt_begin(c, KK_PULL_SPLIT);
       launch(k, k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->vis, b->nx,
-                         b->fr[0], b->fr[1], b->sig, b->mark, b->q, b->ctl, b->cnt, b->sp_e, b->sp_base, b->sp_cap,
+                         b->fr[0], b->fr[1], b->sig, b->mark, b->q, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap,
                          b->part_rows, b->qcap, kt_bytes(c, KK_PULL_SPLIT), b->chunk);
       kt_end(c, KK_PULL_SPLIT);
     }
   }
   kt_begin(c, KK_CLOSE);
-  // a quarter grid: each wave ends with per-root counter atomics, and
-  // fewer waves contend less on them (SCALE 22: 1.15 -> 0.85 ms per batch
-  // against half the grid, 1.83 ms at the full grid)
-  launch(k, k_bclose<NW>, dim3(kBGrid / 4), dim3(kBlock), c->col, c->reach, b->nx, b->vis, b->fr[0],
-                     b->fr[1], b->q, b->qw, b->qoff, b->act, b->ctl, b->cnt, b->seen, b->qcap, c->rcol,
+  // a quarter grid: 64 entries per wave, and its per-wave level slots are
+  // what k_bclose_end adds up
+  launch(k, k_bclose<NW>, dim3(kCloseGrid), dim3(kBlock), c->col, c->reach, b->nx, b->vis, b->fr[0],
+                     b->fr[1], b->q, b->qw, b->qoff, b->ctl, b->wslot, b->lslot, b->seen, b->qcap, c->rcol,
                      kt_bytes(c, KK_CLOSE));
   kt_end(c, KK_CLOSE);
-  launch(k, k_bclose_end, dim3(1), dim3(64), b->ctl, b->qoff, b->qcap, b->pull16);
+  launch(k, k_bclose_end, dim3(1), dim3(kBlock), b->ctl, b->qoff, b->qcap, b->pull16, (const unsigned long long *)b->lslot,
+         b->act, NW);
 }
 
 // One backward depth step (the depth comes from the device; the last
@@ -2174,6 +2282,8 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   const uint64_t all1 = NW == 1 || nb <= 64 ? 0ull : (nb >= 128 ? ~0ull : (1ull << (nb - 64)) - 1ull);
   int zeroed = std::min(b->lev_guess + 2, (int)n + 2);
   HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)zeroed * NW, s));
+  // per-wave counter slots start at zero
+  hipLaunchKernelGGL(k_bzero, dim3(1024), dim3(kBlock), 0, s, b->wslot, (size_t)4 * kSlotWaves * NW * 64);
   hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
                      b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, (unsigned long long)c->ned, b->epoch,
                      all0, all1, c->reach, b->dlt);
@@ -2206,6 +2316,8 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     if ((uint64_t)L > (uint64_t)n + 1) return fail(AXBC_ESTATE, "batch forward did not terminate%s (line %d)", "", __LINE__);
     chunk *= 2;
   }
+  // per-root counters from the wave slots of the forward sweep
+  hipLaunchKernelGGL(k_bcnt_reduce<NW>, dim3(64, 4), dim3(64 * NW), 0, s, (const unsigned long long *)b->wslot, b->cnt);
   const int empty = (int)b->h_ctl->empty_lev;
   const int dmax = empty - 1;  // deepest non-empty level over the batch
   b->levels = empty;
