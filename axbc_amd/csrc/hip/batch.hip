@@ -72,6 +72,13 @@ constexpr int kSlotWaves = kBGrid * kBlock / kWave;  // waves of the largest lev
 constexpr int kCloseGrid = kBGrid / 4;
 constexpr int kCloseWaves = kCloseGrid * kBlock / kWave;
 constexpr int kAppendBuf = 256;   // list ids buffered per wave before one tail atomic
+constexpr int kLevelSlot = 8;     // 64-bit fields per wave in the level slots of k_bclose
+// Sparse levels: at most kSparseRatio roots per list entry on average (high-
+// diameter graphs, where the roots' frontiers rarely meet).  Their entries
+// of at most kSparseDeg edges run one (entry, root) pair per lane instead of
+// one root per lane, which would leave most lanes idle.
+constexpr int kSparseRatio = 4;
+constexpr uint32_t kSparseDeg = 32;
 
 struct BCtl {
   uint32_t tail;       // list length
@@ -91,7 +98,7 @@ struct BCtl {
   uint32_t stamp;                  // batch stamp of `seen`
   int depth;                       // backward depth the next kernels run
   int dmin;                        // last backward depth (0 only for single-level roots)
-  uint32_t pad2;
+  uint32_t big_tail;               // entries of a sparse depth left for k_bdep_big
   unsigned long long all[2];       // the batch's roots per word
   unsigned long long rootmask[2];  // roots whose deepest level is 1 (depth 0)
 };
@@ -118,7 +125,9 @@ struct BatchState {
   uint64_t part_rows = 0;
   unsigned long long *cnt = nullptr;  // per root: nvisited, all, e_r, e_dag
   unsigned long long *wslot = nullptr;  // 4 x kSlotWaves x B per-wave partials of cnt
-  unsigned long long *lslot = nullptr;  // kCloseWaves x 4: per-wave level sums of k_bclose
+  unsigned long long *lslot = nullptr;  // kCloseWaves x kLevelSlot: per-wave level sums of k_bclose
+  uint8_t *lmode = nullptr;             // per level: 1 when sparse (few roots per entry)
+  uint32_t *bigq = nullptr;             // list indices of a sparse depth's large entries
   uint32_t *src = nullptr;
   double *prob = nullptr;
   uint32_t *r0 = nullptr;            // reach of each root
@@ -230,13 +239,49 @@ __device__ __forceinline__ void slot_add(unsigned long long *wslot, int k, uint3
     if (v[j]) row[lane + 64 * j] += v[j];
 }
 
+// Adds val to root s of this wave's slot of kind k from any lane: lanes of
+// one wave may hit the same root, so the add is atomic, at workgroup scope
+// (no other wave touches the slot).
+__device__ __forceinline__ void slot_add_root(unsigned long long *wslot, int k, uint32_t gw, int B, int s,
+                                              unsigned long long val) {
+  if (val)
+    __hip_atomic_fetch_add(wslot + ((size_t)k * kSlotWaves + gw) * B + s, val, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Index of the k-th set bit (from 0) of x, which has more than k set bits.
+__device__ __forceinline__ int select_bit(uint64_t x, uint32_t k) {
+  int pos = 0;
+#pragma unroll
+  for (int width = 32; width >= 1; width >>= 1) {
+    const uint32_t c = (uint32_t)__popcll(x & ((1ull << width) - 1ull));
+    if (k >= c) {
+      k -= c;
+      x >>= width;
+      pos += width;
+    }
+  }
+  return pos;
+}
+
+// Root (lane index in the batch) of the k-th set bit of a word set.
+template <int NW>
+__device__ __forceinline__ int select_root(const uint64_t w[NW], uint32_t k) {
+  const uint32_t c0 = (uint32_t)__popcll(w[0]);
+  if (NW == 1 || k < c0) return select_bit(w[0], k);
+  return 64 + select_bit(w[NW - 1], k - c0);
+}
+
 // A wave's buffer of ids to append to the level list, in LDS: ids are
 // collected here and written with one tail atomic per kAppendBuf ids (or at
 // the end) instead of one atomic per step.  The order of ids in the list
 // does not matter to any result.  n is wave-uniform.
 struct Appender {
-  uint32_t *buf;
+  uint32_t *buf;   // this wave's LDS buffer
   uint32_t n;
+  uint32_t *dst;   // list
+  uint32_t *tail;  // its length counter
+  uint32_t cap;    // its capacity (overflow sets the control block's flag)
 };
 
 __device__ __forceinline__ void lds_wave_sync() {
@@ -244,17 +289,17 @@ __device__ __forceinline__ void lds_wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void app_flush(Appender &a, uint32_t *q, axbc::BCtl *ctl, uint32_t qcap) {
+__device__ __forceinline__ void app_flush(Appender &a, axbc::BCtl *ctl) {
   if (a.n == 0) return;
   const int lane = lane_id();
   uint32_t b = 0;
-  if (lane == 0) b = atomicAdd(&ctl->tail, a.n);
+  if (lane == 0) b = atomicAdd(a.tail, a.n);
   b = rdl(b, 0);
   lds_wave_sync();
   for (uint32_t k = (uint32_t)lane; k < a.n; k += kWave) {
     const uint32_t slot = b + k;
-    if (slot < qcap)
-      q[slot] = a.buf[k];
+    if (slot < a.cap)
+      a.dst[slot] = a.buf[k];
     else
       atomicOr(&ctl->overflow, 1u);
   }
@@ -263,12 +308,11 @@ __device__ __forceinline__ void app_flush(Appender &a, uint32_t *q, axbc::BCtl *
 }
 
 // Appends v from every lane with pred (all lanes active).
-__device__ __forceinline__ void app_push(Appender &a, bool pred, uint32_t v, uint32_t *q, axbc::BCtl *ctl,
-                                         uint32_t qcap) {
+__device__ __forceinline__ void app_push(Appender &a, bool pred, uint32_t v, axbc::BCtl *ctl) {
   const unsigned long long m = __ballot(pred);
   if (m == 0ull) return;
   const uint32_t c = (uint32_t)__popcll(m);
-  if (a.n + c > (uint32_t)kAppendBuf) app_flush(a, q, ctl, qcap);
+  if (a.n + c > (uint32_t)kAppendBuf) app_flush(a, ctl);
   const int lane = lane_id();
   if (pred) a.buf[a.n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
   a.n += c;
@@ -469,7 +513,7 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
                          uint32_t *q, uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl,
                          unsigned long long *cnt, uint32_t stamp, uint32_t *seen, unsigned long long in_edges,
                          uint32_t mark_base, uint64_t all0, uint64_t all1, const uint32_t *__restrict__ reach,
-                         double *dlt) {
+                         double *dlt, uint8_t *lmode) {
   constexpr int B = 64 * NW;
   const int s = threadIdx.x + blockIdx.x * blockDim.x;
   if (s < nb) {
@@ -488,6 +532,7 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     cnt[4 * s + 3] = 0;  // e_dag
   }
   if (s == 0) {
+    lmode[0] = 1;  // one root per entry
     qoff[0] = 0;
     qoff[1] = (uint32_t)nb;
     for (int j = 0; j < NW; ++j) {
@@ -579,7 +624,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
     }
   }
   __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
-  Appender app{s_app[threadIdx.x / kWave], 0};
+  Appender app{s_app[threadIdx.x / kWave], 0, q, &ctl->tail, qcap};
   const uint32_t fb = qoff[L - 1], fe = qoff[L];
   for (uint32_t base = fb + gw * kWave; base < fe; base += nwv * kWave) {
     const uint32_t i = base + lane;
@@ -617,10 +662,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
       }
       // append to the list (bounded)
       bytes += 4ull * (unsigned long long)__popcll(__ballot(first));  // appended ids
-      app_push(app, first, w, q, ctl, qcap);
+      app_push(app, first, w, ctl);
     }
   }
-  app_flush(app, q, ctl, qcap);
+  app_flush(app, ctl);
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
@@ -811,6 +856,58 @@ __device__ __forceinline__ void group_sigma(uint32_t st, uint32_t d, const Words
   if (cur >= 0) flush(cur, acc, found);
 }
 
+// Path counts of the small new vertices of a group of 64 (lane = vertex;
+// small: at most kSparseDeg pull edges), one (vertex, root) pair per lane:
+// sigma_s(w) = sum of sigma_s(u) over u in radj(w) with s in FR[u] (uint32,
+// wrapping; exact in any order).  DAG edges go to the wave's slot.
+// Returns the wave's number of terms.
+template <int NW>
+__device__ __forceinline__ uint32_t sigma_pairs(bool small, uint32_t w, const Words<NW> &nw, uint32_t st, uint32_t d,
+                                                const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ frp,
+                                                uint32_t *sig, unsigned long long *wslot, uint32_t gw) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  uint32_t np = 0;
+  if (small)
+#pragma unroll
+    for (int j = 0; j < NW; ++j) np += (uint32_t)__popcll(nw.w[j]);
+  const uint32_t incl = wave_incl_scan(np);
+  const uint32_t total = rdl(incl, kWave - 1);
+  uint32_t terms = 0;
+  for (uint32_t p0 = 0; p0 < total; p0 += kWave) {
+    const uint32_t p = p0 + (uint32_t)lane;
+    const int own = wave_owner(incl, p) & (kWave - 1);
+    const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+    const uint32_t ex = own ? pin : 0u;
+    const uint32_t ow = __shfl(w, own), ost = __shfl(st, own), od = __shfl(d, own);
+    const Words<NW> onw = shflw<NW>(nw, own);
+    if (p < total) {
+      const int s = select_root<NW>(onw.w, p - ex);
+      const int sj = s >> 6;
+      const uint64_t sb = 1ull << (s & 63);
+      uint32_t acc = 0, dag = 0;
+      for (uint32_t e = 0; e < od; e += 4) {
+        uint32_t u[4];
+        bool use[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) u[t] = e + t < od ? rrow[ost + e + t] : 0u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) use[t] = e + t < od && (frp[(size_t)u[t] * NW + sj] & sb);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (use[t]) {
+            acc += sig[(size_t)u[t] * B + s];
+            ++dag;
+          }
+      }
+      sig[(size_t)ow * B + s] = acc;
+      slot_add_root(wslot, 3, gw, B, s, dag);
+      terms += dag;
+    }
+  }
+  return (uint32_t)wave_sum_u64(terms);
+}
+
 template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
                                                    const uint64_t *__restrict__ nx, const uint64_t *fr0,
@@ -818,7 +915,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                    unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
                                                    uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
-                                                   unsigned long long *kb, uint32_t kChunk, int strict) {
+                                                   unsigned long long *kb, uint32_t kChunk, int strict,
+                                                   const uint8_t *__restrict__ lmode) {
   constexpr int B = 64 * NW;
   if (ld_flag(&ctl->done) || ctl->bu) return;
   const int L = ctl->level;
@@ -863,7 +961,9 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
       }
     }
   }
-  // a wave takes 64 new entries of the level at a time
+  // a wave takes 64 new entries of the level at a time (sparse: the level
+  // before had few roots per entry)
+  const bool sparse = lmode[L - 1] != 0;
   for (uint32_t base = nb + gw * kWave; !level1 && base < ne; base += nwv * kWave) {
     const uint32_t i = base + lane;
     uint32_t w = 0, st = 0, d = 0;
@@ -885,7 +985,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
     }
     bytes += 12ull * (unsigned long long)__popcll(__ballot(split));
     uint32_t contrib = 0;
-    group_sigma<NW>(st, d, nw, rrow, frp, sig, dag_tot, &contrib,
+    // sparse levels: the small vertices as (vertex, root) pairs
+    const bool small = sparse && d > 0 && d <= kSparseDeg;
+    if (sparse) contrib += sigma_pairs<NW>(small, w, nw, st, d, rrow, frp, sig, wslot, gw);
+    group_sigma<NW>(st, small ? 0u : d, nw, rrow, frp, sig, dag_tot, &contrib,
                     [&](int o, const uint32_t *acc, const uint64_t *found) {
                       const uint32_t wo = rdl(w, o);
 #pragma unroll
@@ -903,7 +1006,13 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
              4ull * rsum;
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-  slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
+  if (sparse) {
+    // the pairs added to this slot atomically: stay atomic
+#pragma unroll
+    for (int j = 0; j < NW; ++j) slot_add_root(wslot, 3, gw, B, lane + 64 * j, dag_tot[j]);
+  } else {
+    slot_add<NW>(wslot, 3, gw, dag_tot);
+  }
 }
 
 template <int NW>
@@ -1003,7 +1112,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
 #pragma unroll
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
   __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
-  Appender app{s_app[threadIdx.x / kWave], 0};
+  Appender app{s_app[threadIdx.x / kWave], 0, q, &ctl->tail, qcap};
   for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
     const uint32_t v = vb + (uint32_t)lane;
     uint32_t st = 0, d = 0;
@@ -1053,9 +1162,9 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
       bytes += (unsigned long long)dw * (4 + 8 * NW) + 4ull * contrib + 4ull * roots + (roots ? 8 * NW : 0);
     }
     bytes += 4ull * (unsigned long long)__popcll(fresh);
-    app_push(app, bit(fresh, lane), v, q, ctl, qcap);
+    app_push(app, bit(fresh, lane), v, ctl);
   }
-  app_flush(app, q, ctl, qcap);
+  app_flush(app, ctl);
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
   slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
@@ -1154,7 +1263,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   const uint32_t stamp = ctl->stamp;
   uint64_t *frc = (L & 1) ? fr1 : fr0;
   const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
-  unsigned long long bytes = 0, fe = 0, full = 0;
+  unsigned long long bytes = 0, fe = 0, full = 0, pairs = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
@@ -1195,6 +1304,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
         qw[(size_t)i * NW + j] = nw.w[j];
         nx[(size_t)w * NW + j] = 0;
         a_or[j] |= nw.w[j];
+        pairs += (unsigned long long)__popcll(nw.w[j]);
       }
       seen[w] = stamp;
     }
@@ -1217,6 +1327,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   }
   fe = wave_sum_u64(fe);
   full = wave_sum_u64(full);
+  pairs = wave_sum_u64(pairs);
 #pragma unroll
   for (int j = 0; j < NW; ++j) {
 #pragma unroll
@@ -1229,11 +1340,12 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   slot_add<NW>(wslot, 1, gw, c_all);
   slot_add<NW>(wslot, 2, gw, c_er);
   if (lane == 0) {
-    unsigned long long *ls = lslot + 4 * (size_t)gw;
+    unsigned long long *ls = lslot + kLevelSlot * (size_t)gw;
     ls[0] = fe;
     ls[1] = full;
     ls[2] = a_or[0];
     ls[3] = NW > 1 ? a_or[NW - 1] : 0ull;
+    ls[4] = pairs;
   }
 }
 
@@ -1244,19 +1356,21 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
 // of the next level.
 __global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
                                                        const unsigned long long *__restrict__ lslot, uint64_t *act,
-                                                       int nw) {
+                                                       int nw, uint8_t *lmode) {
   if (ld_flag(&ctl->done)) return;
-  __shared__ unsigned long long s_red[kBlock / kWave][4];
-  unsigned long long fe = 0, full = 0, a0 = 0, a1 = 0;
+  __shared__ unsigned long long s_red[kBlock / kWave][5];
+  unsigned long long fe = 0, full = 0, a0 = 0, a1 = 0, pairs = 0;
   for (int g = (int)threadIdx.x; g < kCloseWaves; g += (int)blockDim.x) {
-    const unsigned long long *ls = lslot + 4 * (size_t)g;
+    const unsigned long long *ls = lslot + kLevelSlot * (size_t)g;
     fe += ls[0];
     full += ls[1];
     a0 |= ls[2];
     a1 |= ls[3];
+    pairs += ls[4];
   }
   fe = wave_sum_u64(fe);
   full = wave_sum_u64(full);
+  pairs = wave_sum_u64(pairs);
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     a0 |= __shfl_xor(a0, off);
@@ -1268,6 +1382,7 @@ __global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t
     s_red[wv][1] = full;
     s_red[wv][2] = a0;
     s_red[wv][3] = a1;
+    s_red[wv][4] = pairs;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -1276,8 +1391,10 @@ __global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t
     full += s_red[k][1];
     a0 |= s_red[k][2];
     a1 |= s_red[k][3];
+    pairs += s_red[k][4];
   }
   const int L = ctl->level;
+  lmode[L] = pairs <= (unsigned long long)kSparseRatio * (ctl->tail - qoff[L]) ? 1 : 0;
   act[(size_t)L * nw] = a0;
   if (nw > 1) act[(size_t)L * nw + 1] = a1;
   ctl->cand_edges -= full;
@@ -1436,6 +1553,104 @@ __device__ __forceinline__ uint32_t dep_chunk(uint32_t st, uint32_t cb, uint32_t
   return contrib;
 }
 
+// Dependencies of one entry (v, f) of d edges, lanes = roots: delta_s(v) =
+// reach + sum x_s(w) sigma_s(v) in adjacency order; entries of more than
+// kChunk edges are claimed for k_bdep_split.  Returns the algorithmic bytes.
+template <int NW>
+__device__ __forceinline__ unsigned long long dep_entry(uint32_t i, uint32_t v, const Words<NW> &f, uint32_t st,
+                                                        uint32_t d, const uint32_t *__restrict__ row,
+                                                        const uint64_t *__restrict__ vis,
+                                                        const uint32_t *__restrict__ sig, double *dlt, double *xr,
+                                                        const uint32_t *__restrict__ reach, axbc::BCtl *ctl,
+                                                        uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
+                                                        uint64_t part_rows, uint32_t kChunk) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  if (d > kChunk) {
+    if (lane == 0) claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, part_rows);
+    return 16 + 8 * NW;
+  }
+  double sv[NW], acc[NW];
+  const double r = (double)reach[v];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    // only the entry's roots are used: the other lanes skip the load
+    sv[j] = bit(f.w[j], lane) ? (double)sig[(size_t)v * B + lane + 64 * j] : 1.0;
+    acc[j] = r;
+  }
+  const uint32_t contrib = dep_chunk<NW>(st, 0, d, row, vis, f, xr, sv, acc);
+  uint32_t roots = 0;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    roots += (uint32_t)__popcll(f.w[j]);
+    if (bit(f.w[j], lane)) {
+      dlt[(size_t)v * B + lane + 64 * j] = acc[j];
+      xr[(size_t)v * B + lane + 64 * j] = (acc[j] + 1.0) / sv[j];
+    }
+  }
+  // entry: id 4 + word 8NW + offsets 8 + reach 4; per edge: id 4 + VIS 8NW;
+  // per term: x 8; per root of the entry: sigma 4 + delta 8 + x 8
+  return 16 + 8 * NW + (unsigned long long)d * (4 + 8 * NW) + 8ull * contrib + 20ull * roots;
+}
+
+// Dependencies of the small entries of a group of 64 (lane = entry; small:
+// at most kSparseDeg edges), one (entry, root) pair per lane: the lane folds
+// its root's terms over the entry's edges in adjacency order -- the same
+// operations in the same order as dep_entry, so the same bits.  Returns the
+// algorithmic bytes (the same definition as dep_entry's).
+template <int NW>
+__device__ __forceinline__ unsigned long long dep_pairs(bool small, uint32_t v, const Words<NW> &f, uint32_t st,
+                                                        uint32_t d, const uint32_t *__restrict__ row,
+                                                        const uint64_t *__restrict__ vis,
+                                                        const uint32_t *__restrict__ sig, double *dlt, double *xr,
+                                                        const uint32_t *__restrict__ reach) {
+  constexpr int B = 64 * NW;
+  const int lane = lane_id();
+  uint32_t np = 0;
+  if (small)
+#pragma unroll
+    for (int j = 0; j < NW; ++j) np += (uint32_t)__popcll(f.w[j]);
+  unsigned long long bytes = small ? 16 + 8 * NW + (unsigned long long)d * (4 + 8 * NW) + 20ull * np : 0ull;
+  const uint32_t incl = wave_incl_scan(np);
+  const uint32_t total = rdl(incl, kWave - 1);
+  uint32_t terms = 0;
+  for (uint32_t p0 = 0; p0 < total; p0 += kWave) {
+    const uint32_t p = p0 + (uint32_t)lane;
+    const int own = wave_owner(incl, p) & (kWave - 1);
+    const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+    const uint32_t ex = own ? pin : 0u;
+    const uint32_t ov = __shfl(v, own), ost = __shfl(st, own), od = __shfl(d, own);
+    const Words<NW> of = shflw<NW>(f, own);
+    if (p < total) {
+      const int s = select_root<NW>(of.w, p - ex);
+      const int sj = s >> 6;
+      const uint64_t sb = 1ull << (s & 63);
+      const double sv = (double)sig[(size_t)ov * B + s];
+      double acc = (double)reach[ov];
+      for (uint32_t e = 0; e < od; e += 4) {
+        uint32_t w[4];
+        bool use[4];
+        double x[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] = e + t < od ? row[ost + e + t] : 0u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) use[t] = e + t < od && !(vis[(size_t)w[t] * NW + sj] & sb);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x[t] = use[t] ? xr[(size_t)w[t] * B + s] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (use[t]) {
+            acc += x[t] * sv;
+            ++terms;
+          }
+      }
+      dlt[(size_t)ov * B + s] = acc;
+      xr[(size_t)ov * B + s] = (acc + 1.0) / sv;
+    }
+  }
+  return bytes + 8ull * terms;
+}
+
 template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                  const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
@@ -1443,8 +1658,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
                                                  const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
-                                                 unsigned long long *kb, uint32_t kChunk) {
-  constexpr int B = 64 * NW;
+                                                 unsigned long long *kb, uint32_t kChunk,
+                                                 const uint8_t *__restrict__ lmode, uint32_t *bigq, uint32_t big_cap) {
   const int depth = ctl->depth;
   if (depth < ctl->dmin) return;
   const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
@@ -1452,41 +1667,82 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
   unsigned long long bytes = 0;
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  for (uint32_t i = b + gw; i < e; i += nwv) {
+  if (!lmode[depth]) {
+    // dense level: one entry per wave at a time, lanes = roots
+    for (uint32_t i = b + gw; i < e; i += nwv) {
+      const uint32_t v = q[i];
+      Words<NW> f = ldw<NW>(qw, i);
+      if (depth == 0) {
+        f.w[0] &= root_mask0;
+        if (NW > 1) f.w[NW - 1] &= root_mask1;
+      }
+      if (!f.any()) continue;
+      const uint32_t st = col[v], d = col[v + 1] - st;
+      bytes += dep_entry<NW>(i, v, f, st, d, row, vis, sig, dlt, xr, reach, ctl, sp_e, sp_base, sp_cap, part_rows,
+                             kChunk);
+    }
+  } else {
+    // sparse level: a wave takes 64 entries; the small ones as (entry,
+    // root) pairs; the others go to a list that k_bdep_big deals out one
+    // entry per wave, as on dense levels
+    __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
+    Appender app{s_app[threadIdx.x / kWave], 0, bigq, &ctl->big_tail, big_cap};
+    for (uint32_t base = b + gw * kWave; base < e; base += nwv * kWave) {
+      const uint32_t i = base + lane;
+      uint32_t v = 0, st = 0, d = 0;
+      Words<NW> f;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) f.w[j] = 0;
+      if (i < e) {
+        v = q[i];
+        f = ldw<NW>(qw, i);
+        if (depth == 0) {
+          f.w[0] &= root_mask0;
+          if (NW > 1) f.w[NW - 1] &= root_mask1;
+        }
+        st = col[v];
+        d = col[v + 1] - st;
+      }
+      const bool live = i < e && f.any();
+      const bool small = live && d <= kSparseDeg;
+      bytes += dep_pairs<NW>(small, v, f, st, d, row, vis, sig, dlt, xr, reach);
+      app_push(app, live && !small, i, ctl);
+    }
+    app_flush(app, ctl);
+  }
+  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+}
+
+// The large entries of a sparse depth (listed by k_bdep), one per wave at a
+// time, lanes = roots.
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_big(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                     const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
+                                                     double *dlt, double *xr, const uint32_t *__restrict__ reach,
+                                                     const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                                                     axbc::BCtl *ctl, uint32_t *sp_e, uint64_t *sp_base,
+                                                     uint32_t sp_cap, uint64_t part_rows, unsigned long long *kb,
+                                                     uint32_t kChunk, const uint32_t *__restrict__ bigq,
+                                                     uint32_t big_cap) {
+  const int depth = ctl->depth;
+  if (depth < ctl->dmin) return;
+  const uint32_t nbig = min(ctl->big_tail, big_cap);
+  if (nbig == 0) return;
+  const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
+  const int lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
+  unsigned long long bytes = 0;
+  for (uint32_t k = gw; k < nbig; k += nwv) {
+    const uint32_t i = bigq[k];
     const uint32_t v = q[i];
     Words<NW> f = ldw<NW>(qw, i);
     if (depth == 0) {
       f.w[0] &= root_mask0;
       if (NW > 1) f.w[NW - 1] &= root_mask1;
     }
-    if (!f.any()) continue;
     const uint32_t st = col[v], d = col[v + 1] - st;
-    if (d > kChunk) {
-      if (lane == 0) claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, part_rows);
-      bytes += 16 + 8 * NW;
-      continue;
-    }
-    double sv[NW], acc[NW];
-    const double r = (double)reach[v];
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      // only the entry's roots are used: the other lanes skip the load
-      sv[j] = bit(f.w[j], lane) ? (double)sig[(size_t)v * B + lane + 64 * j] : 1.0;
-      acc[j] = r;
-    }
-    const uint32_t contrib = dep_chunk<NW>(st, 0, d, row, vis, f, xr, sv, acc);
-    uint32_t roots = 0;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      roots += (uint32_t)__popcll(f.w[j]);
-      if (bit(f.w[j], lane)) {
-        dlt[(size_t)v * B + lane + 64 * j] = acc[j];
-        xr[(size_t)v * B + lane + 64 * j] = (acc[j] + 1.0) / sv[j];
-      }
-    }
-    // entry: id 4 + word 8NW + offsets 8 + reach 4; per edge: id 4 + VIS 8NW;
-    // per term: x 8; per root of the entry: sigma 4 + delta 8 + x 8
-    bytes += 16 + 8 * NW + (unsigned long long)d * (4 + 8 * NW) + 8ull * contrib + 20ull * roots;
+    bytes += dep_entry<NW>(i, v, f, st, d, row, vis, sig, dlt, xr, reach, ctl, sp_e, sp_base, sp_cap, part_rows,
+                           kChunk);
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
@@ -1625,6 +1881,7 @@ __global__ void k_bsplit_reset(axbc::BCtl *ctl) {
 __global__ void k_bstep_down(axbc::BCtl *ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->split_claim = 0;
+    ctl->big_tail = 0;
     if (ctl->depth >= ctl->dmin) ctl->depth -= 1;
   }
 }
@@ -1637,6 +1894,7 @@ __global__ void k_bset_depth(axbc::BCtl *ctl, int first, int last, uint64_t root
     ctl->rootmask[0] = rootmask0;
     ctl->rootmask[1] = rootmask1;
     ctl->split_claim = 0;
+    ctl->big_tail = 0;
   }
 }
 
@@ -1897,7 +2155,7 @@ void batch_free(axbc_ctx *c) {
   if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
-               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
+               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->lmode, b->bigq, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -2033,7 +2291,9 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   BA(b->part, b->part_rows * B);
   BA(b->cnt, 4 * B);
   BA(b->wslot, 4ull * kSlotWaves * B);
-  BA(b->lslot, 4ull * kCloseWaves);
+  BA(b->lslot, (size_t)kLevelSlot * kCloseWaves);
+  BA(b->lmode, n + 2);
+  BA(b->bigq, n);
   BA(b->src, B);
   BA(b->prob, B);
   BA(b->r0, B);
@@ -2128,7 +2388,7 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
   kt_begin(c, KK_SIGMA);
   launch(k, k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
                      b->sig, b->q, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
-                     kt_bytes(c, KK_SIGMA), b->chunk, c->pull_strict ? 1 : 0);
+                     kt_bytes(c, KK_SIGMA), b->chunk, c->pull_strict ? 1 : 0, (const uint8_t *)b->lmode);
   kt_end(c, KK_SIGMA);
   if (split_in) {
     kt_begin(c, KK_SIGMA_SPLIT);
@@ -2159,7 +2419,7 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
                      kt_bytes(c, KK_CLOSE));
   kt_end(c, KK_CLOSE);
   launch(k, k_bclose_end, dim3(1), dim3(kBlock), b->ctl, b->qoff, b->qcap, b->pull16, (const unsigned long long *)b->lslot,
-         b->act, NW);
+         b->act, NW, b->lmode);
 }
 
 // One backward depth step (the depth comes from the device; the last
@@ -2170,7 +2430,10 @@ void enqueue_depth(Sink &k, axbc_ctx *c, BatchState *b, bool split_out) {
   kt_begin(c, KK_DEP);
   launch(k, k_bdep<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
                      c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                     kt_bytes(c, KK_DEP), b->chunk);
+                     kt_bytes(c, KK_DEP), b->chunk, (const uint8_t *)b->lmode, b->bigq, c->n);
+  launch(k, k_bdep_big<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
+         c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, kt_bytes(c, KK_DEP),
+         b->chunk, (const uint32_t *)b->bigq, c->n);
   kt_end(c, KK_DEP);
   if (split_out) {
     kt_begin(c, KK_DEP_SPLIT);
@@ -2286,7 +2549,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   hipLaunchKernelGGL(k_bzero, dim3(1024), dim3(kBlock), 0, s, b->wslot, (size_t)4 * kSlotWaves * NW * 64);
   hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
                      b->qw, b->qoff, b->act, b->ctl, b->cnt, b->stamp, b->seen, (unsigned long long)c->ned, b->epoch,
-                     all0, all1, c->reach, b->dlt);
+                     all0, all1, c->reach, b->dlt, b->lmode);
   hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
   HIP_TRY(hipGetLastError());
   // forward levels, speculatively in chunks; kernels past the last level
@@ -2314,7 +2577,10 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
     if (b->h_ctl->done) break;
     if ((uint64_t)L > (uint64_t)n + 1) return fail(AXBC_ESTATE, "batch forward did not terminate%s (line %d)", "", __LINE__);
-    chunk *= 2;
+    // past the previous batch's depth: grow by an eighth of the levels run
+    // (levels launched past the last one still cost a few microseconds per
+    // kernel, so doubling overshot deep graphs by thousands of levels)
+    chunk = std::max(16, L / 8);
   }
   // per-root counters from the wave slots of the forward sweep
   hipLaunchKernelGGL(k_bcnt_reduce<NW>, dim3(64, 4), dim3(64 * NW), 0, s, (const unsigned long long *)b->wslot, b->cnt);
