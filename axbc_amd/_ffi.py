@@ -97,6 +97,7 @@ def hip() -> C.CDLL:
                                             C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64),
                                             C.POINTER(C.c_uint32)]),
         "axbc_set_batch_graphs": (C.c_int, [ctxp, C.c_int]),
+        "axbc_set_batch_persistent": (C.c_int, [ctxp, C.c_int, C.c_double]),
         "axbc_batch_delta": (C.c_int, [ctxp, C.c_int, f64p]),
         "axbc_batch_levels": (C.c_int, [ctxp, C.c_int, i32p, u32p]),
         "axbc_hdn_build": (C.c_int, [ctxp, u32p, u32p, C.POINTER(C.c_uint32)]),

@@ -78,7 +78,15 @@ constexpr int kLevelSlot = 8;     // 64-bit fields per wave in the level slots o
 // of at most kSparseDeg edges run one (entry, root) pair per lane instead of
 // one root per lane, which would leave most lanes idle.
 constexpr int kSparseRatio = 4;
+constexpr int kSparseAvgDeg = 8;   // and at most this many edges per entry on average
 constexpr uint32_t kSparseDeg = 32;
+// A batch runs its levels in persistent cooperative kernels (when enabled
+// and no vertex is split) once the previous batch had more levels than this.
+constexpr int kPersistLevels = 64;
+// Workgroups per CU of a persistent launch: every workgroup's agent-scope
+// fences at each grid barrier write back its XCD's L2, so fewer workgroups
+// make cheaper barriers.
+constexpr int kPersistPerCu = 1;
 
 struct BCtl {
   uint32_t tail;       // list length
@@ -98,7 +106,8 @@ struct BCtl {
   uint32_t stamp;                  // batch stamp of `seen`
   int depth;                       // backward depth the next kernels run
   int dmin;                        // last backward depth (0 only for single-level roots)
-  uint32_t big_tail;               // entries of a sparse depth left for k_bdep_big
+  uint32_t pad2;
+  uint32_t pad3;
   unsigned long long all[2];       // the batch's roots per word
   unsigned long long rootmask[2];  // roots whose deepest level is 1 (depth 0)
 };
@@ -125,9 +134,8 @@ struct BatchState {
   uint64_t part_rows = 0;
   unsigned long long *cnt = nullptr;  // per root: nvisited, all, e_r, e_dag
   unsigned long long *wslot = nullptr;  // 4 x kSlotWaves x B per-wave partials of cnt
-  unsigned long long *lslot = nullptr;  // kCloseWaves x kLevelSlot: per-wave level sums of k_bclose
+  unsigned long long *lslot = nullptr;  // per-wave level sums of k_bclose (kLevelSlot each)
   uint8_t *lmode = nullptr;             // per level: 1 when sparse (few roots per entry)
-  uint32_t *bigq = nullptr;             // list indices of a sparse depth's large entries
   uint32_t *src = nullptr;
   double *prob = nullptr;
   uint32_t *r0 = nullptr;            // reach of each root
@@ -152,6 +160,10 @@ struct BatchState {
   uint64_t gkey = 0;
   size_t bytes = 0;
   int levels = 0;            // levels of the last batch's list (qoff[0 .. levels])
+  int coop_fwd = 0, coop_bwd = 0;  // co-resident grids of the persistent kernels (0: not computed)
+  uint32_t *prog = nullptr;          // per workgroup of a persistent launch: barriers reached
+  uint32_t *bar = nullptr;           // grid barrier words of the persistent launches
+  uint32_t h_bar[4] = {0, 0, 0, 0};
   // scratch of the getters and the utility, allocated on first use and kept
   void *row_out = nullptr;   // N x 8 bytes: one root's delta, or its lvl + sigma
   uint32_t *util_v = nullptr;
@@ -600,7 +612,7 @@ __device__ __forceinline__ bool push_edge(uint32_t w, const Words<NW> &f, const 
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+__device__ __forceinline__ void bpush_body(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                   const uint64_t *__restrict__ vis, uint64_t *nx, uint32_t *mark,
                                                   uint64_t *fr0, uint64_t *fr1, uint32_t *q,
                                                   const uint64_t *__restrict__ qw, const uint32_t *__restrict__ qoff,
@@ -668,6 +680,15 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t 
   app_flush(app, ctl);
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
+
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                  const uint64_t *__restrict__ vis, uint64_t *nx, uint32_t *mark,
+                                                  uint64_t *fr0, uint64_t *fr1, uint32_t *q,
+                                                  const uint64_t *__restrict__ qw, const uint32_t *__restrict__ qoff,
+                                                  axbc::BCtl *ctl, uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
+                                                  uint64_t item_cap, uint32_t qcap, unsigned long long *kb,
+                                                  uint32_t kChunk) { bpush_body<NW>(col, row, vis, nx, mark, fr0, fr1, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, item_cap, qcap, kb, kChunk); }
 
 // Frontier entries with more than kChunk edges, kChunk edges per item.
 template <int NW>
@@ -909,7 +930,7 @@ __device__ __forceinline__ uint32_t sigma_pairs(bool small, uint32_t w, const Wo
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
+__device__ __forceinline__ void bsigma_body(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
                                                    const uint64_t *__restrict__ nx, const uint64_t *fr0,
                                                    const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
                                                    const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
@@ -1016,6 +1037,16 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t
 }
 
 template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
+                                                   const uint64_t *__restrict__ nx, const uint64_t *fr0,
+                                                   const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
+                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                   unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
+                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
+                                                   unsigned long long *kb, uint32_t kChunk, int strict,
+                                                   const uint8_t *__restrict__ lmode) { bsigma_body<NW>(rcol, rrow, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap, kb, kChunk, strict, lmode); }
+
+template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const uint32_t *__restrict__ rcol,
                                                          const uint32_t *__restrict__ rrow,
                                                          const uint64_t *__restrict__ nx, const uint64_t *fr0,
@@ -1083,7 +1114,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const ui
 // appends the new ones with one atomic.  Vertices with more than kChunk
 // in-edges go to k_bpull_split.
 template <int NW>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, const uint32_t *__restrict__ rcol,
+__device__ __forceinline__ void bpull_body(uint32_t n, const uint32_t *__restrict__ rcol,
                                                   const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
                                                   uint64_t *nx, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
                                                   uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
@@ -1169,6 +1200,35 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
   slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, const uint32_t *__restrict__ rcol,
+                                                  const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
+                                                  uint64_t *nx, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
+                                                  uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                  unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
+                                                  uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
+                                                  unsigned long long *kb, uint32_t kChunk) { bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap, kb, kChunk); }
+
+// Level expansion in one launch: the top-down push or the bottom-up pull,
+// as the previous level's close decided (one launch per level instead of
+// two, one of which would exit at once).
+template <int NW>
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bexpand(
+    uint32_t n, const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+    const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow, const uint64_t *__restrict__ vis,
+    uint64_t *nx, uint32_t *mark, uint64_t *fr0, uint64_t *fr1, uint32_t *sig, uint32_t *q,
+    const uint64_t *__restrict__ qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+    unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t item_cap,
+    uint32_t qcap, uint32_t kChunk) {
+  if (ld_flag(&ctl->done)) return;
+  if (ctl->bu)
+    bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap,
+                   nullptr, kChunk);
+  else
+    bpush_body<NW>(col, row, vis, nx, mark, fr0, fr1, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, item_cap, qcap,
+                   nullptr, kChunk);
+}
+
 // Bottom-up chunks of the vertices with more than kChunk in-edges: counts
 // and roots found are added atomically; the chunk that first makes NX[w]
 // non-empty appends w.
@@ -1251,7 +1311,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
 
 // Close level L: frontier words, VIS, list words, counters, x rows.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
+__device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
                                                    uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
                                                    const uint32_t *__restrict__ q,
                                                    uint64_t *qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
@@ -1349,18 +1409,27 @@ __global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ 
   }
 }
 
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
+                                                   uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
+                                                   const uint32_t *__restrict__ q,
+                                                   uint64_t *qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                   unsigned long long *wslot, unsigned long long *lslot,
+                                                   uint32_t *seen, uint32_t qcap,
+                                                   const uint32_t *__restrict__ rcol, unsigned long long *kb) { bclose_body<NW>(col, reach, nx, vis, fr0, fr1, q, qw, qoff, ctl, wslot, lslot, seen, qcap, rcol, kb); }
+
 
 // End of level L, one workgroup, after k_bclose in stream order (so every
 // slot of the level is written without fences): the level's sums from the
 // k_bclose wave slots, the level offsets, the done flag, and the direction
 // of the next level.
-__global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
+__device__ __forceinline__ void bclose_end_body(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
                                                        const unsigned long long *__restrict__ lslot, uint64_t *act,
-                                                       int nw, uint8_t *lmode) {
+                                                       int nw, uint8_t *lmode, int nslots) {
   if (ld_flag(&ctl->done)) return;
   __shared__ unsigned long long s_red[kBlock / kWave][5];
   unsigned long long fe = 0, full = 0, a0 = 0, a1 = 0, pairs = 0;
-  for (int g = (int)threadIdx.x; g < kCloseWaves; g += (int)blockDim.x) {
+  for (int g = (int)threadIdx.x; g < nslots; g += (int)blockDim.x) {
     const unsigned long long *ls = lslot + kLevelSlot * (size_t)g;
     fe += ls[0];
     full += ls[1];
@@ -1394,7 +1463,10 @@ __global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t
     pairs += s_red[k][4];
   }
   const int L = ctl->level;
-  lmode[L] = pairs <= (unsigned long long)kSparseRatio * (ctl->tail - qoff[L]) ? 1 : 0;
+  {
+    const unsigned long long entries = ctl->tail - qoff[L];
+    lmode[L] = pairs <= kSparseRatio * entries && fe <= kSparseAvgDeg * entries ? 1 : 0;
+  }
   act[(size_t)L * nw] = a0;
   if (nw > 1) act[(size_t)L * nw + 1] = a1;
   ctl->cand_edges -= full;
@@ -1415,6 +1487,10 @@ __global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t
   ctl->split_claim = 0;
   if (!ctl->done) ctl->level = L + 1;
 }
+
+__global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
+                                                       const unsigned long long *__restrict__ lslot, uint64_t *act,
+                                                       int nw, uint8_t *lmode) { bclose_end_body(ctl, qoff, qcap, pull16, lslot, act, nw, lmode, kCloseWaves); }
 
 // Per-root counters of the batch: cnt[4s + k] += the sum of the wave slots
 // of kind k.  Workgroup (k, chunk) adds the slots of its chunk of waves for
@@ -1490,9 +1566,8 @@ __global__ void k_broot_deg(const uint32_t *__restrict__ col, const uint32_t *__
 // ---------------------------------------------------------------------------
 
 template <int NW>
-__global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
-                         const uint32_t *__restrict__ qoff, const axbc::BCtl *ctl) {
-  const int depth = ctl->depth;
+__device__ __forceinline__ void bstrip_body(uint64_t *vis, const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                         const uint32_t *__restrict__ qoff, const axbc::BCtl *ctl, int depth) {
   if (depth < ctl->dmin) return;
   const int lev = depth + 1;
   const uint32_t b = qoff[lev], e = qoff[lev + 1];
@@ -1502,6 +1577,10 @@ __global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const ui
     for (int j = 0; j < NW; ++j) vis[(size_t)v * NW + j] &= ~qw[(size_t)i * NW + j];
   }
 }
+
+template <int NW>
+__global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                         const uint32_t *__restrict__ qoff, const axbc::BCtl *ctl) { bstrip_body<NW>(vis, q, qw, qoff, ctl, ctl->depth); }
 
 // Dependencies of one vertex over adj[cb, ce), per root lane, left fold in
 // adjacency order into acc.
@@ -1652,15 +1731,14 @@ __device__ __forceinline__ unsigned long long dep_pairs(bool small, uint32_t v, 
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+__device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                  const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
                                                  double *dlt, double *xr, const uint32_t *__restrict__ reach,
                                                  const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
                                                  unsigned long long *kb, uint32_t kChunk,
-                                                 const uint8_t *__restrict__ lmode, uint32_t *bigq, uint32_t big_cap) {
-  const int depth = ctl->depth;
+                                                 const uint8_t *__restrict__ lmode, int depth) {
   if (depth < ctl->dmin) return;
   const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
   const uint32_t b = qoff[depth], e = qoff[depth + 1];
@@ -1682,11 +1760,9 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
                              kChunk);
     }
   } else {
-    // sparse level: a wave takes 64 entries; the small ones as (entry,
-    // root) pairs; the others go to a list that k_bdep_big deals out one
-    // entry per wave, as on dense levels
-    __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
-    Appender app{s_app[threadIdx.x / kWave], 0, bigq, &ctl->big_tail, big_cap};
+    // sparse level (few roots and few edges per entry): a wave takes 64
+    // entries; the small ones as (entry, root) pairs, the rare others one at
+    // a time (those above the chunk go to k_bdep_split)
     for (uint32_t base = b + gw * kWave; base < e; base += nwv * kWave) {
       const uint32_t i = base + lane;
       uint32_t v = 0, st = 0, d = 0;
@@ -1706,46 +1782,27 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
       const bool live = i < e && f.any();
       const bool small = live && d <= kSparseDeg;
       bytes += dep_pairs<NW>(small, v, f, st, d, row, vis, sig, dlt, xr, reach);
-      app_push(app, live && !small, i, ctl);
+      unsigned long long todo = __ballot(live && !small);
+      while (todo) {
+        const int k = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        bytes += dep_entry<NW>(base + (uint32_t)k, rdl(v, k), rdlw<NW>(f, k), rdl(st, k), rdl(d, k), row, vis, sig, dlt,
+                               xr, reach, ctl, sp_e, sp_base, sp_cap, part_rows, kChunk);
+      }
     }
-    app_flush(app, ctl);
   }
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
-// The large entries of a sparse depth (listed by k_bdep), one per wave at a
-// time, lanes = roots.
 template <int NW>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_big(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
-                                                     const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
-                                                     double *dlt, double *xr, const uint32_t *__restrict__ reach,
-                                                     const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
-                                                     axbc::BCtl *ctl, uint32_t *sp_e, uint64_t *sp_base,
-                                                     uint32_t sp_cap, uint64_t part_rows, unsigned long long *kb,
-                                                     uint32_t kChunk, const uint32_t *__restrict__ bigq,
-                                                     uint32_t big_cap) {
-  const int depth = ctl->depth;
-  if (depth < ctl->dmin) return;
-  const uint32_t nbig = min(ctl->big_tail, big_cap);
-  if (nbig == 0) return;
-  const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
-  const int lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  unsigned long long bytes = 0;
-  for (uint32_t k = gw; k < nbig; k += nwv) {
-    const uint32_t i = bigq[k];
-    const uint32_t v = q[i];
-    Words<NW> f = ldw<NW>(qw, i);
-    if (depth == 0) {
-      f.w[0] &= root_mask0;
-      if (NW > 1) f.w[NW - 1] &= root_mask1;
-    }
-    const uint32_t st = col[v], d = col[v + 1] - st;
-    bytes += dep_entry<NW>(i, v, f, st, d, row, vis, sig, dlt, xr, reach, ctl, sp_e, sp_base, sp_cap, part_rows,
-                           kChunk);
-  }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
-}
+__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                                 const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
+                                                 double *dlt, double *xr, const uint32_t *__restrict__ reach,
+                                                 const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
+                                                 const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                                 uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
+                                                 unsigned long long *kb, uint32_t kChunk,
+                                                 const uint8_t *__restrict__ lmode) { bdep_body<NW>(col, row, vis, sig, dlt, xr, reach, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, part_rows, kb, kChunk, lmode, ctl->depth); }
 
 // Chunks of the split entries: per item the partial sums of its kChunk
 // edges (left fold from 0.0) go to its row of `part`.
@@ -1871,8 +1928,11 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_combine(const ui
   if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
 }
 
+// Between the push's split items and the path-count pull's claims of a
+// top-down level.  On a bottom-up level the claims are the pull's (made by
+// k_bexpand before this kernel) and stay for k_bpull_split.
 __global__ void k_bsplit_reset(axbc::BCtl *ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->bu) {
     ctl->split_claim = 0;
   }
 }
@@ -1881,7 +1941,6 @@ __global__ void k_bsplit_reset(axbc::BCtl *ctl) {
 __global__ void k_bstep_down(axbc::BCtl *ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->split_claim = 0;
-    ctl->big_tail = 0;
     if (ctl->depth >= ctl->dmin) ctl->depth -= 1;
   }
 }
@@ -1894,7 +1953,146 @@ __global__ void k_bset_depth(axbc::BCtl *ctl, int first, int last, uint64_t root
     ctl->rootmask[0] = rootmask0;
     ctl->rootmask[1] = rootmask1;
     ctl->split_claim = 0;
-    ctl->big_tail = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent level loop (graphs without split vertices, many levels)
+// ---------------------------------------------------------------------------
+
+// Current value of a barrier word, read by a compare-and-swap that never
+// matches (the words never hold ~0): a read-modify-write, performed at the
+// coherence point.  A plain or relaxed atomic load of ordinary (coarse-
+// grained) device memory, or an add of 0 (which compiles to a load), can be
+// served by a stale line of the XCD's own L2 and never see another XCD's
+// update.
+__device__ __forceinline__ uint32_t coherent_read(uint32_t *p) { return atomicCAS(p, ~0u, ~0u); }
+
+// Words of the persistent loop's grid barrier, in an allocation of their
+// own that only atomics touch (sharing a cache sector with fields written by
+// plain stores lets that sector's write-back from one XCD's L2 restore stale
+// words).  Arrivals go to kBarGroups counters, one 128-byte line each, so
+// that the workgroups' atomics spread over memory channels instead of
+// queueing on one address; the last arrival of a group arrives at the top
+// counter, and the last top arrival moves to the next generation.
+enum { kBarGen = 0, kBarErr = 1, kBarSeen = 2, kBarTop = 32, kBarGroup0 = 64, kBarGroups = 32,
+       kBarStride = 32, kBarWords = kBarGroup0 + kBarGroups * kBarStride };
+
+// Generation and error words polled with system-scope atomic loads, which
+// are not served from the XCD's own L2 (a relaxed agent-scope load was
+// measured to keep returning a stale generation).
+__device__ __forceinline__ uint32_t poll_read(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident).
+// Each wait is bounded by `limit` ticks of the 100 MHz real-time counter;
+// a workgroup that waits longer, or sees that another one did, sets the
+// error word and gets false, and the kernel ends in every workgroup (the
+// host then reports the error).  prog counts each workgroup's arrivals
+// (diagnostics of a failed wait).
+__device__ __forceinline__ bool grid_sync(uint32_t *bar, uint32_t nblocks, unsigned long long limit,
+                                          uint32_t *prog) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    prog[blockIdx.x] += 1;
+    const uint32_t gen = coherent_read(&bar[kBarGen]);
+    __threadfence();  // this workgroup's writes before its arrival
+    const uint32_t grp = blockIdx.x % kBarGroups;
+    const uint32_t in_grp = nblocks / kBarGroups + (grp < nblocks % kBarGroups ? 1u : 0u);
+    const uint32_t ngrp = nblocks < (uint32_t)kBarGroups ? nblocks : (uint32_t)kBarGroups;
+    bool last = false;
+    if (atomicAdd(&bar[kBarGroup0 + grp * kBarStride], 1u) + 1u == in_grp) {
+      atomicExch(&bar[kBarGroup0 + grp * kBarStride], 0u);
+      if (atomicAdd(&bar[kBarTop], 1u) + 1u == ngrp) {
+        atomicExch(&bar[kBarTop], 0u);
+        last = true;
+      }
+    }
+    if (last) {
+      __threadfence();
+      atomicAdd(&bar[kBarGen], 1u);
+    } else {
+      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+      while (poll_read(&bar[kBarGen]) == gen) {
+        if (poll_read(&bar[kBarErr]) || __builtin_amdgcn_s_memrealtime() - t0 > limit) {
+          if (atomicOr(&bar[kBarErr], 1u) == 0u) atomicExch(&bar[kBarSeen], coherent_read(&bar[kBarTop]));
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();  // the other workgroups' writes after the barrier
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// Resets the barrier words before a persistent launch.
+__global__ void k_bar_reset(uint32_t *bar) {
+  const int t = (int)threadIdx.x;
+  if (blockIdx.x == 0 && t < kBarGroups) atomicExch(&bar[kBarGroup0 + t * kBarStride], 0u);
+  if (blockIdx.x == 0 && t == 0) {
+    atomicExch(&bar[kBarTop], 0u);
+    atomicExch(&bar[kBarErr], 0u);
+    atomicExch(&bar[kBarSeen], 0u);
+  }
+}
+
+// The forward sweep in one launch: per level the push (or the bottom-up
+// pull), the path counts, the close, and the close end in workgroup 0, each
+// followed by a grid barrier; until the done flag.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bfwd_persist(
+    uint32_t n, const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+    const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow, const uint32_t *__restrict__ reach,
+    uint64_t *vis, uint64_t *nx, uint32_t *mark, uint64_t *fr0, uint64_t *fr1, uint32_t *sig, uint32_t *q,
+    uint64_t *qw, uint32_t *qoff, uint64_t *act, axbc::BCtl *ctl, unsigned long long *wslot,
+    unsigned long long *lslot, uint32_t *seen, uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
+    uint64_t item_cap, uint32_t qcap, uint32_t kChunk, int strict, uint8_t *lmode, uint32_t pull16,
+    unsigned long long limit, uint32_t *bar, uint32_t *prog) {
+  const uint32_t nblocks = gridDim.x;
+  const int nslots = (int)(gridDim.x * blockDim.x / kWave);
+  for (;;) {
+    if (ld_flag(&ctl->done)) return;
+    if (ctl->bu)
+      bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap,
+                     qcap, nullptr, kChunk);
+    else
+      bpush_body<NW>(col, row, vis, nx, mark, fr0, fr1, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, item_cap, qcap,
+                     nullptr, kChunk);
+    if (!grid_sync(bar, nblocks, limit, prog)) return;
+    bsigma_body<NW>(rcol, rrow, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap,
+                    nullptr, kChunk, strict, lmode);
+    if (!grid_sync(bar, nblocks, limit, prog)) return;
+    bclose_body<NW>(col, reach, nx, vis, fr0, fr1, q, qw, qoff, ctl, wslot, lslot, seen, qcap, rcol, nullptr);
+    if (!grid_sync(bar, nblocks, limit, prog)) return;
+    if (blockIdx.x == 0) bclose_end_body(ctl, qoff, qcap, pull16, lslot, act, NW, lmode, nslots);
+    if (!grid_sync(bar, nblocks, limit, prog)) return;
+  }
+}
+
+// The backward sweep in one launch: per depth (first down to last) the VIS
+// strip and the dependency pass, each followed by a grid barrier.  The depth is the loop's own; no split vertices here.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_bbwd_persist(
+    const uint32_t *__restrict__ col, const uint32_t *__restrict__ row, uint64_t *vis,
+    const uint32_t *__restrict__ sig, double *dlt, double *xr, const uint32_t *__restrict__ reach,
+    const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw, const uint32_t *__restrict__ qoff,
+    axbc::BCtl *ctl, uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows, uint32_t kChunk,
+    const uint8_t *__restrict__ lmode, int first, int last,
+    unsigned long long limit, uint32_t *bar, uint32_t *prog) {
+  const uint32_t nblocks = gridDim.x;
+  for (int depth = first; depth >= last; --depth) {
+    bstrip_body<NW>(vis, q, qw, qoff, ctl, depth);
+    if (!grid_sync(bar, nblocks, limit, prog)) return;
+    bdep_body<NW>(col, row, vis, sig, dlt, xr, reach, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, part_rows, nullptr,
+                  kChunk, lmode, depth);
+    if (!grid_sync(bar, nblocks, limit, prog)) return;
   }
 }
 
@@ -2155,7 +2353,7 @@ void batch_free(axbc_ctx *c) {
   if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
-               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->lmode, b->bigq, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
+               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->lmode, b->prog, b->bar, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
@@ -2291,9 +2489,10 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   BA(b->part, b->part_rows * B);
   BA(b->cnt, 4 * B);
   BA(b->wslot, 4ull * kSlotWaves * B);
-  BA(b->lslot, (size_t)kLevelSlot * kCloseWaves);
+  BA(b->lslot, (size_t)kLevelSlot * kSlotWaves);
   BA(b->lmode, n + 2);
-  BA(b->bigq, n);
+  BA(b->prog, kBGrid);
+  BA(b->bar, kBarWords);
   BA(b->src, B);
   BA(b->prob, B);
   BA(b->r0, B);
@@ -2314,6 +2513,12 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->bytes += b->bytes;
+  return AXBC_OK;
+}
+
+int pull_bar(axbc_ctx *c) {
+  HIP_TRY(hipMemcpyAsync(c->batch->h_bar, c->batch->bar, sizeof(c->batch->h_bar), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return AXBC_OK;
 }
 
@@ -2366,17 +2571,48 @@ void launch(Sink &k, void (*f)(P...), dim3 g, dim3 b, A... a) {
   if (e != hipSuccess && k.err == hipSuccess) k.err = e;
 }
 
+// Cooperative launch of f<<<g, b>>>(a...) on stream s (all workgroups
+// resident at once; the runtime refuses a grid that cannot be).
+template <typename... P, typename... A>
+hipError_t coop_launch(void (*f)(P...), dim3 g, dim3 b, hipStream_t s, A... a) {
+  static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+  std::tuple<P...> t(a...);
+  void *args[sizeof...(P) ? sizeof...(P) : 1];
+  param_ptrs(t, args, std::index_sequence_for<P...>{});
+  return hipLaunchCooperativeKernel((const void *)f, g, b, args, 0, s);
+}
+
+// Grid of f for a persistent launch: one workgroup per CU fewer than the
+// occupancy query reports, capped at kBGrid.  (At the reported 7 workgroups
+// per CU the last workgroup of every CU of the backward sweep was not
+// resident until the others left the kernel: measured, every barrier then
+// waits for 256 workgroups that cannot start.)
+template <typename F>
+int coop_grid(axbc_ctx *c, F f) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, 0) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) return 0;
+  return std::min(kBGrid, std::min(std::max(1, per_cu - 1), kPersistPerCu) * cus);
+}
+
 // One forward level: the same launch sequence for every level (the kernels
 // read the level from the device), so it can be built into a graph once and
 // replayed.
 template <int NW>
 void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool split_in) {
   const uint32_t n = c->n;
-  kt_begin(c, KK_PUSH);
-  launch(k, k_bpush<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->nx, b->mark, b->fr[0],
-                     b->fr[1], b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
-                     kt_bytes(c, KK_PUSH), b->chunk);
-  kt_end(c, KK_PUSH);
+  const bool merged = !c->ktimer.on && b->pull16;
+  if (merged) {
+    launch(k, k_bexpand<NW>, dim3(kBGrid), dim3(kBlock), n, c->col, c->row, c->rcol, c->rrow, b->vis, b->nx, b->mark,
+           b->fr[0], b->fr[1], b->sig, b->q, b->qw, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap,
+           b->part_rows, b->qcap, b->chunk);
+  } else {
+    kt_begin(c, KK_PUSH);
+    launch(k, k_bpush<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->nx, b->mark, b->fr[0],
+                       b->fr[1], b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                       kt_bytes(c, KK_PUSH), b->chunk);
+    kt_end(c, KK_PUSH);
+  }
   if (split_out) {
     kt_begin(c, KK_PUSH_SPLIT);
     launch(k, k_bpush_split<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->nx, b->mark,
@@ -2398,11 +2634,13 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
     kt_end(c, KK_SIGMA_SPLIT);
   }
   if (b->pull16) {
-    kt_begin(c, KK_PULL);
-    launch(k, k_bpull<NW>, dim3(kBGrid), dim3(kBlock), n, c->rcol, c->rrow, b->vis, b->nx, b->fr[0],
-                       b->fr[1], b->sig, b->q, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                       b->qcap, kt_bytes(c, KK_PULL), b->chunk);
-    kt_end(c, KK_PULL);
+    if (!merged) {
+      kt_begin(c, KK_PULL);
+      launch(k, k_bpull<NW>, dim3(kBGrid), dim3(kBlock), n, c->rcol, c->rrow, b->vis, b->nx, b->fr[0],
+                         b->fr[1], b->sig, b->q, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                         b->qcap, kt_bytes(c, KK_PULL), b->chunk);
+      kt_end(c, KK_PULL);
+    }
     if (split_in) {
       kt_begin(c, KK_PULL_SPLIT);
       launch(k, k_bpull_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->vis, b->nx,
@@ -2430,10 +2668,7 @@ void enqueue_depth(Sink &k, axbc_ctx *c, BatchState *b, bool split_out) {
   kt_begin(c, KK_DEP);
   launch(k, k_bdep<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
                      c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                     kt_bytes(c, KK_DEP), b->chunk, (const uint8_t *)b->lmode, b->bigq, c->n);
-  launch(k, k_bdep_big<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->row, b->vis, b->sig, b->dlt, b->xr,
-         c->reach, b->q, b->qw, b->ctl, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, kt_bytes(c, KK_DEP),
-         b->chunk, (const uint32_t *)b->bigq, c->n);
+                     kt_bytes(c, KK_DEP), b->chunk, (const uint8_t *)b->lmode);
   kt_end(c, KK_DEP);
   if (split_out) {
     kt_begin(c, KK_DEP_SPLIT);
@@ -2488,6 +2723,34 @@ int ensure_graphs(axbc_ctx *c, bool split_out, bool split_in) {
   }
   b->gkey = key;
   return AXBC_OK;
+}
+
+// A barrier of the persistent loop waited past its limit: the sweep stopped
+// part way, so the batch state is dropped (rebuilt by the next batch) and bc
+// is left as it was (the update had not run).
+int persist_failed(axbc_ctx *c) {
+  {
+    // how many workgroups reached how many barriers
+    BatchState *b = c->batch;
+    std::vector<uint32_t> pr(kBGrid);
+    if (hipMemcpy(pr.data(), b->prog, sizeof(uint32_t) * kBGrid, hipMemcpyDeviceToHost) == hipSuccess) {
+      std::vector<std::pair<uint32_t, int>> hist;
+      for (int g = 0; g < kBGrid; ++g) {
+        bool found = false;
+        for (auto &h : hist)
+          if (h.first == pr[g]) ++h.second, found = true;
+        if (!found) hist.push_back({pr[g], 1});
+      }
+      fprintf(stderr, "persistent loop: barriers reached per workgroup (barriers: workgroups):");
+      for (auto &h : hist) fprintf(stderr, " %u: %d", h.first, h.second);
+      fprintf(stderr, "\n");
+    }
+  }
+  char what[192];
+  snprintf(what, sizeof what, " (grid %d/%d workgroups, %u groups arrived, generation %u)", c->batch->coop_fwd,
+           c->batch->coop_bwd, c->batch->h_bar[kBarSeen], c->batch->h_bar[kBarGen]);
+  batch_free(c);
+  return fail(AXBC_EHIP, "persistent level loop: a barrier wait passed its limit%s (line %d)", what, __LINE__);
 }
 
 template <int NW>
@@ -2552,10 +2815,32 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
                      all0, all1, c->reach, b->dlt, b->lmode);
   hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
   HIP_TRY(hipGetLastError());
+  int L = 1, chunk = b->lev_guess;
+  // deep graphs without split vertices: the whole sweep in one persistent
+  // cooperative launch per direction (no launch per level)
+  bool persist = c->batch_persist != 0 && !split_out && !split_in && !c->ktimer.on &&
+                 (c->batch_persist == 2 || b->lev_guess > kPersistLevels);
+  if (persist && !b->coop_fwd) {
+    b->coop_fwd = coop_grid(c, k_bfwd_persist<NW>);
+    b->coop_bwd = coop_grid(c, k_bbwd_persist<NW>);
+  }
+  persist = persist && b->coop_fwd > 0 && b->coop_bwd > 0;
+  const unsigned long long limit = c->batch_persist_limit;
+  if (persist) {
+    HIP_TRY(hipMemsetAsync(b->prog, 0, sizeof(uint32_t) * kBGrid, s));
+    hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, s, b->bar);
+    HIP_TRY(coop_launch(k_bfwd_persist<NW>, dim3(b->coop_fwd), dim3(kBlock), s, n, c->col, c->row, c->rcol, c->rrow,
+                        c->reach, b->vis, b->nx, b->mark, b->fr[0], b->fr[1], b->sig, b->q, b->qw, b->qoff, b->act,
+                        b->ctl, b->wslot, b->lslot, b->seen, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
+                        b->chunk, c->pull_strict ? 1 : 0, b->lmode, b->pull16, limit, b->bar, b->prog));
+    if ((rc = pull_bctl(c)) || (rc = pull_bar(c))) return rc;
+    if (b->h_bar[kBarErr]) return persist_failed(c);
+    if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
+    L = (int)b->h_ctl->empty_lev + 1;  // levels run (the epoch of the level stamps moves past them)
+  }
   // forward levels, speculatively in chunks; kernels past the last level
   // exit on the device done flag
-  int L = 1, chunk = b->lev_guess;
-  for (;;) {
+  for (; !persist;) {
     const int steps = graphs ? (chunk + kGraphSteps - 1) / kGraphSteps * kGraphSteps : chunk;
     // per-level root words of the levels about to run (a BFS has at most
     // n levels: the buffer holds n + 2 rows)
@@ -2615,7 +2900,16 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   if (dsteps > 0) {
     hipLaunchKernelGGL(k_bset_depth, dim3(1), dim3(64), 0, s, b->ctl, dmax - 1, dmin, rootmask[0],
                        NW > 1 ? rootmask[1] : 0ull);
-    if (graphs) {
+    if (persist) {
+      HIP_TRY(hipMemsetAsync(b->prog, 0, sizeof(uint32_t) * kBGrid, s));
+      hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, s, b->bar);
+      HIP_TRY(coop_launch(k_bbwd_persist<NW>, dim3(b->coop_bwd), dim3(kBlock), s, c->col, c->row, b->vis, b->sig,
+                          b->dlt, b->xr, c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap,
+                          b->part_rows, b->chunk, (const uint8_t *)b->lmode, dmax - 1, dmin, limit,
+                          b->bar, b->prog));
+      if ((rc = pull_bar(c))) return rc;
+      if (b->h_bar[kBarErr]) return persist_failed(c);
+    } else if (graphs) {
       for (int j = 0; j < dsteps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gbwd, s));
     } else {
       Sink k;
@@ -2725,6 +3019,15 @@ int axbc_set_batch_chunk(axbc_ctx *c, uint32_t chunk) {
 int axbc_set_batch_graphs(axbc_ctx *c, int enable) {
   if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
   c->use_graphs = enable != 0;
+  return AXBC_OK;
+}
+
+int axbc_set_batch_persistent(axbc_ctx *c, int mode, double limit_seconds) {
+  if (!c || mode < 0 || mode > 2 || !(limit_seconds > 0.0) || limit_seconds > 3600.0)
+    return fail(AXBC_EINVAL, "persistent mode 0..2 and a limit in (0, 3600] s%s (line %d)", "", __LINE__);
+  c->batch_persist = mode;
+  c->batch_persist_limit = (unsigned long long)(limit_seconds * 1e8);  // 100 MHz real-time ticks
+  if (c->batch_persist_limit == 0) c->batch_persist_limit = 1;
   return AXBC_OK;
 }
 

@@ -144,6 +144,8 @@ struct axbc_ctx {
   uint32_t batch_chunk = 256;
   uint32_t batch_pull16 = 32;        // bottom-up ratio x16 for batched levels (0: top-down only)
   bool use_graphs = true;            // batched level loop from captured HIP graphs
+  int batch_persist = 0;             // persistent level loop: 0 never (default), 1 deep graphs, 2 whenever possible
+  unsigned long long batch_persist_limit = 1000000000ull;  // barrier wait limit, 100 MHz ticks (10 s)
   uint32_t max_out = 0, max_in = 0;  // largest out- and in-degree (set with the transpose)
   size_t budget = 0;  // 0: no limit
   axbc::KernelTimer ktimer;

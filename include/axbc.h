@@ -196,6 +196,16 @@ int axbc_set_batch_pull(axbc_ctx *ctx, uint32_t ratio16);
  * the host.  Kernel timing (axbc_kernel_timing) always uses host launches.
  * Results do not depend on it. */
 int axbc_set_batch_graphs(axbc_ctx *ctx, int enable);
+/* Persistent level loop: on graphs without split vertices (every list
+ * below the chunk) whose previous batch ran more than 64 levels (mode 1,
+ * default), or on every such graph (mode 2), each direction of the sweep
+ * runs in one cooperative launch whose workgroups pass the levels (depths)
+ * through grid barriers, instead of several launches per level; mode 0
+ * never.  Every barrier wait is bounded by limit_seconds (default 10): a
+ * longer wait ends the launch in every workgroup and the batch fails with
+ * AXBC_EHIP, bc unchanged and the batch state rebuilt by the next batch.
+ * Results do not depend on it. */
+int axbc_set_batch_persistent(axbc_ctx *ctx, int mode, double limit_seconds);
 int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);

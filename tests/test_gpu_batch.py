@@ -258,3 +258,67 @@ def test_sigma_wraps_to_zero_batched(require_gpu, width):
     g = layered(256, 6)
     roots = np.array([0, 1 + 5 * 256 + 7, 1 + 2 * 256 + 3, 1 + 4 * 256, 5], np.uint32)
     check_batch(g, roots, width, "sigma wraps to 0")
+
+
+def _persist(e, mode, limit=10.0):
+    assert _ffi.hip().axbc_set_batch_persistent(e.ctx, mode, limit) == 0, _ffi.hip().axbc_last_error()
+
+
+@pytest.mark.parametrize("width", [64, 128])
+def test_persistent_level_loop(require_gpu, width):
+    """High-diameter graphs (a 96^2 lattice with 3% of its edges deleted, a
+    long path with pendants): the sweep in persistent cooperative launches
+    (mode 2) against the oracle, and bit-identical to the launch-per-level
+    loop (mode 0)."""
+    lat = Graph.lattice(96, 0.03, seed=3, heuristic=0)
+    n = 1500
+    path = Graph.from_edges(n, np.array([(i, i + 1) for i in range(n - 1)] + [(i, n - 1 - i // 2) for i in range(0, 40, 3)]))
+    for g, label in ((lat, "lattice 96"), (path, "path 1500")):
+        deg = g.degrees()
+        rng = np.random.default_rng(width)
+        roots = rng.choice(np.flatnonzero(deg > 0), size=width - 5, replace=False)
+        probs = np.full(len(roots), 1.0 / g.N)
+        out = {}
+        for mode in (2, 0):
+            bc_ref = np.zeros(g.N)
+            with Engine(g) as e:
+                assert _ffi.hip().axbc_set_batch_width(e.ctx, width) == 0
+                _persist(e, mode)
+                applied, stats = run_batch(e, roots, probs)
+                assert applied == len(roots)
+                if mode == 2:
+                    for k in (0, len(roots) // 2, len(roots) - 1):
+                        lvl, sig, dl, bc_ref, ost = oracle.pivot(g, int(roots[k]), 1.0 / g.N, np.zeros(g.N))
+                        blvl, bsig = e.batch_levels(k)
+                        assert np.array_equal(blvl, lvl) and np.array_equal(bsig, sig), f"{label} root #{k}"
+                        ok, err = close(delta_of(e, k), dl)
+                        assert ok, f"{label} root #{k}: delta rel err {err}"
+                        assert stats[k] == ost, f"{label} root #{k}: {stats[k]} != {ost}"
+                    assert max(s["nlevels"] for s in stats) > 60
+                out[mode] = e.bc()
+        assert np.array_equal(out[2], out[0]), f"{label}: persistent and per-level bc differ"
+
+
+def test_persistent_barrier_limit_fails_cleanly(require_gpu):
+    """A barrier limit far below the time a level takes (forced from the
+    host): the launch ends in every workgroup, the batch reports the error,
+    bc is unchanged, and the next batch with the normal limit rebuilds its
+    state and matches the oracle."""
+    g = Graph.lattice(64, 0.03, seed=3, heuristic=0)
+    deg = g.degrees()
+    roots = np.random.default_rng(1).choice(np.flatnonzero(deg > 0), size=40, replace=False)
+    probs = np.full(len(roots), 1.0 / g.N)
+    L = _ffi.hip()
+    with Engine(g) as e:
+        _persist(e, 2, 1e-9)
+        rc = L.axbc_batch(e.ctx, np.ascontiguousarray(roots, np.uint32), probs, len(roots), 0, C.byref(C.c_int(0)), None)
+        assert rc == -3 and b"limit" in L.axbc_last_error()
+        assert (e.bc() == 0).all()
+        _persist(e, 2, 10.0)
+        applied, _ = run_batch(e, roots, probs)
+        assert applied == len(roots)
+        bc_ref = np.zeros(g.N)
+        for v in roots:
+            _, _, _, bc_ref, _ = oracle.pivot(g, int(v), 1.0 / g.N, bc_ref)
+        ok, err = close(e.bc(), bc_ref)
+        assert ok, err
