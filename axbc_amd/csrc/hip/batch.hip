@@ -163,6 +163,8 @@ struct BatchState {
   int coop_fwd = 0, coop_bwd = 0;  // co-resident grids of the persistent kernels (0: not computed)
   uint32_t *prog = nullptr;          // per workgroup of a persistent launch: barriers reached
   uint32_t *bar = nullptr;           // grid barrier words of the persistent launches
+  bool plan_only = false;            // balloc adds up sizes without allocating
+  bool charged = false;              // bytes counted in the context's total
   uint32_t h_bar[4] = {0, 0, 0, 0};
   // scratch of the getters and the utility, allocated on first use and kept
   void *row_out = nullptr;   // N x 8 bytes: one root's delta, or its lvl + sigma
@@ -2359,6 +2361,7 @@ void batch_free(axbc_ctx *c) {
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   if (b->h_cnt) (void)hipHostFree(b->h_cnt);
   if (b->h_act) (void)hipHostFree(b->h_act);
+  if (b->charged) c->bytes -= std::min(c->bytes, b->bytes);
   delete b;
   c->batch = nullptr;
 }
@@ -2370,9 +2373,25 @@ namespace {
 template <typename T>
 int balloc(BatchState *b, T **p, size_t count) {
   const size_t bytes = sizeof(T) * (count ? count : 1);
+  if (b->plan_only) {
+    b->bytes += bytes;
+    *p = nullptr;
+    return AXBC_OK;
+  }
   if (hipMalloc((void **)p, bytes) != hipSuccess)
     return fail(AXBC_ENOMEM, "batch allocation of %s failed (line %d)", "device memory", __LINE__);
   b->bytes += bytes;
+  return AXBC_OK;
+}
+
+// An allocation for the batch after its creation (getters, utility):
+// checked against the context's budget and counted in its total.
+template <typename T>
+int balloc_more(axbc_ctx *c, T **p, size_t count, const char *what) {
+  int rc;
+  if ((rc = budget_check(c, sizeof(T) * (count ? count : 1), what))) return rc;
+  if ((rc = balloc(c->batch, p, count))) return rc;
+  c->bytes += sizeof(T) * (count ? count : 1);
   return AXBC_OK;
 }
 
@@ -2437,6 +2456,10 @@ int ensure_transpose(axbc_ctx *c) {
     return AXBC_OK;
   }
   c->symmetric = false;
+  {
+    int rc;
+    if ((rc = budget_check(c, sizeof(uint32_t) * (c->n + 1 + c->ned), "the transposed graph"))) return rc;
+  }
   HIP_TRY(hipMalloc((void **)&c->rcol, sizeof(uint32_t) * (c->n + 1)));
   HIP_TRY(hipMalloc((void **)&c->rrow, sizeof(uint32_t) * (c->ned ? c->ned : 1)));
   HIP_TRY(hipMemcpyAsync(c->rcol, rcol.data(), sizeof(uint32_t) * (c->n + 1), hipMemcpyHostToDevice, c->stream));
@@ -2447,30 +2470,13 @@ int ensure_transpose(axbc_ctx *c) {
   return AXBC_OK;
 }
 
-int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
-  int rc;
-  if ((rc = ensure_transpose(c))) return rc;
-  BatchState *b = new BatchState();
-  c->batch = b;
-  b->nw = width / 64;
-  b->B = width;
-  b->n = c->n;
-  const size_t n = c->n, nw = b->nw, B = b->B;
-  // split entries: vertices with more than kChunk edges, per level at most
-  // all of them; chunk rows: at most ned / kChunk + number of them
-  uint32_t nsplit = 0;
-  // sized for the smallest chunk so that the chunk can be tuned later
-  for (size_t v = 0; v < n; ++v) nsplit += (c->h_col[v + 1] - c->h_col[v]) > kChunkMin;
-  b->sp_cap = nsplit + 1;
-  b->part_rows = c->ned / kChunkMin + nsplit + 1;
-  b->chunk = c->batch_chunk;
-  b->pull16 = c->batch_pull16;
-  b->qcap = qcap;
-#define BA(p, cnt)                         \
-  if ((rc = balloc(b, &(p), (cnt))) != 0) { \
-    batch_free(c);                         \
-    return rc;                             \
-  }
+// Every device buffer of a batch state, sized from b's n, nw, B, qcap,
+// sp_cap and part_rows; with b->plan_only set it only adds up the bytes.
+int alloc_buffers(BatchState *b) {
+  const size_t n = b->n, nw = b->nw, B = b->B, qcap = b->qcap;
+  int r;
+#define BA(p, cnt) \
+  if ((r = balloc(b, &(p), (cnt))) != 0) return r;
   BA(b->vis, n * nw);
   BA(b->nx, n * nw);
   BA(b->fr[0], n * nw);
@@ -2481,7 +2487,7 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   if (nw > 1) BA(b->mark, n);
   BA(b->seen, n);
   BA(b->q, qcap);
-  BA(b->qw, (size_t)qcap * nw);
+  BA(b->qw, qcap * nw);
   BA(b->qoff, n + 2);
   BA(b->act, (n + 2) * nw);
   BA(b->sp_e, b->sp_cap);
@@ -2499,6 +2505,51 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   BA(b->lane_of, B);
   BA(b->ctl, 1);
 #undef BA
+  return AXBC_OK;
+}
+
+// Split entries and chunk rows of a graph: vertices with more than
+// kChunkMin edges (sized for the smallest chunk, so that the chunk can be
+// tuned later), and at most ned / kChunkMin + nsplit + 1 chunk rows.
+void size_splits(BatchState *b, const uint32_t *col, size_t n, uint64_t ned) {
+  uint32_t nsplit = 0;
+  for (size_t v = 0; v < n; ++v) nsplit += (col[v + 1] - col[v]) > kChunkMin;
+  b->sp_cap = nsplit + 1;
+  b->part_rows = ned / kChunkMin + nsplit + 1;
+}
+
+// First capacity of the level list: a vertex is listed once per distinct
+// level among the roots, so at most width times; small-world graphs need a
+// few times N, high-diameter ones (roots sharing no levels) up to width
+// times N.
+uint64_t list_bound(uint64_t n, int width) { return std::min<uint64_t>(n * (uint64_t)width + 1024, 0xFFFFFFF0ull); }
+uint32_t list_first(uint64_t n, int width) { return (uint32_t)std::min<uint64_t>(n * 8 + 1024, list_bound(n, width)); }
+
+int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
+  int rc;
+  if ((rc = ensure_transpose(c))) return rc;
+  BatchState *b = new BatchState();
+  c->batch = b;
+  b->nw = width / 64;
+  b->B = width;
+  b->n = c->n;
+  const size_t n = c->n, nw = b->nw, B = b->B;
+  size_splits(b, c->h_col.data(), n, c->ned);
+  b->chunk = c->batch_chunk;
+  b->pull16 = c->batch_pull16;
+  b->qcap = qcap;
+  // every buffer, first sized (checked against the context's budget), then
+  // allocated
+  b->plan_only = true;
+  (void)alloc_buffers(b);
+  const size_t planned = b->bytes;
+  b->plan_only = false;
+  b->bytes = 0;
+  if ((rc = budget_check(c, planned, "the batch state")) || (rc = alloc_buffers(b))) {
+    batch_free(c);
+    return rc;
+  }
+
   HIP_TRY(hipHostMalloc((void **)&b->h_ctl, sizeof(BCtl), hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&b->h_cnt, sizeof(unsigned long long) * 4 * B, hipHostMallocDefault));
   HIP_TRY(hipHostMalloc((void **)&b->h_act, sizeof(uint64_t) * (n + 2) * nw, hipHostMallocDefault));
@@ -2513,6 +2564,7 @@ int batch_create(axbc_ctx *c, int width, uint32_t qcap) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->bytes += b->bytes;
+  b->charged = true;
   return AXBC_OK;
 }
 
@@ -3031,6 +3083,21 @@ int axbc_set_batch_persistent(axbc_ctx *c, int mode, double limit_seconds) {
   return AXBC_OK;
 }
 
+int axbc_batch_state_bytes(uint64_t n, uint64_t ned, const uint32_t *col, int width, size_t *bytes) {
+  if (!col || !bytes || (width != 64 && width != 128) || n == 0 || n > 0xFFFFFFF0ull)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  BatchState b;
+  b.nw = width / 64;
+  b.B = width;
+  b.n = (uint32_t)n;
+  b.qcap = list_first(n, width);
+  size_splits(&b, col, n, ned);
+  b.plan_only = true;
+  (void)alloc_buffers(&b);
+  *bytes = b.bytes;
+  return AXBC_OK;
+}
+
 int axbc_set_batch_pull(axbc_ctx *c, uint32_t ratio16) {
   if (!c || ratio16 > (1u << 16)) return fail(AXBC_EINVAL, "pull ratio must be in [0, 65536]%s (line %d)", "", __LINE__);
   c->batch_pull16 = ratio16;
@@ -3047,13 +3114,9 @@ int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint3
     if (v0s[k] >= c->n) return fail(AXBC_EINVAL, "root out of range%s (line %d)", "", __LINE__);
   HIP_TRY(hipSetDevice(c->device));
   int rc;
-  // list capacity: a vertex is listed once per distinct level among the
-  // roots, so at most width times; small-world graphs need a few times N,
-  // high-diameter ones (roots sharing no levels) up to width times N
-  const uint64_t bound = std::min<uint64_t>((uint64_t)c->n * (uint64_t)c->batch_width + 1024, 0xFFFFFFF0ull);
+  const uint64_t bound = list_bound(c->n, c->batch_width);
   if (!c->batch) {
-    if ((rc = batch_create(c, c->batch_width, (uint32_t)std::min<uint64_t>((uint64_t)c->n * 8 + 1024, bound))))
-      return rc;
+    if ((rc = batch_create(c, c->batch_width, list_first(c->n, c->batch_width)))) return rc;
   }
   for (int attempt = 0; attempt < 8; ++attempt) {
     rc = c->batch->nw == 1 ? run_batch<1>(c, v0s, ps, nb, stop_c, applied, stats)
@@ -3078,19 +3141,23 @@ int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight
   // staging and partial rows kept in the batch state, grown as needed
   if (!b->util_part) {
     int rc;
-    if ((rc = balloc(b, &b->util_part, (size_t)kUtilWaves * b->B)) || (rc = balloc(b, &b->util_out, (size_t)b->B)))
+    if ((rc = balloc_more(c, &b->util_part, (size_t)kUtilWaves * b->B, "utility rows")) ||
+        (rc = balloc_more(c, &b->util_out, (size_t)b->B, "utility rows")))
       return rc;
-    c->bytes += sizeof(double) * ((size_t)kUtilWaves + 1) * b->B;
   }
   if (count > b->util_cap) {
+    const size_t old = (size_t)b->util_cap * (sizeof(uint32_t) + sizeof(double));
     if (b->util_v) (void)hipFree(b->util_v);
     if (b->util_w) (void)hipFree(b->util_w);
     b->util_v = nullptr;
     b->util_w = nullptr;
     b->util_cap = 0;
+    b->bytes -= std::min(b->bytes, old);
+    c->bytes -= std::min(c->bytes, old);
     const uint32_t cap = std::max<uint32_t>(count, 4096);
     int rc;
-    if ((rc = balloc(b, &b->util_v, cap)) || (rc = balloc(b, &b->util_w, cap))) return rc;
+    if ((rc = balloc_more(c, &b->util_v, cap, "utility targets")) || (rc = balloc_more(c, &b->util_w, cap, "utility targets")))
+      return rc;
     b->util_cap = cap;
   }
   return batch_utility_on(c, b->util_v, b->util_w, b->util_part, b->util_out, vertex, weight, count, out);
@@ -3104,8 +3171,7 @@ int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
   int rc;
   HIP_TRY(hipSetDevice(c->device));
   if (!b->row_out) {
-    if ((rc = balloc(b, (double **)&b->row_out, c->n))) return rc;
-    c->bytes += sizeof(double) * c->n;
+    if ((rc = balloc_more(c, (double **)&b->row_out, c->n, "a per-root row copy"))) return rc;
   }
   double *tmp = (double *)b->row_out;
   if (b->nw == 1)
@@ -3127,8 +3193,7 @@ int axbc_batch_levels(axbc_ctx *c, int k, int32_t *lvl, uint32_t *sigma) {
   int rc;
   HIP_TRY(hipSetDevice(c->device));
   if (!b->row_out) {
-    if ((rc = balloc(b, (double **)&b->row_out, c->n))) return rc;
-    c->bytes += sizeof(double) * c->n;
+    if ((rc = balloc_more(c, (double **)&b->row_out, c->n, "a per-root row copy"))) return rc;
   }
   int32_t *dl = (int32_t *)b->row_out;
   uint32_t *ds = (uint32_t *)(dl + c->n);

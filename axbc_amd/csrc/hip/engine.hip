@@ -49,6 +49,13 @@ int fail(int code, const char *fmt, const char *a, int line) {
   return code;
 }
 
+int budget_check(axbc_ctx *c, size_t more, const char *what) {
+  if (!c->budget || c->bytes + more <= c->budget) return AXBC_OK;
+  snprintf(g_err, sizeof(g_err), "device memory budget: %s needs %zu bytes more, %zu of %zu in use", what, more,
+           c->bytes, c->budget);
+  return AXBC_EBUDGET;
+}
+
 static hipEvent_t kt_event(axbc_ctx *c) {
   KernelTimer &t = c->ktimer;
   if (!t.pool.empty()) {
@@ -881,6 +888,18 @@ int axbc_copy_kernel(axbc_ctx *c, uint64_t bytes, int repeats) {
 int axbc_sync(axbc_ctx *c) {
   if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return AXBC_OK;
+}
+
+int axbc_set_memory_budget(axbc_ctx *c, size_t bytes) {
+  if (!c) return fail(AXBC_EINVAL, "null context%s (line %d)", "", __LINE__);
+  c->budget = bytes;
+  return AXBC_OK;
+}
+
+int axbc_device_bytes(axbc_ctx *c, size_t *bytes) {
+  if (!c || !bytes) return fail(AXBC_EINVAL, "null argument%s (line %d)", "", __LINE__);
+  *bytes = c->bytes;
   return AXBC_OK;
 }
 

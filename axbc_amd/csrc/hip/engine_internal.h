@@ -89,6 +89,9 @@ struct KernelTimer {
 
 thread_local extern char g_err[512];
 int fail(int code, const char *fmt, const char *a = "", int line = 0);
+// AXBC_EBUDGET (with a message) when `more` device bytes would take the
+// context past its budget (none when the budget is 0).
+int budget_check(axbc_ctx *c, size_t more, const char *what);
 
 #define HIP_TRY(call)                                                                            \
   do {                                                                                           \
@@ -147,7 +150,7 @@ struct axbc_ctx {
   int batch_persist = 0;             // persistent level loop: 0 never (default), 1 deep graphs, 2 whenever possible
   unsigned long long batch_persist_limit = 1000000000ull;  // barrier wait limit, 100 MHz ticks (10 s)
   uint32_t max_out = 0, max_in = 0;  // largest out- and in-degree (set with the transpose)
-  size_t budget = 0;  // 0: no limit
+  size_t budget = 0;  // device bytes allowed to the context (0: no limit); checked before each allocation after create
   axbc::KernelTimer ktimer;
 };
 

@@ -368,9 +368,13 @@ int axbc_mlp_init(axbc_ctx *c, const float *params) {
   HIP_TRY(hipSetDevice(c->device));
   MlpBufs *b = bufs_of(c);
   if (!b) {
+    const size_t n = c->n;
+    {
+      int rc;
+      if ((rc = budget_check(c, 32 * n, "the MLP buffers"))) return rc;
+    }
     b = new MlpBufs();
     c->mlp = b;
-    const size_t n = c->n;
     HIP_TRY(hipMalloc((void **)&b->x, sizeof(float) * 4 * n));
     HIP_TRY(hipMalloc((void **)&b->y, sizeof(float) * n));
     HIP_TRY(hipMalloc((void **)&b->score, sizeof(float) * n));
@@ -423,6 +427,10 @@ int axbc_mlp_targets(axbc_ctx *c, const uint32_t *vertex, const double *weight, 
   if (!count) return AXBC_OK;
   hipStream_t s = c->stream;
   if (!b->tw) {
+    {
+      int rc;
+      if ((rc = budget_check(c, 8ull * c->n, "the MLP target weights"))) return rc;
+    }
     HIP_TRY(hipMalloc((void **)&b->tw, sizeof(double) * c->n));
     HIP_TRY(hipMalloc((void **)&b->tred, sizeof(double) * (2 * kRedBlocks + 1)));
     c->bytes += 8ull * c->n;

@@ -462,10 +462,14 @@ int sampler_bufs(axbc_ctx *c, SamplerBufs **out) {
     return AXBC_OK;
   }
   HIP_TRY(hipSetDevice(c->device));
-  SamplerBufs *sb = new SamplerBufs();
-  c->sampler = sb;
   const size_t n = c->n;
   const size_t ntiles = (n + kScanTile - 1) / kScanTile + 1;
+  {
+    int rc;
+    if ((rc = budget_check(c, 28 * n + 8 * ntiles, "the sampler tables"))) return rc;
+  }
+  SamplerBufs *sb = new SamplerBufs();
+  c->sampler = sb;
   HIP_TRY(hipMalloc(&sb->tmp, 8 * n));
   HIP_TRY(hipMalloc(&sb->u32a, 4 * n));
   HIP_TRY(hipMalloc(&sb->u32b, 4 * n));
@@ -579,6 +583,7 @@ int axbc_lcc_build(axbc_ctx *c, float *lcc_host, uint64_t *lw_host, uint64_t *lp
   const uint32_t n = c->n;
   hipStream_t s = c->stream;
   if (!sb->lcc) {
+    if ((rc = budget_check(c, 24ull * n, "the LCC tables"))) return rc;
     HIP_TRY(hipMalloc((void **)&sb->lcc, sizeof(float) * n));
     HIP_TRY(hipMalloc((void **)&sb->lcc_cnt, sizeof(uint32_t) * n));
     HIP_TRY(hipMalloc((void **)&sb->lw, sizeof(uint64_t) * n));

@@ -140,6 +140,77 @@ static uint32_t first_geq(const uint32_t *pmax, uint32_t n, uint32_t r) {
   return lo < n ? lo : n - 1;
 }
 
+/* -d: the generated edge list as the reference dumps it (dump_rmat,
+ * src/axbc2d.c:550-576, called at 2556-2559): the generator's edges
+ * mirrored, normalised (duplicates and self-loops dropped) when -H is not 0,
+ * before the 1-degree reduction, sorted by (u, v), one "u<TAB>v" line each,
+ * into rmat_S<scale>_EF<edgef>.txt in the working directory.  The list is
+ * generated again on the host so the dump does not depend on -g. */
+static void dump_rmat(int scale, int edgef, uint64_t s1, uint64_t s2, int undirected, int heuristic, int nthreads) {
+  const double ini[4] = {.57, .19, .19, .05};
+  const uint64_t m = ((uint64_t)1 << scale) * (uint64_t)edgef;
+  uint64_t *pairs = axbc_kronecker_edges(scale, (int64_t)m, s1, s2, ini, nthreads);
+  if (!pairs) die("%s", "edge generation failed");
+  uint64_t *ed = axbc_pack_pairs(pairs, m);
+  free(pairs);
+  uint64_t ned = m;
+  ed = axbc_mirror(ed, &ned, undirected);
+  printf("Dump file...\n");
+  if (heuristic != 0)
+    ned = axbc_norm_graph(ed, ned, nthreads);
+  else
+    axbc_sort_edges(ed, ned, nthreads);
+  char fname[256];
+  snprintf(fname, sizeof(fname), "rmat_S%d_EF%d.txt", scale, edgef);
+  printf("DUMP THE GENERATED RMAT FILE IN %s\n", fname);
+  FILE *f = fopen(fname, "w+");
+  if (!f) die("error opening %s", fname);
+  for (uint64_t i = 0; i < ned; ++i)
+    fprintf(f, "%" PRIu64 "\t%" PRIu64 "\n", (uint64_t)AXBC_SRC(ed[i]), (uint64_t)AXBC_DST(ed[i]));
+  fclose(f);
+  free(ed);
+}
+
+/* -a: the reference's degree statistics (analyze_deg, src/axbc2d.c:1061-1115,
+ * and dump_array, 1040-1057): the CSC column degrees after preprocessing
+ * counted into 16 bins with lower limits 0..10, 100, 1000, 10^4, 10^5, 10^7
+ * (bin b holds lim[b] <= d < lim[b+1]), appended as one line " c0, c1, ...,"
+ * to degree_stats_<id>_0, where <id> is the run's time stamp. */
+static void analyze_deg(const axbc_graph *g, unsigned id) {
+  static const uint32_t lim[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 100, 1000, 10000, 100000, 10000000};
+  uint32_t bins[16] = {0};
+  for (uint64_t v = 0; v < g->N; ++v) {
+    const uint32_t d = g->col[v + 1] - g->col[v];
+    int b = 15;
+    while (d < lim[b]) --b;
+    ++bins[b];
+  }
+  char fname[256];
+  snprintf(fname, sizeof(fname), "degree_stats_%u_0", id);
+  FILE *f = fopen(fname, "a");
+  if (!f) die("error opening %s", fname);
+  for (int b = 0; b < 16; ++b) fprintf(f, " %d,", (int)bins[b]);
+  fprintf(f, "\n");
+  fclose(f);
+}
+
+/* Numeric option values rejected as the reference's sscanf checks reject
+ * them (src/axbc2d.c:2345-2404): a value that does not start with a number
+ * ends the run with the reference's message. */
+static uint64_t num_u64(const char *arg, const char *msg) {
+  char *end = NULL;
+  while (*arg == ' ' || *arg == '\t') ++arg;
+  if (*arg == '-' || *arg == '+') {
+    if (arg[1] < '0' || arg[1] > '9') die(msg, arg);
+  } else if (*arg < '0' || *arg > '9') {
+    die(msg, arg);
+  }
+  const uint64_t v = strtoull(arg, &end, 10);
+  return v;
+}
+
+static int num_int(const char *arg, const char *msg) { return (int)num_u64(arg, msg); }
+
 typedef struct {
   uint32_t v0;
   double p;
@@ -167,7 +238,7 @@ int main(int argc, char **argv) {
       case 'x': approx_n = (uint32_t)strtoul(optarg, NULL, 10); break;
       case 'z': distribution = atoi(optarg); break;
       case 'H':
-        heuristic = atoi(optarg);
+        heuristic = num_int(optarg, "Invalid Heuristic Option (-H): %s");
         if (heuristic >= 2) die("H2 and H3 are not implemented in Approx BC(-H): %s", optarg);
         break;
       case 'o': ofile = strdup(optarg); break;
@@ -189,24 +260,25 @@ int main(int argc, char **argv) {
       case 'n':
         if (gread == 0) die("Unexpected option -%s!", "n");
         gread = 1;
-        N = strtoull(optarg, NULL, 10);
+        N = num_u64(optarg, "Invalid number of vertices (-n): %s");
         break;
       case 'r':
-        startv = strtoull(optarg, NULL, 10);
+        startv = num_u64(optarg, "Invalid root vertex (-r): %s");
         rootset = 1;
         break;
       case 'S':
         if (gread == 1) die("Unexpected option -%s!", "S");
         gread = 0;
-        scale = atoi(optarg);
+        scale = num_int(optarg, "Invalid scale (-S): %s");
+        if (scale < 0 || scale > 32) die("Invalid scale (-S): %s", optarg);
         N = 1ull << scale;
         break;
       case 'E':
         if (gread == 1) die("Unexpected option -%s!", "E");
         gread = 0;
-        edgef = atoi(optarg);
+        edgef = num_int(optarg, "Invalid edge factor (-S): %s");
         break;
-      case 'N': nbfs = strtoull(optarg, NULL, 10); break;
+      case 'N': nbfs = num_u64(optarg, "Invalid number of bfs (-N): %s"); break;
       case 'U': undirected = 0; break;
       case 'a': analyze = 1; break;
       case 'B': width = atoi(optarg); break;
@@ -283,6 +355,8 @@ int main(int argc, char **argv) {
   printf("Edges %s: %" PRIu64 "\n", gread ? "read from file (after mirror)" : "generated", generated);
   printf("Edges in the CSC: %" PRIu64 " (built in %.2f s)\n", g.ned, now_s() - t0);
   if (heuristic == 1) printf("task 0 Total edges removed %u\n", g.rem_ed);
+  if (dump && !gread) dump_rmat(scale, edgef, s1, s2, undirected, heuristic, nthreads);
+  if (analyze) analyze_deg(&g, (unsigned)time(NULL));
   if (gout) {
     FILE *f = fopen(gout, "w");
     if (!f) die("cannot open %s", gout);
@@ -290,29 +364,6 @@ int main(int argc, char **argv) {
       for (uint32_t k = g.col[v]; k < g.col[v + 1]; ++k) fprintf(f, "%u\t%" PRIu64 "\n", g.row[k], v);
     fclose(f);
     return 0;
-  }
-  if (dump && !gread) {
-    char fname[256];
-    snprintf(fname, sizeof(fname), "rmat_S%d_EF%d.txt", scale, edgef);
-    printf("DUMP THE GENERATED RMAT FILE IN %s\n", fname);
-    FILE *f = fopen(fname, "w");
-    if (f) {
-      for (uint64_t v = 0; v < N; ++v)
-        for (uint32_t k = g.col[v]; k < g.col[v + 1]; ++k) fprintf(f, "%u\t%" PRIu64 "\n", g.row[k], v);
-      fclose(f);
-    }
-  }
-  if (analyze) {
-    uint64_t hist[33] = {0};
-    for (uint64_t v = 0; v < N; ++v) {
-      uint32_t d = g.col[v + 1] - g.col[v];
-      int b = 0;
-      while (d >> b) ++b;
-      hist[b]++;
-    }
-    printf("Degree histogram (bin b holds degrees in [2^(b-1), 2^b)):\n");
-    for (int b = 0; b < 33; ++b)
-      if (hist[b]) printf("  %2d %" PRIu64 "\n", b, hist[b]);
   }
 
   /* ---- device --------------------------------------------------------- */

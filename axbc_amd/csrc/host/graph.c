@@ -203,8 +203,11 @@ int axbc_build_csc(const uint64_t *ed, uint64_t ned, uint64_t N, uint32_t **colp
 
 uint64_t *axbc_read_edge_file(const char *path, uint64_t N, int undirected, uint64_t *ned,
                               char *errbuf, size_t errlen) {
-  /* src/axbc2d.c:356-445 at one rank: lines containing '#' are comments,
-   * other lines hold "i j"; ids must be < N. */
+  /* src/axbc2d.c:356-445 at one rank: lines containing '#' are comments
+   * (a "# Nodes: n Edges: m" header is echoed as "N=n E=m"), other lines
+   * hold "i j"; ids must be < N.  Lines that hold no pair are skipped: the
+   * reference appends the previous line's pair again for them, a duplicate
+   * that its normalisation removes before the CSC is built. */
   FILE *f = fopen(path, "r");
   if (!f) {
     snprintf(errbuf, errlen, "Cannot open file %s...", path);
@@ -214,7 +217,14 @@ uint64_t *axbc_read_edge_file(const char *path, uint64_t N, int undirected, uint
   uint64_t *e = (uint64_t *)xalloc(sizeof(uint64_t) * cap);
   char buf[1024];
   while (fgets(buf, sizeof(buf), f)) {
-    if (strchr(buf, '#')) continue;
+    if (strchr(buf, '#')) {
+      if (strstr(buf, "Nodes:")) {
+        unsigned long long hn = 0, he = ~0ull;
+        (void)sscanf(buf, "# Nodes: %llu Edges: %llu", &hn, &he);
+        printf("N=%llu E=%llu\n", hn, he);
+      }
+      continue;
+    }
     unsigned long long i, j;
     if (sscanf(buf, "%llu %llu", &i, &j) != 2) continue;
     ++line;

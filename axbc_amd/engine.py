@@ -18,10 +18,19 @@ class EngineError(RuntimeError):
     pass
 
 
+class BudgetError(EngineError):
+    """An allocation would take the context past its device memory budget
+    (AXBC_EBUDGET); the context stays usable."""
+
+
+AXBC_EBUDGET = -5
+
+
 def _check(rc: int) -> None:
     if rc != 0:
         msg = _ffi.hip().axbc_last_error()
-        raise EngineError(f"axbc error {rc}: {msg.decode() if msg else ''}")
+        text = f"axbc error {rc}: {msg.decode() if msg else ''}"
+        raise BudgetError(text) if rc == AXBC_EBUDGET else EngineError(text)
 
 
 def device_count() -> int:
@@ -47,6 +56,16 @@ class Engine:
                                  C.byref(ctx), C.byref(nbytes)))
         self.ctx = ctx
         self.device_bytes = int(nbytes.value)
+
+    def set_budget(self, nbytes: int) -> None:
+        """Device bytes the context may hold; allocations past it raise
+        BudgetError (0: no limit)."""
+        _check(_ffi.hip().axbc_set_memory_budget(self.ctx, int(nbytes)))
+
+    def bytes_in_use(self) -> int:
+        n = C.c_size_t(0)
+        _check(_ffi.hip().axbc_device_bytes(self.ctx, C.byref(n)))
+        return int(n.value)
 
     def close(self) -> None:
         if getattr(self, "ctx", None):

@@ -14,6 +14,7 @@ No reference counterpart: AxBC runs one job per process.
 """
 from __future__ import annotations
 
+import ctypes as C
 import queue
 import threading
 import time
@@ -22,7 +23,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .engine import Engine
+from . import _ffi
+from .engine import BudgetError, Engine
 from .graph import Graph
 from .pivots import PivotLoop
 
@@ -30,16 +32,26 @@ PLAIN_STATE_BYTES = 4 + 4 + 8 + 8 + 4 + 4 + 4  # lvl, sigma, delta, bc, queue, q
 SAMPLER_BYTES = 28 + 24 + 8                    # sampler scratch, LCC tables, slack per vertex
 
 
-def footprint_bytes(N: int, ned: int, batch: int = 0) -> int:
-    """Device bytes a job needs: CSC + per-vertex state + sampler tables
-    (+ when batched: the root rows, the bit words, the level list at its
-    initial 8N entries, and one partial row per 64-edge chunk)."""
+def footprint_bytes(N: int, ned: int, batch: int = 0, col: np.ndarray | None = None,
+                    symmetric: bool = True) -> int:
+    """Device bytes a job needs: CSC + per-vertex state + sampler tables,
+    and when batched the batch state the library creates for this graph
+    (axbc_batch_state_bytes: root rows, bit words, the level list at its
+    first capacity, split-vertex and chunk rows, per-wave slots) plus the
+    transpose of a graph that is not symmetric.  Without col the split
+    vertices are estimated from a uniform degree."""
     b = 4 * (N + 1) + 4 * ned + N * (PLAIN_STATE_BYTES + SAMPLER_BYTES)
     if batch:
         width = 128 if batch > 64 else 64
-        words = width // 64
-        b += N * (width * (4 + 8 + 8) + 8 * 4 * words + 8) + 8 * N * (4 + 8 * words)
-        b += (ned // 64 + 1) * width * 8
+        if col is None:
+            col = np.minimum(np.arange(N + 1, dtype=np.uint64) * max(ned // max(N, 1), 1), ned).astype(np.uint32)
+        size = C.c_size_t(0)
+        rc = _ffi.hip().axbc_batch_state_bytes(N, ned, np.ascontiguousarray(col, np.uint32), width, C.byref(size))
+        if rc != 0:
+            raise ValueError(_ffi.hip().axbc_last_error().decode())
+        b += int(size.value)
+        if not symmetric:
+            b += 4 * (N + 1 + ned)
     return b
 
 
@@ -71,9 +83,16 @@ class Job:
 
 
 class Server:
+    """slots device workers under a device memory cap of hbm_cap bytes.
+    engine_factory(graph, device) makes a job's engine (an Engine) and
+    runner(engine, job) runs its pivot loop; tests pass stand-ins to exercise
+    admission without a GPU."""
+
     def __init__(self, slots: int = 4, hbm_cap: float = 200e9, device: int = 0, build_threads: int = 4,
-                 keep_bc: bool = True):
+                 keep_bc: bool = True, engine_factory=Engine, runner=None):
         self.slots, self.cap, self.device, self.keep_bc = slots, hbm_cap, device, keep_bc
+        self._factory = engine_factory
+        self._runner = runner or self._pivot_loop
         self._ready: list[Job] = []      # built, waiting for admission (arrival order)
         self._lock = threading.Condition()
         self._used = 0
@@ -129,7 +148,8 @@ class Server:
         for i, job in enumerate(self._ready):
             if job.error:
                 return self._ready.pop(i)
-            need = footprint_bytes(job.graph.N, job.graph.ned, job.batch)
+            need = getattr(job, "_retry_need", 0) or footprint_bytes(
+                job.graph.N, job.graph.ned, job.batch, getattr(job.graph, "col", None))
             if need > self.cap:
                 job.error = f"footprint {need} B exceeds the memory cap {int(self.cap)} B"
                 return self._ready.pop(i)
@@ -150,30 +170,55 @@ class Server:
                     if job is None:
                         self._lock.wait(0.05)
                 self._active += 1
-            job.admitted = time.perf_counter()
+            job.admitted = job.admitted or time.perf_counter()
+            retry = False
             try:
                 if not job.error:
                     self._run(job)
+            except BudgetError as ex:
+                # the reservation was too small: admit once more with twice it
+                if getattr(job, "_retry_need", 0):
+                    job.error = f"BudgetError: {ex}"
+                else:
+                    job._retry_need = 2 * job._need
+                    retry = True
             except Exception as ex:
                 job.error = f"{type(ex).__name__}: {ex}"
             job.finished = time.perf_counter()
             with self._lock:
                 self._used -= getattr(job, "_need", 0)
+                job._need = 0
                 self._active -= 1
-                self._pending -= 1
-                self._done.append(job)
+                if retry:
+                    self._ready.insert(0, job)
+                else:
+                    self._pending -= 1
+                    self._done.append(job)
                 self._lock.notify_all()
 
+    def _charge(self, job: Job, actual: int) -> None:
+        """The context exists: charge its actual device bytes where they
+        exceed the reservation."""
+        with self._lock:
+            if actual > job._need:
+                self._used += actual - job._need
+                job._need = actual
+
     def _run(self, job: Job) -> None:
-        with Engine(job.graph, self.device) as e:
-            loop = PivotLoop(e, job.budget, c=job.c, x=job.x, z=job.z, batch=job.batch)
-            res = loop.run()
-            job.nrounds = res.nrounds
-            job.pivots = res.pivots
-            if self.keep_bc:
-                job.bc = e.bc()
+        with self._factory(job.graph, self.device) as e:
+            self._charge(job, e.device_bytes)
+            e.set_budget(job._need)
+            self._runner(e, job)
         if not self.keep_bc:
             job.graph = None
+
+    def _pivot_loop(self, e: Engine, job: Job) -> None:
+        loop = PivotLoop(e, job.budget, c=job.c, x=job.x, z=job.z, batch=job.batch)
+        res = loop.run()
+        job.nrounds = res.nrounds
+        job.pivots = res.pivots
+        if self.keep_bc:
+            job.bc = e.bc()
 
 
 def poisson_jobs(n: int, rate: float, seed: int = 0, scales=(12, 20), efs=(8, 16, 32), budget=(64, 2048),

@@ -75,6 +75,20 @@ int axbc_set_reach(axbc_ctx *ctx, const uint32_t *reach);
 /* Pinned host memory (CudaMallocHostSet / CudaFreeHost, cu:825-846). */
 void *axbc_host_alloc(size_t bytes);
 void axbc_host_free(void *p);
+/* Device memory of a context: the bytes it holds now, and a budget for
+ * what it allocates after creation (batch state and its list growth,
+ * sampler and LCC tables, MLP buffers, the transposed graph): an
+ * allocation that would take it past the budget fails with AXBC_EBUDGET
+ * and leaves the context usable; 0 removes the limit.  No reference
+ * counterpart (the serving loop admits jobs by device footprint). */
+int axbc_device_bytes(axbc_ctx *ctx, size_t *bytes);
+int axbc_set_memory_budget(axbc_ctx *ctx, size_t bytes);
+/* Device bytes of the batch state that the first batch of a context on
+ * this graph (n vertices, ned CSC entries, column offsets col[n+1]) creates
+ * at width 64 or 128, with the level list at its first capacity; a graph
+ * that is not symmetric adds its transpose, 4 (n + 1 + ned) bytes.  For
+ * admission before a context exists.  No reference counterpart. */
+int axbc_batch_state_bytes(uint64_t n, uint64_t ned, const uint32_t *col, int width, size_t *bytes);
 /* Release everything (fincuda(), cu:1659-1693). */
 int axbc_ctx_destroy(axbc_ctx *ctx);
 

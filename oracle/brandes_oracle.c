@@ -171,10 +171,22 @@ uint64_t oracle_pivots_omp(uint32_t N, const uint32_t *col, const uint32_t *row,
  * counter, i.e. as an unsigned 32-bit value. */
 uint32_t oracle_lim_x_sample(uint64_t N) { return (uint32_t)(N * N); }
 
+/* srand48 / lrand48 (POSIX: X' = (0x5DEECE66D X + 0xB) mod 2^48, seeded
+ * X = seed << 16 | 0x330E, lrand48 = X >> 17) on a per-thread state, so
+ * that loops run on several threads at once each draw their own sequence;
+ * nrand48 on a caller's state is the same generator. */
+static __thread unsigned short rng_x[3];
+void oracle_srand48(long seed) {
+  rng_x[0] = 0x330E;
+  rng_x[1] = (unsigned short)(seed & 0xFFFF);
+  rng_x[2] = (unsigned short)((seed >> 16) & 0xFFFF);
+}
+long oracle_lrand48(void) { return nrand48(rng_x); }
+
 static uint32_t uniform_int(uint32_t max, uint32_t min) {
   /* lrand48() % (max - min + 1) + min in uint32 (axbc2d.c:2156-2164) */
   uint32_t m = max - min + 1u;
-  long r = lrand48();
+  long r = oracle_lrand48();
   return m ? (uint32_t)((uint32_t)r % m) + min : (uint32_t)r + min;
 }
 
@@ -314,7 +326,7 @@ static int run_loop(uint32_t N, const uint32_t *col, const uint32_t *row, const 
   memset(bc, 0, sizeof(double) * N);
   uint64_t nbfs = cfg->nbfs < N ? cfg->nbfs : N;
   /* init_rand (axbc2d.c:2143-2153) */
-  srand48(cfg->seed);
+  oracle_srand48(cfg->seed);
   uint32_t *approx = (uint32_t *)xcalloc(cfg->approx_n, sizeof(uint32_t));
   for (uint32_t i = 0; i < cfg->approx_n; ++i) approx[i] = select_root1(N);
   hdn_tables hdn = {0};
@@ -374,7 +386,7 @@ static int run_loop(uint32_t N, const uint32_t *col, const uint32_t *row, const 
         uint32_t r = uniform_int(hdn.sum, 0);
         v0 = hdn_pick(N, col, row, &hdn, r, &p);
       } else if (k == 2) {
-        v0 = oracle_weighted_sample(N, Wl, Tl, lrand48(), &p);
+        v0 = oracle_weighted_sample(N, Wl, Tl, oracle_lrand48(), &p);
       } else if (cfg->sync_batch && (k == 3 || k == 4 || k == 5)) {
         /* batch-synchronous: the distribution of the group's first round */
         if (frozen_at != rs->nrounds / cfg->sync_batch || !frozen_ok) {
@@ -385,12 +397,12 @@ static int run_loop(uint32_t N, const uint32_t *col, const uint32_t *row, const 
           frozen_total = oracle_fixed_weights(N, wt, W);
           frozen_ok = 1;
         }
-        v0 = oracle_weighted_sample(N, W, frozen_total, lrand48(), &p);
+        v0 = oracle_weighted_sample(N, W, frozen_total, oracle_lrand48(), &p);
       } else {
         for (uint32_t v = 0; v < N; ++v)
           wt[v] = k == 3 ? bc[v] : k == 4 ? delta[v] : 1.0 / (1.0 + bc[v]);
         uint64_t total = oracle_fixed_weights(N, wt, W);
-        v0 = oracle_weighted_sample(N, W, total, lrand48(), &p);
+        v0 = oracle_weighted_sample(N, W, total, oracle_lrand48(), &p);
       }
       if (z == 6) { /* mixture probability and next component (3030-3051) */
         const int in = v0 < N;

@@ -88,6 +88,10 @@ uint32_t oracle_hdn_sample(uint32_t N, const uint32_t *col, const uint32_t *row,
 uint64_t oracle_fixed_weights(uint32_t N, const double *w, uint64_t *W);
 uint32_t oracle_weighted_sample(uint32_t N, const uint64_t *W, uint64_t total, long draw,
                                 double *p_out);
+/* srand48 / lrand48 on the calling thread's own generator state (the
+ * loops seed it with cfg->seed). */
+void oracle_srand48(long seed);
+long oracle_lrand48(void);
 
 #ifdef __cplusplus
 }

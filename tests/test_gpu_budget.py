@@ -1,0 +1,107 @@
+"""The context's device-memory budget (axbc_set_memory_budget): an
+allocation that would pass it fails with AXBC_EBUDGET before anything is
+allocated, bc stays untouched, the bytes in use stay what they were, and
+the context runs normally once the budget allows it."""
+import numpy as np
+import pytest
+
+from axbc_amd import Engine, Graph
+from axbc_amd.engine import BudgetError
+from oracle import oracle
+from tests.parity import close
+
+pytestmark = pytest.mark.gpu
+
+
+def test_require(require_gpu):
+    pass
+
+
+def path_graph(n):
+    e = np.stack([np.arange(n - 1), np.arange(1, n)], axis=1)
+    return Graph.from_edges(n, e, undirected=True, heuristic=0, name=f"path{n}")
+
+
+def bc_of_roots(g, roots):
+    bc = np.zeros(g.N)
+    for v in roots:
+        bc = oracle.pivot(g, int(v), 1.0 / g.N, bc)[3]
+    return bc
+
+
+def test_batch_state_over_budget(require_gpu):
+    g = Graph.kronecker(12, 16, seed=(1, 2), heuristic=1)
+    roots = np.arange(64, dtype=np.uint32) * 7 % g.N
+    probs = np.full(64, 1.0 / g.N)
+    with Engine(g) as e:
+        base = e.bytes_in_use()
+        assert base == e.device_bytes > 0
+        e.set_budget(base + 4096)
+        with pytest.raises(BudgetError, match="the batch state"):
+            e.batch(roots, probs)
+        assert e.bytes_in_use() == base
+        assert not e.bc().any()
+        e.set_budget(0)
+        applied, _ = e.batch(roots, probs)
+        assert applied == 64
+        assert e.bytes_in_use() > base
+        ok, err = close(e.bc(), bc_of_roots(g, roots))
+        assert ok, err
+
+
+def test_level_list_growth_over_budget(require_gpu):
+    """64 roots spread along a path list most vertices at 64 distinct
+    levels: the level list outgrows its first 8N entries.  With the budget
+    at the bytes of the first batch state, the growth fails cleanly."""
+    g = path_graph(4000)
+    roots = (np.arange(64, dtype=np.uint32) * 61 + 5) % g.N
+    probs = np.full(64, 1.0 / g.N)
+    with Engine(g) as e:
+        e.batch(roots[:1], probs[:1])  # creates the batch state at its first capacity
+        first = e.bytes_in_use()
+        e.reset_bc()
+        e.set_budget(first)
+        with pytest.raises(BudgetError, match="the batch state"):
+            e.batch(roots, probs)
+        assert not e.bc().any()
+        assert e.bytes_in_use() <= first
+        e.set_budget(0)
+        applied, _ = e.batch(roots, probs)
+        assert applied == 64
+        assert e.bytes_in_use() > first
+        ok, err = close(e.bc(), bc_of_roots(g, roots))
+        assert ok, err
+
+
+def test_budget_below_use_refuses_only_new_allocations(require_gpu):
+    g = Graph.kronecker(11, 16, seed=(3, 4), heuristic=1)
+    roots = np.arange(32, dtype=np.uint32) * 5 % g.N
+    probs = np.full(32, 1.0 / g.N)
+    with Engine(g) as e:
+        e.batch(roots, probs)
+        held = e.bytes_in_use()
+        e.set_budget(1)  # below what is held: nothing is freed, nothing more allocated
+        applied, _ = e.batch(roots, probs)  # state exists: no allocation
+        assert applied == 32
+        with pytest.raises(BudgetError):
+            e.batch_delta(0)  # the per-root row copy would be a new allocation
+        assert e.bytes_in_use() == held
+        e.set_budget(0)
+        e.batch_delta(0)
+
+
+@pytest.mark.parametrize("width", [64, 128])
+def test_batch_state_bytes_are_exact(require_gpu, width):
+    """The admission estimate of the batch state equals what the first
+    batch allocates."""
+    import ctypes as C
+
+    from axbc_amd import _ffi
+    g = Graph.kronecker(12, 16, seed=(5, 6), heuristic=1)
+    size = C.c_size_t(0)
+    assert _ffi.hip().axbc_batch_state_bytes(g.N, g.ned, g.col, width, C.byref(size)) == 0
+    with Engine(g) as e:
+        e.set_batch_width(width)
+        base = e.bytes_in_use()
+        e.batch(np.arange(4, dtype=np.uint32), np.full(4, 1.0 / g.N))
+        assert e.bytes_in_use() - base == size.value

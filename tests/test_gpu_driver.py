@@ -70,3 +70,49 @@ def test_plain_config_scale16(require_gpu, tmp_path, directed):
     res = np.loadtxt(str(out) + "_1X1_0.log")
     exp = bc / 2.0 / max(rounds, 1)
     assert np.allclose(res[:, 1], exp, rtol=1e-9, atol=0.0051)
+
+
+@pytest.mark.parametrize("flags", [
+    ["-z", "1", "-H", "1", "-c", "1", "-x", "8"],
+    ["-z", "6", "-H", "0", "-c", "0"],
+    ["-z", "0", "-H", "1", "-c", "0", "-B", "64", "-g"],
+])
+def test_driver_on_a_file_matches_oracle(require_gpu, tmp_path, flags):
+    """-f: a text edge list with '#' comment lines and a "# Nodes:" header,
+    pairs tab-separated, read, mirrored and preprocessed (on the host, or
+    on the device with -g); pivots, round and skip counts and the result
+    file match the oracle's loop on the same edges."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    # a sparse random graph with hubs, a few isolated vertices and pendants
+    m = 12000
+    u = rng.integers(0, n - 50, m)
+    v = (u + rng.zipf(1.6, m)) % (n - 50)
+    hubs = rng.integers(0, 20, 400)
+    leaves = rng.integers(0, n - 50, 400)
+    e = np.concatenate([np.stack([u, v], 1), np.stack([hubs, leaves], 1),
+                        np.array([[n - 40 + k, n - 20 + k] for k in range(10)])])
+    src = tmp_path / "graph.txt"
+    with open(src, "w") as f:
+        f.write("# Undirected test graph\n# Nodes: %d Edges: %d\n# FromNodeId\tToNodeId\n" % (n, len(e)))
+        for a, b in e:
+            f.write(f"{a}\t{b}\n")
+    out = tmp_path / "res"
+    nbfs = 200
+    cmd = [DRIVER, "-p", "1x1", "-f", str(src), "-n", str(n), "-N", str(nbfs), "-o", str(out)] + flags
+    print("[driver -f]", " ".join(cmd[1:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert f"N={n} E={len(e)}" in r.stdout
+    piv = [int(x.group(1)) for x in re.finditer(r"^Perform \d+/\d+ BC from (\d+) ", r.stdout, re.M)]
+    rounds = int(re.search(r"task 0 BC rounds: (\d+)", r.stdout).group(1))
+    skipped = int(re.search(r"task 0 BC skipped: (\d+)", r.stdout).group(1))
+    get = lambda fl: flags[flags.index(fl) + 1] if fl in flags else None  # noqa: E731
+    g = Graph.from_edges(n, e, undirected=True, heuristic=int(get("-H")))
+    pv, pr, bc, rs = oracle.run(g, nbfs, c=int(get("-c")), x=int(get("-x") or 1), z=int(get("-z")))
+    assert rounds == rs["nrounds"] and skipped == rs["skipped"]
+    assert piv[:rounds] == [int(x) for x in pv]
+    res = np.loadtxt(str(out) + "_1X1_0.log")
+    assert res.shape == (n, 2)
+    exp = bc / 2.0 / max(rounds, 1)
+    assert np.allclose(res[:, 1], exp, rtol=1e-9, atol=0.0051)

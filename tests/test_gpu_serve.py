@@ -116,3 +116,42 @@ def test_concurrent_graph_builds(require_gpu):
     work(3, alone)
     ok, err = close(out[3][0], alone[3][0])
     assert ok, err
+
+
+def test_serving_workload_vs_oracle(require_gpu):
+    """The serving workload (Poisson arrivals; Kronecker SCALE 12-20, edge
+    factor 8/16/32, budgets 64-2048, the -z 6 mixture in batches of 64
+    roots) through four slots; every job with SCALE <= 16 matches the
+    oracle's batch-synchronous pivot loop (sync_batch 64): same pivots,
+    round count and bc within 1e-9."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from axbc_amd.serve import poisson_jobs
+    from oracle import oracle
+
+    js = [j for _, j in poisson_jobs(10, 50.0, seed=0)]
+    assert {j.z for j in js} == {6} and {j.batch for j in js} == {64}
+    assert min(j.scale for j in js) == 12 and max(j.scale for j in js) == 20
+    srv = Server(slots=4, keep_bc=True)
+    for j in js:
+        srv.submit(j)
+    done = {d.jid: d for d in srv.drain(timeout=600)}
+    srv.close()
+    assert len(done) == len(js)
+    for d in done.values():
+        assert d.error is None, (d.jid, d.error)
+        assert d.nrounds > 0
+    small = [d for d in done.values() if d.scale <= 16]
+    assert len(small) >= 6
+    print(f"[serve] {len(small)} jobs vs the oracle", flush=True)
+
+    def ref(d):
+        return oracle.run(d.graph, d.budget, c=d.c, x=d.x, z=d.z, sync_batch=d.batch)
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        refs = list(ex.map(ref, small))
+    for d, (pv, pr, bc, rs) in zip(small, refs):
+        assert d.nrounds == rs["nrounds"], (d.jid, d.nrounds, rs)
+        assert list(d.pivots) == [int(v) for v in pv], d.jid
+        ok, err = close(d.bc, bc)
+        assert ok, (d.jid, d.scale, err)

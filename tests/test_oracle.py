@@ -232,6 +232,20 @@ def test_lrand48_posix_recurrence():
         assert libc.lrand48() == X >> 17
 
 
+def test_oracle_generator_is_lrand48():
+    # the oracle's per-thread generator draws libc's srand48/lrand48 sequence
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.lrand48.restype = ctypes.c_long
+    L = oracle.lib()
+    L.oracle_lrand48.restype = ctypes.c_long
+    for seed in (0, 1, 4343543, 2**31 - 1, -5):
+        libc.srand48(ctypes.c_long(seed))
+        L.oracle_srand48(ctypes.c_long(seed))
+        for _ in range(200):
+            assert L.oracle_lrand48() == libc.lrand48()
+
+
 def test_stop_test_stops():
     g = Graph.kronecker(8, 8, seed=(1, 2))
     pv, pr, bc, rs = oracle.run(g, g.N, c=1, x=1, z=0)
@@ -373,3 +387,18 @@ def test_sigma_wrapping_to_zero_gives_infinite_dependencies():
     assert (sig[1 + 4 * a:] == 0).all()
     assert (dl[1 + 5 * a:] == 0).all()
     assert np.isnan(dl[1:1 + 5 * a]).all()
+
+
+def test_loops_on_several_threads_draw_their_own_sequences():
+    """The oracle's generator state is per thread: loops run at once on
+    several threads give each the sequence of a loop run alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    gs = [Graph.kronecker(9 + k % 2, 8, seed=(k + 1, k + 2), heuristic=1) for k in range(4)]
+    alone = [oracle.run(g, 120, c=0, x=1, z=6, seed=100 + k) for k, g in enumerate(gs)]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        together = list(ex.map(lambda kg: oracle.run(kg[1], 120, c=0, x=1, z=6, seed=100 + kg[0]),
+                               list(enumerate(gs)) * 2))
+    for k, (pv, pr, bc, rs) in enumerate(together):
+        apv, apr, abc, ars = alone[k % 4]
+        assert np.array_equal(pv, apv) and np.array_equal(pr, apr) and rs == ars
+        assert np.array_equal(bc, abc)

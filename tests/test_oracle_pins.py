@@ -303,8 +303,7 @@ def test_hdn_draw_over_wrapped_prefix(r_case):
     want_u = {"zero": 0, "first": 0, "peak": 1, "peak+1": 3, "after_wrap": 3,
               "leaf": 4 + 1233, "last": N - 1, "none": N - 1}[r_case]
     assert u == want_u
-    libc = ctypes.CDLL(None)
-    libc.srand48(ctypes.c_long(977))
+    oracle.lib().oracle_srand48(ctypes.c_long(977))
     rng = Lrand48(977)
     adj_u = g.row[g.col[u]:g.col[u + 1]]
     v = int(adj_u[0 if deg[u] == 1 else rng.uniform_int(deg[u] - 1)])
