@@ -38,7 +38,10 @@ _host = None
 
 
 def _load(name: str) -> C.CDLL:
-    path = os.path.join(LIB_DIR, name)
+    # AXBC_SAN_DIR: a directory of sanitizer builds of the host libraries
+    # (python -m axbc_amd.build --sanitize), used where it holds the file
+    san = os.environ.get("AXBC_SAN_DIR")
+    path = os.path.join(san, name) if san and os.path.exists(os.path.join(san, name)) else os.path.join(LIB_DIR, name)
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built: run `python -m axbc_amd.build` "
                            "(the engine has no fallback path)")

@@ -8,6 +8,18 @@ produces
     axbc_amd/bin/axbc2d            command-line driver with AxBC's flags
 and the checker builds under oracle/ (the CPU oracle always; the reference
 generator under oracle/_ref only when the reference sources are present).
+
+    python -m axbc_amd.build --sanitize
+
+builds the host-side C (graph tooling, driver, legacy layer) and the CPU
+oracle with AddressSanitizer and UndefinedBehaviorSanitizer into _build_san/;
+the CPU test suite runs against them with
+
+    AXBC_SAN_DIR=$PWD/_build_san LD_PRELOAD=$(gcc -print-file-name=libasan.so) \
+        ASAN_OPTIONS=detect_leaks=0 UBSAN_OPTIONS=halt_on_error=1:print_stacktrace=1 \
+        python -m pytest tests -m "not gpu"
+
+(the HIP library stays the ordinary build: the CPU suite only loads it).
 """
 from __future__ import annotations
 
@@ -118,6 +130,27 @@ def build_oracle() -> None:
         _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REFERENCE}"])
 
 
+SAN = os.path.join(ROOT, "_build_san")
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized() -> str:
+    """Host libraries, driver and oracle with ASan + UBSan, into _build_san/."""
+    os.makedirs(SAN, exist_ok=True)
+    srcs = [os.path.join(HOST_SRC, s) for s in HOST_SOURCES]
+    _run(["gcc", *SAN_FLAGS, "-std=gnu11", "-fopenmp", "-fPIC", "-shared", *srcs,
+          "-o", os.path.join(SAN, "libaxbc_host.so"), "-lm", "-lpthread"])
+    _run(["gcc", *SAN_FLAGS, "-std=gnu11", "-fopenmp", "-fPIC", "-shared",
+          os.path.join(ROOT, "oracle", "brandes_oracle.c"), "-o", os.path.join(SAN, "liboracle.so"), "-lm"])
+    _run(["gcc", *SAN_FLAGS, "-std=gnu11", "-fopenmp", "-I", INCLUDE, "-I", HOST_SRC,
+          os.path.join(HOST_SRC, "axbc2d.c"), "-o", os.path.join(SAN, "axbc2d"),
+          "-L", SAN, "-L", LIB, "-laxbc_host", "-laxbc_hip", f"-Wl,-rpath,{SAN}:{LIB}", "-lm"])
+    _run(["gcc", *SAN_FLAGS, "-std=gnu11", "-fPIC", "-shared", "-I", INCLUDE, os.path.join(HOST_SRC, "legacy.c"),
+          "-o", os.path.join(SAN, "libaxbc_legacy.so"), "-L", LIB, "-laxbc_hip", f"-Wl,-rpath,{LIB}"])
+    return SAN
+
+
 def build_all() -> None:
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
@@ -134,5 +167,8 @@ def build_all() -> None:
 
 
 if __name__ == "__main__":
-    build_all()
-    print("built", LIB, BIN)
+    if "--sanitize" in sys.argv[1:]:
+        print("built", build_sanitized())
+    else:
+        build_all()
+        print("built", LIB, BIN)

@@ -13,6 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "bin", "liboracle.so")
+# a sanitizer build of the oracle (python -m axbc_amd.build --sanitize)
+if os.environ.get("AXBC_SAN_DIR") and os.path.exists(os.path.join(os.environ["AXBC_SAN_DIR"], "liboracle.so")):
+    LIB = os.path.join(os.environ["AXBC_SAN_DIR"], "liboracle.so")
 
 u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

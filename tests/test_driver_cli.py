@@ -15,6 +15,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "axbc_amd", "bin", "axbc2d")
+if os.environ.get("AXBC_SAN_DIR") and os.path.exists(os.path.join(os.environ["AXBC_SAN_DIR"], "axbc2d")):
+    DRIVER = os.path.join(os.environ["AXBC_SAN_DIR"], "axbc2d")  # the sanitizer build
 GOLDEN = os.path.join(ROOT, "tests", "golden", "generator.json")
 
 pytestmark = pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")

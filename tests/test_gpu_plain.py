@@ -81,3 +81,15 @@ def test_sigma_wraps_to_zero(require_gpu):
     # reference's uint32 arithmetic; both sides must agree position by position
     g = layered(256, 6)
     check_rounds(g, [0, 1 + 5 * 256 + 7, 1 + 2 * 256 + 3], "sigma wraps to 0")
+
+
+@pytest.mark.parametrize("undirected", [True, False])
+def test_plain_config_per_round(require_gpu, undirected):
+    """The CPU-runnable plain configuration, round by round: Kronecker SCALE
+    16, edge factor 16, seed pair (1, 2), -z 0 -N 256 (once symmetrised and
+    once with -U), the oracle's pivot sequence; every round's lvl and sigma
+    bit-exact, delta and bc within 1e-9."""
+    g = Graph.kronecker(16, 16, seed=(1, 2), undirected=undirected, heuristic=0)
+    pv, pr, bc, rs = oracle.run(g, 256, c=0, x=1, z=0)
+    assert rs["nrounds"] == len(pv) > 240
+    check_rounds(g, pv, f"scale16-{'sym' if undirected else 'dir'}")
