@@ -3049,6 +3049,30 @@ int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double
 }
 }  // namespace axbc
 
+namespace axbc {
+// Root k's dependency row of the last batch gathered into the batch's row
+// buffer on the device (stream-ordered); *row points at it.
+int batch_delta_row(axbc_ctx *c, int k, const double **row) {
+  if (!c || !c->batch || !row || k < 0 || k >= c->batch->B)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  BatchState *b = c->batch;
+  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
+  int rc;
+  HIP_TRY(hipSetDevice(c->device));
+  if (!b->row_out) {
+    if ((rc = balloc_more(c, (double **)&b->row_out, c->n, "a per-root row copy"))) return rc;
+  }
+  double *tmp = (double *)b->row_out;
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
+  else
+    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
+  HIP_TRY(hipGetLastError());
+  *row = tmp;
+  return AXBC_OK;
+}
+}  // namespace axbc
+
 extern "C" {
 
 int axbc_set_batch_width(axbc_ctx *c, int width) {
@@ -3166,20 +3190,10 @@ int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight
 int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
   if (!c || !c->batch || !delta || k < 0 || k >= c->batch->B)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
-  BatchState *b = c->batch;
-  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
+  const double *row = nullptr;
   int rc;
-  HIP_TRY(hipSetDevice(c->device));
-  if (!b->row_out) {
-    if ((rc = balloc_more(c, (double **)&b->row_out, c->n, "a per-root row copy"))) return rc;
-  }
-  double *tmp = (double *)b->row_out;
-  if (b->nw == 1)
-    hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
-  else
-    hipLaunchKernelGGL(k_bcopy_delta<2>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(delta, tmp, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = batch_delta_row(c, k, &row))) return rc;
+  HIP_TRY(hipMemcpyAsync(delta, row, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AXBC_OK;
 }

@@ -156,6 +156,7 @@ struct axbc_ctx {
 
 namespace axbc {
 int sampler_bufs(axbc_ctx *c, SamplerBufs **out);
+int batch_delta_row(axbc_ctx *c, int k, const double **row);
 void sampler_free(axbc_ctx *c);
 void batch_free(axbc_ctx *c);
 void mlp_free(axbc_ctx *c);

@@ -1,6 +1,7 @@
 // Learned pivot proposal: a 3-layer MLP over per-vertex features
 // (log degree, LCC, running bc, mean neighbour bc) -> score, with BF16 weights,
-// FP32 accumulation, hand-written forward, backward and Adam kernels.
+// FP32 accumulation, hand-written forward, backward and Adam kernels; the
+// 64 x 64 layer and its backward products run on the BF16 matrix cores.
 //
 // Network:  x (4) -> h1 = relu(W1 x + b1) (64) -> h2 = relu(W2 h1 + b2) (64)
 //           -> s = w3 . h2 + b3.
@@ -58,11 +59,13 @@ __device__ __forceinline__ float bf(const __hip_bfloat16 *w, int i) { return __b
 
 // Features of vertex v: log(1 + degree), LCC (or, while targets are set,
 // the target sum t(v) = sum of tw over v's neighbours, normalised below),
-// log(1 + bc * scale) and log(1 + mean neighbour bc * scale).  Thread per
+// log(1 + bc * scale) and log(1 + mean neighbour bc * scale), or, given a
+// last dependency row dlast, log(1 + dlast(v) * dscale).  Thread per
 // vertex; vertices with more than 64 edges are summed by the whole wave.
 __global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                            const float *__restrict__ lcc, const double *__restrict__ bc, double bc_scale,
-                           const double *__restrict__ tw, float *x) {
+                           const double *__restrict__ tw, const double *__restrict__ dlast, double dscale,
+                           float *x) {
   const int lane = threadIdx.x & 63;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n; base += stride) {
@@ -104,7 +107,7 @@ __global__ void k_features(uint32_t n, const uint32_t *__restrict__ col, const u
       x[4 * v + 0] = log1pf((float)d);
       x[4 * v + 1] = tw ? (float)tsum : lcc ? lcc[v] : 0.0f;
       x[4 * v + 2] = (float)log1p(bc[v] * bc_scale);
-      x[4 * v + 3] = d ? (float)log1p(nsum / (double)d * bc_scale) : 0.0f;
+      x[4 * v + 3] = dlast ? (float)log1p(dlast[v] * dscale) : d ? (float)log1p(nsum / (double)d * bc_scale) : 0.0f;
     }
   }
 }
@@ -162,122 +165,257 @@ __global__ void k_scatter_targets(uint32_t m, const uint32_t *__restrict__ vert,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) tw[vert[i]] = w[i];
 }
 
-// weights of the workgroup in LDS as FP32
-struct SharedW {
-  float w[kParams];
+// ---- the 64 x 64 layer on the matrix cores ---------------------------------
+//
+// Layer 2 and its two backward products are 64 x 64 x 64 products per tile
+// of 64 vertices (samples), run as 2 x 2 tiles of 32 x 32 on the BF16 MFMA
+// (v_mfma_f32_32x32x8_bf16: eight k-steps of 8 per tile) with FP32
+// accumulation.  Their inputs are BF16: the weights (kept in BF16 anyway)
+// and the activations h1 and gradients dz2, rounded to nearest even.
+// Operands sit in LDS as BF16 bits, k contiguous, rows padded to kLd.
+// Layer 1 (4 -> 64), the output unit and the small gradients stay on the
+// vector ALUs in FP32.
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kLd = kH + 4;  // row pitch (BF16 elements) of the LDS operands: 136 B, 8-B aligned rows
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// acc += A[0..32)[0..64) x B[0..64)[0..32) for the 32 x 32 output tile, A
+// given by rows (k contiguous), B by its columns (Bt: column n is row n of
+// Bt, k contiguous).  Lane l supplies rows / columns l % 32, k-quad l / 32.
+__device__ __forceinline__ f32x16 mfma_tile(const unsigned short *A, const unsigned short *Bt, f32x16 acc) {
+  const int l = threadIdx.x & 63;
+  const unsigned short *a = A + (l & 31) * kLd + ((l >> 5) << 2);
+  const unsigned short *b = Bt + (l & 31) * kLd + ((l >> 5) << 2);
+#pragma unroll
+  for (int ks = 0; ks < kH; ks += 8)
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(*(const bf16x4 *)(a + ks), *(const bf16x4 *)(b + ks), acc, 0, 0,
+                                                    0);
+  return acc;
+}
+
+// Output element of register r in lane l of a 32 x 32 MFMA tile: row
+// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+__device__ __forceinline__ int tile_row(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+
+// FP32 copies of the small parameters (W1, b1, b2, w3, b3)
+struct SmallW {
+  float w1[kH * kIn], b1[kH], b2[kH], w3[kH], b3;
 };
 
-__device__ void load_weights(const __hip_bfloat16 *__restrict__ wb, SharedW &sw) {
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) sw.w[i] = bf(wb, i);
+// Small parameters as FP32, W2 as BF16 bits by rows ([out][in]) and, when
+// w2t is given, by columns ([in][out]).
+__device__ void load_weights(const __hip_bfloat16 *__restrict__ wb, SmallW &sw, unsigned short *w2,
+                             unsigned short *w2t) {
+  const unsigned short *raw = reinterpret_cast<const unsigned short *>(wb);
+  for (int i = threadIdx.x; i < kH * kIn; i += blockDim.x) sw.w1[i] = bf(wb, oW1 + i);
+  for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+    sw.b1[i] = bf(wb, oB1 + i);
+    sw.b2[i] = bf(wb, oB2 + i);
+    sw.w3[i] = bf(wb, oW3 + i);
+  }
+  if (threadIdx.x == 0) sw.b3 = bf(wb, oB3);
+  for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) {
+    const int j = i / kH, k = i % kH;  // W2[j][k]: unit j of layer 2, input k
+    w2[j * kLd + k] = raw[oW2 + i];
+    if (w2t) w2t[k * kLd + j] = raw[oW2 + i];
+  }
   __syncthreads();
 }
 
-__device__ __forceinline__ float forward_one(const SharedW &sw, const float *x, float h1[kH], float h2[kH]) {
+// Layer 1 of one vertex: h1 = relu(W1 x + b1)
+__device__ __forceinline__ float layer1(const SmallW &sw, const float xv[kIn], int j) {
+  float a = sw.b1[j];
 #pragma unroll
-  for (int j = 0; j < kH; ++j) {
-    float a = sw.w[oB1 + j];
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) a += sw.w[oW1 + j * kIn + k] * x[k];
-    h1[j] = a > 0.0f ? a : 0.0f;
-  }
-  float s = sw.w[oB3];
-  for (int j = 0; j < kH; ++j) {
-    float a = sw.w[oB2 + j];
-#pragma unroll 16
-    for (int k = 0; k < kH; ++k) a += sw.w[oW2 + j * kH + k] * h1[k];
-    h2[j] = a > 0.0f ? a : 0.0f;
-    s += sw.w[oW3 + j] * h2[j];
-  }
-  return s;
+  for (int k = 0; k < kIn; ++k) a += sw.w1[j * kIn + k] * xv[k];
+  return a > 0.0f ? a : 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_mlp_forward(uint32_t n, const float *__restrict__ x,
-                                                     const __hip_bfloat16 *__restrict__ wb, float *score) {
-  __shared__ SharedW sw;
-  load_weights(wb, sw);
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    float h1[kH], h2[kH], xv[kIn];
+// Scores: 128-thread workgroups, each wave a tile of 64 vertices per step
+// (the loop is uniform over the workgroup, so the barriers are reached by
+// both waves alike).
+constexpr int kFwdThreads = 128;
+
+__global__ __launch_bounds__(kFwdThreads) void k_mlp_forward(uint32_t n, const float *__restrict__ x,
+                                                             const __hip_bfloat16 *__restrict__ wb, float *score) {
+  __shared__ SmallW sw;
+  __shared__ __align__(16) unsigned short W2s[kH * kLd];
+  __shared__ __align__(16) unsigned short H1s[kFwdThreads / 64][64 * kLd];
+  __shared__ float Z[kFwdThreads / 64][64][kH + 1];
+  load_weights(wb, sw, W2s, nullptr);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * kFwdThreads; base < n; base += gridDim.x * kFwdThreads) {
+    const uint32_t v = base + threadIdx.x;
+    float xv[kIn] = {0.f, 0.f, 0.f, 0.f};
+    if (v < n)
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) xv[k] = x[4 * v + k];
-    score[v] = forward_one(sw, xv, h1, h2);
+      for (int k = 0; k < kIn; ++k) xv[k] = x[4 * v + k];
+#pragma unroll 8
+    for (int j = 0; j < kH; ++j) H1s[w][l * kLd + j] = bf16_bits(layer1(sw, xv, j));
+    __syncthreads();
+    // z2 = H1 W2^T: rows = vertices of the wave, columns = units
+#pragma unroll
+    for (int vb = 0; vb < 2; ++vb)
+#pragma unroll
+      for (int ob = 0; ob < 2; ++ob) {
+        f32x16 acc = {};
+        acc = mfma_tile(&H1s[w][vb * 32 * kLd], &W2s[ob * 32 * kLd], acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Z[w][vb * 32 + tile_row(r, l)][ob * 32 + (l & 31)] = acc[r];
+      }
+    __syncthreads();
+    float s = sw.b3;
+#pragma unroll 16
+    for (int j = 0; j < kH; ++j) {
+      const float a = Z[w][l][j] + sw.b2[j];
+      s += sw.w3[j] * (a > 0.0f ? a : 0.0f);
+    }
+    if (v < n) score[v] = s;
+    __syncthreads();
   }
 }
 
 // Gradient of 1/M sum (s - y)^2 over the samples idx[0..m).  A workgroup is
-// one wave of 64 threads and takes tiles of 64 samples: thread t runs sample
-// t of the tile forward and backward and leaves its activations and
-// pre-activation gradients in LDS; then thread t accumulates the gradient
-// of the parameters it owns (row t of W2, entries t of b1, b2, w3, row t of
-// W1, and b3 for thread 0) over the tile's samples in registers -- a
-// (64 x 64) x (64 x 64) product per tile, no atomics inside the workgroup.
-// At the end one atomic add per owned parameter.
+// one wave of 64 threads and takes tiles of 64 samples, lane t sample t:
+//   z2 = H1 W2^T                 (MFMA; H1 = BF16 h1 of the tile)
+//   dz2 = [z2 > 0] ds w3^T       (ds = 2 (s - y) / m per sample)
+//   dh1 = DZ2 W2                 (MFMA; DZ2 = BF16 dz2), dz1 = [h1 > 0] dh1
+//   gW2 += DZ2^T H1              (MFMA, accumulated over the tiles in registers)
+// and the small gradients by owner lanes over the tile's samples (lane j:
+// b2[j], w3[j], b1[j], W1[j][.]).  One atomic add per parameter at the end.
 constexpr int kTile = 64;
 
 __global__ __launch_bounds__(64) void k_mlp_grad(uint32_t m, const uint32_t *__restrict__ idx,
                                                  const float *__restrict__ x, const float *__restrict__ y,
                                                  const __hip_bfloat16 *__restrict__ wb, float *grad, float *loss) {
-  __shared__ SharedW sw;
-  __shared__ float H1[kTile][kH + 1], H2[kTile][kH + 1], DZ1[kTile][kH + 1], DZ2[kTile][kH + 1];
+  __shared__ SmallW sw;
+  __shared__ __align__(16) unsigned short W2s[kH * kLd];  // [unit j][input k]
+  __shared__ __align__(16) unsigned short W2t[kH * kLd];  // [input k][unit j]
+  __shared__ __align__(16) unsigned short H1s[kTile * kLd], H1t[kH * kLd];  // [sample][k], [k][sample]
+  __shared__ __align__(16) unsigned short DZs[kTile * kLd], DZt[kH * kLd];  // [sample][j], [j][sample]
+  __shared__ float Z[kTile][kH + 1];  // z2 (pre-bias), then dz1
   __shared__ float X[kTile][kIn], DS[kTile];
-  load_weights(wb, sw);
+  load_weights(wb, sw, W2s, W2t);
   const int t = threadIdx.x;
-  float gW2[kH], gW1[kIn], gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, lsum = 0.f;
+  f32x16 gW2[2][2];
 #pragma unroll
-  for (int k = 0; k < kH; ++k) gW2[k] = 0.f;
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-  for (int k = 0; k < kIn; ++k) gW1[k] = 0.f;
+    for (int b = 0; b < 2; ++b) gW2[a][b] = f32x16{};
+  float gW1[kIn] = {0.f, 0.f, 0.f, 0.f}, gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, lsum = 0.f;
   const uint32_t ntiles = (m + kTile - 1) / kTile;
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t si = tile * kTile + t;
-    // per-sample forward and backward
-    float h1[kH], h2[kH], xv[kIn] = {0.f, 0.f, 0.f, 0.f}, ds = 0.f;
     const bool live = si < m;
+    uint32_t v = 0;
+    float xv[kIn] = {0.f, 0.f, 0.f, 0.f};
     if (live) {
-      const uint32_t v = idx[si];
+      v = idx[si];
 #pragma unroll
       for (int k = 0; k < kIn; ++k) xv[k] = x[4 * v + k];
-      const float s = forward_one(sw, xv, h1, h2);
+    }
+    // layer 1 (a sample outside the range has h1 = 0 and ds = 0)
+    uint64_t pos1 = 0;  // h1 > 0 per unit
+#pragma unroll 8
+    for (int j = 0; j < kH; ++j) {
+      const float h = live ? layer1(sw, xv, j) : 0.0f;
+      pos1 |= (uint64_t)(h > 0.0f) << j;
+      const unsigned short hb = bf16_bits(h);
+      H1s[t * kLd + j] = hb;
+      H1t[j * kLd + t] = hb;
+    }
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) X[t][k] = xv[k];
+    __syncthreads();
+#pragma unroll
+    for (int vb = 0; vb < 2; ++vb)
+#pragma unroll
+      for (int ob = 0; ob < 2; ++ob) {
+        f32x16 acc = {};
+        acc = mfma_tile(&H1s[vb * 32 * kLd], &W2s[ob * 32 * kLd], acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Z[vb * 32 + tile_row(r, t)][ob * 32 + (t & 31)] = acc[r];
+      }
+    __syncthreads();
+    // output, loss and dz2 of sample t
+    float s = sw.b3;
+#pragma unroll 16
+    for (int j = 0; j < kH; ++j) {
+      const float a = Z[t][j] + sw.b2[j];
+      s += sw.w3[j] * (a > 0.0f ? a : 0.0f);
+    }
+    float ds = 0.0f;
+    if (live) {
       const float r = s - y[v];
       lsum += r * r;
       ds = 2.0f * r / (float)m;
-    } else {
-#pragma unroll
-      for (int k = 0; k < kH; ++k) h1[k] = h2[k] = 0.f;
     }
-    float dh1[kH];
-#pragma unroll
-    for (int k = 0; k < kH; ++k) dh1[k] = 0.f;
-    for (int j = 0; j < kH; ++j) {
-      const float dz2 = h2[j] > 0.f ? ds * sw.w[oW3 + j] : 0.f;
-      DZ2[t][j] = dz2;
-      H1[t][j] = h1[j];
-      H2[t][j] = h2[j];
-#pragma unroll 16
-      for (int k = 0; k < kH; ++k) dh1[k] += dz2 * sw.w[oW2 + j * kH + k];
-    }
-#pragma unroll
-    for (int k = 0; k < kH; ++k) DZ1[t][k] = h1[k] > 0.f ? dh1[k] : 0.f;
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) X[t][k] = xv[k];
     DS[t] = ds;
+#pragma unroll 8
+    for (int j = 0; j < kH; ++j) {
+      const float dz = Z[t][j] + sw.b2[j] > 0.0f ? ds * sw.w3[j] : 0.0f;
+      const unsigned short db = bf16_bits(dz);
+      DZs[t * kLd + j] = db;
+      DZt[j * kLd + t] = db;
+    }
     __syncthreads();
-    // thread t: parameters of hidden unit t
-    for (int s2 = 0; s2 < kTile; ++s2) {
-      const float dz2 = DZ2[s2][t], dz1 = DZ1[s2][t];
+    // owner lane j: b2[j] and w3[j] over the tile's samples (FP32)
+    for (int q = 0; q < kTile; ++q) {
+      const float a = Z[q][t] + sw.b2[t];
+      if (a > 0.0f) {
+        gb2 += DS[q] * sw.w3[t];
+        gw3 += DS[q] * a;
+      }
+      if (t == 0) gb3 += DS[q];
+    }
+    // gW2 += DZ2^T H1: rows = units j, columns = inputs k
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) gW2[jb][kb] = mfma_tile(&DZt[jb * 32 * kLd], &H1t[kb * 32 * kLd], gW2[jb][kb]);
+    __syncthreads();  // Z is overwritten below
+    // dh1 = DZ2 W2: rows = samples, columns = inputs k
+#pragma unroll
+    for (int sb = 0; sb < 2; ++sb)
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        f32x16 acc = {};
+        acc = mfma_tile(&DZs[sb * 32 * kLd], &W2t[kb * 32 * kLd], acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Z[sb * 32 + tile_row(r, t)][kb * 32 + (t & 31)] = acc[r];
+      }
+    __syncthreads();
+    // dz1 of sample t, in place
 #pragma unroll 16
-      for (int k = 0; k < kH; ++k) gW2[k] += dz2 * H1[s2][k];
-      gb2 += dz2;
-      gw3 += DS[s2] * H2[s2][t];
+    for (int k = 0; k < kH; ++k)
+      if (!((pos1 >> k) & 1)) Z[t][k] = 0.0f;
+    __syncthreads();
+    // owner lane j: b1[j], W1[j][.]
+    for (int q = 0; q < kTile; ++q) {
+      const float dz1 = Z[q][t];
       gb1 += dz1;
 #pragma unroll
-      for (int k = 0; k < kIn; ++k) gW1[k] += dz1 * X[s2][k];
-      if (t == 0) gb3 += DS[s2];
+      for (int k = 0; k < kIn; ++k) gW1[k] += dz1 * X[q][k];
     }
     __syncthreads();
   }
-#pragma unroll 16
-  for (int k = 0; k < kH; ++k)
-    if (gW2[k] != 0.f) atomicAdd(&grad[oW2 + t * kH + k], gW2[k]);
+#pragma unroll
+  for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float g = gW2[jb][kb][r];
+        if (g != 0.f) atomicAdd(&grad[oW2 + (jb * 32 + tile_row(r, t)) * kH + kb * 32 + (t & 31)], g);
+      }
 #pragma unroll
   for (int k = 0; k < kIn; ++k)
     if (gW1[k] != 0.f) atomicAdd(&grad[oW1 + t * kIn + k], gW1[k]);
@@ -407,12 +545,32 @@ int axbc_mlp_features(axbc_ctx *c, double bc_scale) {
   const float *lcc = c->sampler && c->sampler->lcc_ready ? c->sampler->lcc : nullptr;
   const double *tw = b->targets ? b->tw : nullptr;
   hipLaunchKernelGGL(k_features, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, c->row, lcc, c->bc,
-                     bc_scale, tw, b->x);
+                     bc_scale, tw, (const double *)nullptr, 0.0, b->x);
   if (tw) {
     hipLaunchKernelGGL(k_target_partials, dim3(kRedBlocks), dim3(256), 0, c->stream, c->n, b->x, b->tred);
     hipLaunchKernelGGL(k_target_mean, dim3(1), dim3(64), 0, c->stream, b->tred);
     hipLaunchKernelGGL(k_target_norm, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, b->tred, b->x);
   }
+  HIP_TRY(hipGetLastError());
+  return AXBC_OK;
+}
+
+// The four inputs of the specification: log(1 + degree), LCC (built here
+// when the context has none), log(1 + bc * bc_scale) and the last
+// dependency values, log(1 + delta_k(v) * delta_scale) with delta_k root k's
+// row of the last batch.
+int axbc_mlp_features_spec(axbc_ctx *c, double bc_scale, int k, double delta_scale) {
+  MlpBufs *b = c ? bufs_of(c) : nullptr;
+  if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
+  int rc;
+  if (!(c->sampler && c->sampler->lcc_ready)) {
+    uint64_t total = 0;
+    if ((rc = axbc_lcc_build(c, nullptr, nullptr, nullptr, &total))) return rc;
+  }
+  const double *dl = nullptr;
+  if ((rc = batch_delta_row(c, k, &dl))) return rc;
+  hipLaunchKernelGGL(k_features, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, c->row,
+                     (const float *)c->sampler->lcc, c->bc, bc_scale, (const double *)nullptr, dl, delta_scale, b->x);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }
@@ -512,8 +670,9 @@ int axbc_mlp_params(axbc_ctx *c, float *master_host) {
 int axbc_mlp_forward(axbc_ctx *c) {
   MlpBufs *b = c ? bufs_of(c) : nullptr;
   if (!b) return fail(AXBC_ESTATE, "mlp not initialised%s (line %d)", "", __LINE__);
-  hipLaunchKernelGGL(k_mlp_forward, dim3(grid_of(c->n) < 2048 ? grid_of(c->n) : 2048), dim3(256), 0, c->stream, c->n,
-                     b->x, b->wb, b->score);
+  const uint64_t fwd_groups = (c->n + kFwdThreads - 1) / kFwdThreads;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_groups < 4096 ? (unsigned)fwd_groups : 4096), dim3(kFwdThreads), 0,
+                     c->stream, c->n, b->x, b->wb, b->score);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
 }

@@ -52,6 +52,7 @@ def _hip():
     for name, args in {
         "axbc_mlp_init": [C.c_void_p, C.c_void_p],
         "axbc_mlp_features": [C.c_void_p, C.c_double],
+        "axbc_mlp_features_spec": [C.c_void_p, C.c_double, C.c_int, C.c_double],
         "axbc_mlp_targets": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32],
         "axbc_mlp_set_data": [C.c_void_p, C.c_void_p, C.c_void_p],
         "axbc_mlp_get_data": [C.c_void_p, C.c_void_p, C.c_void_p],
@@ -82,6 +83,10 @@ class Mlp:
 
     def features(self, bc_scale: float):
         _check(self.L.axbc_mlp_features(self.e.ctx, bc_scale))
+
+    def features_spec(self, bc_scale: float, k: int, delta_scale: float):
+        """log(1+deg), LCC, log(1+bc*bc_scale), log(1+delta_k*delta_scale)."""
+        _check(self.L.axbc_mlp_features_spec(self.e.ctx, bc_scale, int(k), delta_scale))
 
     def targets(self, vertex: np.ndarray, weight: np.ndarray):
         v = np.ascontiguousarray(vertex, np.uint32)
@@ -140,9 +145,14 @@ class StaticLoop(PivotLoop):
 
 def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64, eps: float = 1.0,
                 steps: int = 300, lr: float = 3e-3, seed: int = 4343543, top: float = 0.01,
-                target_feature: bool = True) -> dict:
+                target_feature: bool = True, features: str = "targets") -> dict:
     """The learned loop; returns timing, training log and whether it fell
-    back to uniform.  bc on the engine holds the estimate afterwards."""
+    back to uniform.  bc on the engine holds the estimate afterwards.
+    features: "targets" (log degree, target adjacency, running bc, mean
+    neighbour bc) or "spec" (log degree, LCC, running bc, the last
+    warm-up root's dependency row)."""
+    if features not in ("targets", "spec"):
+        raise ValueError("features: 'targets' or 'spec'")
     t0 = time.perf_counter()
     g = e.g
     N = g.N
@@ -150,8 +160,11 @@ def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64,
     e.reset_bc()
     pilot = max(1, warm // 4)
     loop = PivotLoop(e, N, c=0, x=0, z=0, seed=seed, batch=batch)
-    while loop.res.nrounds < pilot and loop.step(min(batch, pilot - loop.res.nrounds)):
-        pass
+    n0 = 0  # first round of the last batch
+    while loop.res.nrounds < pilot:
+        n0 = loop.res.nrounds
+        if not loop.step(min(batch, pilot - n0)):
+            break
     bc_a = e.bc()
     # targets: the vertices around the top-k boundary of the pilot estimate
     k = max(1, int(round(top * N)))
@@ -172,14 +185,21 @@ def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64,
             util[int(r)] = float(out[j]) * float(r1[r]) if len(tv) else 0.0
     bc_w = e.bc()
     t_warm = time.perf_counter() - t0
-    use_targets = target_feature and len(tv) > 0
-    if not use_targets:
+    use_targets = features == "targets" and target_feature and len(tv) > 0
+    if not use_targets and features == "targets":
         _check(L.axbc_lcc_build(e.ctx, None, None, None, C.byref(C.c_uint64(0))))
     mlp = Mlp(e, seed=seed)
     mw = bc_w[bc_w > 0].mean() if (bc_w > 0).any() else 1.0
-    if use_targets:
-        mlp.targets(tv, tw)
-    mlp.features(1.0 / mw)
+    if features == "spec":
+        # the last warm-up root's dependency row, scaled by its mean
+        k_last = max(0, loop.res.nrounds - 1 - n0)
+        d_last = e.batch_delta(k_last)
+        dm = d_last[d_last > 0].mean() if (d_last > 0).any() else 1.0
+        mlp.features_spec(1.0 / mw, k_last, 1.0 / dm)
+    else:
+        if use_targets:
+            mlp.targets(tv, tw)
+        mlp.features(1.0 / mw)
     roots = np.array(sorted(util), np.uint32)
     u = np.array([util[int(r)] for r in roots])
     losses, ok, val, base = [], False, float("nan"), float("nan")
@@ -214,7 +234,7 @@ def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64,
             pass
         nrounds = loop.res.nrounds
     e.sync()
-    return {"nrounds": nrounds, "fallback": not ok, "samples": len(roots), "targets": int(len(tv)),
+    return {"nrounds": nrounds, "fallback": not ok, "samples": len(roots), "targets": int(len(tv)), "features": features,
             "loss_first": losses[0] if losses else None, "loss_final": float(np.mean(losses[-10:])) if losses else None,
             "val_loss": val, "val_baseline": base, "t_warm": t_warm, "t_train": t_train,
             "seconds": time.perf_counter() - t0}

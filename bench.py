@@ -174,10 +174,11 @@ def serve_case(args):
 
 
 def learned_case(args):
-    """Accuracy on SCALE 18 (two generator seeds x two loop seeds): exact
+    """Accuracy on SCALE 18 (two generator seeds x three loop seeds): exact
     Brandes (every vertex a root, bit-packed batches) against the learned
-    loop and the uniform loop at 4096 pivots; then the learned loop's time
-    on SCALE 24."""
+    loop (with the engine's inputs and with the specification's: LCC and
+    the last dependency row), and the uniform loop at 4096 pivots; then the
+    learned loop's time on SCALE 24."""
     from axbc_amd import Engine, Graph
     from axbc_amd.learned import accuracy, exact_bc, run_learned
     from axbc_amd.pivots import PivotLoop
@@ -188,25 +189,33 @@ def learned_case(args):
             t = time.perf_counter()
             ex = exact_bc(e, batch=128)
             t_exact = time.perf_counter() - t
-            for lseed in (4343543, 1):
+            for lseed in (4343543, 1, 2):
                 info = run_learned(e, budget=4096, warm=512, batch=64, seed=lseed)
                 a_l = accuracy(e.bc(), ex)
+                info_s = run_learned(e, budget=4096, warm=512, batch=64, seed=lseed, features="spec")
+                a_s = accuracy(e.bc(), ex)
                 e.reset_bc()
                 loop = PivotLoop(e, g.N, c=0, x=0, z=0, seed=lseed, batch=64)
                 while loop.res.nrounds < 4096 and loop.step(min(64, 4096 - loop.res.nrounds)):
                     pass
                 a_u = accuracy(e.bc(), ex)
                 run = {"graph_seed": gseed, "loop_seed": lseed, "exact_seconds": round(t_exact, 2),
-                       "learned": a_l, "uniform": a_u, "fallback": info["fallback"],
-                       "val_loss": [info["val_baseline"], info["val_loss"]]}
+                       "learned": a_l, "learned_spec_inputs": a_s, "uniform": a_u, "fallback": info["fallback"],
+                       "fallback_spec_inputs": info_s["fallback"],
+                       "val_loss": [info["val_baseline"], info["val_loss"]],
+                       "val_loss_spec_inputs": [info_s["val_baseline"], info_s["val_loss"]]}
                 log(run)
                 out["runs"].append(run)
     rl = [r["learned"]["top_recall"] for r in out["runs"]]
     ru = [r["uniform"]["top_recall"] for r in out["runs"]]
     sl = [r["learned"]["spearman"] for r in out["runs"]]
     su = [r["uniform"]["spearman"] for r in out["runs"]]
+    rs_ = [r["learned_spec_inputs"]["top_recall"] for r in out["runs"]]
+    ss_ = [r["learned_spec_inputs"]["spearman"] for r in out["runs"]]
     out.update({"top1pct_recall_learned": round(float(np.mean(rl)), 4), "top1pct_recall_uniform": round(float(np.mean(ru)), 4),
-                "spearman_learned": round(float(np.mean(sl)), 5), "spearman_uniform": round(float(np.mean(su)), 5)})
+                "top1pct_recall_learned_spec_inputs": round(float(np.mean(rs_)), 4),
+                "spearman_learned": round(float(np.mean(sl)), 5), "spearman_uniform": round(float(np.mean(su)), 5),
+                "spearman_learned_spec_inputs": round(float(np.mean(ss_)), 5)})
     t = time.perf_counter()
     g = Graph.kronecker_device(24, 16, seed=(4, 5), heuristic=1)
     log(f"SCALE 24 built in {time.perf_counter() - t:.1f}s (N={g.N} ned={g.ned})")

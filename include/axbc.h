@@ -310,6 +310,10 @@ int axbc_kronecker_ingest(int device, int scale, uint64_t nedges, uint64_t seed1
  *   axbc_mlp_init        FP32 master parameters (axbc_mlp_param_count()
  *                        floats: W1[64][4], b1, W2[64][64], b2, w3, b3)
  *   axbc_mlp_features    features from the device state (LCC when built)
+ *   axbc_mlp_features_spec  the inputs log(1+deg), LCC (built if needed),
+ *                        log(1+bc*bc_scale) and the last dependency values
+ *                        log(1+delta_k(v)*delta_scale), delta_k being root
+ *                        k's row of the last batch
  *   axbc_mlp_targets     target vertices and weights: while set (count > 0),
  *                        feature 1 is log(1 + t(v) / mean positive t) with
  *                        t(v) the sum of the weights of v's neighbours
@@ -328,6 +332,7 @@ int axbc_kronecker_ingest(int device, int scale, uint64_t nedges, uint64_t seed1
 int axbc_mlp_param_count(void);
 int axbc_mlp_init(axbc_ctx *ctx, const float *params);
 int axbc_mlp_features(axbc_ctx *ctx, double bc_scale);
+int axbc_mlp_features_spec(axbc_ctx *ctx, double bc_scale, int k, double delta_scale);
 int axbc_mlp_targets(axbc_ctx *ctx, const uint32_t *vertex, const double *weight, uint32_t count);
 int axbc_mlp_set_data(axbc_ctx *ctx, const float *x, const float *y);
 int axbc_mlp_get_data(axbc_ctx *ctx, float *x, float *score);

@@ -79,12 +79,36 @@ def test_training_reduces_loss(require_gpu):
     assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:5])
 
 
-def test_learned_vs_uniform(require_gpu):
+def paired_gain(learned, uniform):
+    """Mean and standard error of the paired differences learned - uniform."""
+    d = np.asarray(learned, float) - np.asarray(uniform, float)
+    se = d.std(ddof=1) / np.sqrt(len(d)) if len(d) > 1 else float("inf")
+    return float(d.mean()), float(se)
+
+
+def report_gain(label, recall_gain, spearman_gain, record):
+    """Print the gains, record them on the test, and warn when the mean
+    recall gain is negative: a loss is reported, never passed silently; a
+    loss beyond two standard errors fails."""
+    import warnings
+    (rg, rse), (sg, sse) = recall_gain, spearman_gain
+    msg = (f"[{label}] top-1% recall gain {rg:+.4f} +- {rse:.4f}  spearman gain {sg:+.5f} +- {sse:.5f} "
+           "(learned - uniform, paired over seeds, +- one standard error)")
+    print(msg, flush=True)
+    record("recall_gain", rg)
+    record("recall_gain_se", rse)
+    record("spearman_gain", sg)
+    if rg < 0:
+        warnings.warn(f"learned proposal loses top-1% recall: {msg}")
+    assert rg + 2 * rse >= 0, f"learned proposal loses top-1% recall beyond two standard errors: {msg}"
+
+
+def test_learned_vs_uniform(require_gpu, record_property):
     """What the trained proposal gains over the method without it (uniform
-    roots) at the same budget, scored against exact Brandes: the learned
-    loop must train (not fall back in every run), keep the round count, and
-    not lose top-1% recall beyond the noise of four runs; the gains are
-    printed (on this graph family they are within noise of zero)."""
+    roots) at the same budget, scored against exact Brandes on a small
+    graph: the learned loop must train (not fall back in every run) and
+    keep the round count; the gain is reported (a loss warns, a loss beyond
+    two standard errors fails)."""
     g = Graph.kronecker(13, 16, seed=(4, 5), heuristic=1)
     from axbc_amd.pivots import PivotLoop
     rows = []
@@ -103,8 +127,36 @@ def test_learned_vs_uniform(require_gpu):
             assert info["nrounds"] == 1024
             rows.append((a_l["top_recall"], a_u["top_recall"], a_l["spearman"], a_u["spearman"], info["fallback"]))
     r = np.array(rows, dtype=float)
-    print(f"[learned] recall gain {r[:, 0].mean() - r[:, 1].mean():+.4f}  spearman gain "
-          f"{r[:, 2].mean() - r[:, 3].mean():+.5f}", flush=True)
     assert not r[:, 4].all()  # the network trained (held-out loss below the mean predictor) at least once
     assert (r[:, 2] > 0.5).all() and (r[:, 3] > 0.5).all()
-    assert r[:, 0].mean() >= r[:, 1].mean() - 0.03
+    report_gain("learned scale13", paired_gain(r[:, 0], r[:, 1]), paired_gain(r[:, 2], r[:, 3]), record_property)
+
+
+@pytest.mark.parametrize("features", ["targets", "spec"])
+def test_learned_c5_shape(require_gpu, record_property, features):
+    """The learned-proposal configuration's accuracy shape: Kronecker SCALE
+    18 (edge factor 16, seed pair (4, 5), -H 1), 512 warm-up pivots, 4096
+    in all, against exact Brandes, paired with uniform sampling over four
+    loop seeds; with the engine's inputs (target adjacency, mean neighbour
+    bc) and with the specification's (LCC, last dependency values)."""
+    from axbc_amd.pivots import PivotLoop
+    g = Graph.kronecker(18, 16, seed=(4, 5), heuristic=1)
+    rows = []
+    with Engine(g) as e:
+        ex = exact_bc(e, batch=128)
+        for seed in (4343543, 1, 2, 3):
+            info = run_learned(e, budget=4096, warm=512, batch=64, seed=seed, features=features)
+            a_l = accuracy(e.bc(), ex)
+            assert info["nrounds"] == 4096 and info["features"] == features
+            e.reset_bc()
+            loop = PivotLoop(e, g.N, c=0, x=0, z=0, batch=64, seed=seed)
+            while loop.res.nrounds < 4096 and loop.step(min(64, 4096 - loop.res.nrounds)):
+                pass
+            a_u = accuracy(e.bc(), ex)
+            print(f"[c5 {features}] seed {seed} fallback {info['fallback']} val {info['val_loss']:.4f} "
+                  f"base {info['val_baseline']:.4f} learned {a_l} uniform {a_u}", flush=True)
+            rows.append((a_l["top_recall"], a_u["top_recall"], a_l["spearman"], a_u["spearman"], info["fallback"]))
+    r = np.array(rows, dtype=float)
+    record_property("fallbacks", int(r[:, 4].sum()))
+    assert (r[:, 2] > 0.9).all() and (r[:, 3] > 0.9).all()
+    report_gain(f"c5 {features}", paired_gain(r[:, 0], r[:, 1]), paired_gain(r[:, 2], r[:, 3]), record_property)

@@ -91,5 +91,5 @@ def test_plain_config_per_round(require_gpu, undirected):
     bit-exact, delta and bc within 1e-9."""
     g = Graph.kronecker(16, 16, seed=(1, 2), undirected=undirected, heuristic=0)
     pv, pr, bc, rs = oracle.run(g, 256, c=0, x=1, z=0)
-    assert rs["nrounds"] == len(pv) > 240
+    assert rs["nrounds"] == len(pv) > 150  # isolated vertices drawn are skipped
     check_rounds(g, pv, f"scale16-{'sym' if undirected else 'dir'}")

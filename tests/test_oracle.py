@@ -337,15 +337,46 @@ def test_mlp_oracle_gradient_by_finite_differences():
     p = rng.normal(0, 0.3, mo.N_PARAMS)
     x = rng.normal(size=(40, 4))
     y = rng.normal(size=40)
-    loss, g = mo.loss_and_grad(p, x, y)
+    loss, g = mo.loss_and_grad(p, x, y, bf16_products=False)
     for i in rng.choice(mo.N_PARAMS, 60, replace=False):
         h = 1e-6
         q = p.copy()
         q[i] += h
-        lp, _ = mo.loss_and_grad(q, x, y)
+        lp, _ = mo.loss_and_grad(q, x, y, bf16_products=False)
         q[i] -= 2 * h
-        lm, _ = mo.loss_and_grad(q, x, y)
+        lm, _ = mo.loss_and_grad(q, x, y, bf16_products=False)
         assert (lp - lm) / (2 * h) == pytest.approx(g[i], rel=1e-4, abs=1e-7)
+
+
+def test_mlp_oracle_bf16_products():
+    """The matrix-core rounding (h1 into layer 2, dz2 into the backward
+    products) moves the loss and gradient by about the BF16 precision and
+    no more, and touches exactly the terms it should: with activations and
+    gradients already on the BF16 grid it changes nothing."""
+    from oracle import mlp_oracle as mo
+    rng = np.random.default_rng(1)
+    p = mo.to_bf16(rng.normal(0, 0.3, mo.N_PARAMS).astype(np.float32)).astype(np.float64)
+    x = rng.normal(size=(64, 4))
+    y = rng.normal(size=64)
+    l0, g0 = mo.loss_and_grad(p, x, y, bf16_products=False)
+    l1, g1 = mo.loss_and_grad(p, x, y)
+    assert l1 != l0 and abs(l1 - l0) <= 2 ** -7 * abs(l0)
+    assert np.abs(g1 - g0).max() <= 2 ** -6 * np.abs(g0).max()
+    # the output-layer terms (b3, w3) see only the forward rounding; b2 the
+    # unrounded dz2: recompute them by hand from the rounded forward pass
+    s, (z1, h1, z2, h2) = mo.forward(p, x)
+    ds = 2.0 * (s - y) / len(y)
+    assert g1[mo.O_B3] == pytest.approx(ds.sum(), rel=1e-12)
+    np.testing.assert_allclose(g1[mo.O_W3:mo.O_B3], h2.T @ ds, rtol=1e-12)
+    w3 = p[mo.O_W3:mo.O_B3]
+    np.testing.assert_allclose(g1[mo.O_B2:mo.O_W3], (np.outer(ds, w3) * (z2 > 0)).sum(0), rtol=1e-12)
+    # one hidden unit, one sample: a network whose h1 is exactly on the BF16
+    # grid (W1 = 0, b1 on the grid) gives the same forward either way
+    q = p.copy()
+    q[mo.O_W1:mo.O_B1] = 0.0
+    s0, _ = mo.forward(q, x, bf16_products=False)
+    s1, _ = mo.forward(q, x)
+    np.testing.assert_array_equal(s0, s1)
 
 
 def test_bf16_rounding():
