@@ -187,6 +187,14 @@ __device__ __forceinline__ uint32_t ld_flag(const uint32_t *p) {
 
 __device__ __forceinline__ bool bit(uint64_t w, int i) { return (w >> i) & 1ull; }
 
+// A launch's algorithmic bytes, added by lane 0 of every wave to one of the
+// kind's kKbSlots counters (chosen by wave id; one line each).
+__device__ __forceinline__ void kb_add(unsigned long long *kb, unsigned long long bytes) {
+  if (!kb || !bytes || lane_id() != 0) return;
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  atomicAdd(kb + (size_t)(gw & (kKbSlots - 1)) * kKbStride, bytes);
+}
+
 template <int NW>
 struct Words {
   uint64_t w[NW];
@@ -321,6 +329,35 @@ __device__ __forceinline__ void app_flush(Appender &a, axbc::BCtl *ctl) {
   a.n = 0;
 }
 
+// The last flush of a kernel, by the whole workgroup (every thread calls
+// it): one tail atomic for the ids of all its waves.  Atomics on one
+// address are served one at a time (about 11 ns each on MI355X), so a
+// level whose every wave appends pays for 8192 of them per kernel with
+// per-wave flushes and for 2048 with this one.
+__device__ __forceinline__ void app_flush_group(Appender &a, axbc::BCtl *ctl) {
+  __shared__ uint32_t s_n[kBlock / kWave];
+  __shared__ uint32_t s_base;
+  const int wv = (int)threadIdx.x / kWave, lane = lane_id();
+  if (lane == 0) s_n[wv] = a.n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int k = 0; k < (int)(blockDim.x / kWave); ++k) t += s_n[k];
+    s_base = t ? atomicAdd(a.tail, t) : 0u;
+  }
+  __syncthreads();
+  uint32_t b = s_base;
+  for (int k = 0; k < wv; ++k) b += s_n[k];
+  for (uint32_t k = (uint32_t)lane; k < a.n; k += kWave) {
+    const uint32_t slot = b + k;
+    if (slot < a.cap)
+      a.dst[slot] = a.buf[k];
+    else
+      atomicOr(&ctl->overflow, 1u);
+  }
+  a.n = 0;
+}
+
 // Appends v from every lane with pred (all lanes active).
 __device__ __forceinline__ void app_push(Appender &a, bool pred, uint32_t v, axbc::BCtl *ctl) {
   const unsigned long long m = __ballot(pred);
@@ -337,16 +374,31 @@ __device__ __forceinline__ void app_push(Appender &a, bool pred, uint32_t v, axb
 // ceil(d/ch) chunk items (split_claim = entries << 40 | items), so sp_base[]
 // increases with k.  The split kernels give each wave one contiguous range
 // of items: a binary search over sp_base for the first, then a forward walk.
-__device__ __forceinline__ void claim_split(axbc::BCtl *ctl, uint32_t i, uint32_t nch, uint32_t *sp_e,
-                                            uint64_t *sp_base, uint32_t sp_cap, uint64_t item_cap) {
-  const unsigned long long old = atomicAdd(&ctl->split_claim, (1ull << 40) + nch);
-  const uint32_t k = (uint32_t)(old >> 40);
-  const uint64_t base = old & ((1ull << 40) - 1);
-  if (k < sp_cap && base + nch <= item_cap) {
-    sp_e[k] = i;
-    sp_base[k] = base;
-  } else {
-    atomicOr(&ctl->overflow, 1u);
+// The lanes of a wave claim together (pred: this lane's entry i is split,
+// with nch items): one atomic for the wave, slots and items in lane order.
+// Atomics on one address are served one at a time (about 11 ns each), and a
+// level of a large graph can hold 10^5 split entries.
+__device__ __forceinline__ void claim_split_wave(axbc::BCtl *ctl, bool pred, uint32_t i, uint32_t nch,
+                                                 uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap,
+                                                 uint64_t item_cap) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  const int lane = lane_id();
+  const uint32_t items = pred ? nch : 0u;
+  const uint32_t incl = wave_incl_scan(items);
+  const uint32_t total = rdl(incl, kWave - 1);
+  unsigned long long old = 0;
+  if (lane == 0) old = atomicAdd(&ctl->split_claim, ((unsigned long long)__popcll(m) << 40) + total);
+  old = __shfl(old, 0);
+  if (pred) {
+    const uint32_t k = (uint32_t)(old >> 40) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint64_t base = (old & ((1ull << 40) - 1)) + (incl - items);
+    if (k < sp_cap && base + nch <= item_cap) {
+      sp_e[k] = i;
+      sp_base[k] = base;
+    } else {
+      atomicOr(&ctl->overflow, 1u);
+    }
   }
 }
 
@@ -652,10 +704,9 @@ __device__ __forceinline__ void bpush_body(const uint32_t *__restrict__ col, con
       d = col[u + 1] - st;
       f = ldw<NW>(qw, i);
     }
-    if (d > kChunk) {  // split: its chunks go to k_bpush_split
-      claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
-      d = 0;
-    }
+    const bool spl = d > kChunk;  // split: its chunks go to k_bpush_split
+    claim_split_wave(ctl, spl, i, spl ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, item_cap);
+    if (spl) d = 0;
     const uint32_t incl = wave_incl_scan(d);
     const uint32_t total = rdl(incl, kWave - 1);
     // entries: id 4 + offsets 8 + word 8NW; per edge: id 4 + VIS 8NW
@@ -679,8 +730,8 @@ __device__ __forceinline__ void bpush_body(const uint32_t *__restrict__ col, con
       app_push(app, first, w, ctl);
     }
   }
-  app_flush(app, ctl);
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  app_flush_group(app, ctl);
+  kb_add(kb, bytes);
 }
 
 template <int NW>
@@ -706,13 +757,22 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush_split(const uin
   const uint32_t mark_id = ctl->mark_base + (uint32_t)ctl->level;
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
-  if (r.lo >= r.hi) return;
+  __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
+  Appender app{s_app[threadIdx.x / kWave], 0, q, &ctl->tail, qcap};
   unsigned long long bytes = 0;
-  uint32_t i = sp_e[r.k];
-  uint32_t u = q[i];
-  uint32_t st = col[u], d = col[u + 1] - st;
-  Words<NW> f = ldw<NW>(qw, i);
-  uint64_t base = sp_base[r.k];
+  uint32_t i = 0, u = 0, st = 0, d = 0;
+  Words<NW> f;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) f.w[j] = 0;
+  uint64_t base = 0;
+  if (r.lo < r.hi) {
+    i = sp_e[r.k];
+    u = q[i];
+    st = col[u];
+    d = col[u + 1] - st;
+    f = ldw<NW>(qw, i);
+    base = sp_base[r.k];
+  }
   for (uint64_t item = r.lo; item < r.hi; ++item) {
     if (walk_to(r, sp_base, item)) {
       i = sp_e[r.k];
@@ -733,24 +793,12 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpush_split(const uin
         w = row[st + k];
         first = push_edge<NW>(w, f, vis, nx, mark, mark_id);
       }
-      const unsigned long long m = __ballot(first);
-      if (m) {
-        const int leader = __ffsll((long long)m) - 1;
-        uint32_t b = 0;
-        if (lane == leader) b = atomicAdd(&ctl->tail, (uint32_t)__popcll(m));
-        b = rdl(b, leader);
-        const uint32_t slot = b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (first) {
-          if (slot < qcap)
-            q[slot] = w;
-          else
-            atomicOr(&ctl->overflow, 1u);
-        }
-        bytes += 4ull * (unsigned long long)__popcll(m);
-      }
+      bytes += 4ull * (unsigned long long)__popcll(__ballot(first));
+      app_push(app, first, w, ctl);
     }
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  app_flush_group(app, ctl);
+  kb_add(kb, bytes);
 }
 
 // sigma pull for the vertices new at level L: q[qoff[L] .. tail).  A wave
@@ -993,19 +1041,17 @@ __device__ __forceinline__ void bsigma_body(const uint32_t *__restrict__ rcol, c
     Words<NW> nw;
 #pragma unroll
     for (int j = 0; j < NW; ++j) nw.w[j] = 0;
-    bool split = false;
     if (i < ne) {
       w = q[i];
       st = rcol[w];
       d = rcol[w + 1] - st;
-      if (d > kChunk) {
-        claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
-        split = true;
-        d = 0;
-      } else {
-        nw = ldw<NW>(nx, w);
-      }
     }
+    const bool split = i < ne && d > kChunk;
+    claim_split_wave(ctl, split, i, split ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, item_cap);
+    if (split)
+      d = 0;
+    else if (i < ne)
+      nw = ldw<NW>(nx, w);
     bytes += 12ull * (unsigned long long)__popcll(__ballot(split));
     uint32_t contrib = 0;
     // sparse levels: the small vertices as (vertex, root) pairs
@@ -1028,7 +1074,7 @@ __device__ __forceinline__ void bsigma_body(const uint32_t *__restrict__ rcol, c
     bytes += (unsigned long long)nv * (12 + 8 * NW) + (unsigned long long)dsum * (4 + 8 * NW) + 4ull * contrib +
              4ull * rsum;
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  kb_add(kb, bytes);
   if (sparse) {
     // the pairs added to this slot atomically: stay atomic
 #pragma unroll
@@ -1103,7 +1149,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const ui
     if (first_item) bytes += 8 * NW + 4ull * roots;
     first_item = false;
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  kb_add(kb, bytes);
   slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
@@ -1161,10 +1207,9 @@ __device__ __forceinline__ void bpull_body(uint32_t n, const uint32_t *__restric
     }
     bool cand = d > 0 && open.any();
     bytes += v < n ? 8 + 8 * NW : 0;
-    if (cand && d > kChunk) {
-      claim_split(ctl, v, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, item_cap);
-      cand = false;
-    }
+    const bool spl = cand && d > kChunk;
+    claim_split_wave(ctl, spl, v, spl ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, item_cap);
+    if (spl) cand = false;
     unsigned long long todo = __ballot(cand);
     unsigned long long fresh = 0;
     while (todo) {
@@ -1197,8 +1242,8 @@ __device__ __forceinline__ void bpull_body(uint32_t n, const uint32_t *__restric
     bytes += 4ull * (unsigned long long)__popcll(fresh);
     app_push(app, bit(fresh, lane), v, ctl);
   }
-  app_flush(app, ctl);
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  app_flush_group(app, ctl);
+  kb_add(kb, bytes);
   slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
@@ -1251,7 +1296,9 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
   const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
   const int lane = lane_id();
   ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
-  if (r.lo >= r.hi) return;
+  __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
+  Appender app{s_app[threadIdx.x / kWave], 0, q, &ctl->tail, qcap};
+  const bool has_items = r.lo < r.hi;
   unsigned long long bytes = 0;
   uint32_t dag_tot[NW];
 #pragma unroll
@@ -1296,18 +1343,14 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
             if (found[j]) atomicOr((unsigned long long *)&nx[(size_t)w * NW + j], (unsigned long long)found[j]);
           if (mark[w] != mark_id) first = atomicExch(&mark[w], mark_id) != mark_id;
         }
-        if (first) {
-          const uint32_t slot = atomicAdd(&ctl->tail, 1u);
-          if (slot < qcap)
-            q[slot] = w;
-          else
-            atomicOr(&ctl->overflow, 1u);
-        }
       }
     }
+    app_push(app, lane == 0 && first, w, ctl);
     bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW) + 4ull * contrib;
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  app_flush_group(app, ctl);
+  if (!has_items) return;
+  kb_add(kb, bytes);
   slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
@@ -1395,7 +1438,7 @@ __device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, co
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) a_or[j] |= __shfl_xor(a_or[j], off);
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  kb_add(kb, bytes);
   // per-root counts into this wave's slots; level sums into its level slot
   // (every wave writes its level slot, zero or not; k_bclose_end adds them)
   slot_add<NW>(wslot, 0, gw, c_vis);
@@ -1634,9 +1677,10 @@ __device__ __forceinline__ uint32_t dep_chunk(uint32_t st, uint32_t cb, uint32_t
   return contrib;
 }
 
-// Dependencies of one entry (v, f) of d edges, lanes = roots: delta_s(v) =
-// reach + sum x_s(w) sigma_s(v) in adjacency order; entries of more than
-// kChunk edges are claimed for k_bdep_split.  Returns the algorithmic bytes.
+// Dependencies of one entry (v, f) of d <= kChunk edges, lanes = roots:
+// delta_s(v) = reach + sum x_s(w) sigma_s(v) in adjacency order (entries of
+// more than kChunk edges are claimed for k_bdep_split by the caller).
+// Returns the algorithmic bytes.
 template <int NW>
 __device__ __forceinline__ unsigned long long dep_entry(uint32_t i, uint32_t v, const Words<NW> &f, uint32_t st,
                                                         uint32_t d, const uint32_t *__restrict__ row,
@@ -1647,10 +1691,6 @@ __device__ __forceinline__ unsigned long long dep_entry(uint32_t i, uint32_t v, 
                                                         uint64_t part_rows, uint32_t kChunk) {
   constexpr int B = 64 * NW;
   const int lane = lane_id();
-  if (d > kChunk) {
-    if (lane == 0) claim_split(ctl, i, (d + kChunk - 1) / kChunk, sp_e, sp_base, sp_cap, part_rows);
-    return 16 + 8 * NW;
-  }
   double sv[NW], acc[NW];
   const double r = (double)reach[v];
 #pragma unroll
@@ -1748,18 +1788,38 @@ __device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, cons
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
   if (!lmode[depth]) {
-    // dense level: one entry per wave at a time, lanes = roots
-    for (uint32_t i = b + gw; i < e; i += nwv) {
-      const uint32_t v = q[i];
-      Words<NW> f = ldw<NW>(qw, i);
-      if (depth == 0) {
-        f.w[0] &= root_mask0;
-        if (NW > 1) f.w[NW - 1] &= root_mask1;
+    // dense level: one entry per wave at a time, lanes = roots; the wave's
+    // entries b + gw + t nwv are loaded 64 at a time (lane = entry) so that
+    // their split entries are claimed together
+    for (uint64_t i0 = (uint64_t)b + gw; i0 < e; i0 += (uint64_t)nwv * kWave) {
+      const uint64_t il = i0 + (uint64_t)lane * nwv;
+      const bool in = il < e;
+      const uint32_t i = in ? (uint32_t)il : 0u;
+      uint32_t v = 0, st = 0, d = 0;
+      Words<NW> f;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) f.w[j] = 0;
+      if (in) {
+        v = q[i];
+        f = ldw<NW>(qw, i);
+        if (depth == 0) {
+          f.w[0] &= root_mask0;
+          if (NW > 1) f.w[NW - 1] &= root_mask1;
+        }
+        st = col[v];
+        d = col[v + 1] - st;
       }
-      if (!f.any()) continue;
-      const uint32_t st = col[v], d = col[v + 1] - st;
-      bytes += dep_entry<NW>(i, v, f, st, d, row, vis, sig, dlt, xr, reach, ctl, sp_e, sp_base, sp_cap, part_rows,
-                             kChunk);
+      const bool live = in && f.any();
+      const bool spl = live && d > kChunk;
+      claim_split_wave(ctl, spl, i, spl ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, part_rows);
+      bytes += (unsigned long long)__popcll(__ballot(spl)) * (16 + 8 * NW);
+      unsigned long long todo = __ballot(live && !spl);
+      while (todo) {
+        const int k = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        bytes += dep_entry<NW>(rdl(i, k), rdl(v, k), rdlw<NW>(f, k), rdl(st, k), rdl(d, k), row, vis, sig, dlt, xr,
+                               reach, ctl, sp_e, sp_base, sp_cap, part_rows, kChunk);
+      }
     }
   } else {
     // sparse level (few roots and few edges per entry): a wave takes 64
@@ -1783,8 +1843,11 @@ __device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, cons
       }
       const bool live = i < e && f.any();
       const bool small = live && d <= kSparseDeg;
+      const bool spl = live && d > kChunk;
       bytes += dep_pairs<NW>(small, v, f, st, d, row, vis, sig, dlt, xr, reach);
-      unsigned long long todo = __ballot(live && !small);
+      claim_split_wave(ctl, spl, i, spl ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, part_rows);
+      bytes += (unsigned long long)__popcll(__ballot(spl)) * (16 + 8 * NW);
+      unsigned long long todo = __ballot(live && !small && !spl);
       while (todo) {
         const int k = __ffsll((long long)todo) - 1;
         todo &= todo - 1;
@@ -1793,7 +1856,7 @@ __device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, cons
       }
     }
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  kb_add(kb, bytes);
 }
 
 template <int NW>
@@ -1857,7 +1920,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint
     // per edge: id 4 + VIS 8NW; per term: x 8 (partials are scratch)
     bytes += (unsigned long long)(ce - cb) * (4 + 8 * NW) + 8ull * contrib;
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  kb_add(kb, bytes);
 }
 
 // One wave per split entry: delta = reach + partials in chunk order.
@@ -1927,7 +1990,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_combine(const ui
     // per root of the entry: sigma 4 + delta 8 + x 8
     bytes += 20ull * roots;
   }
-  if (kb && lane == 0 && bytes) atomicAdd(kb, bytes);
+  kb_add(kb, bytes);
 }
 
 // Between the push's split items and the path-count pull's claims of a
@@ -2229,10 +2292,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bupdate(uint32_t n, c
     }
     __syncthreads();
   }
-  if (kb) {
-    bytes = wave_sum_u64(bytes);
-    if (lane_id() == 0 && bytes) atomicAdd(kb, bytes);
-  }
+  if (kb) kb_add(kb, wave_sum_u64(bytes));
 }
 
 // Utility of the batch's roots for a set of target vertices, in a fixed

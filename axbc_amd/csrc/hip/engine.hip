@@ -85,7 +85,7 @@ void kt_end(axbc_ctx *c, int kind) {
 }
 
 unsigned long long *kt_bytes(axbc_ctx *c, int kind) {
-  return kt_timed(c, kind) && c->ktimer.dbytes ? c->ktimer.dbytes + kind : nullptr;
+  return kt_timed(c, kind) && c->ktimer.dbytes ? c->ktimer.dbytes + (size_t)kind * kKbSlots * kKbStride : nullptr;
 }
 
 // Fold finished event pairs into the per-kind totals.
@@ -818,8 +818,9 @@ int axbc_last_timing(axbc_ctx *c, float *f, float *b, float *u) {
 static int kt_enable(axbc_ctx *c, bool enable, uint32_t mask) {
   KernelTimer &t = c->ktimer;
   if (enable) {
-    if (!t.dbytes) HIP_TRY(hipMalloc((void **)&t.dbytes, sizeof(unsigned long long) * KK_COUNT));
-    HIP_TRY(hipMemsetAsync(t.dbytes, 0, sizeof(unsigned long long) * KK_COUNT, c->stream));
+    const size_t nb = sizeof(unsigned long long) * KK_COUNT * kKbSlots * kKbStride;
+    if (!t.dbytes) HIP_TRY(hipMalloc((void **)&t.dbytes, nb));
+    HIP_TRY(hipMemsetAsync(t.dbytes, 0, nb, c->stream));
     int rc = kt_collect(c);
     if (rc) return rc;
     for (int k = 0; k < KK_COUNT; ++k) {
@@ -851,9 +852,12 @@ int axbc_kernel_stats(axbc_ctx *c, int kind, double *ms, uint64_t *launches, uin
   int rc = kt_collect(c);
   if (rc) return rc;
   unsigned long long b = 0;
+  std::vector<unsigned long long> slots((size_t)kKbSlots * kKbStride, 0ull);
   if (c->ktimer.dbytes)
-    HIP_TRY(hipMemcpyAsync(&b, c->ktimer.dbytes + kind, sizeof(b), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(slots.data(), c->ktimer.dbytes + (size_t)kind * kKbSlots * kKbStride,
+                           sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < kKbSlots; ++k) b += slots[(size_t)k * kKbStride];
   if (ms) *ms = c->ktimer.ms[kind];
   if (launches) *launches = c->ktimer.launches[kind];
   if (bytes) *bytes = b;

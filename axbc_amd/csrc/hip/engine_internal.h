@@ -76,11 +76,15 @@ enum KernelKind {
   KK_PULL_SPLIT = 15,  // batched bottom-up level, split vertices (k_bpull_split)
   KK_COUNT = 16
 };
+constexpr int kKbSlots = 64, kKbStride = 16;
 
 struct KernelTimer {
   bool on = false;
   uint32_t mask = ~0u;  // kinds timed while on (bit k = kind k)
-  unsigned long long *dbytes = nullptr;  // device: KK_COUNT counters
+  // device: per kind kKbSlots counters kKbStride words apart (one 128-B line
+  // each), summed when read: one counter per kind would serialise the
+  // adds of every wave of a launch
+  unsigned long long *dbytes = nullptr;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> pool;
   double ms[KK_COUNT] = {0};
