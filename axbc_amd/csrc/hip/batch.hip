@@ -69,8 +69,6 @@ constexpr int kGraphSteps = 8;    // levels (depths) per graph
 // serialise, which made every level cost tens of microseconds whatever its
 // size.
 constexpr int kSlotWaves = kBGrid * kBlock / kWave;  // waves of the largest level grid
-constexpr int kCloseGrid = kBGrid / 4;
-constexpr int kCloseWaves = kCloseGrid * kBlock / kWave;
 constexpr int kAppendBuf = 256;   // list ids buffered per wave before one tail atomic
 constexpr int kLevelSlot = 8;     // 64-bit fields per wave in the level slots of k_bclose
 // Sparse levels: at most kSparseRatio roots per list entry on average (high-
@@ -980,124 +978,9 @@ __device__ __forceinline__ uint32_t sigma_pairs(bool small, uint32_t w, const Wo
 }
 
 template <int NW>
-__device__ __forceinline__ void bsigma_body(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
-                                                   const uint64_t *__restrict__ nx, const uint64_t *fr0,
-                                                   const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
-                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                   unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
-                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
-                                                   unsigned long long *kb, uint32_t kChunk, int strict,
-                                                   const uint8_t *__restrict__ lmode) {
-  constexpr int B = 64 * NW;
-  if (ld_flag(&ctl->done) || ctl->bu) return;
-  const int L = ctl->level;
-  const uint64_t *frp = (L & 1) ? fr0 : fr1;  // level L-1
-  unsigned long long bytes = 0;
-  const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
-  const int lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  uint32_t dag_tot[NW];
-#pragma unroll
-  for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
-  // Level 1 over strictly increasing lists: the frontier is the roots
-  // (sigma 1 each), and a new vertex w has exactly one edge from every root
-  // in NX[w], so sigma_s(w) = 1 there without scanning w's in-edges (a hub
-  // next to the roots would be scanned whole).  The same DAG edges count.
-  const bool level1 = strict && L == 1;
-  for (uint32_t base = nb + gw * kWave; level1 && base < ne; base += nwv * kWave) {
-    const uint32_t i = base + lane;
-    uint32_t w = 0;
-    Words<NW> nw;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) nw.w[j] = 0;
-    if (i < ne) {
-      w = q[i];
-      nw = ldw<NW>(nx, w);
-    }
-    unsigned long long todo = __ballot(nw.any());
-    // per entry: id 4 + NX 8NW; per root reached: sigma write 4
-    bytes += (unsigned long long)__popcll(todo) * (4 + 8 * NW);
-    while (todo) {
-      const int k = __ffsll((long long)todo) - 1;
-      todo &= todo - 1;
-      const uint32_t wk = rdl(w, k);
-#pragma unroll
-      for (int j = 0; j < NW; ++j) {
-        const uint64_t mk = rdl(nw.w[j], k);
-        bytes += 4ull * (unsigned long long)__popcll(mk);
-        if (bit(mk, lane)) {
-          sig[(size_t)wk * B + lane + 64 * j] = 1u;
-          dag_tot[j] += 1;
-        }
-      }
-    }
-  }
-  // a wave takes 64 new entries of the level at a time (sparse: the level
-  // before had few roots per entry)
-  const bool sparse = lmode[L - 1] != 0;
-  for (uint32_t base = nb + gw * kWave; !level1 && base < ne; base += nwv * kWave) {
-    const uint32_t i = base + lane;
-    uint32_t w = 0, st = 0, d = 0;
-    Words<NW> nw;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) nw.w[j] = 0;
-    if (i < ne) {
-      w = q[i];
-      st = rcol[w];
-      d = rcol[w + 1] - st;
-    }
-    const bool split = i < ne && d > kChunk;
-    claim_split_wave(ctl, split, i, split ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, item_cap);
-    if (split)
-      d = 0;
-    else if (i < ne)
-      nw = ldw<NW>(nx, w);
-    bytes += 12ull * (unsigned long long)__popcll(__ballot(split));
-    uint32_t contrib = 0;
-    // sparse levels: the small vertices as (vertex, root) pairs
-    const bool small = sparse && d > 0 && d <= kSparseDeg;
-    if (sparse) contrib += sigma_pairs<NW>(small, w, nw, st, d, rrow, frp, sig, wslot, gw);
-    group_sigma<NW>(st, small ? 0u : d, nw, rrow, frp, sig, dag_tot, &contrib,
-                    [&](int o, const uint32_t *acc, const uint64_t *found) {
-                      const uint32_t wo = rdl(w, o);
-#pragma unroll
-                      for (int j = 0; j < NW; ++j)
-                        if (bit(found[j], lane)) sig[(size_t)wo * B + lane + 64 * j] = acc[j];
-                    });
-    // vertex: id 4 + offsets 8 + word 8NW + sigma write 4 per root; per
-    // edge: id 4 + frontier word 8NW; per term: sigma 4
-    uint32_t roots = 0;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) roots += d ? (uint32_t)__popcll(nw.w[j]) : 0u;
-    const uint32_t dsum = (uint32_t)wave_sum_u64(d), rsum = (uint32_t)wave_sum_u64(roots);
-    const uint32_t nv = (uint32_t)__popcll(__ballot(i < ne && d > 0));
-    bytes += (unsigned long long)nv * (12 + 8 * NW) + (unsigned long long)dsum * (4 + 8 * NW) + 4ull * contrib +
-             4ull * rsum;
-  }
-  kb_add(kb, bytes);
-  if (sparse) {
-    // the pairs added to this slot atomically: stay atomic
-#pragma unroll
-    for (int j = 0; j < NW; ++j) slot_add_root(wslot, 3, gw, B, lane + 64 * j, dag_tot[j]);
-  } else {
-    slot_add<NW>(wslot, 3, gw, dag_tot);
-  }
-}
-
-template <int NW>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma(const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
-                                                   const uint64_t *__restrict__ nx, const uint64_t *fr0,
-                                                   const uint64_t *fr1, uint32_t *sig, const uint32_t *__restrict__ q,
-                                                   const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                   unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
-                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
-                                                   unsigned long long *kb, uint32_t kChunk, int strict,
-                                                   const uint8_t *__restrict__ lmode) { bsigma_body<NW>(rcol, rrow, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap, kb, kChunk, strict, lmode); }
-
-template <int NW>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const uint32_t *__restrict__ rcol,
                                                          const uint32_t *__restrict__ rrow,
-                                                         const uint64_t *__restrict__ nx, const uint64_t *fr0,
+                                                         const uint64_t *__restrict__ qw, const uint64_t *fr0,
                                                          const uint64_t *fr1, uint32_t *sig,
                                                          const uint32_t *__restrict__ q, axbc::BCtl *ctl,
                                                          unsigned long long *wslot, const uint32_t *__restrict__ sp_e,
@@ -1115,7 +998,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const ui
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
   uint32_t w = q[sp_e[r.k]];
   uint32_t st = rcol[w], d = rcol[w + 1] - st;
-  Words<NW> nw = ldw<NW>(nx, w);
+  Words<NW> nw = ldw<NW>(qw, sp_e[r.k]);
   uint64_t base = sp_base[r.k];
   bool first_item = r.lo == base;
   for (uint64_t item = r.lo; item < r.hi; ++item) {
@@ -1123,7 +1006,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bsigma_split(const ui
       w = q[sp_e[r.k]];
       st = rcol[w];
       d = rcol[w + 1] - st;
-      nw = ldw<NW>(nx, w);
+      nw = ldw<NW>(qw, sp_e[r.k]);
       base = sp_base[r.k];
       first_item = true;
     }
@@ -1354,35 +1237,48 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
   slot_add<NW>(wslot, 3, (blockIdx.x * blockDim.x + threadIdx.x) / kWave, dag_tot);
 }
 
-// Close level L: frontier words, VIS, list words, counters, x rows.
+// Close level L and count its paths, in one pass over the level's new
+// entries q[qoff[L] .. tail) (lane = entry, 64 at a time): the entry's roots
+// NX[w] become its list word, its frontier word and part of VIS[w] (NX is
+// cleared); the per-root counters and the level sums for k_bclose_end are
+// taken; and, unless the level was found bottom-up (whose pull counted the
+// paths already), sigma_s(w) = sum of sigma_s(u) over u in radj(w) with s
+// in FR[u] (level L-1) for the vertices of at most kChunk pull edges (the
+// others are claimed for k_bsigma_split, which reads their roots from the
+// list word).  The path counts read only level L-1 state, the close writes
+// only level L state, so the two need no order between them.
 template <int NW>
 __device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
-                                                   uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
-                                                   const uint32_t *__restrict__ q,
-                                                   uint64_t *qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                   unsigned long long *wslot, unsigned long long *lslot,
-                                                   uint32_t *seen, uint32_t qcap,
-                                                   const uint32_t *__restrict__ rcol, unsigned long long *kb) {
+                                            const uint32_t *__restrict__ rcol, const uint32_t *__restrict__ rrow,
+                                            uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
+                                            const uint32_t *__restrict__ q, uint64_t *qw,
+                                            const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+                                            unsigned long long *wslot, unsigned long long *lslot, uint32_t *seen,
+                                            uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t item_cap,
+                                            uint32_t qcap, unsigned long long *kb, uint32_t kChunk, int strict,
+                                            const uint8_t *__restrict__ lmode) {
+  constexpr int B = 64 * NW;
   if (ld_flag(&ctl->done)) return;
   const int L = ctl->level;
+  const bool counts = ctl->bu == 0u;
   const uint32_t stamp = ctl->stamp;
-  uint64_t *frc = (L & 1) ? fr1 : fr0;
+  uint64_t *frc = (L & 1) ? fr1 : fr0;        // level L
+  const uint64_t *frp = (L & 1) ? fr0 : fr1;  // level L-1
   const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
+  const bool level1 = strict && L == 1;
+  const bool sparse = lmode[L - 1] != 0;
   unsigned long long bytes = 0, fe = 0, full = 0, pairs = 0;
   const uint32_t nb = qoff[L], ne = min(ctl->tail, qcap);
   const int lane = lane_id();
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
-  uint32_t c_vis[NW], c_all[NW], c_er[NW];
+  uint32_t c_vis[NW], c_all[NW], c_er[NW], dag_tot[NW];
   uint64_t a_or[NW];
 #pragma unroll
-  for (int j = 0; j < NW; ++j) c_vis[j] = c_all[j] = c_er[j] = 0, a_or[j] = 0;
-  // a wave takes 64 entries at a time: per-entry words with lane = entry,
-  // then the per-root counts entry by entry with lane = root (x is set by
-  // the dependency pass, or by k_bleaf on the deepest level)
+  for (int j = 0; j < NW; ++j) c_vis[j] = c_all[j] = c_er[j] = dag_tot[j] = 0, a_or[j] = 0;
   for (uint32_t base = nb + gw * kWave; base < ne; base += nwv * kWave) {
     const uint32_t i = base + lane;
     const bool valid = i < ne;
-    uint32_t w = 0, rw = 0, dw = 0;
+    uint32_t w = 0, rw = 0, dw = 0, st = 0, d = 0;
     Words<NW> nw;
 #pragma unroll
     for (int j = 0; j < NW; ++j) nw.w[j] = 0;
@@ -1392,6 +1288,8 @@ __device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, co
       const Words<NW> vo = ldw<NW>(vis, w);
       rw = reach[w];
       dw = col[w + 1] - col[w];
+      st = rcol[w];
+      d = rcol[w + 1] - st;
       // direction bookkeeping: out-edges of the level, and in-edges of the
       // vertices every root has now reached
       bool was_full = true, is_full = true;
@@ -1401,7 +1299,7 @@ __device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, co
         is_full &= ((vo.w[j] | nw.w[j]) & all[j]) == all[j];
       }
       fe += dw;
-      if (is_full && !was_full) full += rcol[w + 1] - rcol[w];
+      if (is_full && !was_full) full += d;
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         frc[(size_t)w * NW + j] = nw.w[j];
@@ -1429,6 +1327,58 @@ __device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, co
     // entry: id 4 + NX 8NW + reach 4 + offsets 8 + seen 4; writes FR, VIS
     // (read+write), list word, NX: 5 * 8NW
     bytes += (unsigned long long)cntv * (20 + 48 * NW);
+    if (!counts) continue;
+    if (level1) {
+      // level 1 over strictly increasing lists: the frontier is the roots
+      // (sigma 1 each), and a new vertex has exactly one edge from every
+      // root in its word, so sigma_s(w) = 1 there without scanning its
+      // in-edges (a hub next to the roots would be scanned whole); the same
+      // DAG edges count
+      unsigned long long todo = __ballot(nw.any());
+      while (todo) {
+        const int k = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        const uint32_t wk = rdl(w, k);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const uint64_t mk = rdl(nw.w[j], k);
+          bytes += 4ull * (unsigned long long)__popcll(mk);
+          if (bit(mk, lane)) {
+            sig[(size_t)wk * B + lane + 64 * j] = 1u;
+            dag_tot[j] += 1;
+          }
+        }
+      }
+      continue;
+    }
+    const bool split = valid && d > kChunk;
+    claim_split_wave(ctl, split, i, split ? (d + kChunk - 1) / kChunk : 0u, sp_e, sp_base, sp_cap, item_cap);
+    bytes += 12ull * (unsigned long long)__popcll(__ballot(split));
+    Words<NW> snw = nw;  // the roots counted here (none for a split vertex)
+    if (split) {
+      d = 0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) snw.w[j] = 0;
+    }
+    uint32_t contrib = 0;
+    // sparse levels: the small vertices as (vertex, root) pairs
+    const bool small = sparse && d > 0 && d <= kSparseDeg;
+    if (sparse) contrib += sigma_pairs<NW>(small, w, snw, st, d, rrow, frp, sig, wslot, gw);
+    group_sigma<NW>(st, small ? 0u : d, snw, rrow, frp, sig, dag_tot, &contrib,
+                    [&](int o, const uint32_t *acc, const uint64_t *found) {
+                      const uint32_t wo = rdl(w, o);
+#pragma unroll
+                      for (int j = 0; j < NW; ++j)
+                        if (bit(found[j], lane)) sig[(size_t)wo * B + lane + 64 * j] = acc[j];
+                    });
+    // per vertex: pull offsets 8 + sigma write 4 per root; per edge: id 4 +
+    // frontier word 8NW; per term: sigma 4
+    uint32_t roots = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) roots += d ? (uint32_t)__popcll(snw.w[j]) : 0u;
+    const uint32_t dsum = (uint32_t)wave_sum_u64(d), rsum = (uint32_t)wave_sum_u64(roots);
+    const uint32_t nv = (uint32_t)__popcll(__ballot(valid && d > 0));
+    bytes += (unsigned long long)nv * 8 + (unsigned long long)dsum * (4 + 8 * NW) + 4ull * contrib + 4ull * rsum;
   }
   fe = wave_sum_u64(fe);
   full = wave_sum_u64(full);
@@ -1439,34 +1389,59 @@ __device__ __forceinline__ void bclose_body(const uint32_t *__restrict__ col, co
     for (int off = kWave / 2; off > 0; off >>= 1) a_or[j] |= __shfl_xor(a_or[j], off);
   }
   kb_add(kb, bytes);
-  // per-root counts into this wave's slots; level sums into its level slot
-  // (every wave writes its level slot, zero or not; k_bclose_end adds them)
+  // per-root counts into this wave's slots
   slot_add<NW>(wslot, 0, gw, c_vis);
   slot_add<NW>(wslot, 1, gw, c_all);
   slot_add<NW>(wslot, 2, gw, c_er);
+  if (counts && !level1 && sparse) {
+    // the pairs added to this slot atomically: stay atomic
+#pragma unroll
+    for (int j = 0; j < NW; ++j) slot_add_root(wslot, 3, gw, B, lane + 64 * j, dag_tot[j]);
+  } else {
+    slot_add<NW>(wslot, 3, gw, dag_tot);
+  }
+  // level sums: one slot per workgroup (every workgroup writes its slot,
+  // zero or not; k_bclose_end adds them)
+  __shared__ unsigned long long s_ls[kBlock / kWave][5];
+  const int wv = (int)threadIdx.x / kWave;
   if (lane == 0) {
-    unsigned long long *ls = lslot + kLevelSlot * (size_t)gw;
-    ls[0] = fe;
-    ls[1] = full;
-    ls[2] = a_or[0];
-    ls[3] = NW > 1 ? a_or[NW - 1] : 0ull;
-    ls[4] = pairs;
+    s_ls[wv][0] = fe;
+    s_ls[wv][1] = full;
+    s_ls[wv][2] = a_or[0];
+    s_ls[wv][3] = NW > 1 ? a_or[NW - 1] : 0ull;
+    s_ls[wv][4] = pairs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < (int)(blockDim.x / kWave); ++k) {
+      t[0] += s_ls[k][0];
+      t[1] += s_ls[k][1];
+      t[2] |= s_ls[k][2];
+      t[3] |= s_ls[k][3];
+      t[4] += s_ls[k][4];
+    }
+    unsigned long long *ls = lslot + kLevelSlot * (size_t)blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ls[k] = t[k];
   }
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_bclose(const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach,
-                                                   uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1,
-                                                   const uint32_t *__restrict__ q,
-                                                   uint64_t *qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
-                                                   unsigned long long *wslot, unsigned long long *lslot,
-                                                   uint32_t *seen, uint32_t qcap,
-                                                   const uint32_t *__restrict__ rcol, unsigned long long *kb) { bclose_body<NW>(col, reach, nx, vis, fr0, fr1, q, qw, qoff, ctl, wslot, lslot, seen, qcap, rcol, kb); }
-
+__global__ __launch_bounds__(kBlock) void k_bclose(
+    const uint32_t *__restrict__ col, const uint32_t *__restrict__ reach, const uint32_t *__restrict__ rcol,
+    const uint32_t *__restrict__ rrow, uint64_t *nx, uint64_t *vis, uint64_t *fr0, uint64_t *fr1, uint32_t *sig,
+    const uint32_t *__restrict__ q, uint64_t *qw, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
+    unsigned long long *wslot, unsigned long long *lslot, uint32_t *seen, uint32_t *sp_e, uint64_t *sp_base,
+    uint32_t sp_cap, uint64_t item_cap, uint32_t qcap, unsigned long long *kb, uint32_t kChunk, int strict,
+    const uint8_t *__restrict__ lmode) {
+  bclose_body<NW>(col, reach, rcol, rrow, nx, vis, fr0, fr1, sig, q, qw, qoff, ctl, wslot, lslot, seen, sp_e, sp_base,
+                  sp_cap, item_cap, qcap, kb, kChunk, strict, lmode);
+}
 
 // End of level L, one workgroup, after k_bclose in stream order (so every
 // slot of the level is written without fences): the level's sums from the
-// k_bclose wave slots, the level offsets, the done flag, and the direction
+// k_bclose workgroup slots, the level offsets, the done flag, and the direction
 // of the next level.
 __device__ __forceinline__ void bclose_end_body(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
                                                        const unsigned long long *__restrict__ lslot, uint64_t *act,
@@ -1535,7 +1510,7 @@ __device__ __forceinline__ void bclose_end_body(axbc::BCtl *ctl, uint32_t *qoff,
 
 __global__ __launch_bounds__(kBlock) void k_bclose_end(axbc::BCtl *ctl, uint32_t *qoff, uint32_t qcap, uint32_t pull16,
                                                        const unsigned long long *__restrict__ lslot, uint64_t *act,
-                                                       int nw, uint8_t *lmode) { bclose_end_body(ctl, qoff, qcap, pull16, lslot, act, nw, lmode, kCloseWaves); }
+                                                       int nw, uint8_t *lmode) { bclose_end_body(ctl, qoff, qcap, pull16, lslot, act, nw, lmode, kBGrid); }
 
 // Per-root counters of the batch: cnt[4s + k] += the sum of the wave slots
 // of kind k.  Workgroup (k, chunk) adds the slots of its chunk of waves for
@@ -2121,7 +2096,6 @@ __global__ __launch_bounds__(kBlock) void k_bfwd_persist(
     uint64_t item_cap, uint32_t qcap, uint32_t kChunk, int strict, uint8_t *lmode, uint32_t pull16,
     unsigned long long limit, uint32_t *bar, uint32_t *prog) {
   const uint32_t nblocks = gridDim.x;
-  const int nslots = (int)(gridDim.x * blockDim.x / kWave);
   for (;;) {
     if (ld_flag(&ctl->done)) return;
     if (ctl->bu)
@@ -2131,12 +2105,10 @@ __global__ __launch_bounds__(kBlock) void k_bfwd_persist(
       bpush_body<NW>(col, row, vis, nx, mark, fr0, fr1, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, item_cap, qcap,
                      nullptr, kChunk);
     if (!grid_sync(bar, nblocks, limit, prog)) return;
-    bsigma_body<NW>(rcol, rrow, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap,
-                    nullptr, kChunk, strict, lmode);
+    bclose_body<NW>(col, reach, rcol, rrow, nx, vis, fr0, fr1, sig, q, qw, qoff, ctl, wslot, lslot, seen, sp_e, sp_base,
+                    sp_cap, item_cap, qcap, nullptr, kChunk, strict, lmode);
     if (!grid_sync(bar, nblocks, limit, prog)) return;
-    bclose_body<NW>(col, reach, nx, vis, fr0, fr1, q, qw, qoff, ctl, wslot, lslot, seen, qcap, rcol, nullptr);
-    if (!grid_sync(bar, nblocks, limit, prog)) return;
-    if (blockIdx.x == 0) bclose_end_body(ctl, qoff, qcap, pull16, lslot, act, NW, lmode, nslots);
+    if (blockIdx.x == 0) bclose_end_body(ctl, qoff, qcap, pull16, lslot, act, NW, lmode, (int)nblocks);
     if (!grid_sync(bar, nblocks, limit, prog)) return;
   }
 }
@@ -2733,18 +2705,6 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
     kt_end(c, KK_PUSH_SPLIT);
     launch(k, k_bsplit_reset, dim3(1), dim3(64), b->ctl);
   }
-  kt_begin(c, KK_SIGMA);
-  launch(k, k_bsigma<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0], b->fr[1],
-                     b->sig, b->q, b->qoff, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows, b->qcap,
-                     kt_bytes(c, KK_SIGMA), b->chunk, c->pull_strict ? 1 : 0, (const uint8_t *)b->lmode);
-  kt_end(c, KK_SIGMA);
-  if (split_in) {
-    kt_begin(c, KK_SIGMA_SPLIT);
-    launch(k, k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->nx, b->fr[0],
-                       b->fr[1], b->sig, b->q, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
-                       kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
-    kt_end(c, KK_SIGMA_SPLIT);
-  }
   if (b->pull16) {
     if (!merged) {
       kt_begin(c, KK_PULL);
@@ -2761,13 +2721,20 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
       kt_end(c, KK_PULL_SPLIT);
     }
   }
+  // close and path counts (k_bclose_end adds up its per-workgroup level slots)
   kt_begin(c, KK_CLOSE);
-  // a quarter grid: 64 entries per wave, and its per-wave level slots are
-  // what k_bclose_end adds up
-  launch(k, k_bclose<NW>, dim3(kCloseGrid), dim3(kBlock), c->col, c->reach, b->nx, b->vis, b->fr[0],
-                     b->fr[1], b->q, b->qw, b->qoff, b->ctl, b->wslot, b->lslot, b->seen, b->qcap, c->rcol,
-                     kt_bytes(c, KK_CLOSE));
+  launch(k, k_bclose<NW>, dim3(kBGrid), dim3(kBlock), c->col, c->reach, c->rcol, c->rrow, b->nx, b->vis, b->fr[0],
+         b->fr[1], b->sig, b->q, b->qw, b->qoff, b->ctl, b->wslot, b->lslot, b->seen, b->sp_e, b->sp_base,
+         b->sp_cap, b->part_rows, b->qcap, kt_bytes(c, KK_CLOSE), b->chunk, c->pull_strict ? 1 : 0,
+         (const uint8_t *)b->lmode);
   kt_end(c, KK_CLOSE);
+  if (split_in) {
+    kt_begin(c, KK_SIGMA_SPLIT);
+    launch(k, k_bsigma_split<NW>, dim3(kBGrid), dim3(kBlock), c->rcol, c->rrow, b->qw, b->fr[0],
+                       b->fr[1], b->sig, b->q, b->ctl, b->wslot, b->sp_e, b->sp_base, b->sp_cap, b->part_rows,
+                       kt_bytes(c, KK_SIGMA_SPLIT), b->chunk);
+    kt_end(c, KK_SIGMA_SPLIT);
+  }
   launch(k, k_bclose_end, dim3(1), dim3(kBlock), b->ctl, b->qoff, b->qcap, b->pull16, (const unsigned long long *)b->lslot,
          b->act, NW, b->lmode);
 }

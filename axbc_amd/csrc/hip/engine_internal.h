@@ -59,8 +59,8 @@ struct BatchState;  // batch.hip
 // accumulated by the kernels themselves (counted only while enabled).
 enum KernelKind {
   KK_PUSH = 0,      // batched forward marking (k_bpush)
-  KK_SIGMA = 1,     // batched path-count pull (k_bsigma)
-  KK_CLOSE = 2,     // batched level close (k_bclose)
+  KK_SIGMA = 1,     // batched path-count pull (now inside k_bclose; no launches)
+  KK_CLOSE = 2,     // batched level close and path counts (k_bclose)
   KK_DEP = 3,       // batched dependency pull (k_bdep)
   KK_UPDATE = 4,    // batched bc update (k_bupdate)
   KK_FWD = 5,       // plain forward expansion (k_fwd_expand)
