@@ -102,9 +102,9 @@ struct BCtl {
   int level;                       // forward level the next kernels run
   uint32_t mark_base;              // level stamps of `mark`: mark_base + level
   uint32_t stamp;                  // batch stamp of `seen`
-  int depth;                       // backward depth the next kernels run
+  int depth;                       // backward depth the dependency kernels run (set by k_bstrip)
   int dmin;                        // last backward depth (0 only for single-level roots)
-  uint32_t pad2;
+  int sdepth;                      // backward depth the next k_bstrip runs (set by k_bdep)
   uint32_t pad3;
   unsigned long long all[2];       // the batch's roots per word
   unsigned long long rootmask[2];  // roots whose deepest level is 1 (depth 0)
@@ -1600,7 +1600,18 @@ __device__ __forceinline__ void bstrip_body(uint64_t *vis, const uint32_t *__res
 
 template <int NW>
 __global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const uint64_t *__restrict__ qw,
-                         const uint32_t *__restrict__ qoff, const axbc::BCtl *ctl) { bstrip_body<NW>(vis, q, qw, qoff, ctl, ctl->depth); }
+                         const uint32_t *__restrict__ qoff, axbc::BCtl *ctl) {
+  // a depth step is this kernel, k_bdep and the split kernels: the step's
+  // depth comes from k_bdep of the step before (sdepth), and is handed to
+  // the kernels after this one (depth) with the split claims cleared -- no
+  // kernel reads the field that the kernel before it writes
+  const int d = ctl->sdepth;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->depth = d;
+    ctl->split_claim = 0;
+  }
+  bstrip_body<NW>(vis, q, qw, qoff, ctl, d);
+}
 
 // Dependencies of one vertex over adj[cb, ce), per root lane, left fold in
 // adjacency order into acc.
@@ -1842,7 +1853,12 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
                                                  unsigned long long *kb, uint32_t kChunk,
-                                                 const uint8_t *__restrict__ lmode) { bdep_body<NW>(col, row, vis, sig, dlt, xr, reach, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, part_rows, kb, kChunk, lmode, ctl->depth); }
+                                                 const uint8_t *__restrict__ lmode) {
+  const int d = ctl->depth;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->sdepth = d >= ctl->dmin ? d - 1 : d;  // the next step's
+  bdep_body<NW>(col, row, vis, sig, dlt, xr, reach, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, part_rows, kb, kChunk,
+                lmode, d);
+}
 
 // Chunks of the split entries: per item the partial sums of its kChunk
 // edges (left fold from 0.0) go to its row of `part`.
@@ -1977,18 +1993,11 @@ __global__ void k_bsplit_reset(axbc::BCtl *ctl) {
   }
 }
 
-// End of a backward depth step: next depth.
-__global__ void k_bstep_down(axbc::BCtl *ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctl->split_claim = 0;
-    if (ctl->depth >= ctl->dmin) ctl->depth -= 1;
-  }
-}
-
 // Backward sweep set-up: depths dmax-1 .. dmin.
 __global__ void k_bset_depth(axbc::BCtl *ctl, int first, int last, uint64_t rootmask0, uint64_t rootmask1) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->depth = first;
+    ctl->sdepth = first;
     ctl->dmin = last;
     ctl->rootmask[0] = rootmask0;
     ctl->rootmask[1] = rootmask1;
@@ -2739,8 +2748,9 @@ void enqueue_level(Sink &k, axbc_ctx *c, BatchState *b, bool split_out, bool spl
          b->act, NW, b->lmode);
 }
 
-// One backward depth step (the depth comes from the device; the last
-// kernel moves it down).  Before depth d the level d+1 bits leave VIS.
+// One backward depth step (the depth comes from the device: k_bstrip takes
+// it from k_bdep of the step before).  Before depth d the level d+1 bits
+// leave VIS.
 template <int NW>
 void enqueue_depth(Sink &k, axbc_ctx *c, BatchState *b, bool split_out) {
   launch(k, k_bstrip<NW>, dim3(kBGrid / 2), dim3(kBlock), b->vis, b->q, b->qw, b->qoff, b->ctl);
@@ -2761,7 +2771,6 @@ void enqueue_depth(Sink &k, axbc_ctx *c, BatchState *b, bool split_out) {
                        kt_bytes(c, KK_DEP_COMBINE), b->chunk);
     kt_end(c, KK_DEP_COMBINE);
   }
-  launch(k, k_bstep_down, dim3(1), dim3(64), b->ctl);
 }
 
 void graphs_free(BatchState *b) {
