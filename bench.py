@@ -86,9 +86,10 @@ def log_kernels(kstats):
             log(f"  {KIND_NAMES[k]}: {n} launches, {ms:.2f} ms, avg {ms / n * 1e3:.1f} us{rate}")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, batch, chunk, pull):
     """HBM bytes per launch of `kernel` from the committed counter summary,
-    or None when the kernel was not measured."""
+    or None when the kernel was not measured at this batch width, split
+    threshold and pull ratio."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -96,6 +97,8 @@ def measured_traffic(kernel):
         return None
     k = t.get("kernels", {}).get(kernel)
     if not k or t.get("workload") != WORKLOAD["workload"]:
+        return None
+    if (t.get("batch_width"), t.get("chunk"), t.get("pull")) != (batch, chunk, pull):
         return None
     return int(k["hbm_bytes_per_launch"])
 
@@ -134,6 +137,10 @@ def grid_case(args):
     g = Graph.lattice(4096, 0.03, seed=3, heuristic=0)
     log(f"lattice: N={g.N} ned={g.ned} built in {time.perf_counter() - t:.1f}s")
     e = Engine(g)
+    if args.no_graphs:
+        from axbc_amd import _ffi
+        if _ffi.hip().axbc_set_batch_graphs(e.ctx, 0) != 0:
+            raise RuntimeError(_ffi.hip().axbc_last_error())
     out = {"case": "grid-4096-del3-c0", "n": g.N, "ned": g.ned}
 
     def timed(loop, steps):
@@ -333,9 +340,9 @@ def main():
     if not batch:
         from axbc_amd.autotune import tune
         rng = np.random.default_rng(0)
-        sample = rng.choice(np.flatnonzero(np.diff(g.col) > 0), 128 * 3, replace=False)
+        sample = rng.choice(np.flatnonzero(np.diff(g.col) > 0), 128 * 4, replace=False)
         t = time.perf_counter()
-        tuned = tune(e, sample, reps=1)
+        tuned = tune(e, sample, reps=3)  # 3 timed batches per setting: one batch alone is too noisy to rank them
         batch = tuned["width"]
         log(f"autotune {time.perf_counter() - t:.1f}s: width {tuned['width']} chunk {tuned['chunk']} "
             f"pull {tuned['pull']} ({tuned['s_per_root'] * 1e3:.3f} ms/root)")
@@ -347,6 +354,10 @@ def main():
             raise RuntimeError(L.axbc_last_error())
         if args.pull >= 0 and L.axbc_set_batch_pull(e.ctx, args.pull) != 0:
             raise RuntimeError(L.axbc_last_error())
+
+    # the split threshold and pull ratio in force (the engine's defaults are 256 and 32)
+    cur_chunk = tuned["chunk"] if tuned else (args.chunk or 256)
+    cur_pull = tuned["pull"] if tuned else (args.pull if args.pull >= 0 else 32)
 
     # ---- batched pivot loop ---------------------------------------------
     # the warmup batches run with events on every kernel (the per-kernel
@@ -390,7 +401,8 @@ def main():
     ms, n, b = kstats[dom]
     achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(KIND_NAMES[dom]),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(KIND_NAMES[dom], batch, cur_chunk, cur_pull),
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(ms / max(n, 1) * 1e3, 2), "launches": n,
                 "algorithmic_bytes_per_launch": int(b / max(n, 1)),
                 "kernel_ms_share": round(wstats[dom][0] / max(warm_ms, 1e-9), 3)}
