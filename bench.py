@@ -296,7 +296,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
                     help="serving case instead: Poisson 20 req/s, SCALE 12-20, ef 8/16/32, budgets 64-2048, -z 6, "
                          "200 GB cap; prints p50/p99 latency and aggregate pivots/s")
-    ap.add_argument("--slots", type=int, default=16, help="device workers of --serve")
+    ap.add_argument("--slots", type=int, default=8,
+                    help="device workers of --serve (two streams per hardware queue at HIP's 4 queues per process)")
     ap.add_argument("--learned", action="store_true",
                     help="learned-proposal case instead: accuracy vs exact Brandes on SCALE 18 (learned vs uniform at "
                          "4096 pivots) and the learned loop's throughput on SCALE 24")
