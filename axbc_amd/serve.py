@@ -29,7 +29,18 @@ from .graph import Graph
 from .pivots import PivotLoop
 
 PLAIN_STATE_BYTES = 4 + 4 + 8 + 8 + 4 + 4 + 4  # lvl, sigma, delta, bc, queue, qoff, reach per vertex
+# device scratch of building a graph on the device (generator output, mirrored keys and three sort/select
+# buffers of 8 B per mirrored edge, flags, radix-sort temporaries), per generated edge and per vertex
+BUILD_BYTES_PER_EDGE = 112
+BUILD_BYTES_PER_VERTEX = 48
 SAMPLER_BYTES = 28 + 24 + 8                    # sampler scratch, LCC tables, slack per vertex
+
+
+def build_bytes(scale: int, edgefactor: int) -> int:
+    """Upper bound of the device scratch of Graph.kronecker_device, held
+    while the graph is built (freed before the job is admitted)."""
+    n = 1 << scale
+    return n * edgefactor * BUILD_BYTES_PER_EDGE + n * BUILD_BYTES_PER_VERTEX
 
 
 def footprint_bytes(N: int, ned: int, batch: int = 0, col: np.ndarray | None = None,
@@ -84,15 +95,18 @@ class Job:
 
 class Server:
     """slots device workers under a device memory cap of hbm_cap bytes.
-    engine_factory(graph, device) makes a job's engine (an Engine) and
-    runner(engine, job) runs its pivot loop; tests pass stand-ins to exercise
-    admission without a GPU."""
+    engine_factory(graph, device) makes a job's engine (an Engine),
+    runner(engine, job) runs its pivot loop and builder(job, device) builds a
+    job's graph on the device; tests pass stand-ins to exercise admission
+    without a GPU.  A device build holds its scratch (build_bytes) under the
+    same cap while it runs."""
 
     def __init__(self, slots: int = 4, hbm_cap: float = 200e9, device: int = 0, build_threads: int = 4,
-                 keep_bc: bool = True, engine_factory=Engine, runner=None):
+                 keep_bc: bool = True, engine_factory=Engine, runner=None, builder=None):
         self.slots, self.cap, self.device, self.keep_bc = slots, hbm_cap, device, keep_bc
         self._factory = engine_factory
         self._runner = runner or self._pivot_loop
+        self._builder_fn = builder or self._kronecker
         self._ready: list[Job] = []      # built, waiting for admission (arrival order)
         self._lock = threading.Condition()
         self._used = 0
@@ -131,13 +145,30 @@ class Server:
         self._builder.shutdown(wait=True)
 
     # -- internals ----------------------------------------------------------
+    def _kronecker(self, job: Job, device: int) -> Graph:
+        return Graph.kronecker_device(job.scale, job.edgefactor, seed=job.seed, heuristic=job.heuristic,
+                                      device=device)
+
     def _build(self, job: Job) -> None:
+        scratch = build_bytes(job.scale, job.edgefactor) if job.graph is None else 0
+        if scratch > self.cap:
+            job.error = f"graph build needs {scratch} B of device scratch, above the memory cap {int(self.cap)} B"
+            scratch = 0
+        elif scratch:
+            with self._lock:  # the build's scratch counts against the cap while it runs
+                while self._used + scratch > self.cap and not self._stop:
+                    self._lock.wait(0.05)
+                self._used += scratch
         try:
-            if job.graph is None:  # host generator, preprocessing on the device
-                job.graph = Graph.kronecker_device(job.scale, job.edgefactor, seed=job.seed,
-                                                   heuristic=job.heuristic, device=self.device)
+            if job.graph is None and not job.error:  # generated and preprocessed on the device
+                job.graph = self._builder_fn(job, self.device)
         except Exception as ex:  # the job fails, the server goes on
             job.error = f"graph build failed: {type(ex).__name__}: {ex}"
+        finally:
+            if scratch:
+                with self._lock:
+                    self._used -= scratch
+                    self._lock.notify_all()
         with self._lock:
             self._ready.append(job)
             self._lock.notify_all()

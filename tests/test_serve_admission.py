@@ -147,3 +147,35 @@ def test_poisson_jobs_distribution():
     b = np.array([j.budget for _, j in js])
     assert b.min() >= 64 and b.max() <= 2048
     assert {j.edgefactor for _, j in js} == {8, 16, 32}
+
+
+def test_device_builds_hold_their_scratch_under_the_cap():
+    """Graph builds on the device reserve build_bytes while they run: with
+    a cap for one build at a time, builds never overlap, and a build larger
+    than the cap fails at once."""
+    from axbc_amd.serve import build_bytes
+    ledger = Ledger()
+    live = []
+    peak = [0]
+    lock = threading.Lock()
+
+    def builder(job, device):
+        with lock:
+            live.append(job.jid)
+            peak[0] = max(peak[0], len(live))
+        time.sleep(0.02)
+        with lock:
+            live.remove(job.jid)
+        return FakeGraph(1 << job.scale, 16 << job.scale)
+
+    cap = int(1.5 * build_bytes(12, 16))
+    srv = Server(slots=2, hbm_cap=cap, build_threads=4, keep_bc=False,
+                 engine_factory=lambda g, d: FakeEngine(ledger, g, d), runner=runner, builder=builder)
+    for i in range(6):
+        srv.submit(Job(jid=i, budget=5, scale=12, edgefactor=16))
+    srv.submit(Job(jid=99, budget=5, scale=24, edgefactor=32))
+    done = {d.jid: d for d in srv.drain(timeout=30)}
+    srv.close()
+    assert all(not done[i].error for i in range(6)), [done[i].error for i in range(6)]
+    assert peak[0] == 1
+    assert "above the memory cap" in done[99].error
