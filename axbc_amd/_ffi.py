@@ -103,10 +103,11 @@ def hip() -> C.CDLL:
         "axbc_set_batch_persistent": (C.c_int, [ctxp, C.c_int, C.c_double]),
         "axbc_set_memory_budget": (C.c_int, [ctxp, C.c_size_t]),
         "axbc_device_bytes": (C.c_int, [ctxp, C.POINTER(C.c_size_t)]),
+        "axbc_draw_window": (C.c_int, [ctxp, C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p]),
         "axbc_batch_state_bytes": (C.c_int, [C.c_uint64, C.c_uint64, u32p, C.c_int, C.POINTER(C.c_size_t)]),
         "axbc_batch_delta": (C.c_int, [ctxp, C.c_int, f64p]),
         "axbc_batch_levels": (C.c_int, [ctxp, C.c_int, i32p, u32p]),
-        "axbc_hdn_build": (C.c_int, [ctxp, u32p, u32p, C.POINTER(C.c_uint32)]),
+        "axbc_hdn_build": (C.c_int, [ctxp, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
         "axbc_hdn_pick": (C.c_int, [ctxp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "axbc_weights": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_uint64)]),
         "axbc_weights_pick": (C.c_int, [ctxp, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),

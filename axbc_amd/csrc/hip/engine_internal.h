@@ -50,6 +50,11 @@ struct SamplerBufs {
   uint64_t ltotal = 0;
   int lcc_ready = 0;
   int lcc_sym = -1;  // CSC symmetric, loop-free, strictly sorted: 1 yes, 0 no, -1 not checked
+  // draw windows (axbc_draw_window): their own stream, so that a window is
+  // searched while a batch runs on the context stream
+  hipStream_t wstream = nullptr;
+  uint32_t *win = nullptr;  // 2 x win_cap: lrand48 values, picked vertices
+  uint32_t win_cap = 0;
 };
 
 struct BatchState;  // batch.hip

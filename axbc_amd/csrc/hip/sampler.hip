@@ -14,6 +14,8 @@
 //    A draw r in [0, total] picks the first v with prefix(v) >= r.
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -446,6 +448,59 @@ __global__ void k_first_geq(const T *__restrict__ P, uint32_t n, T r, uint32_t *
   *out = lo < n ? lo : n - 1;
 }
 
+// Draw windows: lrand48 at positions 0..count-1 ahead of state X (each
+// value advances X = (0x5DEECE66D X + 0xB) mod 2^48 and is X >> 17; the map
+// to position p is the affine map raised to p + 1 by squaring, in 64-bit
+// arithmetic -- exact mod 2^48), and the vertex a first draw there picks:
+// table 1, r = lr % (total + 1) as uint32 (uniform_int(total, 0), the raw
+// value when total + 1 wraps to 0) and the first i with the running maximum
+// of the wrapped deg^2 prefix >= r; table 2, r = lr / 2147483647 * total
+// (uniform_double, clamped to total) and the first i with the LCC weight
+// prefix >= r; n-1 when none.
+constexpr uint64_t kLcgA = 0x5DEECE66Dull, kLcgC = 0xBull, kLcgMask = (1ull << 48) - 1;
+
+__global__ void k_draw_window(int table, uint64_t state, uint64_t total, uint32_t count,
+                              const uint32_t *__restrict__ pmax, const uint64_t *__restrict__ lp, uint32_t n,
+                              uint32_t *lr_out, uint32_t *v_out) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
+    uint64_t A = 1, Cc = 0, ap = kLcgA, cp = kLcgC;
+    for (uint64_t e = (uint64_t)p + 1; e; e >>= 1) {
+      if (e & 1) {
+        A = ap * A;
+        Cc = ap * Cc + cp;
+      }
+      cp = ap * cp + cp;
+      ap = ap * ap;
+    }
+    const uint64_t x = (A * state + Cc) & kLcgMask;
+    const uint32_t lr = (uint32_t)(x >> 17);
+    uint32_t lo = 0, hi = n;
+    if (table == 1) {
+      const uint32_t m = (uint32_t)total + 1u;
+      const uint32_t r = m ? lr % m : lr;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (pmax[mid] >= r)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+    } else {
+      const double rd = (double)lr / 2147483647.0 * (double)total;
+      const uint64_t r = rd >= (double)total ? total : (uint64_t)rd;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (lp[mid] >= r)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+    }
+    lr_out[p] = lr;
+    v_out[p] = lo < n ? lo : n - 1;
+  }
+}
+
 int grid_n(uint64_t n) {
   uint64_t g = (n + 255) / 256;
   return (int)(g < 1 ? 1 : g > 4096 ? 4096 : g);
@@ -494,6 +549,14 @@ void sampler_free(axbc_ctx *c) {
     if (p) (void)hipFree(p);
   if (sb->pick_host) (void)hipHostFree(sb->pick_host);
   if (sb->host64) (void)hipHostFree(sb->host64);
+  if (sb->wstream) {
+    (void)hipStreamSynchronize(sb->wstream);
+    (void)hipStreamDestroy(sb->wstream);
+  }
+  if (sb->win) {
+    (void)hipFree(sb->win);
+    c->bytes -= std::min<size_t>(c->bytes, 8ull * sb->win_cap);
+  }
   delete sb;
   c->sampler = nullptr;
 }
@@ -542,6 +605,40 @@ int axbc_hdn_pick(axbc_ctx *c, uint32_t r, uint32_t *u, uint32_t *sdn_u) {
     HIP_TRY(hipStreamSynchronize(s));
     *sdn_u = *sb->pick_host;
   }
+  return AXBC_OK;
+}
+
+int axbc_draw_window(axbc_ctx *c, int table, uint64_t state, uint64_t total, uint32_t count, uint32_t *lr,
+                     uint32_t *v) {
+  SamplerBufs *sb = nullptr;
+  int rc = sampler_bufs(c, &sb);
+  if (rc) return rc;
+  if ((table != 1 && table != 2) || !lr || !v || count == 0 || count > (1u << 20))
+    return fail(AXBC_EINVAL, "table 1 or 2, 1..2^20 draws%s (line %d)", "", __LINE__);
+  if ((table == 1 && !sb->hdn_ready) || (table == 2 && !sb->lcc_ready))
+    return fail(AXBC_ESTATE, "tables not built%s (line %d)", "", __LINE__);
+  if (state >> 48) return fail(AXBC_EINVAL, "a 48-bit generator state%s (line %d)", "", __LINE__);
+  HIP_TRY(hipSetDevice(c->device));
+  if (!sb->wstream) HIP_TRY(hipStreamCreateWithFlags(&sb->wstream, hipStreamNonBlocking));
+  if (count > sb->win_cap) {
+    const uint32_t cap = std::max<uint32_t>(count, 1024);
+    const size_t old = 8ull * sb->win_cap;
+    if ((rc = budget_check(c, 8ull * cap - old, "the draw window"))) return rc;
+    if (sb->win) HIP_TRY(hipFree(sb->win));
+    sb->win = nullptr;
+    sb->win_cap = 0;
+    c->bytes -= std::min<size_t>(c->bytes, old);
+    HIP_TRY(hipMalloc((void **)&sb->win, 8ull * cap));
+    sb->win_cap = cap;
+    c->bytes += 8ull * cap;
+  }
+  hipStream_t s = sb->wstream;
+  hipLaunchKernelGGL(k_draw_window, dim3(grid_n(count)), dim3(256), 0, s, table, state, total, count,
+                     (const uint32_t *)sb->u32b, (const uint64_t *)sb->lp, c->n, sb->win, sb->win + sb->win_cap);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(lr, sb->win, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(v, sb->win + sb->win_cap, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return AXBC_OK;
 }
 

@@ -112,32 +112,62 @@ static int device_generate(int scale, int edgef, uint64_t s1, uint64_t s2, int u
   return rc;
 }
 
+/* srand48 / lrand48 of the reference's loop, on a state of its own
+ * (nrand48: the same generator), so that the device can compute values
+ * ahead of it; pos counts the values drawn. */
+static unsigned short g_rng[3];
+static uint64_t g_rng_pos;
+static void drv_srand48(long seed) {
+  g_rng[0] = 0x330E;
+  g_rng[1] = (unsigned short)(seed & 0xFFFF);
+  g_rng[2] = (unsigned short)((seed >> 16) & 0xFFFF);
+  g_rng_pos = 0;
+}
+static long drv_lrand48(void) {
+  ++g_rng_pos;
+  return nrand48(g_rng);
+}
+static uint64_t drv_state48(void) {
+  return (uint64_t)g_rng[0] | ((uint64_t)g_rng[1] << 16) | ((uint64_t)g_rng[2] << 32);
+}
+
 /* uniform_int(max, min) = lrand48() % (max - min + 1) + min in uint32
  * (src/axbc2d.c:2156-2164); an empty modulus uses the raw draw. */
 static uint32_t uniform_int(uint32_t max, uint32_t min) {
   uint32_t m = max - min + 1u;
-  long r = lrand48();
+  long r = drv_lrand48();
   return m ? (uint32_t)((uint32_t)r % m) + min : (uint32_t)r + min;
 }
 
 /* r = lrand48()/RAND_MAX * total (uniform_double, src/axbc2d.c:2167-2172)
  * on the fixed-point scale of the weights, clamped to total */
 static uint64_t fixed_draw(uint64_t total) {
-  double rd = (double)lrand48() / 2147483647.0 * (double)total;
+  double rd = (double)drv_lrand48() / 2147483647.0 * (double)total;
   return rd >= (double)total ? total : (uint64_t)rd;
 }
 
-/* first i with pmax[i] >= r; N-1 when none (findsample_LOCINT + fallback) */
-static uint32_t first_geq(const uint32_t *pmax, uint32_t n, uint32_t r) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = lo + (hi - lo) / 2;
-    if (pmax[mid] >= r)
-      hi = mid;
-    else
-      lo = mid + 1;
+/* The vertex a first draw of table 1 (-z 1, over the deg^2 prefix) or 2
+ * (-z 2, over the LCC prefix) picks: searched on the device a window of
+ * generator positions at a time (axbc_draw_window); consumes the draw and
+ * checks it against the device's value. */
+#define DRAW_WINDOW 1024
+typedef struct {
+  uint64_t base;
+  uint32_t count;
+  uint32_t lr[DRAW_WINDOW], v[DRAW_WINDOW];
+} draw_window;
+static draw_window g_win[3];
+
+static uint32_t window_pick(axbc_ctx *ctx, int table, uint64_t total) {
+  draw_window *w = &g_win[table];
+  if (!w->count || g_rng_pos < w->base || g_rng_pos >= w->base + w->count) {
+    check(axbc_draw_window(ctx, table, drv_state48(), total, DRAW_WINDOW, w->lr, w->v), "axbc_draw_window");
+    w->base = g_rng_pos;
+    w->count = DRAW_WINDOW;
   }
-  return lo < n ? lo : n - 1;
+  const uint64_t k = g_rng_pos - w->base;
+  if ((uint32_t)drv_lrand48() != w->lr[k]) die("%s", "device draw window out of step with the generator");
+  return w->v[k];
 }
 
 /* -d: the generated edge list as the reference dumps it (dump_rmat,
@@ -374,7 +404,7 @@ int main(int argc, char **argv) {
   printf("Device memory allocated (MB): %zu\n", dev_bytes >> 20);
 
   /* ---- pivot loop (src/axbc2d.c:2700-3156) --------------------------------- */
-  srand48(random_mode == 1 ? (long)(getpid() + time(NULL)) : DEFAULT_SEED);
+  drv_srand48(random_mode == 1 ? (long)(getpid() + time(NULL)) : DEFAULT_SEED);
   uint32_t *approx = (uint32_t *)calloc(approx_n ? approx_n : 1, sizeof(uint32_t));
   for (uint32_t i = 0; i < approx_n; ++i) approx[i] = uniform_int((uint32_t)N, 0);
   if (bader > 0) {
@@ -385,8 +415,8 @@ int main(int argc, char **argv) {
     printf("Stop computation after %" PRIu64 " good samples\n", nbfs);
   }
   check(axbc_set_checks(ctx, approx, approx_n), "axbc_set_checks");
-  uint32_t *pmax = NULL, *sdn = NULL, sum_deg = 0;
-  uint64_t *lw = NULL, *lp = NULL, ltotal = 0;
+  uint32_t *sdn = NULL, sum_deg = 0;
+  uint64_t *lw = NULL, ltotal = 0;
   uint32_t dist_mix = (uint32_t)distribution;
   if (distribution == 0) {
     (void)uniform_int((uint32_t)N, 0);
@@ -401,14 +431,12 @@ int main(int argc, char **argv) {
   if (distribution == 2 || distribution == 6) {
     double tl = now_s();
     lw = (uint64_t *)malloc(sizeof(uint64_t) * N);
-    lp = (uint64_t *)malloc(sizeof(uint64_t) * N);
-    check(axbc_lcc_build(ctx, NULL, lw, lp, &ltotal), "axbc_lcc_build");
+    check(axbc_lcc_build(ctx, NULL, lw, NULL, &ltotal), "axbc_lcc_build");
     printf("LCC done in %f secs\n", now_s() - tl);
   }
   if (distribution == 1 || distribution == 6) {
-    pmax = (uint32_t *)malloc(sizeof(uint32_t) * N);
     sdn = (uint32_t *)malloc(sizeof(uint32_t) * N);
-    check(axbc_hdn_build(ctx, pmax, sdn, &sum_deg), "axbc_hdn_build");
+    check(axbc_hdn_build(ctx, NULL, sdn, &sum_deg), "axbc_hdn_build");
   }
   nbfs = nbfs < N ? nbfs : N;
   const uint32_t lim = (uint32_t)(N * N); /* int LIMxSAMPLE = N*N vs uint32 BSM */
@@ -450,8 +478,7 @@ int main(int argc, char **argv) {
         v0 = uniform_int((uint32_t)N, 0);
         p = 1.0 / (double)N;
       } else if (k == 1) {
-        uint32_t r = uniform_int(sum_deg, 0);
-        uint32_t u = first_geq(pmax, (uint32_t)N, r);
+        uint32_t u = window_pick(ctx, 1, sum_deg);
         uint32_t d = g.col[u + 1] - g.col[u];
         if (d > 0) {
           uint32_t nb = d == 1 ? 0 : uniform_int(d - 1, 0);
@@ -460,16 +487,7 @@ int main(int argc, char **argv) {
         v0 = u;
         p = (double)sdn[v0] / (double)sum_deg;
       } else if (k == 2) {
-        uint64_t r = fixed_draw(ltotal);
-        uint32_t lo = 0, hi = (uint32_t)N;
-        while (lo < hi) {
-          uint32_t mid = lo + (hi - lo) / 2;
-          if (lp[mid] >= r)
-            hi = mid;
-          else
-            lo = mid + 1;
-        }
-        v0 = lo < N ? lo : (uint32_t)N - 1;
+        v0 = window_pick(ctx, 2, ltotal);
         p = (double)lw[v0] / (double)ltotal;
       } else {
         if (!wready || distribution == 6) {
@@ -597,10 +615,8 @@ int main(int argc, char **argv) {
   free(bc);
   free(vs);
   free(approx);
-  free(pmax);
   free(sdn);
   free(lw);
-  free(lp);
   free(pending);
   free(bv);
   free(bp);

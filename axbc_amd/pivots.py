@@ -35,9 +35,23 @@ class Rand48:
     def __init__(self, seed: int):
         s = seed & 0xFFFFFFFF
         self.x = (C.c_ushort * 3)(0x330E, s & 0xFFFF, (s >> 16) & 0xFFFF)
+        self.pos = 0  # values drawn so far
 
     def lrand48(self) -> int:
+        self.pos += 1
         return int(_libc.nrand48(C.byref(self.x)))
+
+    def state48(self) -> int:
+        """The 48-bit state X (the next value is of the advanced X)."""
+        return self.x[0] | (self.x[1] << 16) | (self.x[2] << 32)
+
+    def snapshot(self):
+        return tuple(self.x), self.pos
+
+    def restore(self, snap) -> None:
+        x, self.pos = snap
+        for j, v in enumerate(x):
+            self.x[j] = v
 
     def uniform_int(self, mx: int, mn: int = 0) -> int:
         """lrand48() % (max - min + 1) + min in uint32 (src/axbc2d.c:2156-2164)."""
@@ -90,19 +104,17 @@ class PivotLoop:
             uniform_int(self.N)  # discarded initial draw (axbc2d.c:2740)
         if z == 6:
             self.mix = uniform_int(4)  # initial mixture component (axbc2d.c:2798)
+        # the tables and their prefix stay on the device (draws are searched
+        # there); the host keeps the per-vertex weights for the probabilities
         if z in (2, 6):
-            self.lcc = np.empty(self.N, np.float32)
             self.lw = np.empty(self.N, np.uint64)
-            self.lp = np.empty(self.N, np.uint64)
             t = C.c_uint64(0)
-            _check(_ffi.hip().axbc_lcc_build(engine.ctx, self.lcc.ctypes.data, self.lw.ctypes.data,
-                                             self.lp.ctypes.data, C.byref(t)))
+            _check(_ffi.hip().axbc_lcc_build(engine.ctx, None, self.lw.ctypes.data, None, C.byref(t)))
             self.ltotal = int(t.value)
         if z in (1, 6):
-            self.pmax = np.empty(self.N, np.uint32)
             self.sdn = np.empty(self.N, np.uint32)
             s = C.c_uint32(0)
-            _check(_ffi.hip().axbc_hdn_build(engine.ctx, self.pmax, self.sdn, C.byref(s)))
+            _check(_ffi.hip().axbc_hdn_build(engine.ctx, None, self.sdn.ctypes.data, C.byref(s)))
             self.sum_deg = int(s.value)
         self.lim = (self.N * self.N) & 0xFFFFFFFF
         self.vs = np.zeros(self.N, np.uint8)
@@ -110,8 +122,28 @@ class PivotLoop:
         self.res = LoopResult()
         self._marks = None  # vertices marked in vs while drawing ahead
         self._ahead = None  # roots drawn during the previous batch
+        self._win = {}      # table -> (first generator position, lrand48 values, picked vertices)
 
     # -- sampling ---------------------------------------------------------
+    WINDOW = 1024  # draws searched on the device at a time
+
+    def _window_pick(self, table: int, total: int) -> int:
+        """The vertex a first draw of table 1 (-z 1) or 2 (-z 2) picks at the
+        generator's next position, searched on the device a window of
+        positions at a time (axbc_draw_window); consumes the draw."""
+        pos = self.rng.pos
+        w = self._win.get(table)
+        if w is None or not (w[0] <= pos < w[0] + len(w[1])):
+            lr = np.empty(self.WINDOW, np.uint32)
+            v = np.empty(self.WINDOW, np.uint32)
+            _check(_ffi.hip().axbc_draw_window(self.e.ctx, table, self.rng.state48(), int(total), self.WINDOW, lr, v))
+            w = (pos, lr, v)
+            self._win[table] = w
+        k = pos - w[0]
+        if self.rng.lrand48() != int(w[1][k]):
+            raise RuntimeError("device draw window out of step with the generator")
+        return int(w[2][k])
+
     def _fixed_draw(self, total: int) -> int:
         """r = lrand48()/RAND_MAX * total (uniform_double), clamped."""
         rd = float(self.rng.lrand48()) / 2147483647.0 * float(total)
@@ -166,16 +198,10 @@ class PivotLoop:
             _check(_ffi.hip().axbc_weights_pick(self.e.ctx, r, C.byref(v), C.byref(p)))
             return int(v.value), float(p.value)
         if z == 2:
-            r = self._fixed_draw(self.ltotal)
-            u = int(np.searchsorted(self.lp, np.uint64(r), side="left"))
-            if u >= N:
-                u = N - 1
+            u = self._window_pick(2, self.ltotal)
             return u, float(self.lw[u]) / float(self.ltotal)
         if z == 1:
-            r = uniform_int(self.sum_deg)
-            u = int(np.searchsorted(self.pmax, np.uint32(r), side="left"))
-            if u >= N:
-                u = N - 1
+            u = self._window_pick(1, self.sum_deg)
             d = int(self.deg[u])
             if d > 0:
                 nb = 0 if d == 1 else uniform_int(d - 1)
@@ -239,7 +265,7 @@ class PivotLoop:
         """Start drawing the next `want` roots: (thread, box, want, state
         before).  The thread leaves the roots, the counts after them and the
         marks it set in box."""
-        snap = (tuple(self.rng.x), self.ui, self.res.skipped)
+        snap = (self.rng.snapshot(), self.ui, self.res.skipped)
         box = {}
 
         def work():
@@ -276,11 +302,10 @@ class PivotLoop:
         """Discard the roots drawn ahead and restore the sampler state."""
         if self._ahead is None:
             return
-        t, box, _, (x, ui, skipped) = self._ahead
+        t, box, _, (rs, ui, skipped) = self._ahead
         t.join()
         self._ahead = None
-        for j, v in enumerate(x):
-            self.rng.x[j] = v
+        self.rng.restore(rs)
         self.ui, self.res.skipped = ui, skipped
         for v in box.get("marks", ()):
             self.vs[v] = 0

@@ -152,7 +152,7 @@ class Server:
     def _build(self, job: Job) -> None:
         scratch = build_bytes(job.scale, job.edgefactor) if job.graph is None else 0
         if scratch > self.cap:
-            job.error = f"graph build needs {scratch} B of device scratch, above the memory cap {int(self.cap)} B"
+            job.error = f"graph build needs {scratch} B of device scratch, which exceeds the memory cap {int(self.cap)} B"
             scratch = 0
         elif scratch:
             with self._lock:  # the build's scratch counts against the cap while it runs

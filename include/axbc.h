@@ -181,6 +181,23 @@ int axbc_weights_copy(axbc_ctx *ctx, uint64_t *w);
  * prefix.  Host copies (N entries each) may be NULL.
  */
 int axbc_lcc_build(axbc_ctx *ctx, float *lcc_host, uint64_t *lw_host, uint64_t *lp_host, uint64_t *total);
+/*
+ * Draws of the static distributions searched on the device, a window at a
+ * time (findsample_LOCINT, src/axbc2d.c:2188-2198, over the -z 1 table of
+ * 2758-2786; the -z 2 LCC draw of 2872-2894): for the next `count` values
+ * of the lrand48 stream with 48-bit state `state` (X; a value advances
+ * X = (0x5DEECE66D X + 0xB) mod 2^48 and is X >> 17), lr[p] is the value
+ * at position p and v[p] the vertex that a first draw there picks -- table
+ * 1: first i with the running maximum of the wrapped (uint32)deg^2 prefix
+ * >= lr % (total + 1), total the deg^2 sum (axbc_hdn_build); table 2: first
+ * i with the LCC weight prefix >= lr / 2147483647 * total (clamped), total
+ * from axbc_lcc_build; N-1 when none.  A host loop consumes its own lrand48
+ * values, checks them against lr, and takes v where a first draw falls.
+ * Runs on a stream of its own, so a window can be searched while a batch
+ * runs.
+ */
+int axbc_draw_window(axbc_ctx *ctx, int table, uint64_t state, uint64_t total, uint32_t count, uint32_t *lr,
+                     uint32_t *v);
 
 /*
  * Bit-packed multi-source rounds: nb (1..width) roots in one sweep, width

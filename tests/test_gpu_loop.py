@@ -128,7 +128,7 @@ def test_hdn_pick_over_wrapped_prefix(require_gpu):
         pmax = np.empty(g.N, np.uint32)
         sdn = np.empty(g.N, np.uint32)
         s = C.c_uint32(0)
-        assert L.axbc_hdn_build(e.ctx, pmax, sdn, C.byref(s)) == 0
+        assert L.axbc_hdn_build(e.ctx, pmax.ctypes.data, sdn.ctypes.data, C.byref(s)) == 0
         assert s.value == S
         assert np.array_equal(pmax.astype(np.uint64), runmax)
         rng = np.random.default_rng(9)
@@ -142,3 +142,60 @@ def test_hdn_pick_over_wrapped_prefix(require_gpu):
             assert u.value == want, f"r = {r}: device {u.value}, first match {want}"
             want_sdn = sum(deg[w] for w in g.row[g.col[want]:g.col[want + 1]]) & 0xFFFFFFFF
             assert su.value == want_sdn
+
+
+def test_draw_windows_on_the_device(require_gpu):
+    """axbc_draw_window: the lrand48 values ahead of a generator state equal
+    the generator's own (Rand48, itself pinned against libc), and the vertex
+    at each position is the first match of that draw -- over the wrapped
+    deg^2 prefix of a hub graph (table 1, linear first match in Python
+    integers) and over the LCC weight prefix of a Kronecker graph (table 2,
+    uniform_double restated)."""
+    import ctypes as C
+    from axbc_amd import _ffi
+    from axbc_amd.pivots import Rand48
+    from tests.test_oracle_pins import hub_graph, hdn_expect
+    L = _ffi.hip()
+    g = hub_graph()
+    deg, prefix = hdn_expect(g)
+    pre = np.array(prefix, np.uint64)
+    S = prefix[-1]
+    with Engine(g) as e:
+        s = C.c_uint32(0)
+        sdn = np.empty(g.N, np.uint32)
+        assert L.axbc_hdn_build(e.ctx, None, sdn.ctypes.data, C.byref(s)) == 0
+        for seed, skip, count in ((4343543, 0, 1000), (77, 12345, 4096), (1, 3, 1)):
+            rng = Rand48(seed)
+            for _ in range(skip):
+                rng.lrand48()
+            lr = np.empty(count, np.uint32)
+            v = np.empty(count, np.uint32)
+            assert L.axbc_draw_window(e.ctx, 1, rng.state48(), S, count, lr, v) == 0
+            for p in range(count):
+                x = rng.lrand48()
+                assert lr[p] == x, (seed, p)
+                r = x % ((S + 1) & 0xFFFFFFFF)
+                hits = np.flatnonzero(pre >= r)
+                assert v[p] == (int(hits[0]) if hits.size else g.N - 1), (seed, p, r)
+        bad = np.empty(1, np.uint32)
+        assert L.axbc_draw_window(e.ctx, 2, 0, 1, 1, bad, bad) != 0  # LCC table not built
+    g = Graph.kronecker(12, 16, seed=(3, 4), heuristic=1)
+    with Engine(g) as e:
+        lw = np.empty(g.N, np.uint64)
+        lp = np.empty(g.N, np.uint64)
+        t = C.c_uint64(0)
+        assert L.axbc_lcc_build(e.ctx, None, lw.ctypes.data, lp.ctypes.data, C.byref(t)) == 0
+        total = int(t.value)
+        assert total > 0 and np.array_equal(np.cumsum(lw, dtype=np.uint64), lp)
+        rng = Rand48(4343543)
+        count = 3000
+        lr = np.empty(count, np.uint32)
+        v = np.empty(count, np.uint32)
+        assert L.axbc_draw_window(e.ctx, 2, rng.state48(), total, count, lr, v) == 0
+        for p in range(count):
+            x = rng.lrand48()
+            assert lr[p] == x
+            rd = float(x) / 2147483647.0 * float(total)
+            r = total if rd >= float(total) else int(rd)
+            u = int(np.searchsorted(lp, np.uint64(r), side="left"))
+            assert v[p] == min(u, g.N - 1), (p, r)

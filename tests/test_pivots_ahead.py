@@ -30,11 +30,29 @@ class FakeLib:
         return 0
 
     def axbc_hdn_build(self, ctx, pmax, sdn, s):
+        n = self.g.N
         deg = np.diff(self.g.col).astype(np.uint64)
         sq = (deg * deg) & np.uint64(0xFFFFFFFF)
-        pmax[:] = (np.cumsum(sq) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
-        sdn[:] = (deg + 1).astype(np.uint32)
-        s._obj.value = int(pmax[-1])
+        pref = (np.cumsum(sq) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        self.runmax = np.maximum.accumulate(pref)
+        if pmax:
+            np.ctypeslib.as_array((C.c_uint32 * n).from_address(pmax))[:] = self.runmax
+        if sdn:
+            np.ctypeslib.as_array((C.c_uint32 * n).from_address(sdn))[:] = (deg + 1).astype(np.uint32)
+        s._obj.value = int(pref[-1])
+        return 0
+
+    def axbc_draw_window(self, ctx, table, state, total, count, lr, v):
+        # the windows' contract, restated: lrand48 values ahead of the state
+        # and the first running-max entry >= r
+        x = state
+        m = (total + 1) & 0xFFFFFFFF
+        for p in range(count):
+            x = (0x5DEECE66D * x + 0xB) % (1 << 48)
+            lr[p] = x >> 17
+            r = int(lr[p]) % m if m else int(lr[p])
+            v[p] = min(int(np.searchsorted(self.runmax, np.uint32(r), side="left")), self.g.N - 1)
+        self.windows = getattr(self, "windows", 0) + 1
         return 0
 
     def axbc_stop_test(self, ctx, c, out):
@@ -114,3 +132,32 @@ def test_draw_ahead_is_invisible(monkeypatch, z, case):
         assert a_res.skipped > 0
     if case in ("cut", "stop"):
         assert a_res.stopped
+
+
+@pytest.mark.parametrize("window", [7, 37, 1024])
+@pytest.mark.parametrize("ahead", [False, True])
+def test_draw_windows_equal_per_draw_search(monkeypatch, window, ahead):
+    """-z 1 draws taken from device windows (here the stand-in's) equal the
+    per-draw search over the running maximum, whatever the window size and
+    with or without drawing ahead -- windows cross batch boundaries and
+    draw-ahead undos."""
+    g = graph()
+    monkeypatch.setattr(pv.PivotLoop, "WINDOW", window)
+    sizes = [64, 64, 32, 10, 64]
+    res, _ = run(monkeypatch, g, ahead, sizes, 1, nbfs=300)
+    # the same loop restated with a search per draw
+    fake = FakeLib(g)
+    monkeypatch.setattr(pv._ffi, "hip", lambda: fake)
+    ref = pv.PivotLoop(FakeEngine(g), 300, c=0, x=2, z=1, batch=64)
+
+    def per_draw(table, total):
+        r = ref.rng.uniform_int(total)
+        return min(int(np.searchsorted(fake.runmax, np.uint32(r), side="left")), g.N - 1)
+
+    ref._window_pick = per_draw
+    ref.draw_ahead = False
+    for m in sizes + [None] * 40:
+        if ref.step(m) == 0 and ref.ui >= ref.nbfs:
+            break
+    assert res.pivots == ref.res.pivots and res.probs == ref.res.probs
+    assert res.skipped == ref.res.skipped and len(res.pivots) > 100
