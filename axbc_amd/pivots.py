@@ -235,6 +235,20 @@ class PivotLoop:
         self.vs[v0] = 1
         return v0, p
 
+    def sample_roots(self, n: int) -> np.ndarray:
+        """The next n roots of the loop's draw sequence (distinct, of positive
+        degree), drawn without running rounds; for static distributions
+        (-z 0, 1, 2) the loop's own roots would follow them."""
+        if self.z not in (0, 1, 2):
+            raise ValueError("roots of a static distribution only")
+        out = []
+        while len(out) < n and self.ui < self.nbfs:
+            r = self._next_root()
+            self.ui += 1
+            if r is not None:
+                out.append(r[0])
+        return np.array(out, np.uint32)
+
     def _collect(self, want: int):
         """Draw the next roots, up to `want` for a batch and one otherwise:
         (roots, attempt index after each, skipped count before each)."""
