@@ -174,11 +174,12 @@ def grid_case(args):
 
 def serve_case(args):
     from axbc_amd.serve import poisson_jobs, run_workload
-    jobs = poisson_jobs(args.serve, 20.0, seed=0)
+    jobs = poisson_jobs(args.serve, args.rate, seed=0)
     t = time.perf_counter()
     r = run_workload(jobs, slots=args.slots, hbm_cap=200e9)
     log(f"serving: {r} in {time.perf_counter() - t:.1f}s")
-    print(json.dumps(dict({"case": "serving-poisson20-s12to20-z6", "requests": args.serve, "slots": args.slots},
+    case = f"serving-poisson{args.rate:g}-s12to20-z6"
+    print(json.dumps(dict({"case": case, "requests": args.serve, "slots": args.slots, "rate": args.rate},
                           **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()})))
 
 
@@ -298,6 +299,7 @@ def main():
     ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
                     help="serving case instead: Poisson 20 req/s, SCALE 12-20, ef 8/16/32, budgets 64-2048, -z 6, "
                          "200 GB cap; prints p50/p99 latency and aggregate pivots/s")
+    ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second of --serve (the workload's is 20)")
     ap.add_argument("--slots", type=int, default=8,
                     help="device workers of --serve (two streams per hardware queue at HIP's 4 queues per process)")
     ap.add_argument("--learned", action="store_true",
