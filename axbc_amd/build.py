@@ -43,7 +43,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-HIP_SOURCES = ["engine.hip", "batch.hip", "sampler.hip", "mlp.hip", "ingest.hip", "kronecker.hip", "noalloc.hip"]
+HIP_SOURCES = ["engine.hip", "batch.hip", "wgsweep.hip", "sampler.hip", "mlp.hip", "ingest.hip", "kronecker.hip",
+               "noalloc.hip"]
 HOST_SOURCES = ["kronecker.c", "graph.c"]
 
 

@@ -169,6 +169,18 @@ struct BatchState {
   uint32_t *util_v = nullptr;
   double *util_w = nullptr, *util_part = nullptr, *util_out = nullptr;
   uint32_t util_cap = 0;
+  // workgroup sweep (one workgroup per root, wgsweep.hip), allocated on
+  // first use: per lane the state words, deltas, level list and level
+  // offsets (wlcap per lane), the deepest level per lane and the overflow flag
+  uint64_t *wst = nullptr;
+  double *wdl = nullptr;
+  uint32_t *wlst = nullptr, *wloff = nullptr, *wovf = nullptr;
+  int32_t *wnlev = nullptr;
+  int32_t *h_wnlev = nullptr;  // pinned: B deepest levels + the overflow flag
+  uint32_t wlcap = 0;
+  uint32_t wtag = 1;       // tag of level 0 for the next workgroup sweep
+  uint32_t wtag_last = 0;  // tag of level 0 of the last batch (when it was a workgroup sweep)
+  bool last_wg = false;    // the last batch ran as a workgroup sweep
 };
 
 }  // namespace axbc
@@ -2396,9 +2408,11 @@ void batch_free(axbc_ctx *c) {
   if (b->gbwd) (void)hipGraphExecDestroy(b->gbwd);
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
-               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->lmode, b->prog, b->bar, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out};
+               b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->lmode, b->prog, b->bar, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out,
+               b->wst, b->wdl, b->wlst, b->wloff, b->wnlev};  // wovf lies inside wnlev
   for (void *p : d)
     if (p) (void)hipFree(p);
+  if (b->h_wnlev) (void)hipHostFree(b->h_wnlev);
   if (b->h_ctl) (void)hipHostFree(b->h_ctl);
   if (b->h_cnt) (void)hipHostFree(b->h_cnt);
   if (b->h_act) (void)hipHostFree(b->h_act);
@@ -2841,48 +2855,136 @@ int persist_failed(axbc_ctx *c) {
   return fail(AXBC_EHIP, "persistent level loop: a barrier wait passed its limit%s (line %d)", what, __LINE__);
 }
 
+// Automatic choice of the workgroup sweep: graphs whose previous batch ran
+// more than kWgLevels levels (high diameter: the roots' frontiers rarely
+// meet, so the bit-packed sweep shares little and pays a grid-wide launch
+// per level), and whose lists are short (a thread walks a vertex's list).
+constexpr int kWgLevels = 256;
+constexpr uint32_t kWgMaxDeg = 64;
+constexpr uint32_t kWgFirstLevels = 16384;  // first level-offset capacity per root
+
+bool use_wg(const axbc_ctx *c) {
+  if (c->batch_mode == 2) return true;
+  if (c->batch_mode == 1) return false;
+  return c->batch->lev_guess > kWgLevels && c->max_out <= kWgMaxDeg;
+}
+
+// The workgroup sweep's buffers, all or none: released (and their bytes
+// taken off the totals) when one of them cannot be had.
+void wg_release(axbc_ctx *c) {
+  BatchState *b = c->batch;
+  const size_t B = (size_t)b->B, n = c->n;
+  size_t held = 0;
+  if (b->wst) held += B * n * 8;
+  if (b->wdl) held += B * n * 8;
+  if (b->wlst) held += B * n * 4;
+  if (b->wloff) held += B * b->wlcap * 4;
+  if (b->wnlev) held += (B + 1) * 4;
+  void *d[] = {b->wst, b->wdl, b->wlst, b->wloff, b->wnlev};
+  for (void *p : d)
+    if (p) (void)hipFree(p);
+  if (b->h_wnlev) (void)hipHostFree(b->h_wnlev);
+  b->wst = nullptr;
+  b->wdl = nullptr;
+  b->wlst = b->wloff = b->wovf = nullptr;
+  b->wnlev = b->h_wnlev = nullptr;
+  b->bytes -= std::min(b->bytes, held);
+  c->bytes -= std::min(c->bytes, held);
+  b->wlcap = 0;
+}
+
+int wg_alloc(axbc_ctx *c) {
+  BatchState *b = c->batch;
+  const size_t B = (size_t)b->B, n = c->n;
+  const uint32_t lcap = (uint32_t)std::min<uint64_t>((uint64_t)n + 2, kWgFirstLevels);
+  int rc;
+  if ((rc = budget_check(c, B * n * (8 + 8 + 4) + B * lcap * 4 + (B + 1) * 4, "the workgroup sweep state")))
+    return rc;
+  b->wlcap = lcap;
+  const size_t before = b->bytes;
+  if ((rc = balloc(b, &b->wst, B * n)) || (rc = balloc(b, &b->wdl, B * n)) || (rc = balloc(b, &b->wlst, B * n)) ||
+      (rc = balloc(b, &b->wloff, B * lcap)) || (rc = balloc(b, &b->wnlev, B + 1))) {
+    c->bytes += b->bytes - before;
+    wg_release(c);
+    return rc;
+  }
+  c->bytes += b->bytes - before;
+  b->wovf = (uint32_t *)(b->wnlev + B);
+  if (hipHostMalloc((void **)&b->h_wnlev, sizeof(int32_t) * (B + 1), hipHostMallocDefault) != hipSuccess ||
+      hipMemsetAsync(b->wst, 0, sizeof(uint64_t) * B * n, c->stream) != hipSuccess) {
+    wg_release(c);
+    return fail(AXBC_EHIP, "workgroup sweep state set-up failed%s (line %d)", "", __LINE__);
+  }
+  b->wtag = 1;
+  return AXBC_OK;
+}
+
+// Workgroup sweep of a batch whose roots and lanes are on the device: the
+// per-root rounds (wgsweep.hip), then the batch's delta rows and seen
+// stamps, which the stop test and the bc update read as after a bit-packed
+// sweep.  A root deeper than the level-offset capacity runs the batch again
+// with four times the capacity (bc untouched: the update has not run).
 template <int NW>
-int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
-              axbc_pivot_stats *stats) {
+int wg_batch(axbc_ctx *c, int nb, std::vector<int> &nlev) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  const size_t B = (size_t)b->B;
+  int rc;
+  if (!b->wst && (rc = wg_alloc(c))) return rc;
+  b->have = true;
+  b->stamp += 1;
+  for (;;) {
+    // a batch uses the tags wtag .. wtag + its deepest level: start over
+    // (every word unvisited) before they could wrap
+    if ((uint64_t)b->wtag + n + 2 > 0xFFFFFFFFull) {
+      HIP_TRY(hipMemsetAsync(b->wst, 0, sizeof(uint64_t) * B * n, s));
+      b->wtag = 1;
+    }
+    const uint32_t tag0 = b->wtag;
+    HIP_TRY(hipMemsetAsync(b->wovf, 0, sizeof(uint32_t), s));
+    kt_begin(c, KK_WGSWEEP);
+    HIP_TRY(wg_sweep(s, nb, n, c->col, c->row, c->nr ? c->reach : nullptr, b->src, tag0, b->wst, b->wdl, b->wlst,
+                     b->wloff, b->wlcap, b->cnt, b->wnlev, b->wovf, kt_bytes(c, KK_WGSWEEP)));
+    kt_end(c, KK_WGSWEEP);
+    HIP_TRY(hipMemcpyAsync(b->h_wnlev, b->wnlev, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int maxl = 0;
+    for (int l = 0; l < nb; ++l) maxl = std::max(maxl, (int)b->h_wnlev[l]);
+    b->wtag += (uint32_t)maxl + 2u;
+    if (b->h_wnlev[B] == 0) {
+      HIP_TRY(wg_out(s, NW, n, nb, tag0, b->wst, b->wdl, c->reach, b->dlt, b->seen, b->stamp));
+      for (int l = 0; l < nb; ++l) nlev[l] = b->h_wnlev[l];
+      b->wtag_last = tag0;
+      b->levels = maxl + 1;
+      b->lev_guess = maxl + 2;
+      b->last_wg = true;
+      return AXBC_OK;
+    }
+    if (b->wlcap >= n + 2)
+      return fail(AXBC_ESTATE, "workgroup sweep: more levels than vertices%s (line %d)", "", __LINE__);
+    const size_t old = sizeof(uint32_t) * B * b->wlcap;
+    (void)hipFree(b->wloff);
+    b->wloff = nullptr;
+    b->bytes -= std::min(b->bytes, old);
+    c->bytes -= std::min(c->bytes, old);
+    b->wlcap = (uint32_t)std::min<uint64_t>((uint64_t)n + 2, (uint64_t)b->wlcap * 4);
+    if ((rc = balloc_more(c, &b->wloff, B * b->wlcap, "the workgroup sweep level offsets"))) {
+      wg_release(c);  // the next batch starts over
+      return rc;
+    }
+  }
+}
+
+// The bit-packed sweep of a batch whose roots, probabilities and lanes are
+// on the device: forward levels, per-root counters, the delta rows and the
+// backward depths.  nlev[l] receives lane l's deepest level.
+template <int NW>
+int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs, std::vector<int> &nlev) {
   BatchState *b = c->batch;
   hipStream_t s = c->stream;
   const uint32_t n = c->n;
   int rc;
-  std::vector<uint32_t> r0(nb);
-  for (int k = 0; k < nb; ++k) r0[k] = c->h_reach[v0s[k]];
-  // lanes: the roots by degree, highest first (ties in draw order).  Roots
-  // of similar degree have similar level structure, so the roots that meet
-  // at an edge of the sweep sit in fewer cache lines of the root-minor rows.
-  std::vector<uint32_t> root_at(nb);
-  for (int k = 0; k < nb; ++k) root_at[k] = (uint32_t)k;
-  std::stable_sort(root_at.begin(), root_at.end(), [&](uint32_t a, uint32_t b2) {
-    return c->h_col[v0s[a] + 1] - c->h_col[v0s[a]] > c->h_col[v0s[b2] + 1] - c->h_col[v0s[b2]];
-  });
-  std::vector<uint32_t> &lane_of = b->h_lane_of;
-  lane_of.resize(b->B);
-  for (int l = 0; l < b->B; ++l) lane_of[l] = (uint32_t)l;  // lanes past nb are unused
-  std::vector<uint32_t> src_l(nb), r0_l(nb);
-  std::vector<double> p_l(nb);
-  for (int l = 0; l < nb; ++l) {
-    const uint32_t k = root_at[l];
-    lane_of[k] = (uint32_t)l;
-    src_l[l] = v0s[k];
-    r0_l[l] = r0[k];
-    p_l[l] = ps[k];
-  }
-  // split kernels only where some vertex has more edges than a chunk
-  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
-  // the level loop runs from prebuilt graphs unless per-kernel timing is on
-  const bool graphs = c->use_graphs && !c->ktimer.on;
-  if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
-  HIP_TRY(hipMemcpyAsync(b->src, src_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->prob, p_l.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->r0, r0_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->lane_of, lane_of.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  // reset the previous batch, then level 0
-  if (b->have)
-    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
-                       b->vis, b->fr[0], b->fr[1], b->sig);
   // rows of vertices above the split threshold start at zero: clear them
   // all when the threshold dropped below the one they were cleared for
   if (b->chunk < b->sig_zero_above)
@@ -2967,7 +3069,6 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   HIP_TRY(hipMemcpyAsync(b->h_act, b->act, sizeof(uint64_t) * (size_t)(empty + 1) * NW, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   uint64_t rootmask[2] = {0, 0};
-  std::vector<int> nlev(nb, 0);  // per lane
   for (int l = 0; l < nb; ++l) {
     for (int lv = 1; lv <= dmax; ++lv)
       if ((b->h_act[(size_t)lv * NW + l / 64] >> (l % 64)) & 1ull) nlev[l] = lv;
@@ -3007,6 +3108,60 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     }
   }
   HIP_TRY(hipGetLastError());
+  b->last_wg = false;
+  return AXBC_OK;
+}
+
+template <int NW>
+int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
+              axbc_pivot_stats *stats) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  int rc;
+  std::vector<uint32_t> r0(nb);
+  for (int k = 0; k < nb; ++k) r0[k] = c->h_reach[v0s[k]];
+  // lanes: the roots by degree, highest first (ties in draw order).  Roots
+  // of similar degree have similar level structure, so the roots that meet
+  // at an edge of the sweep sit in fewer cache lines of the root-minor rows.
+  std::vector<uint32_t> root_at(nb);
+  for (int k = 0; k < nb; ++k) root_at[k] = (uint32_t)k;
+  std::stable_sort(root_at.begin(), root_at.end(), [&](uint32_t a, uint32_t b2) {
+    return c->h_col[v0s[a] + 1] - c->h_col[v0s[a]] > c->h_col[v0s[b2] + 1] - c->h_col[v0s[b2]];
+  });
+  std::vector<uint32_t> &lane_of = b->h_lane_of;
+  lane_of.resize(b->B);
+  for (int l = 0; l < b->B; ++l) lane_of[l] = (uint32_t)l;  // lanes past nb are unused
+  std::vector<uint32_t> src_l(nb), r0_l(nb);
+  std::vector<double> p_l(nb);
+  for (int l = 0; l < nb; ++l) {
+    const uint32_t k = root_at[l];
+    lane_of[k] = (uint32_t)l;
+    src_l[l] = v0s[k];
+    r0_l[l] = r0[k];
+    p_l[l] = ps[k];
+  }
+  // split kernels only where some vertex has more edges than a chunk
+  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
+  const bool wg = use_wg(c);
+  // the level loop runs from prebuilt graphs unless per-kernel timing is on
+  const bool graphs = !wg && c->use_graphs && !c->ktimer.on;
+  if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
+  HIP_TRY(hipMemcpyAsync(b->src, src_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->prob, p_l.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->r0, r0_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->lane_of, lane_of.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  // undo the previous bit-packed sweep (a workgroup sweep leaves the
+  // bit-packed state as it found it)
+  if (b->have && !b->last_wg)
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
+                       b->vis, b->fr[0], b->fr[1], b->sig);
+  std::vector<int> nlev(nb, 0);  // per lane
+  if (wg)
+    rc = wg_batch<NW>(c, nb, nlev);
+  else
+    rc = packed_sweep<NW>(c, nb, split_out, split_in, graphs, nlev);
+  if (rc) return rc;
   // exact stop test per root, then the bc update for the applied roots
   int nr = nb;
   if (stop_c > 0 && c->ncheck > 0) {
@@ -3143,6 +3298,12 @@ int axbc_set_batch_persistent(axbc_ctx *c, int mode, double limit_seconds) {
   return AXBC_OK;
 }
 
+int axbc_set_batch_mode(axbc_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 2) return fail(AXBC_EINVAL, "batch mode must be 0, 1 or 2%s (line %d)", "", __LINE__);
+  c->batch_mode = mode;
+  return AXBC_OK;
+}
+
 int axbc_batch_state_bytes(uint64_t n, uint64_t ned, const uint32_t *col, int width, size_t *bytes) {
   if (!col || !bytes || (width != 64 && width != 128) || n == 0 || n > 0xFFFFFFF0ull)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
@@ -3248,13 +3409,18 @@ int axbc_batch_levels(axbc_ctx *c, int k, int32_t *lvl, uint32_t *sigma) {
   int32_t *dl = (int32_t *)b->row_out;
   uint32_t *ds = (uint32_t *)(dl + c->n);
   hipStream_t s = c->stream;
-  hipLaunchKernelGGL(k_blevels_init, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->n, dl, ds);
-  if (b->nw == 1)
-    hipLaunchKernelGGL(k_blevels<1>, dim3(2048), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, b->levels, lane, b->sig,
-                       dl, ds);
-  else
-    hipLaunchKernelGGL(k_blevels<2>, dim3(2048), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, b->levels, lane, b->sig,
-                       dl, ds);
+  if (b->last_wg) {
+    // workgroup sweep: the lane's own state words
+    HIP_TRY(wg_levels(s, c->n, b->wtag_last, b->wst + (size_t)lane * c->n, dl, ds));
+  } else {
+    hipLaunchKernelGGL(k_blevels_init, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->n, dl, ds);
+    if (b->nw == 1)
+      hipLaunchKernelGGL(k_blevels<1>, dim3(2048), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, b->levels, lane, b->sig,
+                         dl, ds);
+    else
+      hipLaunchKernelGGL(k_blevels<2>, dim3(2048), dim3(kBlock), 0, s, b->q, b->qw, b->qoff, b->levels, lane, b->sig,
+                         dl, ds);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(lvl, dl, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(sigma, ds, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, s));

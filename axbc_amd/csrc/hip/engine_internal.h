@@ -79,7 +79,8 @@ enum KernelKind {
   KK_BWD_HUBS = 13,    // plain dependency pull, hub chunks (k_bwd_hubs)
   KK_PULL = 14,        // batched bottom-up level (k_bpull)
   KK_PULL_SPLIT = 15,  // batched bottom-up level, split vertices (k_bpull_split)
-  KK_COUNT = 16
+  KK_WGSWEEP = 16,     // workgroup sweep, one workgroup per root (k_wgsweep)
+  KK_COUNT = 17
 };
 constexpr int kKbSlots = 64, kKbStride = 16;
 
@@ -157,6 +158,7 @@ struct axbc_ctx {
   uint32_t batch_pull16 = 32;        // bottom-up ratio x16 for batched levels (0: top-down only)
   bool use_graphs = true;            // batched level loop from captured HIP graphs
   int batch_persist = 0;             // persistent level loop: 0 never (default), 1 deep graphs, 2 whenever possible
+  int batch_mode = 0;                // 0 automatic, 1 bit-packed sweep only, 2 workgroup sweep only
   unsigned long long batch_persist_limit = 1000000000ull;  // barrier wait limit, 100 MHz ticks (10 s)
   uint32_t max_out = 0, max_in = 0;  // largest out- and in-degree (set with the transpose)
   size_t budget = 0;  // device bytes allowed to the context (0: no limit); checked before each allocation after create
@@ -181,6 +183,19 @@ constexpr int kUtilWaves = 1024;
 // partial rows, dout B values.
 int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
                      const double *weight, uint32_t count, double *out);
+// Workgroup sweep (wgsweep.hip): one workgroup per root of the batch, the
+// whole round of each root in one launch, per-root state in the wst / wdl /
+// wlst / wloff arrays of n (wloff: lcap) entries per lane.
+hipError_t wg_sweep(hipStream_t s, int nb, uint32_t n, const uint32_t *col, const uint32_t *row,
+                    const uint32_t *reach, const uint32_t *src, uint32_t tag0, uint64_t *pst, double *pdl,
+                    uint32_t *plst, uint32_t *ploff, uint32_t lcap, unsigned long long *cnt, int32_t *nlev,
+                    uint32_t *ovf, unsigned long long *kb);
+// The batch's root-minor delta rows (every lane: delta where root l < nb
+// reached the vertex, reach elsewhere) and seen = stamp where some root did.
+hipError_t wg_out(hipStream_t s, int nw, uint32_t n, int nb, uint32_t tag0, const uint64_t *pst, const double *pdl,
+                  const uint32_t *reach, double *dlt, uint32_t *seen, uint32_t stamp);
+// One lane's levels and path counts from its state words.
+hipError_t wg_levels(hipStream_t s, uint32_t n, uint32_t tag0, const uint64_t *st, int32_t *lvl, uint32_t *sigma);
 // Event pair around one launch of kind k when instrumentation is on.
 void kt_begin(axbc_ctx *c, int kind);
 void kt_end(axbc_ctx *c, int kind);

@@ -93,9 +93,14 @@ class Engine:
     def set_batch_width(self, width: int) -> None:
         _check(_ffi.hip().axbc_set_batch_width(self.ctx, int(width)))
 
+    def set_batch_mode(self, mode: int) -> None:
+        """0 automatic, 1 bit-packed sweep, 2 one workgroup per root."""
+        _check(_ffi.hip().axbc_set_batch_mode(self.ctx, int(mode)))
+
     def batch(self, v0s, ps, stop_c: int = 0, stats: bool = False):
-        """Bit-packed multi-source rounds: the same result as one pivot()
-        per root in order.  Returns (applied, stats or None)."""
+        """Multi-source rounds (bit-packed, or one workgroup per root): the
+        same result as one pivot() per root in order.  Returns (applied,
+        stats or None)."""
         L = _ffi.hip()
         v = np.ascontiguousarray(v0s, dtype=np.uint32)
         p = np.ascontiguousarray(ps, dtype=np.float64)

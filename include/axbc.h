@@ -237,6 +237,14 @@ int axbc_set_batch_graphs(axbc_ctx *ctx, int enable);
  * AXBC_EHIP, bc unchanged and the batch state rebuilt by the next batch.
  * Results do not depend on it. */
 int axbc_set_batch_persistent(axbc_ctx *ctx, int mode, double limit_seconds);
+/* How a batch sweeps: 1 the bit-packed sweep above; 2 one workgroup per root
+ * (the whole round of each root in one launch, levels separated by
+ * workgroup barriers, per-root state in HBM: 24 bytes per vertex and lane
+ * plus 4 per level and lane, allocated on first use); 0 (default) chooses 2
+ * on graphs whose previous batch ran more than 256 levels and whose lists
+ * hold at most 64 edges (high-diameter graphs, where the roots' frontiers
+ * rarely meet), 1 otherwise.  Results do not depend on it. */
+int axbc_set_batch_mode(axbc_ctx *ctx, int mode);
 int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,
                int *applied, axbc_pivot_stats *stats);

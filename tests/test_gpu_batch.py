@@ -31,12 +31,15 @@ def delta_of(e, k):
     return out
 
 
-def check_batch(g, roots, width, label, batches=1, pull=None, chunks=None):
-    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)} pull={pull} chunks={chunks}", flush=True)
+def check_batch(g, roots, width, label, batches=1, pull=None, chunks=None, mode=None):
+    print(f"[{label}] N={g.N} ned={g.ned} width={width} roots={len(roots)} pull={pull} chunks={chunks} mode={mode}",
+          flush=True)
     bc_ref = np.zeros(g.N)
     probs = np.full(len(roots), 1.0 / g.N)
     with Engine(g) as e:
         assert _ffi.hip().axbc_set_batch_width(e.ctx, width) == 0
+        if mode is not None:
+            assert _ffi.hip().axbc_set_batch_mode(e.ctx, mode) == 0
         if pull is not None:
             assert _ffi.hip().axbc_set_batch_pull(e.ctx, pull) == 0
         step = (len(roots) + batches - 1) // batches

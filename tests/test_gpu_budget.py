@@ -57,11 +57,39 @@ def test_level_list_growth_over_budget(require_gpu):
     roots = (np.arange(64, dtype=np.uint32) * 61 + 5) % g.N
     probs = np.full(64, 1.0 / g.N)
     with Engine(g) as e:
+        e.set_batch_mode(1)  # the bit-packed sweep (its level list)
         e.batch(roots[:1], probs[:1])  # creates the batch state at its first capacity
         first = e.bytes_in_use()
         e.reset_bc()
         e.set_budget(first)
         with pytest.raises(BudgetError, match="the batch state"):
+            e.batch(roots, probs)
+        assert not e.bc().any()
+        assert e.bytes_in_use() <= first
+        e.set_budget(0)
+        applied, _ = e.batch(roots, probs)
+        assert applied == 64
+        assert e.bytes_in_use() > first
+        ok, err = close(e.bc(), bc_of_roots(g, roots))
+        assert ok, err
+
+
+def test_workgroup_sweep_state_over_budget(require_gpu):
+    """The workgroup sweep allocates its per-root state on first use: with
+    the budget at the bytes in use after a bit-packed batch, the first
+    workgroup-sweep batch fails cleanly (bc untouched); with no budget it
+    runs and matches the oracle."""
+    g = path_graph(3000)
+    roots = (np.arange(64, dtype=np.uint32) * 37 + 3) % g.N
+    probs = np.full(64, 1.0 / g.N)
+    with Engine(g) as e:
+        e.set_batch_mode(1)
+        e.batch(roots[:8], probs[:8])
+        e.reset_bc()
+        first = e.bytes_in_use()
+        e.set_budget(first)
+        e.set_batch_mode(2)
+        with pytest.raises(BudgetError, match="workgroup sweep"):
             e.batch(roots, probs)
         assert not e.bc().any()
         assert e.bytes_in_use() <= first
