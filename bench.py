@@ -33,7 +33,8 @@ WORKLOAD = {
 }
 KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_expand",
               6: "k_bwd_pull", 7: "k_copy", 8: "k_bpush_split", 9: "k_bsigma_split", 10: "k_bdep_split",
-              11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split"}
+              11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split",
+              16: "k_wgsweep"}
 BATCH_KINDS = (0, 8, 1, 9, 14, 15, 2, 3, 10, 11, 4)
 PLAIN_KINDS = (5, 12, 6, 13)
 HBM_PEAK_GBS = 8000.0
@@ -127,44 +128,66 @@ def grid_case(args):
     """4096 x 4096 lattice, 4-neighbour, 3% of edges deleted (seed 3), edges of
     the largest component kept with the lattice ids (N = 2^24), -c 0 (the
     default adaptive stop c=1 x=1 holds after the first rounds here).  Two
-    distributions: uniform (-z 0, exact batches of 64) and delta sampling
-    (-z 4, batch-synchronous batches of 64).  With N = 2^24 the reference's
+    distributions: uniform (-z 0, exact batches of 128) and delta sampling
+    (-z 4, batch-synchronous batches of 128).  With N = 2^24 the reference's
     retry cap (int)(N*N) is 0, so a duplicate draw consumes an attempt; the
     concentrated delta distribution makes most -z 4 attempts duplicates, so
-    its pivots/s count rounds that actually ran.  Batched levels replay
-    captured graphs; plain rounds launch per level."""
-    from axbc_amd import Engine, Graph
+    its pivots/s count rounds that actually ran.  Batches choose their sweep
+    themselves: after the first (bit-packed) batch ran ~8000 levels, one
+    workgroup per root; the bit-packed sweep (batches of 64, level loop from
+    HIP graphs) is timed on the same -z 0 roots for comparison.  Plain rounds
+    launch per level."""
+    from axbc_amd import Engine, Graph, _ffi
     from axbc_amd.pivots import PivotLoop
+    L = _ffi.hip()
     t = time.perf_counter()
     g = Graph.lattice(4096, 0.03, seed=3, heuristic=0)
     log(f"lattice: N={g.N} ned={g.ned} built in {time.perf_counter() - t:.1f}s")
     e = Engine(g)
     if args.no_graphs:
-        from axbc_amd import _ffi
-        if _ffi.hip().axbc_set_batch_graphs(e.ctx, 0) != 0:
-            raise RuntimeError(_ffi.hip().axbc_last_error())
+        if L.axbc_set_batch_graphs(e.ctx, 0) != 0:
+            raise RuntimeError(L.axbc_last_error())
     out = {"case": "grid-4096-del3-c0", "n": g.N, "ned": g.ned}
 
     def timed(loop, steps):
-        loop.step()  # warm-up (allocation, list growth, graph capture)
+        loop.step()  # warm-up (allocation, list growth, graph capture, sweep choice)
         e.sync()
         r0, a0 = loop.res.nrounds, loop.res.attempts
+        ms0, n0, b0 = kernel_stats(e, 16)
         t = time.perf_counter()
         for _ in range(steps):
             if not loop.step():
                 break
         e.sync()
         dt = time.perf_counter() - t
-        return {"rounds": loop.res.nrounds - r0, "attempts": loop.res.attempts - a0, "seconds": round(dt, 2),
-                "pivots_per_s": round((loop.res.nrounds - r0) / dt, 3) if dt > 0 else 0.0}
+        ms1, n1, b1 = kernel_stats(e, 16)
+        r = {"rounds": loop.res.nrounds - r0, "attempts": loop.res.attempts - a0, "seconds": round(dt, 2),
+             "pivots_per_s": round((loop.res.nrounds - r0) / dt, 3) if dt > 0 else 0.0}
+        if n1 > n0:
+            r["k_wgsweep"] = {"launches": n1 - n0, "avg_launch_ms": round((ms1 - ms0) / (n1 - n0), 2),
+                              "algorithmic_gb_per_s": round((b1 - b0) / ((ms1 - ms0) * 1e-3) / 1e9, 1)}
+        return r
 
     for z in (0, 4):
+        assert L.axbc_set_batch_width(e.ctx, 128) == 0
+        assert L.axbc_set_batch_mode(e.ctx, 0) == 0
         e.reset_bc()
-        b = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=64), args.steps)
+        # events and byte counters on the workgroup sweep only (with timing
+        # on, a bit-packed level loop launches from the host)
+        set_timing_kinds(e, [16])
+        b = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=128), args.steps)
+        set_timing_kinds(e, [])
+        if z == 0:
+            assert L.axbc_set_batch_width(e.ctx, 64) == 0
+            assert L.axbc_set_batch_mode(e.ctx, 1) == 0
+            e.reset_bc()
+            bp = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=64), max(2, args.steps // 6))
         e.reset_bc()
         p = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=0), args.plain_rounds)
         out[f"z{z}"] = {"batched": b, "plain": p,
                         "batched_over_plain": round(b["pivots_per_s"] / p["pivots_per_s"], 2) if p["pivots_per_s"] else None}
+        if z == 0:
+            out["z0"]["bit_packed_width64"] = bp
         log(f"z{z}: {out[f'z{z}']}")
     st = e.pivot(int(np.flatnonzero(np.diff(g.col) > 0)[0]), 1.0 / g.N, stats=True)
     out["levels_of_a_round"] = st["nlevels"]
@@ -294,8 +317,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="split threshold with --batch (default 256)")
     ap.add_argument("--pull", type=int, default=-1, help="bottom-up ratio x16 with --batch (default 32)")
     ap.add_argument("--grid", action="store_true",
-                    help="high-diameter case instead: 4096^2 lattice, 3%% edges deleted (seed 3), -z 4, batch-synchronous "
-                         "batches of 64 vs plain rounds")
+                    help="high-diameter case instead: 4096^2 lattice, 3%% edges deleted (seed 3), -z 0 and -z 4 "
+                         "(batch-synchronous) in batches of 128 vs plain rounds")
     ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
                     help="serving case instead: Poisson 20 req/s, SCALE 12-20, ef 8/16/32, budgets 64-2048, -z 6, "
                          "200 GB cap; prints p50/p99 latency and aggregate pivots/s")
