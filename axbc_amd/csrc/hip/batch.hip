@@ -2945,7 +2945,7 @@ int wg_batch(axbc_ctx *c, int nb, std::vector<int> &nlev) {
     HIP_TRY(hipMemsetAsync(b->wovf, 0, sizeof(uint32_t), s));
     kt_begin(c, KK_WGSWEEP);
     HIP_TRY(wg_sweep(s, nb, n, c->col, c->row, c->nr ? c->reach : nullptr, b->src, tag0, b->wst, b->wdl, b->wlst,
-                     b->wloff, b->wlcap, b->cnt, b->wnlev, b->wovf, kt_bytes(c, KK_WGSWEEP)));
+                     b->wloff, b->wlcap, b->cnt, b->wnlev, b->wovf, kt_bytes(c, KK_WGSWEEP), c->ned));
     kt_end(c, KK_WGSWEEP);
     HIP_TRY(hipMemcpyAsync(b->h_wnlev, b->wnlev, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -189,7 +189,7 @@ int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double
 hipError_t wg_sweep(hipStream_t s, int nb, uint32_t n, const uint32_t *col, const uint32_t *row,
                     const uint32_t *reach, const uint32_t *src, uint32_t tag0, uint64_t *pst, double *pdl,
                     uint32_t *plst, uint32_t *ploff, uint32_t lcap, unsigned long long *cnt, int32_t *nlev,
-                    uint32_t *ovf, unsigned long long *kb);
+                    uint32_t *ovf, unsigned long long *kb, uint64_t ned);
 // The batch's root-minor delta rows (every lane: delta where root l < nb
 // reached the vertex, reach elsewhere) and seen = stamp where some root did.
 hipError_t wg_out(hipStream_t s, int nw, uint32_t n, int nb, uint32_t tag0, const uint64_t *pst, const double *pdl,

@@ -53,6 +53,115 @@ __device__ __forceinline__ uint32_t ld_wg(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Both list offsets of a vertex in one 8-byte load (4-byte aligned).
+__device__ __forceinline__ uint2 ld_pair(const uint32_t *__restrict__ p) {
+  uint2 r;
+  __builtin_memcpy(&r, p, sizeof(r));
+  return r;
+}
+
+// The ids of a list [b, b + d) with d <= 4 into slots O .. O+3 of w, from at
+// most two 16-byte aligned loads ([b, b + 4) lies in [q, q + 8) for
+// q = b & ~3); slots past d get 0.
+// ned_al = the list array's length when it starts 16-byte aligned, else 0:
+// lists whose window would pass the end (or any list, unaligned) load entry
+// by entry.
+template <int O>
+__device__ __forceinline__ void short_list(const uint32_t *__restrict__ row, uint64_t ned_al, uint32_t b, uint32_t d,
+                                           uint32_t (&w)[kEdgeRun]) {
+  const uint32_t q = b & ~3u;
+  if ((uint64_t)q + 8 > ned_al) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[O + t] = (uint32_t)t < d ? row[b + t] : 0u;
+    return;
+  }
+  // both windows lie inside the array here; loaded unconditionally, as a
+  // conditional vector load is split into one load per slot
+  uint4 lo = *(const uint4 *)(row + q);
+  uint4 hi = *(const uint4 *)(row + q + 4);
+  // without this the compiler turns the selects below back into one 4-byte
+  // load per slot at the selected address
+  asm volatile("" : "+v"(lo.x), "+v"(lo.y), "+v"(lo.z), "+v"(lo.w), "+v"(hi.x), "+v"(hi.y), "+v"(hi.z));
+  uint32_t x0 = lo.x, x1 = lo.y, x2 = lo.z, x3 = lo.w, x4 = hi.x, x5 = hi.y, x6 = hi.z;
+  const uint32_t sh = b - q;
+  if (sh & 2u) {
+    x0 = x2;
+    x1 = x3;
+    x2 = x4;
+    x3 = x5;
+    x4 = x6;
+  }
+  if (sh & 1u) {
+    x0 = x1;
+    x1 = x2;
+    x2 = x3;
+    x3 = x4;
+  }
+  w[O + 0] = d > 0 ? x0 : 0u;
+  w[O + 1] = d > 1 ? x1 : 0u;
+  w[O + 2] = d > 2 ? x2 : 0u;
+  w[O + 3] = d > 3 ? x3 : 0u;
+}
+
+// Forward over one run of edges (slot t valid when bit t of `valid`, its
+// frontier vertex's path count su[t]): every neighbour word, then every
+// claim, then the appends and additions, so the run costs three memory
+// round trips.  A vertex unvisited in this batch is claimed with tag and
+// path count at once (on failure word[t] receives the winner's word); later
+// visitors of the same level add to the word's low half (uint32, wrapping).
+// Returns the run's DAG edges; *all gains the reach of the claimed vertices.
+__device__ __forceinline__ uint32_t push_run(uint64_t *st, uint32_t *lst, uint32_t e, uint32_t *here,
+                                             const uint32_t *__restrict__ reach, uint32_t tag0, uint32_t tnext,
+                                             const uint32_t (&w)[kEdgeRun], uint32_t valid,
+                                             const uint32_t (&su)[kEdgeRun], unsigned long long *all) {
+  uint64_t word[kEdgeRun];
+  bool won[kEdgeRun];
+  uint32_t dag = 0;
+#pragma unroll
+  for (int t = 0; t < kEdgeRun; ++t) word[t] = (valid >> t) & 1u ? ld_wg(st + w[t]) : ~0ull;
+#pragma unroll
+  for (int t = 0; t < kEdgeRun; ++t)
+    won[t] = (uint32_t)(word[t] >> 32) < tag0 &&
+             __hip_atomic_compare_exchange_strong(st + w[t], &word[t], ((uint64_t)tnext << 32) | su[t],
+                                                  __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+  for (int t = 0; t < kEdgeRun; ++t) {
+    if (won[t]) {
+      lst[e + atomicAdd(here, 1u)] = w[t];
+      ++dag;
+      if (reach) *all += reach[w[t]];
+    } else if ((uint32_t)(word[t] >> 32) == tnext) {
+      __hip_atomic_fetch_add((uint32_t *)(st + w[t]), su[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      ++dag;
+    }
+  }
+  return dag;
+}
+
+// Backward over one run of edges: slots below `first1` belong to the first
+// vertex (path count sv0, sum acc0), the others to the second; each sum
+// takes its terms in slot order, i.e. in its own adjacency order.
+__device__ __forceinline__ void pull_run(const uint64_t *st, const double *dl, uint32_t tnext,
+                                         const uint32_t (&w)[kEdgeRun], uint32_t valid, int first1, double sv0,
+                                         double sv1, double &acc0, double &acc1, uint32_t &terms) {
+  uint64_t word[kEdgeRun];
+  double x[kEdgeRun];
+#pragma unroll
+  for (int t = 0; t < kEdgeRun; ++t) word[t] = (valid >> t) & 1u ? st[w[t]] : 0ull;
+#pragma unroll
+  for (int t = 0; t < kEdgeRun; ++t) x[t] = (uint32_t)(word[t] >> 32) == tnext ? dl[w[t]] : 0.0;
+#pragma unroll
+  for (int t = 0; t < kEdgeRun; ++t)
+    if ((uint32_t)(word[t] >> 32) == tnext) {
+      const double xt = (x[t] + 1.0) / (double)(uint32_t)word[t];
+      if (t < first1)
+        acc0 += xt * sv0;
+      else
+        acc1 += xt * sv1;
+      ++terms;
+    }
+}
+
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long x, unsigned long long *slot) {
   x = wave_sum_u64(x);
   if (lane_id() == 0 && x) atomicAdd(slot, x);
@@ -71,7 +180,7 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
                                                       const uint32_t *__restrict__ src, uint32_t tag0, uint64_t *pst,
                                                       double *pdl, uint32_t *plst, uint32_t *ploff, uint32_t lcap,
                                                       unsigned long long *cnt, int32_t *nlev, uint32_t *ovf,
-                                                      unsigned long long *kb) {
+                                                      unsigned long long *kb, uint64_t ned_al) {
   const uint32_t l = blockIdx.x;
   const uint32_t tid = threadIdx.x;
   uint64_t *st = pst + (size_t)l * n;
@@ -102,51 +211,37 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
     const uint32_t tnext = tag0 + (uint32_t)L + 1u;
     uint32_t *here = &s_cnt[L % 3];
     if (tid == 0) s_cnt[(L + 1) % 3] = 0;
-    // two frontier vertices per thread and their edges as one list, walked
-    // kEdgeRun at a time: every load of a run is issued before any result
-    // is used, so a level costs a few memory round trips, not one per edge
+    // two frontier vertices per thread; lists of at most 4 edges (all of a
+    // lattice) in two 4-slot halves of one run, longer ones as one list
+    // walked kEdgeRun at a time
     for (uint32_t i0 = b + tid; i0 < e; i0 += 2 * kWgBlock) {
       const uint32_t i1 = i0 + kWgBlock;
       const bool h1 = i1 < e;
       const uint32_t u0 = lst[i0], u1 = h1 ? lst[i1] : 0u;
       const uint32_t s0 = (uint32_t)ld_wg(st + u0), s1 = h1 ? (uint32_t)ld_wg(st + u1) : 0u;
-      const uint32_t b0 = col[u0], d0 = col[u0 + 1] - b0;
-      const uint32_t b1 = h1 ? col[u1] : 0u, d1 = h1 ? col[u1 + 1] - b1 : 0u;
+      const uint2 c0 = ld_pair(col + u0), c1 = h1 ? ld_pair(col + u1) : make_uint2(0u, 0u);
+      const uint32_t b0 = c0.x, d0 = c0.y - c0.x, b1 = c1.x, d1 = c1.y - c1.x;
       er += (unsigned long long)d0 + d1;
+      uint32_t w[kEdgeRun], su[kEdgeRun];
+      if (d0 <= 4 && d1 <= 4) {
+        short_list<0>(row, ned_al, b0, d0, w);
+        short_list<4>(row, ned_al, b1, d1, w);
+#pragma unroll
+        for (int t = 0; t < kEdgeRun; ++t) su[t] = t < 4 ? s0 : s1;
+        dag += push_run(st, lst, e, here, reach, tag0, tnext, w, ((1u << d0) - 1u) | (((1u << d1) - 1u) << 4), su,
+                        &all);
+        continue;
+      }
       for (uint32_t p0 = 0; p0 < d0 + d1; p0 += kEdgeRun) {
-        uint32_t w[kEdgeRun];
-        uint64_t word[kEdgeRun];
+        uint32_t valid = 0;
 #pragma unroll
         for (int t = 0; t < kEdgeRun; ++t) {
           const uint32_t q = p0 + t;
           w[t] = q < d0 ? row[b0 + q] : (q < d0 + d1 ? row[b1 + (q - d0)] : 0u);
+          su[t] = q < d0 ? s0 : s1;
+          valid |= (q < d0 + d1 ? 1u : 0u) << t;
         }
-#pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t) word[t] = p0 + t < d0 + d1 ? ld_wg(st + w[t]) : ~0ull;
-        // unvisited in this batch: claim with tag and sigma at once (on
-        // failure word[t] receives the winner's word)
-        bool won[kEdgeRun];
-#pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t) {
-          const uint32_t su = p0 + t < d0 ? s0 : s1;
-          won[t] = (uint32_t)(word[t] >> 32) < tag0 &&
-                   __hip_atomic_compare_exchange_strong(st + w[t], &word[t], ((uint64_t)tnext << 32) | su,
-                                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-#pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t) {
-          const uint32_t su = p0 + t < d0 ? s0 : s1;
-          if (won[t]) {
-            lst[e + atomicAdd(here, 1u)] = w[t];
-            ++dag;
-            if (reach) all += reach[w[t]];
-          } else if ((uint32_t)(word[t] >> 32) == tnext) {
-            // the low half of the word: sigma (uint32, wrapping)
-            __hip_atomic_fetch_add((uint32_t *)(st + w[t]), su, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            ++dag;
-          }
-        }
+        dag += push_run(st, lst, e, here, reach, tag0, tnext, w, valid, su, &all);
       }
     }
     __syncthreads();
@@ -189,41 +284,35 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
   for (int d = L - 1; d >= dmin; --d) {
     const uint32_t db = off[d], de = off[d + 1];
     const uint32_t tnext = tag0 + (uint32_t)d + 1u;
-    // two vertices per thread, their edges walked kEdgeRun at a time with
-    // the loads of a run issued together; each vertex folds its terms in
-    // its own adjacency order
+    // two vertices per thread, their edges walked as in the forward sweep;
+    // each vertex folds its terms in its own adjacency order
     for (uint32_t i0 = db + tid; i0 < de; i0 += 2 * kWgBlock) {
       const uint32_t i1 = i0 + kWgBlock;
       const bool h1 = i1 < de;
       const uint32_t v0 = lst[i0], v1 = h1 ? lst[i1] : 0u;
       const double sv0 = (double)(uint32_t)st[v0], sv1 = h1 ? (double)(uint32_t)st[v1] : 1.0;
       double acc0 = reach ? (double)reach[v0] : 0.0, acc1 = reach && h1 ? (double)reach[v1] : 0.0;
-      const uint32_t b0 = col[v0], d0 = col[v0 + 1] - b0;
-      const uint32_t b1 = h1 ? col[v1] : 0u, d1 = h1 ? col[v1 + 1] - b1 : 0u;
+      const uint2 c0 = ld_pair(col + v0), c1 = h1 ? ld_pair(col + v1) : make_uint2(0u, 0u);
+      const uint32_t b0 = c0.x, d0 = c0.y - c0.x, b1 = c1.x, d1 = c1.y - c1.x;
       uint32_t terms = 0;
-      for (uint32_t p0 = 0; p0 < d0 + d1; p0 += kEdgeRun) {
-        uint32_t w[kEdgeRun];
-        uint64_t word[kEdgeRun];
-        double x[kEdgeRun];
+      uint32_t w[kEdgeRun];
+      if (d0 <= 4 && d1 <= 4) {
+        short_list<0>(row, ned_al, b0, d0, w);
+        short_list<4>(row, ned_al, b1, d1, w);
+        pull_run(st, dl, tnext, w, ((1u << d0) - 1u) | (((1u << d1) - 1u) << 4), 4, sv0, sv1, acc0, acc1, terms);
+      } else {
+        for (uint32_t p0 = 0; p0 < d0 + d1; p0 += kEdgeRun) {
+          uint32_t valid = 0;
 #pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t) {
-          const uint32_t q = p0 + t;
-          w[t] = q < d0 ? row[b0 + q] : (q < d0 + d1 ? row[b1 + (q - d0)] : 0u);
-        }
-#pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t) word[t] = p0 + t < d0 + d1 ? st[w[t]] : 0ull;
-#pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t) x[t] = (uint32_t)(word[t] >> 32) == tnext ? dl[w[t]] : 0.0;
-#pragma unroll
-        for (int t = 0; t < kEdgeRun; ++t)
-          if ((uint32_t)(word[t] >> 32) == tnext) {
-            const double xt = (x[t] + 1.0) / (double)(uint32_t)word[t];
-            if (p0 + t < d0)
-              acc0 += xt * sv0;
-            else
-              acc1 += xt * sv1;
-            ++terms;
+          for (int t = 0; t < kEdgeRun; ++t) {
+            const uint32_t q = p0 + t;
+            w[t] = q < d0 ? row[b0 + q] : (q < d0 + d1 ? row[b1 + (q - d0)] : 0u);
+            valid |= (q < d0 + d1 ? 1u : 0u) << t;
           }
+          // slots of the first vertex: those below d0 - p0
+          const int first1 = p0 >= d0 ? 0 : (int)min(d0 - p0, (uint32_t)kEdgeRun);
+          pull_run(st, dl, tnext, w, valid, first1, sv0, sv1, acc0, acc1, terms);
+        }
       }
       dl[v0] = acc0;
       if (h1) dl[v1] = acc1;
@@ -302,9 +391,11 @@ namespace axbc {
 hipError_t wg_sweep(hipStream_t s, int nb, uint32_t n, const uint32_t *col, const uint32_t *row,
                     const uint32_t *reach, const uint32_t *src, uint32_t tag0, uint64_t *pst, double *pdl,
                     uint32_t *plst, uint32_t *ploff, uint32_t lcap, unsigned long long *cnt, int32_t *nlev,
-                    uint32_t *ovf, unsigned long long *kb) {
+                    uint32_t *ovf, unsigned long long *kb, uint64_t ned) {
+  // short lists load 16-byte windows of row only when it starts 16-byte aligned
+  const uint64_t ned_al = ((uintptr_t)row & 15u) ? 0ull : ned;
   hipLaunchKernelGGL(k_wgsweep, dim3(nb), dim3(kWgBlock), 0, s, n, col, row, reach, src, tag0, pst, pdl, plst, ploff,
-                     lcap, cnt, nlev, ovf, kb);
+                     lcap, cnt, nlev, ovf, kb, ned_al);
   return hipGetLastError();
 }
 
