@@ -176,8 +176,17 @@ struct BatchState {
   double *wdl = nullptr;
   uint32_t *wlst = nullptr, *wloff = nullptr, *wovf = nullptr;
   int32_t *wnlev = nullptr;
-  int32_t *h_wnlev = nullptr;  // pinned: B deepest levels + the overflow flag
+  int32_t *h_wnlev = nullptr;  // pinned: wlanes deepest levels + the overflow flag
+  uint32_t *wsrc = nullptr;    // roots of the sweep by sweep lane (batches above B roots)
+  unsigned long long *wcnt = nullptr;  // their counters (4 per sweep lane)
   uint32_t wlcap = 0;
+  int wlanes = 0;          // sweep lanes: B, or 2B at width 2B
+  std::vector<uint32_t> h_wlane;  // sweep lane of the last batch's k-th root (when it was a workgroup sweep)
+  // a batch of more than B roots: its halves' lane maps (draw index within
+  // the half -> lane), the last half's also in h_lane_of
+  bool wg_pair = false;
+  int pair_nb[2] = {0, 0};
+  std::vector<uint32_t> h_lane_half[2];
   uint32_t wtag = 1;       // tag of level 0 for the next workgroup sweep
   uint32_t wtag_last = 0;  // tag of level 0 of the last batch (when it was a workgroup sweep)
   bool last_wg = false;    // the last batch ran as a workgroup sweep
@@ -2409,7 +2418,7 @@ void batch_free(axbc_ctx *c) {
   void *d[] = {b->vis, b->nx, b->fr[0], b->fr[1], b->sig, b->dlt, b->xr, b->mark, b->seen, b->q, b->qw, b->qoff,
                b->act, b->sp_e, b->sp_base, b->part, b->cnt, b->src, b->prob, b->r0,
                b->lane_of, b->check, b->ctl, b->wslot, b->lslot, b->lmode, b->prog, b->bar, b->row_out, b->util_v, b->util_w, b->util_part, b->util_out,
-               b->wst, b->wdl, b->wlst, b->wloff, b->wnlev};  // wovf lies inside wnlev
+               b->wst, b->wdl, b->wlst, b->wloff, b->wnlev, b->wsrc, b->wcnt};  // wovf lies inside wnlev
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_wnlev) (void)hipHostFree(b->h_wnlev);
@@ -2873,45 +2882,54 @@ bool use_wg(const axbc_ctx *c) {
 // taken off the totals) when one of them cannot be had.
 void wg_release(axbc_ctx *c) {
   BatchState *b = c->batch;
-  const size_t B = (size_t)b->B, n = c->n;
+  const size_t W = (size_t)b->wlanes, n = c->n;
   size_t held = 0;
-  if (b->wst) held += B * n * 8;
-  if (b->wdl) held += B * n * 8;
-  if (b->wlst) held += B * n * 4;
-  if (b->wloff) held += B * b->wlcap * 4;
-  if (b->wnlev) held += (B + 1) * 4;
-  void *d[] = {b->wst, b->wdl, b->wlst, b->wloff, b->wnlev};
+  if (b->wst) held += W * n * 8;
+  if (b->wdl) held += W * n * 8;
+  if (b->wlst) held += W * n * 4;
+  if (b->wloff) held += W * b->wlcap * 4;
+  if (b->wnlev) held += (W + 1) * 4;
+  if (b->wsrc) held += W * 4;
+  if (b->wcnt) held += W * 4 * 8;
+  void *d[] = {b->wst, b->wdl, b->wlst, b->wloff, b->wnlev, b->wsrc, b->wcnt};
   for (void *p : d)
     if (p) (void)hipFree(p);
   if (b->h_wnlev) (void)hipHostFree(b->h_wnlev);
   b->wst = nullptr;
   b->wdl = nullptr;
-  b->wlst = b->wloff = b->wovf = nullptr;
+  b->wlst = b->wloff = b->wovf = b->wsrc = nullptr;
   b->wnlev = b->h_wnlev = nullptr;
+  b->wcnt = nullptr;
   b->bytes -= std::min(b->bytes, held);
   c->bytes -= std::min(c->bytes, held);
   b->wlcap = 0;
+  b->wlanes = 0;
 }
 
+// Sweep lanes: B, or 2B when the context's width is twice the state's
+// (width 256 with the workgroup sweep: roots on all CUs).
 int wg_alloc(axbc_ctx *c) {
   BatchState *b = c->batch;
-  const size_t B = (size_t)b->B, n = c->n;
+  const size_t W = (size_t)(c->batch_width > b->B ? 2 * b->B : b->B), n = c->n;
   const uint32_t lcap = (uint32_t)std::min<uint64_t>((uint64_t)n + 2, kWgFirstLevels);
   int rc;
-  if ((rc = budget_check(c, B * n * (8 + 8 + 4) + B * lcap * 4 + (B + 1) * 4, "the workgroup sweep state")))
+  if ((rc = budget_check(c, W * n * (8 + 8 + 4) + W * lcap * 4 + (W + 1) * 4 + W * 4 + W * 32,
+                         "the workgroup sweep state")))
     return rc;
   b->wlcap = lcap;
+  b->wlanes = (int)W;
   const size_t before = b->bytes;
-  if ((rc = balloc(b, &b->wst, B * n)) || (rc = balloc(b, &b->wdl, B * n)) || (rc = balloc(b, &b->wlst, B * n)) ||
-      (rc = balloc(b, &b->wloff, B * lcap)) || (rc = balloc(b, &b->wnlev, B + 1))) {
+  if ((rc = balloc(b, &b->wst, W * n)) || (rc = balloc(b, &b->wdl, W * n)) || (rc = balloc(b, &b->wlst, W * n)) ||
+      (rc = balloc(b, &b->wloff, W * lcap)) || (rc = balloc(b, &b->wnlev, W + 1)) || (rc = balloc(b, &b->wsrc, W)) ||
+      (rc = balloc(b, &b->wcnt, 4 * W))) {
     c->bytes += b->bytes - before;
     wg_release(c);
     return rc;
   }
   c->bytes += b->bytes - before;
-  b->wovf = (uint32_t *)(b->wnlev + B);
-  if (hipHostMalloc((void **)&b->h_wnlev, sizeof(int32_t) * (B + 1), hipHostMallocDefault) != hipSuccess ||
-      hipMemsetAsync(b->wst, 0, sizeof(uint64_t) * B * n, c->stream) != hipSuccess) {
+  b->wovf = (uint32_t *)(b->wnlev + W);
+  if (hipHostMalloc((void **)&b->h_wnlev, sizeof(int32_t) * (W + 1), hipHostMallocDefault) != hipSuccess ||
+      hipMemsetAsync(b->wst, 0, sizeof(uint64_t) * W * n, c->stream) != hipSuccess) {
     wg_release(c);
     return fail(AXBC_EHIP, "workgroup sweep state set-up failed%s (line %d)", "", __LINE__);
   }
@@ -2919,61 +2937,77 @@ int wg_alloc(axbc_ctx *c) {
   return AXBC_OK;
 }
 
-// Workgroup sweep of a batch whose roots and lanes are on the device: the
-// per-root rounds (wgsweep.hip), then the batch's delta rows and seen
-// stamps, which the stop test and the bc update read as after a bit-packed
-// sweep.  A root deeper than the level-offset capacity runs the batch again
-// with four times the capacity (bc untouched: the update has not run).
-template <int NW>
-int wg_batch(axbc_ctx *c, int nb, std::vector<int> &nlev) {
+// The per-root rounds of nr roots (src: device array by sweep lane) into the
+// workgroup sweep state; per-lane counters to cnt, deepest levels to
+// nlev[0 .. nr).  Returns the tag of level 0.  A root deeper than the
+// level-offset capacity makes the sweep run again with four times the
+// capacity (nothing outside the sweep state has changed).
+int wg_run(axbc_ctx *c, int nr, const uint32_t *src, unsigned long long *cnt, std::vector<int> &nlev,
+           uint32_t *tag_out) {
   BatchState *b = c->batch;
   hipStream_t s = c->stream;
   const uint32_t n = c->n;
-  const size_t B = (size_t)b->B;
+  const size_t W = (size_t)b->wlanes;
   int rc;
-  if (!b->wst && (rc = wg_alloc(c))) return rc;
-  b->have = true;
-  b->stamp += 1;
   for (;;) {
     // a batch uses the tags wtag .. wtag + its deepest level: start over
     // (every word unvisited) before they could wrap
     if ((uint64_t)b->wtag + n + 2 > 0xFFFFFFFFull) {
-      HIP_TRY(hipMemsetAsync(b->wst, 0, sizeof(uint64_t) * B * n, s));
+      HIP_TRY(hipMemsetAsync(b->wst, 0, sizeof(uint64_t) * W * n, s));
       b->wtag = 1;
     }
     const uint32_t tag0 = b->wtag;
     HIP_TRY(hipMemsetAsync(b->wovf, 0, sizeof(uint32_t), s));
     kt_begin(c, KK_WGSWEEP);
-    HIP_TRY(wg_sweep(s, nb, n, c->col, c->row, c->nr ? c->reach : nullptr, b->src, tag0, b->wst, b->wdl, b->wlst,
-                     b->wloff, b->wlcap, b->cnt, b->wnlev, b->wovf, kt_bytes(c, KK_WGSWEEP), c->ned));
+    HIP_TRY(wg_sweep(s, nr, n, c->col, c->row, c->nr ? c->reach : nullptr, src, tag0, b->wst, b->wdl, b->wlst,
+                     b->wloff, b->wlcap, cnt, b->wnlev, b->wovf, kt_bytes(c, KK_WGSWEEP), c->ned));
     kt_end(c, KK_WGSWEEP);
-    HIP_TRY(hipMemcpyAsync(b->h_wnlev, b->wnlev, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(b->h_wnlev, b->wnlev, sizeof(int32_t) * (W + 1), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     int maxl = 0;
-    for (int l = 0; l < nb; ++l) maxl = std::max(maxl, (int)b->h_wnlev[l]);
+    for (int l = 0; l < nr; ++l) maxl = std::max(maxl, (int)b->h_wnlev[l]);
     b->wtag += (uint32_t)maxl + 2u;
-    if (b->h_wnlev[B] == 0) {
-      HIP_TRY(wg_out(s, NW, n, nb, tag0, b->wst, b->wdl, c->reach, b->dlt, b->seen, b->stamp));
-      for (int l = 0; l < nb; ++l) nlev[l] = b->h_wnlev[l];
+    if (b->h_wnlev[W] == 0) {
+      nlev.assign(b->h_wnlev, b->h_wnlev + nr);
       b->wtag_last = tag0;
       b->levels = maxl + 1;
       b->lev_guess = maxl + 2;
-      b->last_wg = true;
+      *tag_out = tag0;
       return AXBC_OK;
     }
     if (b->wlcap >= n + 2)
       return fail(AXBC_ESTATE, "workgroup sweep: more levels than vertices%s (line %d)", "", __LINE__);
-    const size_t old = sizeof(uint32_t) * B * b->wlcap;
+    const size_t old = sizeof(uint32_t) * W * b->wlcap;
     (void)hipFree(b->wloff);
     b->wloff = nullptr;
     b->bytes -= std::min(b->bytes, old);
     c->bytes -= std::min(c->bytes, old);
     b->wlcap = (uint32_t)std::min<uint64_t>((uint64_t)n + 2, (uint64_t)b->wlcap * 4);
-    if ((rc = balloc_more(c, &b->wloff, B * b->wlcap, "the workgroup sweep level offsets"))) {
+    if ((rc = balloc_more(c, &b->wloff, W * b->wlcap, "the workgroup sweep level offsets"))) {
       wg_release(c);  // the next batch starts over
       return rc;
     }
   }
+}
+
+// Workgroup sweep of a batch of at most B roots whose roots and lanes are on
+// the device: the per-root rounds, then the batch's delta rows and seen
+// stamps, which the stop test and the bc update read as after a bit-packed
+// sweep.
+template <int NW>
+int wg_batch(axbc_ctx *c, int nb, std::vector<int> &nlev) {
+  BatchState *b = c->batch;
+  int rc;
+  if (!b->wst && (rc = wg_alloc(c))) return rc;
+  b->have = true;
+  b->stamp += 1;
+  uint32_t tag0 = 0;
+  if ((rc = wg_run(c, nb, b->src, b->cnt, nlev, &tag0))) return rc;
+  HIP_TRY(wg_out(c->stream, NW, c->n, nb, 0, tag0, b->wst, b->wdl, c->reach, b->dlt, b->seen, b->stamp));
+  b->h_wlane.assign(b->h_lane_of.begin(), b->h_lane_of.end());
+  b->last_wg = true;
+  b->wg_pair = false;
+  return AXBC_OK;
 }
 
 // The bit-packed sweep of a batch whose roots, probabilities and lanes are
@@ -3109,60 +3143,62 @@ int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs
   }
   HIP_TRY(hipGetLastError());
   b->last_wg = false;
+  b->wg_pair = false;
   return AXBC_OK;
 }
 
-template <int NW>
-int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
-              axbc_pivot_stats *stats) {
-  BatchState *b = c->batch;
-  hipStream_t s = c->stream;
-  const uint32_t n = c->n;
-  int rc;
-  std::vector<uint32_t> r0(nb);
-  for (int k = 0; k < nb; ++k) r0[k] = c->h_reach[v0s[k]];
-  // lanes: the roots by degree, highest first (ties in draw order).  Roots
-  // of similar degree have similar level structure, so the roots that meet
-  // at an edge of the sweep sit in fewer cache lines of the root-minor rows.
+// Lanes of a group of at most B roots: by degree, highest first (ties in
+// draw order).  Roots of similar degree have similar level structure, so the
+// roots that meet at an edge of the sweep sit in fewer cache lines of the
+// root-minor rows.  lane_of[k] = lane of the k-th root (B entries, identity
+// past nb); src/p/r0 by lane.
+struct Lanes {
+  std::vector<uint32_t> lane_of, src, r0;
+  std::vector<double> p;
+};
+
+Lanes assign_lanes(const axbc_ctx *c, int B, const uint32_t *v0s, const double *ps, int nb) {
+  Lanes ln;
   std::vector<uint32_t> root_at(nb);
   for (int k = 0; k < nb; ++k) root_at[k] = (uint32_t)k;
   std::stable_sort(root_at.begin(), root_at.end(), [&](uint32_t a, uint32_t b2) {
     return c->h_col[v0s[a] + 1] - c->h_col[v0s[a]] > c->h_col[v0s[b2] + 1] - c->h_col[v0s[b2]];
   });
-  std::vector<uint32_t> &lane_of = b->h_lane_of;
-  lane_of.resize(b->B);
-  for (int l = 0; l < b->B; ++l) lane_of[l] = (uint32_t)l;  // lanes past nb are unused
-  std::vector<uint32_t> src_l(nb), r0_l(nb);
-  std::vector<double> p_l(nb);
+  ln.lane_of.resize(B);
+  for (int l = 0; l < B; ++l) ln.lane_of[l] = (uint32_t)l;  // lanes past nb are unused
+  ln.src.resize(nb);
+  ln.r0.resize(nb);
+  ln.p.resize(nb);
   for (int l = 0; l < nb; ++l) {
     const uint32_t k = root_at[l];
-    lane_of[k] = (uint32_t)l;
-    src_l[l] = v0s[k];
-    r0_l[l] = r0[k];
-    p_l[l] = ps[k];
+    ln.lane_of[k] = (uint32_t)l;
+    ln.src[l] = v0s[k];
+    ln.r0[l] = c->h_reach[v0s[k]];
+    ln.p[l] = ps[k];
   }
-  // split kernels only where some vertex has more edges than a chunk
-  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
-  const bool wg = use_wg(c);
-  // the level loop runs from prebuilt graphs unless per-kernel timing is on
-  const bool graphs = !wg && c->use_graphs && !c->ktimer.on;
-  if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
-  HIP_TRY(hipMemcpyAsync(b->src, src_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->prob, p_l.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->r0, r0_l.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->lane_of, lane_of.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
-  // undo the previous bit-packed sweep (a workgroup sweep leaves the
-  // bit-packed state as it found it)
-  if (b->have && !b->last_wg)
-    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
-                       b->vis, b->fr[0], b->fr[1], b->sig);
-  std::vector<int> nlev(nb, 0);  // per lane
-  if (wg)
-    rc = wg_batch<NW>(c, nb, nlev);
-  else
-    rc = packed_sweep<NW>(c, nb, split_out, split_in, graphs, nlev);
-  if (rc) return rc;
-  // exact stop test per root, then the bc update for the applied roots
+  return ln;
+}
+
+int upload_lanes(axbc_ctx *c, const Lanes &ln, int nb) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(b->src, ln.src.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->prob, ln.p.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->r0, ln.r0.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->lane_of, ln.lane_of.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  return AXBC_OK;
+}
+
+// After a sweep of nb roots whose lanes, counters (b->cnt) and delta rows
+// (b->dlt) are in place: the exact stop test per root, the bc update of the
+// applied roots, the context's delta row, and the stats (nb entries, by
+// draw order; nlev by lane).  *applied = roots applied.
+template <int NW>
+int finish_batch(axbc_ctx *c, int nb, uint32_t stop_c, const Lanes &ln, const std::vector<int> &nlev, int *applied,
+                 axbc_pivot_stats *stats) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
   int nr = nb;
   if (stop_c > 0 && c->ncheck > 0) {
     unsigned long long *okb = (unsigned long long *)b->part;  // scratch, rows are free now
@@ -3188,7 +3224,7 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
   // the context's delta becomes the last applied root's (the -z 4
   // distribution of the next draw); plain rounds re-initialise it
   if (nr > 0) {
-    hipLaunchKernelGGL(k_bcopy_delta<NW>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (int)lane_of[nr - 1], b->dlt,
+    hipLaunchKernelGGL(k_bcopy_delta<NW>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, (int)ln.lane_of[nr - 1], b->dlt,
                        c->delta);
     c->dirty_delta = true;
   }
@@ -3197,14 +3233,111 @@ int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32
     HIP_TRY(hipMemcpyAsync(b->h_cnt, b->cnt, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int k = 0; k < nb; ++k) {
-      const uint32_t l = lane_of[k];
+      const uint32_t l = ln.lane_of[k];
       stats[k].nvisited = b->h_cnt[4 * l + 0];
       stats[k].nlevels = (uint64_t)nlev[l];
       stats[k].e_r = b->h_cnt[4 * l + 2];
       stats[k].e_dag = b->h_cnt[4 * l + 3];
-      stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * l + 1] * (c->heuristic == 1 || c->heuristic == 3) + r0[k]);
+      stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * l + 1] * (c->heuristic == 1 || c->heuristic == 3) + ln.r0[l]);
     }
   }
+  return AXBC_OK;
+}
+
+template <int NW>
+int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
+              axbc_pivot_stats *stats) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  int rc;
+  const Lanes ln = assign_lanes(c, b->B, v0s, ps, nb);
+  b->h_lane_of = ln.lane_of;
+  // split kernels only where some vertex has more edges than a chunk
+  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
+  const bool wg = use_wg(c);
+  // the level loop runs from prebuilt graphs unless per-kernel timing is on
+  const bool graphs = !wg && c->use_graphs && !c->ktimer.on;
+  if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
+  if ((rc = upload_lanes(c, ln, nb))) return rc;
+  // undo the previous bit-packed sweep (a workgroup sweep leaves the
+  // bit-packed state as it found it)
+  if (b->have && !b->last_wg)
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
+                       b->vis, b->fr[0], b->fr[1], b->sig);
+  std::vector<int> nlev(nb, 0);  // per lane
+  if (wg)
+    rc = wg_batch<NW>(c, nb, nlev);
+  else
+    rc = packed_sweep<NW>(c, nb, split_out, split_in, graphs, nlev);
+  if (rc) return rc;
+  return finish_batch<NW>(c, nb, stop_c, ln, nlev, applied, stats);
+}
+
+// A batch of B < nb <= 2B roots (width 2B, workgroup sweep only): one sweep
+// of all of them (a workgroup each), then the stop test and the bc update
+// for the first B roots in draw order and, unless the stop test held among
+// them, for the rest -- the sequence of one-root rounds, as for any batch.
+template <int NW>
+int run_wg_pair(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
+                axbc_pivot_stats *stats) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  const int B = b->B;
+  int rc;
+  if (!b->wst && (rc = wg_alloc(c))) return rc;
+  if (b->wlanes < nb) return fail(AXBC_ESTATE, "workgroup sweep state has too few lanes%s (line %d)", "", __LINE__);
+  const int nh[2] = {B, nb - B};
+  const Lanes ln[2] = {assign_lanes(c, B, v0s, ps, nh[0]), assign_lanes(c, B, v0s + B, ps + B, nh[1])};
+  // sweep lane of half h's lane l: h B + l
+  std::vector<uint32_t> src(ln[0].src);
+  src.insert(src.end(), ln[1].src.begin(), ln[1].src.end());
+  HIP_TRY(hipMemcpyAsync(b->wsrc, src.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, s));
+  if (b->have && !b->last_wg)
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
+                       b->vis, b->fr[0], b->fr[1], b->sig);
+  b->have = true;
+  b->stamp += 1;
+  std::vector<int> nlev_all;
+  uint32_t tag0 = 0;
+  if ((rc = wg_run(c, nb, b->wsrc, b->wcnt, nlev_all, &tag0))) return rc;
+  b->last_wg = true;
+  b->wg_pair = true;
+  b->h_wlane.resize(nb);
+  for (int k = 0; k < nb; ++k) b->h_wlane[k] = (uint32_t)((k / B) * B) + ln[k / B].lane_of[k % B];
+  int done = 0;
+  for (int h = 0; h < 2; ++h) {
+    b->pair_nb[h] = nh[h];
+    b->h_lane_half[h] = ln[h].lane_of;
+    b->h_lane_of = ln[h].lane_of;
+    if (h == 1 && done < B) continue;  // the stop test held in the first half
+    if ((rc = upload_lanes(c, ln[h], nh[h]))) return rc;
+    HIP_TRY(hipMemcpyAsync(b->cnt, b->wcnt + (size_t)4 * B * h, sizeof(unsigned long long) * 4 * nh[h],
+                           hipMemcpyDeviceToDevice, s));
+    HIP_TRY(wg_out(s, NW, n, nh[h], h * B, tag0, b->wst, b->wdl, c->reach, b->dlt, b->seen, b->stamp));
+    const std::vector<int> nlev(nlev_all.begin() + h * B, nlev_all.begin() + h * B + nh[h]);
+    int ap = 0;
+    if ((rc = finish_batch<NW>(c, nh[h], stop_c, ln[h], nlev, &ap, stats ? stats + h * B : nullptr))) return rc;
+    done += ap;
+  }
+  if (stats && done < B) {
+    // the second half ran but was not applied: its counters all the same
+    std::vector<unsigned long long> h(4 * (size_t)nh[1]);
+    HIP_TRY(hipMemcpyAsync(h.data(), b->wcnt + (size_t)4 * B, sizeof(unsigned long long) * h.size(),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < nh[1]; ++k) {
+      const uint32_t l = ln[1].lane_of[k];
+      axbc_pivot_stats &st = stats[B + k];
+      st.nvisited = h[4 * l + 0];
+      st.nlevels = (uint64_t)nlev_all[B + l];
+      st.e_r = h[4 * l + 2];
+      st.e_dag = h[4 * l + 3];
+      st.all = (uint32_t)((uint32_t)h[4 * l + 1] * (c->heuristic == 1 || c->heuristic == 3) + ln[1].r0[l]);
+    }
+  }
+  if (applied) *applied = done;
   return AXBC_OK;
 }
 
@@ -3238,22 +3371,48 @@ int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double
   for (int k = 0; k < B; ++k) out[k] = per_lane[b->h_lane_of.empty() ? k : b->h_lane_of[k]];
   return AXBC_OK;
 }
+
+// The utility of the last batch's roots, out[k] for k < the context's
+// width: after a workgroup sweep of more than B roots, half by half, each
+// half's delta rows written again from the sweep state first (the rows and
+// lane map left in place are the last half's, as after the batch).
+int batch_utility_any(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
+                      const double *weight, uint32_t count, double *out) {
+  BatchState *b = c->batch;
+  if (!b->wg_pair) return batch_utility_on(c, dv, dw, part, dout, vertex, weight, count, out);
+  int rc;
+  for (int h = 0; h < 2; ++h) {
+    HIP_TRY(wg_out(c->stream, b->nw, c->n, b->pair_nb[h], h * b->B, b->wtag_last, b->wst, b->wdl, c->reach, b->dlt,
+                   b->seen, b->stamp));
+    b->h_lane_of = b->h_lane_half[h];
+    if ((rc = batch_utility_on(c, dv, dw, part, dout, vertex, weight, count, out + (size_t)h * b->B))) return rc;
+  }
+  return AXBC_OK;
+}
 }  // namespace axbc
 
 namespace axbc {
 // Root k's dependency row of the last batch gathered into the batch's row
 // buffer on the device (stream-ordered); *row points at it.
 int batch_delta_row(axbc_ctx *c, int k, const double **row) {
-  if (!c || !c->batch || !row || k < 0 || k >= c->batch->B)
+  if (!c || !c->batch || !row || k < 0 || k >= c->batch_width)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
   BatchState *b = c->batch;
-  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
   int rc;
   HIP_TRY(hipSetDevice(c->device));
   if (!b->row_out) {
     if ((rc = balloc_more(c, (double **)&b->row_out, c->n, "a per-root row copy"))) return rc;
   }
   double *tmp = (double *)b->row_out;
+  if (b->last_wg && k < (int)b->h_wlane.size()) {
+    // workgroup sweep: from the root's own state (reach where it did not reach)
+    const size_t wl = b->h_wlane[k];
+    HIP_TRY(wg_delta(c->stream, c->n, b->wtag_last, b->wst + wl * c->n, b->wdl + wl * c->n, c->reach, tmp));
+    *row = tmp;
+    return AXBC_OK;
+  }
+  if (k >= b->B) return fail(AXBC_EINVAL, "no such root in the last batch%s (line %d)", "", __LINE__);
+  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
   if (b->nw == 1)
     hipLaunchKernelGGL(k_bcopy_delta<1>, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, lane, b->dlt, tmp);
   else
@@ -3267,8 +3426,13 @@ int batch_delta_row(axbc_ctx *c, int k, const double **row) {
 extern "C" {
 
 int axbc_set_batch_width(axbc_ctx *c, int width) {
-  if (!c || (width != 64 && width != 128)) return fail(AXBC_EINVAL, "batch width must be 64 or 128%s (line %d)", "", __LINE__);
-  if (c->batch && c->batch->B != width) batch_free(c);
+  if (!c || (width != 64 && width != 128 && width != 256))
+    return fail(AXBC_EINVAL, "batch width must be 64, 128 or 256%s (line %d)", "", __LINE__);
+  if (width == 256 && c->batch_mode != 2)
+    return fail(AXBC_EINVAL, "batch width 256 needs the workgroup sweep (axbc_set_batch_mode 2)%s (line %d)", "",
+                __LINE__);
+  // the state's lanes (and the workgroup sweep's) follow the width
+  if (c->batch && c->batch_width != width) batch_free(c);
   c->batch_width = width;
   return AXBC_OK;
 }
@@ -3300,6 +3464,8 @@ int axbc_set_batch_persistent(axbc_ctx *c, int mode, double limit_seconds) {
 
 int axbc_set_batch_mode(axbc_ctx *c, int mode) {
   if (!c || mode < 0 || mode > 2) return fail(AXBC_EINVAL, "batch mode must be 0, 1 or 2%s (line %d)", "", __LINE__);
+  if (c->batch_width > 128 && mode != 2)
+    return fail(AXBC_EINVAL, "batch width 256 runs only as a workgroup sweep%s (line %d)", "", __LINE__);
   c->batch_mode = mode;
   return AXBC_OK;
 }
@@ -3335,10 +3501,15 @@ int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint3
     if (v0s[k] >= c->n) return fail(AXBC_EINVAL, "root out of range%s (line %d)", "", __LINE__);
   HIP_TRY(hipSetDevice(c->device));
   int rc;
-  const uint64_t bound = list_bound(c->n, c->batch_width);
+  // the bit-packed state holds at most 128 lanes; width 256 sweeps the
+  // roots of a batch above 128 in one workgroup sweep and applies them in
+  // two halves
+  const int sw = std::min(c->batch_width, 128);
+  const uint64_t bound = list_bound(c->n, sw);
   if (!c->batch) {
-    if ((rc = batch_create(c, c->batch_width, list_first(c->n, c->batch_width)))) return rc;
+    if ((rc = batch_create(c, sw, list_first(c->n, sw)))) return rc;
   }
+  if (nb > c->batch->B) return run_wg_pair<2>(c, v0s, ps, nb, stop_c, applied, stats);
   for (int attempt = 0; attempt < 8; ++attempt) {
     rc = c->batch->nw == 1 ? run_batch<1>(c, v0s, ps, nb, stop_c, applied, stats)
                            : run_batch<2>(c, v0s, ps, nb, stop_c, applied, stats);
@@ -3381,11 +3552,11 @@ int axbc_batch_utility(axbc_ctx *c, const uint32_t *vertex, const double *weight
       return rc;
     b->util_cap = cap;
   }
-  return batch_utility_on(c, b->util_v, b->util_w, b->util_part, b->util_out, vertex, weight, count, out);
+  return batch_utility_any(c, b->util_v, b->util_w, b->util_part, b->util_out, vertex, weight, count, out);
 }
 
 int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
-  if (!c || !c->batch || !delta || k < 0 || k >= c->batch->B)
+  if (!c || !c->batch || !delta || k < 0 || k >= c->batch_width)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
   const double *row = nullptr;
   int rc;
@@ -3396,11 +3567,15 @@ int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
 }
 
 int axbc_batch_levels(axbc_ctx *c, int k, int32_t *lvl, uint32_t *sigma) {
-  if (!c || !c->batch || !lvl || !sigma || k < 0 || k >= c->batch->B)
+  if (!c || !c->batch || !lvl || !sigma || k < 0 || k >= c->batch_width)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
   BatchState *b = c->batch;
   if (!b->have || b->levels <= 0) return fail(AXBC_ESTATE, "no batch has run%s (line %d)", "", __LINE__);
-  const int lane = b->h_lane_of.empty() ? k : (int)b->h_lane_of[k];
+  if (!(b->last_wg && k < (int)b->h_wlane.size()) && k >= b->B)
+    return fail(AXBC_EINVAL, "no such root in the last batch%s (line %d)", "", __LINE__);
+  const int lane = b->last_wg && k < (int)b->h_wlane.size() ? (int)b->h_wlane[k]
+                   : b->h_lane_of.empty()                    ? k
+                                                             : (int)b->h_lane_of[k];
   int rc;
   HIP_TRY(hipSetDevice(c->device));
   if (!b->row_out) {

@@ -183,6 +183,10 @@ constexpr int kUtilWaves = 1024;
 // partial rows, dout B values.
 int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
                      const double *weight, uint32_t count, double *out);
+// The same for every root of the last batch (out: the context's width),
+// also after a workgroup sweep of more than 128 roots.
+int batch_utility_any(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
+                      const double *weight, uint32_t count, double *out);
 // Workgroup sweep (wgsweep.hip): one workgroup per root of the batch, the
 // whole round of each root in one launch, per-root state in the wst / wdl /
 // wlst / wloff arrays of n (wloff: lcap) entries per lane.
@@ -190,10 +194,14 @@ hipError_t wg_sweep(hipStream_t s, int nb, uint32_t n, const uint32_t *col, cons
                     const uint32_t *reach, const uint32_t *src, uint32_t tag0, uint64_t *pst, double *pdl,
                     uint32_t *plst, uint32_t *ploff, uint32_t lcap, unsigned long long *cnt, int32_t *nlev,
                     uint32_t *ovf, unsigned long long *kb, uint64_t ned);
-// The batch's root-minor delta rows (every lane: delta where root l < nb
-// reached the vertex, reach elsewhere) and seen = stamp where some root did.
-hipError_t wg_out(hipStream_t s, int nw, uint32_t n, int nb, uint32_t tag0, const uint64_t *pst, const double *pdl,
-                  const uint32_t *reach, double *dlt, uint32_t *seen, uint32_t stamp);
+// The batch's root-minor delta rows from sweep lanes lane0 .. lane0+nb-1
+// (row lane l: delta where that root reached the vertex, reach elsewhere and
+// for l >= nb) and seen = stamp where one of them did.
+hipError_t wg_out(hipStream_t s, int nw, uint32_t n, int nb, int lane0, uint32_t tag0, const uint64_t *pst,
+                  const double *pdl, const uint32_t *reach, double *dlt, uint32_t *seen, uint32_t stamp);
+// One sweep lane's delta (reach where its root did not reach).
+hipError_t wg_delta(hipStream_t s, uint32_t n, uint32_t tag0, const uint64_t *st, const double *dl,
+                    const uint32_t *reach, double *out);
 // One lane's levels and path counts from its state words.
 hipError_t wg_levels(hipStream_t s, uint32_t n, uint32_t tag0, const uint64_t *st, int32_t *lvl, uint32_t *sigma);
 // Event pair around one launch of kind k when instrumentation is on.

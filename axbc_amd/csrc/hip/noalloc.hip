@@ -71,6 +71,6 @@ extern "C" int axbc_batch_utility_noalloc(axbc_ctx *c, const uint32_t *vertex, c
   unsigned char *base = nullptr;
   int rc;
   if ((rc = scratch_of_device(c->device, &base))) return rc;
-  return batch_utility_on(c, (uint32_t *)(base + kScrVert), (double *)(base + kScrWeight),
-                          (double *)(base + kScrPart), (double *)(base + kScrOut), vertex, weight, count, out);
+  return batch_utility_any(c, (uint32_t *)(base + kScrVert), (double *)(base + kScrWeight),
+                           (double *)(base + kScrPart), (double *)(base + kScrOut), vertex, weight, count, out);
 }

@@ -340,6 +340,7 @@ template <int NW>
 __global__ __launch_bounds__(kBlock) void k_wgout(uint32_t n, int nb, uint32_t tag0, const uint64_t *__restrict__ pst,
                                                   const double *__restrict__ pdl, const uint32_t *__restrict__ reach,
                                                   double *dlt, uint32_t *seen, uint32_t stamp) {
+  // pst / pdl start at the first sweep lane of these rows
   constexpr int B = 64 * NW;
   constexpr int TV = 32;
   __shared__ double tile[TV][B + 1];
@@ -372,6 +373,14 @@ __global__ __launch_bounds__(kBlock) void k_wgout(uint32_t n, int nb, uint32_t t
   }
 }
 
+// One root's delta from its state words and deltas: reach where it did not
+// reach.
+__global__ void k_wgdelta(uint32_t n, uint32_t tag0, const uint64_t *__restrict__ st, const double *__restrict__ dl,
+                          const uint32_t *__restrict__ reach, double *out) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    out[v] = (uint32_t)(st[v] >> 32) >= tag0 ? dl[v] : (double)reach[v];
+}
+
 // One root's levels (-1 unreached) and path counts from its state words.
 __global__ void k_wglevels(uint32_t n, uint32_t tag0, const uint64_t *__restrict__ st, int32_t *lvl,
                            uint32_t *sigma) {
@@ -399,13 +408,21 @@ hipError_t wg_sweep(hipStream_t s, int nb, uint32_t n, const uint32_t *col, cons
   return hipGetLastError();
 }
 
-hipError_t wg_out(hipStream_t s, int nw, uint32_t n, int nb, uint32_t tag0, const uint64_t *pst, const double *pdl,
-                  const uint32_t *reach, double *dlt, uint32_t *seen, uint32_t stamp) {
+hipError_t wg_out(hipStream_t s, int nw, uint32_t n, int nb, int lane0, uint32_t tag0, const uint64_t *pst,
+                  const double *pdl, const uint32_t *reach, double *dlt, uint32_t *seen, uint32_t stamp) {
   const int grid = grid_for((n + 31) / 32, 4096);
+  pst += (size_t)lane0 * n;
+  pdl += (size_t)lane0 * n;
   if (nw == 1)
     hipLaunchKernelGGL(k_wgout<1>, dim3(grid), dim3(kBlock), 0, s, n, nb, tag0, pst, pdl, reach, dlt, seen, stamp);
   else
     hipLaunchKernelGGL(k_wgout<2>, dim3(grid), dim3(kBlock), 0, s, n, nb, tag0, pst, pdl, reach, dlt, seen, stamp);
+  return hipGetLastError();
+}
+
+hipError_t wg_delta(hipStream_t s, uint32_t n, uint32_t tag0, const uint64_t *st, const double *dl,
+                    const uint32_t *reach, double *out) {
+  hipLaunchKernelGGL(k_wgdelta, dim3(grid_for(n)), dim3(kBlock), 0, s, n, tag0, st, dl, reach, out);
   return hipGetLastError();
 }
 

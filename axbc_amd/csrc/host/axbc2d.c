@@ -10,7 +10,8 @@
  * through libaxbc_hip.so (include/axbc.h).
  *
  * Flags of this engine only:
- *   -B W   roots per batch: 0 one root per round (default), 64 or 128
+ *   -B W   roots per batch: 0 one root per round (default), 64 or 128, or 256
+ *          (one workgroup per root: the workgroup sweep, for deep graphs)
  *   -s S   generator seed pair (S, S+1) (default: the reference's 23, 24)
  *   -T T   host threads for graph construction
  *   -q     no per-pivot lines
@@ -68,7 +69,7 @@ static void usage(const char *p) {
           "\t\t -H <0|1>       1-degree reduction\n"
           "\t\t -U             DO NOT make graph Undirected\n"
           "\t\t -R <0|1>       fixed / time-based seed\n"
-          "\t\t -B <0|64|128>  roots per GPU batch\n"
+          "\t\t -B <0|64|128|256>  roots per GPU batch (256: one workgroup per root)\n"
           "\t\t -g             graph generation and preprocessing on the GPU\n",
           p);
   exit(EXIT_FAILURE);
@@ -334,7 +335,7 @@ int main(int argc, char **argv) {
   if (R != 1 || C != 1) die("%s", "R and C must be 1: this engine runs the 1x1 path on one GPU (-p 1x1)");
   if (startv >= N) die("Invalid start vertex: %s", "");
   if (N > 0xFFFFFFF0ull) die("%s", "Number of vertices too big for 32-bit ids");
-  if (width != 0 && width != 64 && width != 128) die("%s", "-B must be 0, 64 or 128");
+  if (width != 0 && width != 64 && width != 128 && width != 256) die("%s", "-B must be 0, 64, 128 or 256");
   if (width && !(distribution == 0 || distribution == 1))
     die("%s", "-B batches need a static distribution (-z 0 or -z 1)");
   if (rootset) random_mode = 0;
@@ -400,6 +401,7 @@ int main(int argc, char **argv) {
   axbc_ctx *ctx = NULL;
   size_t dev_bytes = 0;
   check(axbc_ctx_create(0, N, g.ned, g.col, g.row, g.reach, heuristic, &ctx, &dev_bytes), "axbc_ctx_create");
+  if (width == 256) check(axbc_set_batch_mode(ctx, 2), "axbc_set_batch_mode");
   if (width) check(axbc_set_batch_width(ctx, width), "axbc_set_batch_width");
   printf("Device memory allocated (MB): %zu\n", dev_bytes >> 20);
 

@@ -174,7 +174,7 @@ def run_learned(e: Engine, budget: int = 4096, warm: int = 512, batch: int = 64,
     tw = np.ascontiguousarray(1.0 / bc_a[tv], np.float64)
     r1 = g.reach.astype(np.float64) + 1.0 if g.reach is not None else np.ones(N)
     util = {}
-    out = np.zeros(128)
+    out = np.zeros(256)  # one entry per lane of the widest batch
     while loop.res.nrounds < warm:
         n0 = loop.res.nrounds
         if not loop.step(min(batch, warm - n0)):

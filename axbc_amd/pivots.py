@@ -96,9 +96,9 @@ class PivotLoop:
         self.checks = np.array([uniform_int(self.N) for _ in range(x)], np.uint32)
         engine.set_checks(self.checks)
         if batch:
-            if batch not in (32, 64, 128):
-                raise ValueError("roots per batch: 32, 64 or 128")
-            _check(_ffi.hip().axbc_set_batch_width(engine.ctx, 128 if batch == 128 else 64))
+            if batch not in (32, 64, 128, 256):
+                raise ValueError("roots per batch: 32, 64, 128 or 256 (256: workgroup sweep, batch mode 2)")
+            _check(_ffi.hip().axbc_set_batch_width(engine.ctx, batch if batch >= 128 else 64))
         self.mix = z
         if z == 0:
             uniform_int(self.N)  # discarded initial draw (axbc2d.c:2740)

@@ -128,15 +128,14 @@ def grid_case(args):
     """4096 x 4096 lattice, 4-neighbour, 3% of edges deleted (seed 3), edges of
     the largest component kept with the lattice ids (N = 2^24), -c 0 (the
     default adaptive stop c=1 x=1 holds after the first rounds here).  Two
-    distributions: uniform (-z 0, exact batches of 128) and delta sampling
-    (-z 4, batch-synchronous batches of 128).  With N = 2^24 the reference's
+    distributions: uniform (-z 0, exact batches of 256) and delta sampling
+    (-z 4, batch-synchronous batches of 256).  With N = 2^24 the reference's
     retry cap (int)(N*N) is 0, so a duplicate draw consumes an attempt; the
     concentrated delta distribution makes most -z 4 attempts duplicates, so
-    its pivots/s count rounds that actually ran.  Batches choose their sweep
-    themselves: after the first (bit-packed) batch ran ~8000 levels, one
-    workgroup per root; the bit-packed sweep (batches of 64, level loop from
-    HIP graphs) is timed on the same -z 0 roots for comparison.  Plain rounds
-    launch per level."""
+    its pivots/s count rounds that actually ran.  Batches of 256 roots run as
+    workgroup sweeps (one workgroup, i.e. one CU, per root); the bit-packed
+    sweep (batches of 64, level loop from HIP graphs) is timed on the same
+    -z 0 roots for comparison.  Plain rounds launch per level."""
     from axbc_amd import Engine, Graph, _ffi
     from axbc_amd.pivots import PivotLoop
     L = _ffi.hip()
@@ -169,13 +168,14 @@ def grid_case(args):
         return r
 
     for z in (0, 4):
-        assert L.axbc_set_batch_width(e.ctx, 128) == 0
-        assert L.axbc_set_batch_mode(e.ctx, 0) == 0
+        # the workgroup sweep at width 256: one root per CU
+        assert L.axbc_set_batch_mode(e.ctx, 2) == 0
+        assert L.axbc_set_batch_width(e.ctx, 256) == 0
         e.reset_bc()
         # events and byte counters on the workgroup sweep only (with timing
         # on, a bit-packed level loop launches from the host)
         set_timing_kinds(e, [16])
-        b = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=128), args.steps)
+        b = timed(PivotLoop(e, 2048, c=0, x=1, z=z, batch=256), args.steps)
         set_timing_kinds(e, [])
         if z == 0:
             assert L.axbc_set_batch_width(e.ctx, 64) == 0
@@ -318,7 +318,7 @@ def main():
     ap.add_argument("--pull", type=int, default=-1, help="bottom-up ratio x16 with --batch (default 32)")
     ap.add_argument("--grid", action="store_true",
                     help="high-diameter case instead: 4096^2 lattice, 3%% edges deleted (seed 3), -z 0 and -z 4 "
-                         "(batch-synchronous) in batches of 128 vs plain rounds")
+                         "(batch-synchronous) in batches of 256 vs plain rounds")
     ap.add_argument("--serve", type=int, default=0, metavar="REQUESTS",
                     help="serving case instead: Poisson 20 req/s, SCALE 12-20, ef 8/16/32, budgets 64-2048, -z 6, "
                          "200 GB cap; prints p50/p99 latency and aggregate pivots/s")

@@ -52,7 +52,7 @@ def drive(args, cwd):
     (["-p", "1x1", "-f", "g.txt", "-n", "ten"], "Invalid number of vertices (-n): ten"),
     (["-p", "1x1", "-S", "8", "-r", "a"], "Invalid root vertex (-r): a"),
     (["-p", "1x1", "-S", "8", "-m"], "-m selects the reference's 2D multi-GPU path"),
-    (["-p", "1x1", "-S", "8", "-B", "32"], "-B must be 0, 64 or 128"),
+    (["-p", "1x1", "-S", "8", "-B", "32"], "-B must be 0, 64, 128 or 256"),
     (["-p", "1x1", "-S", "8", "-B", "64", "-z", "4"], "-B batches need a static distribution"),
     (["-p", "1x1", "-S", "8", "-h"], "Usage:"),
     (["-p", "1x1", "-S", "8", "-Y"], "Usage:"),

@@ -186,3 +186,84 @@ def test_automatic_choice_on_a_deep_lattice(require_gpu):
         assert n.value == 2, f"workgroup sweeps: {n.value}"
         ok, err = close(e.bc(), bc_ref)
         assert ok, err
+
+
+def test_width_256_needs_the_workgroup_sweep(require_gpu):
+    g = Graph.lattice(32, 0.03, seed=3, heuristic=0)
+    L = _ffi.hip()
+    with Engine(g) as e:
+        assert L.axbc_set_batch_width(e.ctx, 256) != 0 and b"workgroup sweep" in L.axbc_last_error()
+        _mode(e, 2)
+        assert L.axbc_set_batch_width(e.ctx, 256) == 0
+        assert L.axbc_set_batch_mode(e.ctx, 1) != 0
+        assert L.axbc_set_batch_mode(e.ctx, 0) != 0
+        assert L.axbc_set_batch_width(e.ctx, 128) == 0
+        _mode(e, 1)
+
+
+@pytest.mark.parametrize("stop_at", [None, 40, 170])
+def test_batches_of_256(require_gpu, stop_at):
+    """Width 256: one workgroup sweep of up to 256 roots, applied in two
+    halves of draw order.  200 roots on a 96^2 lattice (and a Kronecker
+    graph): every root's levels, path counts, delta and counters, the
+    utility of all of them and the bc against the oracle; with a stop test
+    that holds after root 40 (first half) or 170 (second half), exactly the
+    roots up to it applied."""
+    lat = Graph.lattice(96, 0.03, seed=3, heuristic=0)
+    kron = Graph.kronecker(10, 16, seed=(1, 2), heuristic=1)
+    for g in ((lat, kron) if stop_at is None else (lat,)):
+        deg = g.degrees()
+        roots = np.random.default_rng(9).choice(np.flatnonzero(deg > 0), size=200, replace=False)
+        probs = np.full(200, 1.0 / g.N)
+        refs = []
+        bc_ref = np.zeros(g.N)
+        bcs = []
+        for v0 in roots:
+            lvl, sig, dl, bc_ref, ost = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)
+            refs.append((lvl, sig, dl, ost))
+            bcs.append(bc_ref.copy())
+        checks, stop_c, want = None, 0, None
+        if stop_at is not None:
+            # a check vertex whose bc grows at root stop_at; probabilities
+            # scaled so that bc/2 crosses c*N (c = 1) between roots stop_at-1
+            # and stop_at (the lattice has no reach terms: bc scales with 1/p)
+            a = next(int(v) for v in np.argsort(-bcs[-1]) if bcs[stop_at][v] > bcs[stop_at - 1][v])
+            mid = 0.5 * (bcs[stop_at][a] + bcs[stop_at - 1][a])
+            probs = probs * (mid / 2.0) / g.N
+            bcs = []
+            bc_ref = np.zeros(g.N)
+            for v0 in roots:
+                _, _, _, bc_ref, _ = oracle.pivot(g, int(v0), probs[0], bc_ref)
+                bcs.append(bc_ref.copy())
+            want = next(k for k in range(200) if bcs[k][a] / 2.0 >= g.N)
+            assert want == stop_at
+            checks, stop_c = np.array([a], np.uint32), 1
+        with Engine(g) as e:
+            _mode(e, 2)
+            assert _ffi.hip().axbc_set_batch_width(e.ctx, 256) == 0
+            if checks is not None:
+                e.set_checks(checks)
+            applied, stats = run_batch(e, roots, probs, stop_c=stop_c)
+            assert applied == (200 if want is None else want + 1), (applied, want)
+            for k in range(200):
+                lvl, sig, dl, ost = refs[k]
+                assert stats[k] == ost, f"root #{k}: {stats[k]} != {ost}"
+                blvl, bsig = e.batch_levels(k)
+                assert np.array_equal(blvl, lvl) and np.array_equal(bsig, sig), f"root #{k}"
+                if checks is None:
+                    ok, err = close(delta_of(e, k), dl)
+                    assert ok, f"root #{k}: delta rel err {err}"
+            if checks is None:
+                ok, err = close(e.bc(), bcs[-1])
+                assert ok, f"bc rel err {err}"
+                verts = np.random.default_rng(3).choice(g.N, size=300, replace=False).astype(np.uint32)
+                wts = np.random.default_rng(4).uniform(0.5, 1.5, size=300)
+                out = np.zeros(256)
+                assert _ffi.hip().axbc_batch_utility(e.ctx, verts.ctypes.data, wts.ctypes.data, 300,
+                                                     out.ctypes.data) == 0
+                for k in range(200):
+                    want_u = float(np.dot(wts, refs[k][2][verts]))
+                    assert abs(out[k] - want_u) <= 1e-9 * max(abs(want_u), 1.0), (k, out[k], want_u)
+            else:
+                ok, err = close(e.bc(), bcs[applied - 1])
+                assert ok, f"bc after the stop: rel err {err}"
