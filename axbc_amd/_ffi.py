@@ -102,6 +102,7 @@ def hip() -> C.CDLL:
         "axbc_set_batch_graphs": (C.c_int, [ctxp, C.c_int]),
         "axbc_set_batch_persistent": (C.c_int, [ctxp, C.c_int, C.c_double]),
         "axbc_set_batch_mode": (C.c_int, [ctxp, C.c_int]),
+        "axbc_set_plain_pull": (C.c_int, [ctxp, C.c_uint32]),
         "axbc_set_memory_budget": (C.c_int, [ctxp, C.c_size_t]),
         "axbc_device_bytes": (C.c_int, [ctxp, C.POINTER(C.c_size_t)]),
         "axbc_draw_window": (C.c_int, [ctxp, C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p]),

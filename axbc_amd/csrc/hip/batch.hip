@@ -2410,6 +2410,26 @@ __global__ __launch_bounds__(kBlock) void k_mirror_check(const uint32_t *__restr
 
 namespace axbc {
 
+int mirror_symmetric(axbc_ctx *c, bool *sym) {
+  if (c->mirror_state < 0) {
+    unsigned int *bad = nullptr;
+    HIP_TRY(hipMalloc((void **)&bad, sizeof(unsigned int)));
+    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(unsigned int), c->stream));
+    hipLaunchKernelGGL(k_mirror_check, dim3(4096), dim3(kBlock), 0, c->stream, c->col, c->row, c->n, bad);
+    unsigned int hb = 1;
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e3 = hipStreamSynchronize(c->stream);
+    (void)hipFree(bad);
+    HIP_TRY(e1);
+    HIP_TRY(e2);
+    HIP_TRY(e3);
+    c->mirror_state = hb ? 0 : 1;
+  }
+  *sym = c->mirror_state == 1;
+  return AXBC_OK;
+}
+
 void batch_free(axbc_ctx *c) {
   BatchState *b = c->batch;
   if (!b) return;
@@ -2466,19 +2486,10 @@ int ensure_transpose(axbc_ctx *c) {
   // strictly increasing lists in which every edge has its reverse: the
   // transpose is the CSC itself, decided on the device without copying it
   {
-    unsigned int *bad = nullptr;
-    HIP_TRY(hipMalloc((void **)&bad, sizeof(unsigned int)));
-    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(unsigned int), c->stream));
-    hipLaunchKernelGGL(k_mirror_check, dim3(4096), dim3(kBlock), 0, c->stream, c->col, c->row, c->n, bad);
-    unsigned int hb = 1;
-    const hipError_t e1 = hipGetLastError();
-    const hipError_t e2 = hipMemcpyAsync(&hb, bad, sizeof(hb), hipMemcpyDeviceToHost, c->stream);
-    const hipError_t e3 = hipStreamSynchronize(c->stream);
-    (void)hipFree(bad);
-    HIP_TRY(e1);
-    HIP_TRY(e2);
-    HIP_TRY(e3);
-    if (!hb) {
+    bool sym = false;
+    int rc;
+    if ((rc = mirror_symmetric(c, &sym))) return rc;
+    if (sym) {
       uint32_t mx = 0;
       for (uint32_t v = 0; v < c->n; ++v) mx = std::max(mx, c->h_col[v + 1] - c->h_col[v]);
       c->max_out = c->max_in = mx;
@@ -3482,6 +3493,12 @@ int axbc_batch_state_bytes(uint64_t n, uint64_t ned, const uint32_t *col, int wi
   b.plan_only = true;
   (void)alloc_buffers(&b);
   *bytes = b.bytes;
+  return AXBC_OK;
+}
+
+int axbc_set_plain_pull(axbc_ctx *c, uint32_t ratio16) {
+  if (!c || ratio16 > (1u << 16)) return fail(AXBC_EINVAL, "pull ratio must be in [0, 65536]%s (line %d)", "", __LINE__);
+  c->plain_pull16 = ratio16;
   return AXBC_OK;
 }
 

@@ -161,9 +161,12 @@ __global__ void k_reset_all(uint32_t n, const uint32_t *__restrict__ reach, int 
   }
 }
 
+// pull16: bottom-up ratio x16 of the round's levels (0: all top-down; only
+// for graphs whose lists are their in-lists).
 __global__ void k_set_root(uint32_t v0, int *lvl, uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
-                           DevCtl *ctl) {
+                           DevCtl *ctl, const uint32_t *__restrict__ col, unsigned long long ned, uint32_t pull16) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t d0 = col[v0 + 1] - col[v0];
   lvl[v0] = 0;
   sigma[v0] = 1;
   queue[0] = v0;
@@ -175,8 +178,12 @@ __global__ void k_set_root(uint32_t v0, int *lvl, uint32_t *sigma, uint32_t *que
   ctl->hub_claim = 0;
   ctl->wg_done = 0;
   ctl->all = 0;
-  ctl->e_r = 0;
+  ctl->e_r = d0;  // E_r: edges of every vertex when it is found
   ctl->e_dag = 0;
+  ctl->ef = 0;
+  ctl->eu = ned - d0;
+  ctl->bu = 0;  // level 1 (the root's neighbours) top-down
+  ctl->pull16 = pull16;
 }
 
 // ---------------------------------------------------------------------------
@@ -268,12 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
                                                        const uint32_t *__restrict__ qoff,
                                                        DevCtl *ctl, uint32_t *hubs, uint64_t *hub_base,
                                                        uint32_t hub_cap, uint64_t item_cap, int L) {
-  if (ld_flag(&ctl->done)) return;
+  if (ld_flag(&ctl->done) || ld_flag(&ctl->bu)) return;
   const uint32_t fb = qoff[L - 1], fe = qoff[L];
   const int lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
-  unsigned long long er = 0, dag = 0;
+  unsigned long long ef = 0, dag = 0;
   for (uint32_t base = fb + wid * kWave; base < fe; base += nw * kWave) {
     const uint32_t i = base + lane;
     uint32_t u = 0, st = 0, d = 0, su = 0;
@@ -283,7 +290,6 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
       d = col[u + 1] - st;
       su = sigma[u];
     }
-    er += d;
     if (d > kHubDeg) {
       claim_hub(ctl, u, d, hubs, hub_base, hub_cap, item_cap);
       d = 0;
@@ -302,14 +308,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
       if (e < total) {
         w = row[ost + (e - oex)];
         fresh = visit_edge(w, osu, L, lvl, sigma, dag);
+        if (fresh) ef += col[w + 1] - col[w];
       }
       wave_append(fresh, w, queue, &ctl->tail);
     }
   }
-  er = wave_sum_u64(er);
+  ef = wave_sum_u64(ef);
   dag = wave_sum_u64(dag);
-  if (lane == 0 && (er | dag)) {
-    atomicAdd(&ctl->e_r, er);
+  if (lane == 0 && (ef | dag)) {
+    atomicAdd(&ctl->ef, ef);
     atomicAdd(&ctl->e_dag, dag);
   }
 }
@@ -320,9 +327,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
                                                      DevCtl *ctl, const uint32_t *__restrict__ hubs,
                                                      const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
                                                      uint64_t item_cap, int L) {
-  if (ld_flag(&ctl->done)) return;
+  if (ld_flag(&ctl->done) || ld_flag(&ctl->bu)) return;
   const int lane = lane_id();
-  unsigned long long dag = 0;
+  unsigned long long dag = 0, ef = 0;
   HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);
   for (uint64_t item = r.lo; item < r.hi; ++item) {
     hub_step(r, hub_base, item);
@@ -336,12 +343,153 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
       if (k < ce) {
         w = row[st + k];
         fresh = visit_edge(w, su, L, lvl, sigma, dag);
+        if (fresh) ef += col[w + 1] - col[w];
       }
       wave_append(fresh, w, queue, &ctl->tail);
     }
   }
   dag = wave_sum_u64(dag);
-  if (lane == 0 && dag) atomicAdd(&ctl->e_dag, dag);
+  ef = wave_sum_u64(ef);
+  if (lane == 0 && (dag | ef)) {
+    atomicAdd(&ctl->e_dag, dag);
+    atomicAdd(&ctl->ef, ef);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bottom-up level L (graphs whose lists are their in-lists): every vertex w
+// not yet found sums the path counts of its neighbours at level L-1,
+//   sigma[w] = sum_{u in adj(w), lvl[u] = L-1} sigma[u]   (uint32, wrapping),
+// the same sum the top-down push adds up edge by edge, and is found at L when
+// it has one.  A wave takes 64 consecutive vertices and deals their lists 64
+// edges per step with a segmented scan, so each vertex is written by one wave
+// with no atomics; lists above the hub degree are claimed as hub chunks
+// (k_fwd_pull_hubs).  Chosen per level by k_fwd_end when the lists of the
+// vertices not yet found are at most pull16/16 times those a top-down level
+// would scan.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_fwd_pull(const uint32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ row, int *lvl, uint32_t *sigma,
+                                                     uint32_t *queue, DevCtl *ctl, uint32_t *hubs,
+                                                     uint64_t *hub_base, uint32_t hub_cap, uint64_t item_cap,
+                                                     uint32_t n, int L) {
+  if (ld_flag(&ctl->done) || !ld_flag(&ctl->bu)) return;
+  const int lane = lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
+  unsigned long long dag = 0, ef = 0;
+  for (uint32_t base = wid * kWave; base < n; base += nw * kWave) {
+    const uint32_t w = base + lane;
+    uint32_t st = 0, d = 0;
+    if (w < n && lvl[w] == -1) {
+      st = col[w];
+      d = col[w + 1] - st;
+    }
+    const uint32_t dw = d;
+    if (d > kHubDeg) {
+      claim_hub(ctl, w, d, hubs, hub_base, hub_cap, item_cap);
+      d = 0;
+    }
+    const uint32_t incl = wave_incl_scan(d);
+    const uint32_t excl = incl - d;
+    const uint32_t total = __shfl(incl, kWave - 1);
+    uint32_t acc = 0, cnt = 0;
+    for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+      const uint32_t e = e0 + lane;
+      const int own = wave_owner(incl, e) & (kWave - 1);
+      const uint32_t ost = __shfl(st, own);
+      const uint32_t pin = __shfl(incl, (own + kWave - 1) & (kWave - 1));
+      const uint32_t oex = own ? pin : 0u;
+      uint32_t t = 0, tc = 0;
+      int seg = kWave + lane;  // inactive lanes form singleton segments
+      if (e < total) {
+        seg = own;
+        const uint32_t u = row[ost + (e - oex)];
+        if (lvl[u] == L - 1) {
+          t = sigma[u];
+          tc = 1;
+        }
+      }
+      // segmented inclusive scan over contiguous owner segments
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(t, off), yc = __shfl_up(tc, off);
+        const int sy = __shfl_up(seg, off);
+        if (lane >= off && sy == seg) {
+          t += y;
+          tc += yc;
+        }
+      }
+      const bool has = d > 0 && excl < e0 + kWave && incl > e0;
+      const uint32_t last = (incl < e0 + kWave ? incl : e0 + kWave) - 1u - e0;
+      const uint32_t got = __shfl(t, has ? (int)last : lane), gotc = __shfl(tc, has ? (int)last : lane);
+      if (has) {
+        acc += got;
+        cnt += gotc;
+      }
+    }
+    const bool fresh = d > 0 && cnt > 0;
+    if (fresh) {
+      lvl[w] = L;
+      sigma[w] = acc;
+      dag += cnt;
+      ef += dw;
+    }
+    wave_append(fresh, w, queue, &ctl->tail);
+  }
+  dag = wave_sum_u64(dag);
+  ef = wave_sum_u64(ef);
+  if (lane == 0 && (dag | ef)) {
+    atomicAdd(&ctl->e_dag, dag);
+    atomicAdd(&ctl->ef, ef);
+  }
+}
+
+// Hub chunks of a bottom-up level: per item the sum and count of its
+// neighbours at level L-1; the first item with any claims the hub (CAS on its
+// level, then the append), every one adds its sum (uint32, order-free).
+__global__ __launch_bounds__(kBlock) void k_fwd_pull_hubs(const uint32_t *__restrict__ col,
+                                                          const uint32_t *__restrict__ row, int *lvl,
+                                                          uint32_t *sigma, uint32_t *queue, DevCtl *ctl,
+                                                          const uint32_t *__restrict__ hubs,
+                                                          const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                                          uint64_t item_cap, int L) {
+  if (ld_flag(&ctl->done) || !ld_flag(&ctl->bu)) return;
+  const int lane = lane_id();
+  unsigned long long dag = 0, ef = 0;
+  HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);
+  for (uint64_t item = r.lo; item < r.hi; ++item) {
+    hub_step(r, hub_base, item);
+    const uint32_t w = hubs[r.k];
+    const uint32_t st = col[w], d = col[w + 1] - st;
+    const uint32_t cb = (uint32_t)(item - hub_base[r.k]) * kHubChunk, ce = min(d, cb + kHubChunk);
+    uint32_t t = 0, tc = 0;
+    for (uint32_t k = cb + lane; k < ce; k += kWave) {
+      const uint32_t u = row[st + k];
+      if (lvl[u] == L - 1) {
+        t += sigma[u];
+        ++tc;
+      }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      t += __shfl_down(t, off);
+      tc += __shfl_down(tc, off);
+    }
+    if (lane == 0 && tc > 0) {
+      atomicAdd(&sigma[w], t);
+      dag += tc;
+      if (atomicCAS(&lvl[w], -1, L) == -1) {
+        queue[atomicAdd(&ctl->tail, 1u)] = w;
+        ef += d;
+      }
+    }
+  }
+  if (lane == 0 && (dag | ef)) {
+    atomicAdd(&ctl->e_dag, dag);
+    atomicAdd(&ctl->ef, ef);
+  }
 }
 
 // End of forward level L, one thread, after the level's kernels in stream
@@ -355,6 +503,13 @@ __global__ void k_fwd_end(uint32_t *qoff, DevCtl *ctl, int L) {
     ctl->done = 1u;
   }
   ctl->hub_claim = 0;
+  // the next level's direction: bottom-up scans the lists of the vertices
+  // not yet found, top-down those of the ones this level found
+  const unsigned long long ef = ctl->ef;
+  ctl->e_r += ef;
+  ctl->eu -= ef < ctl->eu ? ef : ctl->eu;
+  ctl->bu = ctl->pull16 && ctl->eu * 16ull <= (unsigned long long)ctl->pull16 * ef ? 1u : 0u;
+  ctl->ef = 0;
 }
 
 // Clears the hub claims of a backward depth.
@@ -567,16 +722,26 @@ int pull_ctl(axbc_ctx *c) {
   return AXBC_OK;
 }
 
+// A forward level: the top-down and the bottom-up kernels are both enqueued,
+// and the one the direction flag does not select exits at once.
 int enqueue_forward(axbc_ctx *c, int L) {
   kt_begin(c, KK_FWD);
   hipLaunchKernelGGL(k_fwd_expand, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
                      c->lvl, c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap,
                      c->hub_items, L);
   kt_end(c, KK_FWD);
+  kt_begin(c, KK_FWD_PULL);
+  hipLaunchKernelGGL(k_fwd_pull, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl, c->sigma,
+                     c->queue, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, c->n, L);
+  kt_end(c, KK_FWD_PULL);
   kt_begin(c, KK_FWD_HUBS);
   hipLaunchKernelGGL(k_fwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
                      c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
   kt_end(c, KK_FWD_HUBS);
+  kt_begin(c, KK_FWD_PULL_HUBS);
+  hipLaunchKernelGGL(k_fwd_pull_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl, c->sigma,
+                     c->queue, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
+  kt_end(c, KK_FWD_PULL_HUBS);
   hipLaunchKernelGGL(k_fwd_end, dim3(1), dim3(64), 0, c->stream, c->qoff, c->ctl, L);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;
@@ -608,8 +773,12 @@ int enqueue_reset_and_root(axbc_ctx *c, uint32_t v0) {
   } else if (c->have_round)
     hipLaunchKernelGGL(k_reset_visited, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->queue,
                        c->ctl, c->reach, c->lvl, c->sigma, c->delta);
+  // bottom-up levels only where a vertex's list is its in-list
+  bool sym = false;
+  int rc;
+  if (c->plain_pull16 && (rc = mirror_symmetric(c, &sym))) return rc;
   hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, c->stream, v0, c->lvl, c->sigma, c->queue,
-                     c->qoff, c->ctl);
+                     c->qoff, c->ctl, c->col, (unsigned long long)c->ned, sym ? c->plain_pull16 : 0u);
   HIP_TRY(hipGetLastError());
   c->have_round = true;
   return AXBC_OK;
@@ -1023,8 +1192,10 @@ int axbc_begin_round(axbc_ctx *c) {
 
 int axbc_set_root(axbc_ctx *c, uint32_t v0) {
   if (!c || v0 >= c->n) return fail(AXBC_EINVAL, "bad root%s (line %d)", "", __LINE__);
+  // the per-level entry points keep every level top-down (the reference's
+  // scan_col_mono2 order of work)
   hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, c->stream, v0, c->lvl, c->sigma, c->queue, c->qoff,
-                     c->ctl);
+                     c->ctl, c->col, (unsigned long long)c->ned, 0u);
   c->have_round = true;
   c->last_v0 = v0;
   return axbc_sync(c);

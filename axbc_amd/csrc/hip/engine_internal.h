@@ -27,6 +27,13 @@ struct DevCtl {
   uint32_t overflow;   // a bounded list ran out of room
   unsigned long long e_r, e_dag;
   unsigned long long hub_claim;  // hubs of the current pass << 40 | chunk items
+  // direction of the forward levels: ef = edges of the vertices found in the
+  // current level (what a top-down next level scans), eu = edges of the
+  // vertices not yet found (what a bottom-up next level scans); bu = the
+  // current level runs bottom-up; pull16 = bottom-up when eu <= pull16/16 ef
+  // (0: never)
+  unsigned long long ef, eu;
+  uint32_t bu, pull16;
 };
 
 // Device buffers of the samplers (allocated on first use).
@@ -80,7 +87,9 @@ enum KernelKind {
   KK_PULL = 14,        // batched bottom-up level (k_bpull)
   KK_PULL_SPLIT = 15,  // batched bottom-up level, split vertices (k_bpull_split)
   KK_WGSWEEP = 16,     // workgroup sweep, one workgroup per root (k_wgsweep)
-  KK_COUNT = 17
+  KK_FWD_PULL = 17,    // plain forward, bottom-up level (k_fwd_pull)
+  KK_FWD_PULL_HUBS = 18,  // plain forward, bottom-up level, hub chunks (k_fwd_pull_hubs)
+  KK_COUNT = 19
 };
 constexpr int kKbSlots = 64, kKbStride = 16;
 
@@ -156,6 +165,8 @@ struct axbc_ctx {
   int batch_width = 64;
   uint32_t batch_chunk = 256;
   uint32_t batch_pull16 = 32;        // bottom-up ratio x16 for batched levels (0: top-down only)
+  uint32_t plain_pull16 = 256;       // the same for the levels of one-root rounds
+  int mirror_state = -1;             // CSC strictly sorted and symmetric: 1 yes, 0 no, -1 not checked
   bool use_graphs = true;            // batched level loop from captured HIP graphs
   int batch_persist = 0;             // persistent level loop: 0 never (default), 1 deep graphs, 2 whenever possible
   int batch_mode = 0;                // 0 automatic, 1 bit-packed sweep only, 2 workgroup sweep only
@@ -168,6 +179,9 @@ struct axbc_ctx {
 namespace axbc {
 int sampler_bufs(axbc_ctx *c, SamplerBufs **out);
 int batch_delta_row(axbc_ctx *c, int k, const double **row);
+// Whether every list is strictly increasing and every edge has its reverse
+// (then the in-edges of a vertex are its list); checked on the device once.
+int mirror_symmetric(axbc_ctx *c, bool *sym);
 void sampler_free(axbc_ctx *c);
 void batch_free(axbc_ctx *c);
 void mlp_free(axbc_ctx *c);

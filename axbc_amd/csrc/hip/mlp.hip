@@ -33,7 +33,7 @@ namespace {
 constexpr int kIn = 4, kH = 64;
 // parameter layout (floats): W1[64][4], b1[64], W2[64][64], b2[64], w3[64], b3
 constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
-constexpr int kParams = oB3 + 1;  // 4481
+constexpr int kParams = oB3 + 1;  // 4545
 
 struct MlpBufs {
   float *x = nullptr;        // N x 4 features
@@ -50,6 +50,8 @@ struct MlpBufs {
   uint32_t *idx = nullptr;
   uint32_t idx_cap = 0;
   int step = 0;
+  float *gpart = nullptr;    // kGradBlocks x (kParams + 1): per-workgroup gradients and loss
+  double *wpart = nullptr;   // per-wave (weight, count) sums of the proposal
 };
 
 MlpBufs *bufs_of(axbc_ctx *c) { return reinterpret_cast<MlpBufs *>(c->mlp); }
@@ -293,9 +295,15 @@ __global__ __launch_bounds__(kFwdThreads) void k_mlp_forward(uint32_t n, const f
 // b2[j], w3[j], b1[j], W1[j][.]).  One atomic add per parameter at the end.
 constexpr int kTile = 64;
 
+// Workgroups of the gradient pass: each writes its whole gradient (and its
+// loss) to its own row of gpart, and k_grad_reduce adds the rows in
+// workgroup order, so a gradient does not depend on the run.
+constexpr int kGradBlocks = 1024;
+constexpr int kGradRow = kParams + 1;
+
 __global__ __launch_bounds__(64) void k_mlp_grad(uint32_t m, const uint32_t *__restrict__ idx,
                                                  const float *__restrict__ x, const float *__restrict__ y,
-                                                 const __hip_bfloat16 *__restrict__ wb, float *grad, float *loss) {
+                                                 const __hip_bfloat16 *__restrict__ wb, float *gpart) {
   __shared__ SmallW sw;
   __shared__ __align__(16) unsigned short W2s[kH * kLd];  // [unit j][input k]
   __shared__ __align__(16) unsigned short W2t[kH * kLd];  // [input k][unit j]
@@ -407,25 +415,37 @@ __global__ __launch_bounds__(64) void k_mlp_grad(uint32_t m, const uint32_t *__r
     }
     __syncthreads();
   }
+  // every entry of the row is written: each parameter has one owner lane
+  float *grad = gpart + (size_t)blockIdx.x * kGradRow;
 #pragma unroll
   for (int jb = 0; jb < 2; ++jb)
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float g = gW2[jb][kb][r];
-        if (g != 0.f) atomicAdd(&grad[oW2 + (jb * 32 + tile_row(r, t)) * kH + kb * 32 + (t & 31)], g);
-      }
+      for (int r = 0; r < 16; ++r)
+        grad[oW2 + (jb * 32 + tile_row(r, t)) * kH + kb * 32 + (t & 31)] = gW2[jb][kb][r];
 #pragma unroll
-  for (int k = 0; k < kIn; ++k)
-    if (gW1[k] != 0.f) atomicAdd(&grad[oW1 + t * kIn + k], gW1[k]);
-  if (gb1 != 0.f) atomicAdd(&grad[oB1 + t], gb1);
-  if (gb2 != 0.f) atomicAdd(&grad[oB2 + t], gb2);
-  if (gw3 != 0.f) atomicAdd(&grad[oW3 + t], gw3);
-  if (t == 0 && gb3 != 0.f) atomicAdd(&grad[oB3], gb3);
+  for (int k = 0; k < kIn; ++k) grad[oW1 + t * kIn + k] = gW1[k];
+  grad[oB1 + t] = gb1;
+  grad[oB2 + t] = gb2;
+  grad[oW3 + t] = gw3;
+  if (t == 0) grad[oB3] = gb3;
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-  if (t == 0) atomicAdd(loss, lsum / (float)m);
+  if (t == 0) grad[kParams] = lsum / (float)m;
+}
+
+// grad[i] += the rows' entries i in workgroup order (i = kParams: the loss,
+// stored instead of added).
+__global__ void k_grad_reduce(const float *__restrict__ gpart, int rows, float *grad, float *loss) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= kParams; i += gridDim.x * blockDim.x) {
+    float t = 0.f;
+    for (int r = 0; r < rows; ++r) t += gpart[(size_t)r * kGradRow + i];
+    if (i < kParams)
+      grad[i] += t;
+    else
+      *loss = t;
+  }
 }
 
 __global__ void k_adam(int n, float *master, __hip_bfloat16 *wb, float *grad, float *m, float *v, float lr,
@@ -460,9 +480,11 @@ __global__ void k_proposal(uint32_t n, const uint32_t *__restrict__ col, const f
     w[v] = col[v + 1] > col[v] ? score_weight(score[v]) + floor_w : 0.0;
 }
 
-// sum[0] += weights, sum[1] += vertices of positive degree
+// Per wave (weights, vertices of positive degree) into part[2 gw ..], then
+// k_weight_total adds the waves' pairs in wave order: the proposal's
+// normalisation does not depend on the run.
 __global__ void k_weight_sum(uint32_t n, const uint32_t *__restrict__ col, const float *__restrict__ score,
-                             double *sum) {
+                             double *part) {
   double a = 0.0, c = 0.0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     if (col[v + 1] > col[v]) {
@@ -474,9 +496,35 @@ __global__ void k_weight_sum(uint32_t n, const uint32_t *__restrict__ col, const
     a += __shfl_xor(a, off);
     c += __shfl_xor(c, off);
   }
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (lane_id() == 0) {
-    atomicAdd(&sum[0], a);
-    atomicAdd(&sum[1], c);
+    part[2 * gw] = a;
+    part[2 * gw + 1] = c;
+  }
+}
+
+// One workgroup of 256: thread t adds waves t, t + 256, ... in order, then a
+// fixed tree over the threads.
+__global__ __launch_bounds__(256) void k_weight_total(const double *__restrict__ part, uint32_t waves, double *sum) {
+  __shared__ double sa[256], sc[256];
+  double a = 0.0, c = 0.0;
+  for (uint32_t w = threadIdx.x; w < waves; w += 256) {
+    a += part[2 * w];
+    c += part[2 * w + 1];
+  }
+  sa[threadIdx.x] = a;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+      sc[threadIdx.x] += sc[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sum[0] = sa[0];
+    sum[1] = sc[0];
   }
 }
 
@@ -491,7 +539,8 @@ namespace axbc {
 void mlp_free(axbc_ctx *c) {
   MlpBufs *b = bufs_of(c);
   if (!b) return;
-  void *d[] = {b->x, b->y, b->score, b->master, b->wb, b->grad, b->m, b->v, b->loss, b->idx, b->prop, b->tw, b->tred};
+  void *d[] = {b->x,   b->y,    b->score, b->master, b->wb,   b->grad,  b->m,    b->v,
+               b->loss, b->idx, b->prop,  b->tw,     b->tred, b->gpart, b->wpart};
   for (void *p : d)
     if (p) (void)hipFree(p);
   delete b;
@@ -636,11 +685,12 @@ int axbc_mlp_gradient(axbc_ctx *c, const uint32_t *idx, uint32_t m, float *grad_
     HIP_TRY(hipMalloc((void **)&b->idx, sizeof(uint32_t) * m));
     b->idx_cap = m;
   }
+  if (!b->gpart) HIP_TRY(hipMalloc((void **)&b->gpart, sizeof(float) * kGradBlocks * kGradRow));
   HIP_TRY(hipMemcpyAsync(b->idx, idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemsetAsync(b->loss, 0, sizeof(float), s));
   const uint32_t tiles = (m + 63) / 64;
-  hipLaunchKernelGGL(k_mlp_grad, dim3(tiles < 1024 ? tiles : 1024), dim3(64), 0, s, m, b->idx, b->x, b->y, b->wb,
-                     b->grad, b->loss);
+  const int rows = tiles < (uint32_t)kGradBlocks ? (int)tiles : kGradBlocks;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(rows), dim3(64), 0, s, m, b->idx, b->x, b->y, b->wb, b->gpart);
+  hipLaunchKernelGGL(k_grad_reduce, dim3((kParams + 256) / 256), dim3(256), 0, s, b->gpart, rows, b->grad, b->loss);
   HIP_TRY(hipGetLastError());
   if (grad_host) HIP_TRY(hipMemcpyAsync(grad_host, b->grad, sizeof(float) * kParams, hipMemcpyDeviceToHost, s));
   if (loss_host) HIP_TRY(hipMemcpyAsync(loss_host, b->loss, sizeof(float), hipMemcpyDeviceToHost, s));
@@ -686,13 +736,16 @@ int axbc_mlp_proposal(axbc_ctx *c, double eps, uint64_t *total) {
   SamplerBufs *sb = nullptr;
   int rc = sampler_bufs(c, &sb);
   if (rc) return rc;
-  double *dsum = nullptr, hsum[2] = {0, 0};
-  HIP_TRY(hipMalloc((void **)&dsum, 2 * sizeof(double)));
-  HIP_TRY(hipMemsetAsync(dsum, 0, 2 * sizeof(double), c->stream));
-  hipLaunchKernelGGL(k_weight_sum, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, b->score, dsum);
+  double hsum[2] = {0, 0};
+  const int wgroups = grid_of(c->n);
+  const uint32_t waves = (uint32_t)wgroups * (256 / kWave);
+  // per-wave pairs, then the total in the last two entries
+  if (!b->wpart) HIP_TRY(hipMalloc((void **)&b->wpart, sizeof(double) * (2 * 4096 * (256 / kWave) + 2)));
+  double *dsum = b->wpart + 2 * 4096 * (256 / kWave);
+  hipLaunchKernelGGL(k_weight_sum, dim3(wgroups), dim3(256), 0, c->stream, c->n, c->col, b->score, b->wpart);
+  hipLaunchKernelGGL(k_weight_total, dim3(1), dim3(256), 0, c->stream, b->wpart, waves, dsum);
   HIP_TRY(hipMemcpyAsync(hsum, dsum, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipFree(dsum));
   if (!(hsum[1] > 0.0)) return fail(AXBC_ESTATE, "no vertex of positive degree%s (line %d)", "", __LINE__);
   double *w = b->prop;
   hipLaunchKernelGGL(k_proposal, dim3(grid_of(c->n)), dim3(256), 0, c->stream, c->n, c->col, b->score,

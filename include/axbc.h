@@ -223,6 +223,16 @@ int axbc_set_batch_chunk(axbc_ctx *ctx, uint32_t chunk);
  * top-down push would scan; 0 keeps every level top-down.  Default 32.
  * Results do not depend on it. */
 int axbc_set_batch_pull(axbc_ctx *ctx, uint32_t ratio16);
+/* The same switch for the levels of one-root rounds (axbc_pivot): a level
+ * runs bottom-up (every vertex not yet found sums its neighbours' path counts
+ * at the level before) when the lists of the vertices not yet found hold at
+ * most ratio16/16 times the edges of the vertices the last level found;
+ * only on graphs whose every list is strictly increasing and holds each
+ * edge's reverse (checked once on the device); 0 keeps every level
+ * top-down, as do the per-level entry points.  Default 256 (SCALE 22,
+ * high-degree-neighbourhood roots: 155 pivots/s at 32, 173 at 128, 176 at
+ * 256, 177 at 1024).  Results do not depend on it. */
+int axbc_set_plain_pull(axbc_ctx *ctx, uint32_t ratio16);
 /* Level loop of the batched sweep: 1 (default) replays prebuilt HIP graphs
  * of 8 levels (depths) each, the kernels reading the level from device
  * memory, so a level costs no host launches; 0 launches every kernel from

@@ -12,10 +12,13 @@ from oracle import oracle
 from tests.parity import close
 
 pytestmark = pytest.mark.gpu
-def check_rounds(g, roots, label):
-    print(f"[{label}] N={g.N} ned={g.ned} roots={len(roots)}", flush=True)
+def check_rounds(g, roots, label, pull=None):
+    print(f"[{label}] N={g.N} ned={g.ned} roots={len(roots)} pull={pull}", flush=True)
     bc_ref = np.zeros(g.N)
     with Engine(g) as e:
+        if pull is not None:
+            from axbc_amd import _ffi
+            assert _ffi.hip().axbc_set_plain_pull(e.ctx, pull) == 0
         for k, v0 in enumerate(roots):
             p = 1.0 / g.N
             st = e.pivot(int(v0), p, stats=True)
@@ -34,27 +37,47 @@ def test_require(require_gpu):
     pass
 
 
+# pull ratio (x16) of the plain levels: 0 = every level top-down, 32 = the
+# default switch, 65536 = bottom-up from level 2 on (symmetric graphs only;
+# the directed graph keeps every level top-down whatever the ratio)
+@pytest.mark.parametrize("pull", [0, 32, 65536])
 @pytest.mark.parametrize("scale,ef,seed,undirected,heur", [
     (10, 16, (1, 2), True, 0),
     (12, 16, (23, 24), True, 1),
     (12, 8, (3, 4), False, 0),
 ])
-def test_kronecker_rounds(require_gpu, scale, ef, seed, undirected, heur):
+def test_kronecker_rounds(require_gpu, scale, ef, seed, undirected, heur, pull):
     g = Graph.kronecker(scale, ef, seed=seed, undirected=undirected, heuristic=heur)
     deg = g.degrees()
     rng = np.random.default_rng(scale * 7 + heur)
     cand = np.flatnonzero(deg > 0)
     roots = list(rng.choice(cand, size=8, replace=False)) + [int(np.argmax(deg))]
-    check_rounds(g, roots, f"kron s{scale} ef{ef} H{heur} U{int(not undirected)}")
+    check_rounds(g, roots, f"kron s{scale} ef{ef} H{heur} U{int(not undirected)}", pull=pull)
 
 
-def test_hub_paths(require_gpu):
+@pytest.mark.parametrize("pull", [0, 65536])
+def test_hub_paths(require_gpu, pull):
     # star + clique core: the centre exceeds the hub degree threshold, so
-    # both forward and backward hub passes run
+    # both forward and backward hub passes run (bottom-up: the centre's
+    # list is dealt in hub chunks while it is not yet found)
     n = 6000
     edges = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]
     g = Graph.from_edges(n, np.array(edges), undirected=True)
-    check_rounds(g, [0, 1, 2999, n - 1], "star-path")
+    check_rounds(g, [0, 1, 2999, n - 1], "star-path", pull=pull)
+
+
+@pytest.mark.parametrize("pull", [0, 65536])
+def test_hub_found_bottom_up(require_gpu, pull):
+    """A hub of 5000 leaves three steps from the root: with bottom-up levels
+    from level 2 on, the hub is found while its list is dealt in hub chunks
+    (the chunk that sees its predecessor claims it, every chunk adds its
+    path counts), and its leaves one level later."""
+    n = 5010
+    hub = 5
+    edges = [(0, 1), (1, 2), (2, 3), (3, hub), (0, 4), (4, 3)] + [(hub, i) for i in range(10, n)]
+    edges += [(i, i + 1) for i in range(10, n - 1, 2)]  # leaf pairs: two paths into each from the hub
+    g = Graph.from_edges(n, np.array(edges), undirected=True)
+    check_rounds(g, [0, 1, 10, n - 1], "path into a hub", pull=pull)
 
 
 def test_path_depths(require_gpu):
