@@ -267,3 +267,15 @@ def test_batches_of_256(require_gpu, stop_at):
             else:
                 ok, err = close(e.bc(), bcs[applied - 1])
                 assert ok, f"bc after the stop: rel err {err}"
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_roots_without_edges(require_gpu, mode):
+    """Roots of degree 0 in a batch (the ABI takes any vertex; the pivot
+    loop skips them): a one-vertex round, delta = reach, counters as the
+    oracle's, next to ordinary roots."""
+    n = 400
+    e = [(i, i + 1) for i in range(0, 300)] + [(0, k) for k in range(301, 340)]
+    g = Graph.from_edges(n, np.array(e), heuristic=0)
+    assert (g.degrees()[350:] == 0).all()
+    check_batch(g, np.array([350, 0, 399, 150, 360], np.uint32), 64, f"isolated roots mode {mode}", mode=mode)
