@@ -121,6 +121,7 @@ def hip() -> C.CDLL:
         "axbc_kernel_stats": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]),
         "axbc_copy_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
+        "axbc_read_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
     })
     optional = {}
     for name, (res, args) in sig.items():

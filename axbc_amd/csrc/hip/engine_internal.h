@@ -89,7 +89,8 @@ enum KernelKind {
   KK_WGSWEEP = 16,     // workgroup sweep, one workgroup per root (k_wgsweep)
   KK_FWD_PULL = 17,    // plain forward, bottom-up level (k_fwd_pull)
   KK_FWD_PULL_HUBS = 18,  // plain forward, bottom-up level, hub chunks (k_fwd_pull_hubs)
-  KK_COUNT = 19
+  KK_READ = 19,        // calibration read kernel
+  KK_COUNT = 20
 };
 constexpr int kKbSlots = 64, kKbStride = 16;
 

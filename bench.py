@@ -34,7 +34,7 @@ WORKLOAD = {
 KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_expand",
               6: "k_bwd_pull", 7: "k_copy", 8: "k_bpush_split", 9: "k_bsigma_split", 10: "k_bdep_split",
               11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split",
-              16: "k_wgsweep", 17: "k_fwd_pull", 18: "k_fwd_pull_hubs"}
+              16: "k_wgsweep", 17: "k_fwd_pull", 18: "k_fwd_pull_hubs", 19: "k_read"}
 BATCH_KINDS = (0, 8, 1, 9, 14, 15, 2, 3, 10, 11, 4)
 PLAIN_KINDS = (5, 17, 12, 18, 6, 13)
 HBM_PEAK_GBS = 8000.0
@@ -117,11 +117,17 @@ def copy_calibration(mib):
     if L.axbc_copy_kernel(e.ctx, mib << 20, 8) != 0:
         raise RuntimeError(L.axbc_last_error())
     ms, n, b = kernel_stats(e, 7)
+    if L.axbc_read_kernel(e.ctx, mib << 20, 8) != 0:
+        raise RuntimeError(L.axbc_last_error())
+    rms, rn, rb = kernel_stats(e, 19)
     set_timing(e, False)
     e.close()
     print(json.dumps({"kernel": "k_copy", "bytes_read_per_launch": mib << 20, "bytes_written_per_launch": mib << 20,
                       "launches": n, "avg_launch_us": round(ms / n * 1e3, 2),
-                      "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1)}))
+                      "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1),
+                      "read_kernel": {"bytes_read_per_launch": mib << 20, "launches": rn,
+                                      "avg_launch_us": round(rms / rn * 1e3, 2),
+                                      "gb_per_s": round(rb / (rms * 1e-3) / 1e9, 1)}}))
 
 
 def grid_case(args):

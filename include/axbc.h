@@ -295,18 +295,22 @@ int axbc_batch_utility_noalloc(axbc_ctx *ctx, const uint32_t *vertex, const doub
  * path-count pull of split vertices, 10 batched dependency pull of split
  * vertices, 11 their chunk combine, 12 plain forward hub chunks, 13 plain
  * dependency pull hub chunks, 14 batched bottom-up level, 15 its split
- * vertices.  axbc_kernel_stats returns the summed milliseconds,
+ * vertices, 16 workgroup sweep, 17 plain bottom-up level, 18 its hub
+ * chunks, 19 calibration read.  axbc_kernel_stats returns the summed milliseconds,
  * the launch count and the byte count of one kind since enabling.
  * axbc_kernel_timing_kinds enables the same for the kinds whose bit is set
  * in `mask` only (0 disables), so that a timed run carries the events of
  * the kernel it measures and no others.
  * axbc_copy_kernel runs a plain device copy of `bytes` bytes (kind 7) to
- * calibrate hardware counters against a known traffic.
+ * calibrate hardware counters against a known traffic; axbc_read_kernel
+ * streams `bytes` bytes without writing them back (kind 19): the attainable
+ * read rate, the bound of the read-dominated frontier kernels.
  */
 int axbc_kernel_timing(axbc_ctx *ctx, int enable);
 int axbc_kernel_timing_kinds(axbc_ctx *ctx, uint32_t mask);
 int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, uint64_t *bytes);
 int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
+int axbc_read_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 
 /*
  * Graph construction on the device (no reference device counterpart; the
