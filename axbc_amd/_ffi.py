@@ -122,6 +122,7 @@ def hip() -> C.CDLL:
                                         C.POINTER(C.c_uint64)]),
         "axbc_copy_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
         "axbc_read_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
+        "axbc_gather_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
     })
     optional = {}
     for name, (res, args) in sig.items():

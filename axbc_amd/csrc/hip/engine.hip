@@ -734,6 +734,36 @@ __global__ __launch_bounds__(kBlock) void k_read(const u32x4 *__restrict__ src, 
   if (acc == never) sink[0] = acc;
 }
 
+// Random-line gather: every lane reads one double from its own random
+// 128-byte line among the first `lines` lines of src (xorshift32 per lane),
+// kGatherRun loads in flight per thread, `iters` rounds.  The rate of line
+// fetches this reaches over a region of a given size bounds the kernels
+// whose rows are read at scattered vertices.
+constexpr int kGatherRun = 4;
+constexpr int kGatherGrid = 4096;
+constexpr uint32_t kGatherIters = 32;
+
+__global__ __launch_bounds__(kBlock) void k_gather(const double *__restrict__ src, uint32_t lines, uint32_t iters,
+                                                   uint32_t never, uint32_t *sink) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t s = gid * 2654435761u + 12345u;
+  if (s == 0) s = 1;
+  double acc = 0.0;
+  for (uint32_t it = 0; it < iters; ++it) {
+    double v[kGatherRun];
+#pragma unroll
+    for (int u = 0; u < kGatherRun; ++u) {
+      s ^= s << 13;
+      s ^= s >> 17;
+      s ^= s << 5;
+      v[u] = src[(size_t)(s % lines) * 16 + (threadIdx.x & 15)];
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherRun; ++u) acc += v[u];
+  }
+  if (__double_as_longlong(acc) == (long long)never) sink[0] = 1;
+}
+
 __global__ void k_degrees(const uint32_t *__restrict__ col, uint32_t n, uint32_t *deg) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     deg[v] = col[v + 1] - col[v];
@@ -1086,6 +1116,38 @@ int axbc_read_kernel(axbc_ctx *c, uint64_t bytes, int repeats) {
   }
   if (unsigned long long *kb = kt_bytes(c, KK_READ)) {
     const unsigned long long add = (bytes / 16) * 16 * (unsigned long long)repeats;
+    unsigned long long cur = 0;
+    HIP_TRY(hipMemcpyAsync(&cur, kb, sizeof(cur), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    cur += add;
+    HIP_TRY(hipMemcpyAsync(kb, &cur, sizeof(cur), hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(a));
+  HIP_TRY(hipFree(sink));
+  return AXBC_OK;
+}
+
+int axbc_gather_kernel(axbc_ctx *c, uint64_t window, int repeats) {
+  if (!c || window < 128 || window / 128 > 0xffffffffull)
+    return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  const uint64_t lines = window / 128;
+  void *a = nullptr;
+  uint32_t *sink = nullptr;
+  HIP_TRY(hipMalloc(&a, lines * 128));
+  HIP_TRY(hipMalloc((void **)&sink, sizeof(uint32_t)));
+  // zeros: the sum stays +0.0, whose bits are not `never`
+  HIP_TRY(hipMemsetAsync(a, 0, lines * 128, c->stream));
+  for (int r = 0; r < repeats; ++r) {
+    kt_begin(c, KK_GATHER);
+    hipLaunchKernelGGL(k_gather, dim3(kGatherGrid), dim3(kBlock), 0, c->stream, (const double *)a, (uint32_t)lines,
+                       kGatherIters, 7u, sink);
+    kt_end(c, KK_GATHER);
+  }
+  if (unsigned long long *kb = kt_bytes(c, KK_GATHER)) {
+    // the bytes of the lines fetched: one 128-byte line per load
+    const unsigned long long add =
+        128ull * kGatherGrid * kBlock * kGatherIters * kGatherRun * (unsigned long long)repeats;
     unsigned long long cur = 0;
     HIP_TRY(hipMemcpyAsync(&cur, kb, sizeof(cur), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

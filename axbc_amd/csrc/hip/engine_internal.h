@@ -90,7 +90,8 @@ enum KernelKind {
   KK_FWD_PULL = 17,    // plain forward, bottom-up level (k_fwd_pull)
   KK_FWD_PULL_HUBS = 18,  // plain forward, bottom-up level, hub chunks (k_fwd_pull_hubs)
   KK_READ = 19,        // calibration read kernel
-  KK_COUNT = 20
+  KK_GATHER = 20,      // calibration random-line gather kernel
+  KK_COUNT = 21
 };
 constexpr int kKbSlots = 64, kKbStride = 16;
 

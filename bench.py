@@ -34,7 +34,7 @@ WORKLOAD = {
 KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_expand",
               6: "k_bwd_pull", 7: "k_copy", 8: "k_bpush_split", 9: "k_bsigma_split", 10: "k_bdep_split",
               11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split",
-              16: "k_wgsweep", 17: "k_fwd_pull", 18: "k_fwd_pull_hubs", 19: "k_read"}
+              16: "k_wgsweep", 17: "k_fwd_pull", 18: "k_fwd_pull_hubs", 19: "k_read", 20: "k_gather"}
 BATCH_KINDS = (0, 8, 1, 9, 14, 15, 2, 3, 10, 11, 4)
 PLAIN_KINDS = (5, 17, 12, 18, 6, 13)
 HBM_PEAK_GBS = 8000.0
@@ -120,6 +120,19 @@ def copy_calibration(mib):
     if L.axbc_read_kernel(e.ctx, mib << 20, 8) != 0:
         raise RuntimeError(L.axbc_last_error())
     rms, rn, rb = kernel_stats(e, 19)
+    # random 128-byte line fetches over regions of growing size: one XCD's
+    # L2 (4 MiB), the eight L2s, the 256 MiB Infinity Cache, and the 4 GiB
+    # of a SCALE 22 dependency row array at width 128
+    gather = []
+    for wmib in (4, 32, 256, 4096):
+        set_timing(e, False)
+        set_timing(e, True)
+        if L.axbc_gather_kernel(e.ctx, wmib << 20, 8) != 0:
+            raise RuntimeError(L.axbc_last_error())
+        gms, gn, gb = kernel_stats(e, 20)
+        gather.append({"window_mib": wmib, "launches": gn, "avg_launch_us": round(gms / gn * 1e3, 2),
+                       "glines_per_s": round(gb / 128 / (gms * 1e-3) / 1e9, 2),
+                       "gb_per_s": round(gb / (gms * 1e-3) / 1e9, 1)})
     set_timing(e, False)
     e.close()
     print(json.dumps({"kernel": "k_copy", "bytes_read_per_launch": mib << 20, "bytes_written_per_launch": mib << 20,
@@ -127,7 +140,8 @@ def copy_calibration(mib):
                       "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1),
                       "read_kernel": {"bytes_read_per_launch": mib << 20, "launches": rn,
                                       "avg_launch_us": round(rms / rn * 1e3, 2),
-                                      "gb_per_s": round(rb / (rms * 1e-3) / 1e9, 1)}}))
+                                      "gb_per_s": round(rb / (rms * 1e-3) / 1e9, 1)},
+                      "gather_kernel": gather}))
 
 
 def grid_case(args):

@@ -296,7 +296,8 @@ int axbc_batch_utility_noalloc(axbc_ctx *ctx, const uint32_t *vertex, const doub
  * vertices, 11 their chunk combine, 12 plain forward hub chunks, 13 plain
  * dependency pull hub chunks, 14 batched bottom-up level, 15 its split
  * vertices, 16 workgroup sweep, 17 plain bottom-up level, 18 its hub
- * chunks, 19 calibration read.  axbc_kernel_stats returns the summed milliseconds,
+ * chunks, 19 calibration read, 20 calibration gather.  axbc_kernel_stats
+ * returns the summed milliseconds,
  * the launch count and the byte count of one kind since enabling.
  * axbc_kernel_timing_kinds enables the same for the kinds whose bit is set
  * in `mask` only (0 disables), so that a timed run carries the events of
@@ -304,13 +305,19 @@ int axbc_batch_utility_noalloc(axbc_ctx *ctx, const uint32_t *vertex, const doub
  * axbc_copy_kernel runs a plain device copy of `bytes` bytes (kind 7) to
  * calibrate hardware counters against a known traffic; axbc_read_kernel
  * streams `bytes` bytes without writing them back (kind 19): the attainable
- * read rate, the bound of the read-dominated frontier kernels.
+ * read rate, the bound of the read-dominated frontier kernels;
+ * axbc_gather_kernel reads one double from each of 134 million random
+ * 128-byte lines of a `window`-byte buffer per launch (kind 20, counting
+ * 128 bytes per line): the attainable rate of scattered line fetches over a
+ * region of that size, the bound of the kernels that gather rows at
+ * scattered vertices.
  */
 int axbc_kernel_timing(axbc_ctx *ctx, int enable);
 int axbc_kernel_timing_kinds(axbc_ctx *ctx, uint32_t mask);
 int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, uint64_t *bytes);
 int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 int axbc_read_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
+int axbc_gather_kernel(axbc_ctx *ctx, uint64_t window, int repeats);
 
 /*
  * Graph construction on the device (no reference device counterpart; the
