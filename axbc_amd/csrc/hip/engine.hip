@@ -269,13 +269,10 @@ __device__ __forceinline__ void hub_step(HubWalk &r, const uint64_t *__restrict_
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restrict__ col,
-                                                       const uint32_t *__restrict__ row, int *lvl,
-                                                       uint32_t *sigma, uint32_t *queue,
-                                                       const uint32_t *__restrict__ qoff,
-                                                       DevCtl *ctl, uint32_t *hubs, uint64_t *hub_base,
-                                                       uint32_t hub_cap, uint64_t item_cap, int L) {
-  if (ld_flag(&ctl->done) || ld_flag(&ctl->bu)) return;
+__device__ __forceinline__ void fwd_expand(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                           int *lvl, uint32_t *sigma, uint32_t *queue,
+                                           const uint32_t *__restrict__ qoff, DevCtl *ctl, uint32_t *hubs,
+                                           uint64_t *hub_base, uint32_t hub_cap, uint64_t item_cap, int L) {
   const uint32_t fb = qoff[L - 1], fe = qoff[L];
   const int lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -321,13 +318,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_expand(const uint32_t *__restric
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict__ col,
-                                                     const uint32_t *__restrict__ row, int *lvl,
-                                                     uint32_t *sigma, uint32_t *queue, uint32_t *qoff,
-                                                     DevCtl *ctl, const uint32_t *__restrict__ hubs,
-                                                     const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
-                                                     uint64_t item_cap, int L) {
-  if (ld_flag(&ctl->done) || ld_flag(&ctl->bu)) return;
+__device__ __forceinline__ void fwd_hubs(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                         int *lvl, uint32_t *sigma, uint32_t *queue, DevCtl *ctl,
+                                         const uint32_t *__restrict__ hubs, const uint64_t *__restrict__ hub_base,
+                                         uint32_t hub_cap, uint64_t item_cap, int L) {
   const int lane = lane_id();
   unsigned long long dag = 0, ef = 0;
   HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);
@@ -364,17 +358,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubs(const uint32_t *__restrict_
 // it has one.  A wave takes 64 consecutive vertices and deals their lists 64
 // edges per step with a segmented scan, so each vertex is written by one wave
 // with no atomics; lists above the hub degree are claimed as hub chunks
-// (k_fwd_pull_hubs).  Chosen per level by k_fwd_end when the lists of the
+// (fwd_pull_hubs).  Chosen per level by k_fwd_end when the lists of the
 // vertices not yet found are at most pull16/16 times those a top-down level
 // would scan.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_fwd_pull(const uint32_t *__restrict__ col,
-                                                     const uint32_t *__restrict__ row, int *lvl, uint32_t *sigma,
-                                                     uint32_t *queue, DevCtl *ctl, uint32_t *hubs,
-                                                     uint64_t *hub_base, uint32_t hub_cap, uint64_t item_cap,
-                                                     uint32_t n, int L) {
-  if (ld_flag(&ctl->done) || !ld_flag(&ctl->bu)) return;
+__device__ __forceinline__ void fwd_pull(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                         int *lvl, uint32_t *sigma, uint32_t *queue, DevCtl *ctl, uint32_t *hubs,
+                                         uint64_t *hub_base, uint32_t hub_cap, uint64_t item_cap, uint32_t n,
+                                         int L) {
   const int lane = lane_id();
   const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nw = (gridDim.x * blockDim.x) / kWave;
@@ -449,13 +441,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pull(const uint32_t *__restrict_
 // Hub chunks of a bottom-up level: per item the sum and count of its
 // neighbours at level L-1; the first item with any claims the hub (CAS on its
 // level, then the append), every one adds its sum (uint32, order-free).
-__global__ __launch_bounds__(kBlock) void k_fwd_pull_hubs(const uint32_t *__restrict__ col,
-                                                          const uint32_t *__restrict__ row, int *lvl,
-                                                          uint32_t *sigma, uint32_t *queue, DevCtl *ctl,
-                                                          const uint32_t *__restrict__ hubs,
-                                                          const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
-                                                          uint64_t item_cap, int L) {
-  if (ld_flag(&ctl->done) || !ld_flag(&ctl->bu)) return;
+__device__ __forceinline__ void fwd_pull_hubs(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
+                                              int *lvl, uint32_t *sigma, uint32_t *queue, DevCtl *ctl,
+                                              const uint32_t *__restrict__ hubs,
+                                              const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                              uint64_t item_cap, int L) {
   const int lane = lane_id();
   unsigned long long dag = 0, ef = 0;
   HubWalk r = hub_walk(ctl, hub_base, hub_cap, item_cap);
@@ -490,6 +480,35 @@ __global__ __launch_bounds__(kBlock) void k_fwd_pull_hubs(const uint32_t *__rest
     atomicAdd(&ctl->e_dag, dag);
     atomicAdd(&ctl->ef, ef);
   }
+}
+
+// Forward level L: the direction k_fwd_end chose for it (the flag bu),
+// read once per wave, so one launch serves both and no empty kernel is
+// launched per level.
+__global__ __launch_bounds__(kBlock) void k_fwd_level(const uint32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ row, int *lvl, uint32_t *sigma,
+                                                      uint32_t *queue, const uint32_t *__restrict__ qoff,
+                                                      DevCtl *ctl, uint32_t *hubs, uint64_t *hub_base,
+                                                      uint32_t hub_cap, uint64_t item_cap, uint32_t n, int L) {
+  if (ld_flag(&ctl->done)) return;
+  if (ld_flag(&ctl->bu))
+    fwd_pull(col, row, lvl, sigma, queue, ctl, hubs, hub_base, hub_cap, item_cap, n, L);
+  else
+    fwd_expand(col, row, lvl, sigma, queue, qoff, ctl, hubs, hub_base, hub_cap, item_cap, L);
+}
+
+// The hub chunks of forward level L, in the level's direction.
+__global__ __launch_bounds__(kBlock) void k_fwd_hub_level(const uint32_t *__restrict__ col,
+                                                          const uint32_t *__restrict__ row, int *lvl,
+                                                          uint32_t *sigma, uint32_t *queue, DevCtl *ctl,
+                                                          const uint32_t *__restrict__ hubs,
+                                                          const uint64_t *__restrict__ hub_base, uint32_t hub_cap,
+                                                          uint64_t item_cap, int L) {
+  if (ld_flag(&ctl->done)) return;
+  if (ld_flag(&ctl->bu))
+    fwd_pull_hubs(col, row, lvl, sigma, queue, ctl, hubs, hub_base, hub_cap, item_cap, L);
+  else
+    fwd_hubs(col, row, lvl, sigma, queue, ctl, hubs, hub_base, hub_cap, item_cap, L);
 }
 
 // End of forward level L, one thread, after the level's kernels in stream
@@ -788,26 +807,17 @@ int pull_ctl(axbc_ctx *c) {
   return AXBC_OK;
 }
 
-// A forward level: the top-down and the bottom-up kernels are both enqueued,
-// and the one the direction flag does not select exits at once.
+// A forward level: two launches whatever its direction (the device flag
+// selects top-down or bottom-up inside them).
 int enqueue_forward(axbc_ctx *c, int L) {
   kt_begin(c, KK_FWD);
-  hipLaunchKernelGGL(k_fwd_expand, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row,
-                     c->lvl, c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap,
-                     c->hub_items, L);
+  hipLaunchKernelGGL(k_fwd_level, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
+                     c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, c->n, L);
   kt_end(c, KK_FWD);
-  kt_begin(c, KK_FWD_PULL);
-  hipLaunchKernelGGL(k_fwd_pull, dim3(kExpandGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl, c->sigma,
-                     c->queue, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, c->n, L);
-  kt_end(c, KK_FWD_PULL);
   kt_begin(c, KK_FWD_HUBS);
-  hipLaunchKernelGGL(k_fwd_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
-                     c->sigma, c->queue, c->qoff, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
+  hipLaunchKernelGGL(k_fwd_hub_level, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl,
+                     c->sigma, c->queue, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
   kt_end(c, KK_FWD_HUBS);
-  kt_begin(c, KK_FWD_PULL_HUBS);
-  hipLaunchKernelGGL(k_fwd_pull_hubs, dim3(kHubGrid), dim3(kBlock), 0, c->stream, c->col, c->row, c->lvl, c->sigma,
-                     c->queue, c->ctl, c->hubs, c->hub_base, c->nhub_cap, c->hub_items, L);
-  kt_end(c, KK_FWD_PULL_HUBS);
   hipLaunchKernelGGL(k_fwd_end, dim3(1), dim3(64), 0, c->stream, c->qoff, c->ctl, L);
   HIP_TRY(hipGetLastError());
   return AXBC_OK;

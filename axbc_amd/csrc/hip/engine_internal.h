@@ -75,20 +75,20 @@ enum KernelKind {
   KK_CLOSE = 2,     // batched level close and path counts (k_bclose)
   KK_DEP = 3,       // batched dependency pull (k_bdep)
   KK_UPDATE = 4,    // batched bc update (k_bupdate)
-  KK_FWD = 5,       // plain forward expansion (k_fwd_expand)
+  KK_FWD = 5,       // plain forward level, either direction (k_fwd_level)
   KK_BWD = 6,       // plain dependency pull (k_bwd_pull)
   KK_COPY = 7,      // calibration copy kernel
   KK_PUSH_SPLIT = 8,   // batched forward marking, split vertices (k_bpush_split)
   KK_SIGMA_SPLIT = 9,  // batched path-count pull, split vertices (k_bsigma_split)
   KK_DEP_SPLIT = 10,   // batched dependency pull, split vertices (k_bdep_split)
   KK_DEP_COMBINE = 11, // batched dependency chunk combine (k_bdep_combine)
-  KK_FWD_HUBS = 12,    // plain forward, hub chunks (k_fwd_hubs)
+  KK_FWD_HUBS = 12,    // plain forward, hub chunks, either direction (k_fwd_hub_level)
   KK_BWD_HUBS = 13,    // plain dependency pull, hub chunks (k_bwd_hubs)
   KK_PULL = 14,        // batched bottom-up level (k_bpull)
   KK_PULL_SPLIT = 15,  // batched bottom-up level, split vertices (k_bpull_split)
   KK_WGSWEEP = 16,     // workgroup sweep, one workgroup per root (k_wgsweep)
-  KK_FWD_PULL = 17,    // plain forward, bottom-up level (k_fwd_pull)
-  KK_FWD_PULL_HUBS = 18,  // plain forward, bottom-up level, hub chunks (k_fwd_pull_hubs)
+  KK_FWD_PULL = 17,    // reserved: the plain level kernels (kinds 5, 12) run bottom-up levels too
+  KK_FWD_PULL_HUBS = 18,  // reserved, as 17
   KK_READ = 19,        // calibration read kernel
   KK_GATHER = 20,      // calibration random-line gather kernel
   KK_COUNT = 21

@@ -31,12 +31,12 @@ WORKLOAD = {
              "symmetrized, 1-degree reduction",
     "pivots": "high-degree-neighbourhood sampling (-z 1), srand48(4343543), adaptive stop c=1 x=1024, max 4096",
 }
-KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_expand",
+KIND_NAMES = {0: "k_bpush", 1: "k_bsigma", 2: "k_bclose", 3: "k_bdep", 4: "k_bupdate", 5: "k_fwd_level",
               6: "k_bwd_pull", 7: "k_copy", 8: "k_bpush_split", 9: "k_bsigma_split", 10: "k_bdep_split",
-              11: "k_bdep_combine", 12: "k_fwd_hubs", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split",
+              11: "k_bdep_combine", 12: "k_fwd_hub_level", 13: "k_bwd_hubs", 14: "k_bpull", 15: "k_bpull_split",
               16: "k_wgsweep", 17: "k_fwd_pull", 18: "k_fwd_pull_hubs", 19: "k_read", 20: "k_gather"}
 BATCH_KINDS = (0, 8, 1, 9, 14, 15, 2, 3, 10, 11, 4)
-PLAIN_KINDS = (5, 17, 12, 18, 6, 13)
+PLAIN_KINDS = (5, 12, 6, 13)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (KiB per dispatch) on this workload, scaled by the same counters' reading

@@ -290,13 +290,14 @@ int axbc_batch_utility_noalloc(axbc_ctx *ctx, const uint32_t *vertex, const doub
  * bracketed by HIP events on the context stream and the kernels add their
  * algorithmic byte counts to device counters.  Kinds: 0 batched push,
  * 1 batched path-count pull, 2 batched level close, 3 batched dependency
- * pull, 4 batched bc update, 5 plain forward, 6 plain dependency pull,
+ * pull, 4 batched bc update, 5 plain forward level (top-down or
+ * bottom-up), 6 plain dependency pull,
  * 7 calibration copy, 8 batched push of split vertices, 9 batched
  * path-count pull of split vertices, 10 batched dependency pull of split
  * vertices, 11 their chunk combine, 12 plain forward hub chunks, 13 plain
  * dependency pull hub chunks, 14 batched bottom-up level, 15 its split
- * vertices, 16 workgroup sweep, 17 plain bottom-up level, 18 its hub
- * chunks, 19 calibration read, 20 calibration gather.  axbc_kernel_stats
+ * vertices, 16 workgroup sweep, 17 and 18 reserved (not launched), 19
+ * calibration read, 20 calibration gather.  axbc_kernel_stats
  * returns the summed milliseconds,
  * the launch count and the byte count of one kind since enabling.
  * axbc_kernel_timing_kinds enables the same for the kinds whose bit is set
