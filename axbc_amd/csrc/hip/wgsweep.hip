@@ -6,7 +6,7 @@
 // pays several grid-wide kernel boundaries per level.  Here each root of the
 // batch owns one workgroup (one CU) for its whole round: the levels are
 // separated by workgroup barriers only, so no level waits for another root
-// or for a kernel launch, and up to 128 roots run side by side.
+// or for a kernel launch, and up to 256 roots (one per CU) run side by side.
 //
 // Per root (lane l) the state is root-major, so that the neighbours of a
 // vertex share cache lines:

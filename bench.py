@@ -88,22 +88,24 @@ def log_kernels(kstats):
 
 
 def measured_traffic(kernel, batch, chunk, pull):
-    """HBM bytes per launch of `kernel` from the committed counter summary,
-    or None when the kernel was not measured at this batch width, split
-    threshold and pull ratio."""
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (one entry per batch width and split threshold the autotuner can pick),
+    or None when the kernel was not measured at these settings."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("kernels", {}).get(kernel)
-    if not k or t.get("workload") != WORKLOAD["workload"]:
+    if t.get("workload") != WORKLOAD["workload"]:
         return None
     # the backward sweep and the update do not depend on the forward direction switch
     backward = kernel.startswith("k_bdep") or kernel == "k_bupdate"
-    if (t.get("batch_width"), t.get("chunk")) != (batch, chunk) or (not backward and t.get("pull") != pull):
-        return None
-    return int(k["hbm_bytes_per_launch"])
+    for s in t.get("settings", []):
+        if (s.get("batch_width"), s.get("chunk")) != (batch, chunk) or (not backward and s.get("pull") != pull):
+            continue
+        k = s.get("kernels", {}).get(kernel)
+        return int(k["hbm_bytes_per_launch"]) if k else None
+    return None
 
 
 def copy_calibration(mib):
