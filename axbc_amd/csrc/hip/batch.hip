@@ -86,6 +86,9 @@ constexpr int kPersistLevels = 64;
 // make cheaper barriers.
 constexpr int kPersistPerCu = 1;
 
+constexpr int kHandSlices = 8;   // item counters of a split kernel: one per XCD
+constexpr int kHandStride = 16;  // 128 bytes between counters
+
 struct BCtl {
   uint32_t tail;       // list length
   uint32_t done;
@@ -97,6 +100,11 @@ struct BCtl {
   uint32_t pad;
   unsigned long long split_claim;  // (entry index << 40) | chunk base
   unsigned long long cand_edges;   // in-edges of vertices not yet reached by every root
+  // chunk-item hand-out of k_bdep_split (dep, cleared by k_bstrip) and of
+  // k_bpull_split (pull, cleared at each level's end): counter c (one per
+  // XCD, each on its own 128-byte line) hands out slice c of the items
+  unsigned long long hand_dep[kHandSlices * kHandStride];
+  unsigned long long hand_pull[kHandSlices * kHandStride];
   // device-resident loop state, so that every level (depth) runs the same
   // launch sequence and a prebuilt graph can replay it
   int level;                       // forward level the next kernels run
@@ -391,8 +399,11 @@ __device__ __forceinline__ void app_push(Appender &a, bool pred, uint32_t v, axb
 // Split vertices.  An entry with more than `ch` edges is claimed with one
 // 64-bit atomic that returns both its slot k and the first of its
 // ceil(d/ch) chunk items (split_claim = entries << 40 | items), so sp_base[]
-// increases with k.  The split kernels give each wave one contiguous range
-// of items: a binary search over sp_base for the first, then a forward walk.
+// increases with k, and the item -> slot map after the slots is filled.  The
+// push and path-count split kernels give each wave one contiguous range of
+// items (a binary search over sp_base for the first, then a forward walk);
+// the bottom-up and dependency split kernels hand items out one at a time
+// (hand_next) and find their slot in the map.
 // The lanes of a wave claim together (pred: this lane's entry i is split,
 // with nch items): one atomic for the wave, slots and items in lane order.
 // Atomics on one address are served one at a time (about 11 ns each), and a
@@ -409,16 +420,74 @@ __device__ __forceinline__ void claim_split_wave(axbc::BCtl *ctl, bool pred, uin
   unsigned long long old = 0;
   if (lane == 0) old = atomicAdd(&ctl->split_claim, ((unsigned long long)__popcll(m) << 40) + total);
   old = __shfl(old, 0);
+  const uint32_t k = (uint32_t)(old >> 40) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  const uint64_t base = (old & ((1ull << 40) - 1)) + (incl - items);
+  const bool fits = k < sp_cap && base + nch <= item_cap;
   if (pred) {
-    const uint32_t k = (uint32_t)(old >> 40) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    const uint64_t base = (old & ((1ull << 40) - 1)) + (incl - items);
-    if (k < sp_cap && base + nch <= item_cap) {
+    if (fits) {
       sp_e[k] = i;
       sp_base[k] = base;
     } else {
       atomicOr(&ctl->overflow, 1u);
     }
   }
+  // the item -> slot map after the sp_cap slots of sp_e (item_cap words):
+  // each claimed entry's items, written by the whole wave
+  uint32_t *item_slot = sp_e + sp_cap;
+  unsigned long long todo = __ballot(pred && fits);
+  while (todo) {
+    const int src = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    const uint32_t ks = rdl(k, src), ns = rdl(nch, src);
+    const uint64_t bs = __shfl(base, src);
+    for (uint32_t t = (uint32_t)lane; t < ns; t += kWave) item_slot[bs + t] = ks;
+  }
+}
+
+// Dynamic hand-out of `total` chunk items, one per claim: the items are cut
+// into kHandSlices slices with a counter each, a wave starts at its XCD's
+// slice (workgroups are dealt to the XCDs round robin) and moves on to the
+// next slice when its own is used up.  Atomics on one address are served
+// one at a time, so a single counter would bound the launch at ~10 ns per
+// item.  Returns false when every slice is used up.
+struct Hand {
+  uint64_t per;
+  int c, left;
+};
+
+__device__ __forceinline__ Hand hand_begin(uint64_t total) {
+  Hand h;
+  h.per = (total + kHandSlices - 1) / kHandSlices;
+  h.c = (int)(blockIdx.x % kHandSlices);
+  h.left = kHandSlices;
+  return h;
+}
+
+__device__ __forceinline__ bool hand_next(Hand &h, unsigned long long *ctr, uint64_t total, uint64_t &item) {
+  while (h.left > 0) {
+    const uint64_t lo = (uint64_t)h.c * h.per, hi = min(total, lo + h.per);
+    unsigned long long got = ~0ull;
+    if (lo < hi && lane_id() == 0) got = atomicAdd(&ctr[h.c * kHandStride], 1ull);
+    got = __shfl(got, 0);
+    if (lo < hi && lo + got < hi) {
+      item = lo + got;
+      return true;
+    }
+    h.c = (h.c + 1) % kHandSlices;
+    --h.left;
+  }
+  return false;
+}
+
+// The slot of chunk item `item` and its item range [base, next) (total =
+// the items claimed, ns = the slots claimed).
+__device__ __forceinline__ uint32_t item_entry(const uint32_t *__restrict__ sp_e,
+                                               const uint64_t *__restrict__ sp_base, uint32_t sp_cap, uint32_t ns,
+                                               uint64_t total, uint64_t item, uint64_t &base, uint64_t &next) {
+  const uint32_t k = min(sp_e[sp_cap + item], ns - 1);
+  base = sp_base[k];
+  next = k + 1 < ns ? sp_base[k + 1] : total;
+  return k;
 }
 
 struct ItemWalk {
@@ -632,6 +701,7 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     ctl->overflow = 0;
     ctl->wg_done = 0;
     ctl->split_claim = 0;
+    for (int c = 0; c < kHandSlices; ++c) ctl->hand_pull[c * kHandStride] = 0;
     ctl->bu = 0;
     ctl->cand_edges = in_edges;
     ctl->level = 1;
@@ -1199,28 +1269,32 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull_split(const uin
   const uint32_t mark_id = ctl->mark_base + (uint32_t)ctl->level;
   const uint64_t all[2] = {ctl->all[0], ctl->all[1]};
   const int lane = lane_id();
-  ItemWalk r = walk_begin(ctl, sp_base, sp_cap, item_cap);
+  const unsigned long long claim = ctl->split_claim;
+  const uint32_t ns = min((uint32_t)(claim >> 40), sp_cap);
+  const uint64_t total = min((uint64_t)(claim & ((1ull << 40) - 1)), item_cap);
+  if (ns == 0 || total == 0) return;
   __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
   Appender app{s_app[threadIdx.x / kWave], 0, q, &ctl->tail, qcap};
-  const bool has_items = r.lo < r.hi;
+  bool has_items = false;
   unsigned long long bytes = 0;
   uint32_t dag_tot[NW];
 #pragma unroll
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
   uint32_t w = 0, st = 0, d = 0;
-  uint64_t base = 0;
+  uint64_t base = 0, next = 0;  // the current entry's items: [base, next)
   Words<NW> open;
-  bool fresh_entry = true;
-  for (uint64_t item = r.lo; item < r.hi; ++item) {
-    if (walk_to(r, sp_base, item) || fresh_entry) {
-      w = sp_e[r.k];
+  // items handed out one at a time, as in k_bdep_split
+  Hand hand = hand_begin(total);
+  uint64_t item = 0;
+  while (hand_next(hand, ctl->hand_pull, total, item)) {
+    has_items = true;
+    if (item < base || item >= next) {
+      w = sp_e[item_entry(sp_e, sp_base, sp_cap, ns, total, item, base, next)];
       st = rcol[w];
       d = rcol[w + 1] - st;
-      base = sp_base[r.k];
       const Words<NW> vw = ldw<NW>(vis, w);
 #pragma unroll
       for (int j = 0; j < NW; ++j) open.w[j] = all[j] & ~vw.w[j];
-      fresh_entry = false;
     }
     const uint32_t cb = (uint32_t)(item - base) * kChunk, ce = min(d, cb + kChunk);
     if (cb >= ce) continue;
@@ -1526,6 +1600,7 @@ __device__ __forceinline__ void bclose_end_body(axbc::BCtl *ctl, uint32_t *qoff,
   const unsigned long long cand = ctl->cand_edges;
   ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe ? 1u : 0u;
   ctl->split_claim = 0;
+  for (int c = 0; c < kHandSlices; ++c) ctl->hand_pull[c * kHandStride] = 0;
   if (!ctl->done) ctl->level = L + 1;
 }
 
@@ -1630,6 +1705,7 @@ __global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const ui
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->depth = d;
     ctl->split_claim = 0;
+    for (int c = 0; c < kHandSlices; ++c) ctl->hand_dep[c * kHandStride] = 0;
   }
   bstrip_body<NW>(vis, q, qw, qoff, ctl, d);
 }
@@ -1884,10 +1960,14 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
 // Chunks of the split entries: per item the partial sums of its kChunk
 // edges (left fold from 0.0) go to its row of `part`.
 template <int NW>
+// Items are handed out one at a time (hand_next), not in fixed ranges per
+// wave: their costs differ with the roots that have terms there, and with
+// fixed ranges the launch waits for its slowest wave.  An item's partial row
+// is the same whichever wave computes it.
 __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint32_t *__restrict__ col, const uint32_t *__restrict__ row,
                                                        const uint64_t *__restrict__ vis, const uint32_t *__restrict__ sig,
                                                        const double *__restrict__ xr, const uint32_t *__restrict__ q,
-                                                       const uint64_t *__restrict__ qw, const axbc::BCtl *ctl,
+                                                       const uint64_t *__restrict__ qw, axbc::BCtl *ctl,
                                                        const uint32_t *__restrict__ sp_e,
                                                        const uint64_t *__restrict__ sp_base, uint32_t sp_cap,
                                                        uint64_t part_rows, double *part, unsigned long long *kb,
@@ -1897,21 +1977,23 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint
   if (depth < ctl->dmin) return;
   const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
   const int lane = lane_id();
-  ItemWalk r = walk_begin(ctl, sp_base, sp_cap, part_rows);
-  if (r.lo >= r.hi) return;
+  const unsigned long long claim = ctl->split_claim;
+  const uint32_t ns = min((uint32_t)(claim >> 40), sp_cap);
+  const uint64_t total = min((uint64_t)(claim & ((1ull << 40) - 1)), part_rows);
+  if (ns == 0 || total == 0) return;
   unsigned long long bytes = 0;
   uint32_t v = 0, st = 0, d = 0;
-  uint64_t base = 0;
+  uint64_t base = 0, next = 0;  // the current entry's items: [base, next)
   Words<NW> f;
   double sv[NW];
-  bool fresh = true;
-  for (uint64_t item = r.lo; item < r.hi; ++item) {
-    if (walk_to(r, sp_base, item) || fresh) {
-      const uint32_t i = sp_e[r.k];
+  Hand hand = hand_begin(total);
+  uint64_t item = 0;
+  while (hand_next(hand, ctl->hand_dep, total, item)) {
+    if (item < base || item >= next) {
+      const uint32_t i = sp_e[item_entry(sp_e, sp_base, sp_cap, ns, total, item, base, next)];
       v = q[i];
       st = col[v];
       d = col[v + 1] - st;
-      base = sp_base[r.k];
       f = ldw<NW>(qw, i);
       if (depth == 0) {
         f.w[0] &= root_mask0;
@@ -1919,7 +2001,6 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep_split(const uint
       }
 #pragma unroll
       for (int j = 0; j < NW; ++j) sv[j] = bit(f.w[j], lane) ? (double)sig[(size_t)v * B + lane + 64 * j] : 1.0;
-      fresh = false;
     }
     const uint32_t cb = (uint32_t)(item - base) * kChunk, ce = min(d, cb + kChunk);
     if (cb >= ce) continue;
@@ -2565,7 +2646,7 @@ int alloc_buffers(BatchState *b) {
   BA(b->qw, qcap * nw);
   BA(b->qoff, n + 2);
   BA(b->act, (n + 2) * nw);
-  BA(b->sp_e, b->sp_cap);
+  BA(b->sp_e, b->sp_cap + b->part_rows);  // the slots' entries, then the item -> slot map
   BA(b->sp_base, b->sp_cap);
   BA(b->part, b->part_rows * B);
   BA(b->cnt, 4 * B);
