@@ -105,6 +105,10 @@ struct BCtl {
   // XCD, each on its own 128-byte line) hands out slice c of the items
   unsigned long long hand_dep[kHandSlices * kHandStride];
   unsigned long long hand_pull[kHandSlices * kHandStride];
+  // 64-entry groups of k_bdep's dense levels (cleared by k_bstrip) and
+  // 64-vertex groups of the bottom-up pull (cleared at each level's end)
+  unsigned long long hand_ent[kHandSlices * kHandStride];
+  unsigned long long hand_vert[kHandSlices * kHandStride];
   // device-resident loop state, so that every level (depth) runs the same
   // launch sequence and a prebuilt graph can replay it
   int level;                       // forward level the next kernels run
@@ -701,7 +705,7 @@ __global__ void k_broots(const uint32_t *__restrict__ src, int nb, uint64_t *vis
     ctl->overflow = 0;
     ctl->wg_done = 0;
     ctl->split_claim = 0;
-    for (int c = 0; c < kHandSlices; ++c) ctl->hand_pull[c * kHandStride] = 0;
+    for (int c = 0; c < kHandSlices; ++c) ctl->hand_pull[c * kHandStride] = ctl->hand_vert[c * kHandStride] = 0;
     ctl->bu = 0;
     ctl->cand_edges = in_edges;
     ctl->level = 1;
@@ -1142,7 +1146,8 @@ __device__ __forceinline__ void bpull_body(uint32_t n, const uint32_t *__restric
                                                   uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                   unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
-                                                  unsigned long long *kb, uint32_t kChunk) {
+                                                  unsigned long long *kb, uint32_t kChunk,
+                                                  unsigned long long *hand) {
   constexpr int B = 64 * NW;
   if (ld_flag(&ctl->done) || !ctl->bu) return;
   const int L = ctl->level;
@@ -1166,7 +1171,19 @@ __device__ __forceinline__ void bpull_body(uint32_t n, const uint32_t *__restric
   for (int j = 0; j < NW; ++j) dag_tot[j] = 0;
   __shared__ uint32_t s_app[kBlock / kWave][kAppendBuf];
   Appender app{s_app[threadIdx.x / kWave], 0, q, &ctl->tail, qcap};
-  for (uint32_t vb = gw * kWave; vb < n; vb += nwv * kWave) {
+  // groups of 64 vertices: handed out one at a time (hand), else the groups
+  // gw + t nwv
+  const uint64_t ng = ((uint64_t)n + kWave - 1) / kWave;
+  Hand h = hand_begin(ng);
+  uint64_t g = gw;
+  for (;;) {
+    if (hand) {
+      if (!hand_next(h, hand, ng, g)) break;
+    } else if (g >= ng) {
+      break;
+    }
+    const uint32_t vb = (uint32_t)(g * kWave);
+    if (!hand) g += nwv;
     const uint32_t v = vb + (uint32_t)lane;
     uint32_t st = 0, d = 0;
     Words<NW> open;  // roots that have not reached v
@@ -1228,7 +1245,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bpull(uint32_t n, con
                                                   uint32_t *q, const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                   unsigned long long *wslot, uint32_t *sp_e, uint64_t *sp_base,
                                                   uint32_t sp_cap, uint64_t item_cap, uint32_t qcap,
-                                                  unsigned long long *kb, uint32_t kChunk) { bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap, kb, kChunk); }
+                                                  unsigned long long *kb, uint32_t kChunk) { bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap, kb, kChunk, ctl->hand_vert); }
 
 // Level expansion in one launch: the top-down push or the bottom-up pull,
 // as the previous level's close decided (one launch per level instead of
@@ -1244,7 +1261,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bexpand(
   if (ld_flag(&ctl->done)) return;
   if (ctl->bu)
     bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap, qcap,
-                   nullptr, kChunk);
+                   nullptr, kChunk, ctl->hand_vert);
   else
     bpush_body<NW>(col, row, vis, nx, mark, fr0, fr1, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, item_cap, qcap,
                    nullptr, kChunk);
@@ -1600,7 +1617,7 @@ __device__ __forceinline__ void bclose_end_body(axbc::BCtl *ctl, uint32_t *qoff,
   const unsigned long long cand = ctl->cand_edges;
   ctl->bu = pull16 && cand * 16ull <= (unsigned long long)pull16 * fe ? 1u : 0u;
   ctl->split_claim = 0;
-  for (int c = 0; c < kHandSlices; ++c) ctl->hand_pull[c * kHandStride] = 0;
+  for (int c = 0; c < kHandSlices; ++c) ctl->hand_pull[c * kHandStride] = ctl->hand_vert[c * kHandStride] = 0;
   if (!ctl->done) ctl->level = L + 1;
 }
 
@@ -1705,7 +1722,7 @@ __global__ void k_bstrip(uint64_t *vis, const uint32_t *__restrict__ q, const ui
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->depth = d;
     ctl->split_claim = 0;
-    for (int c = 0; c < kHandSlices; ++c) ctl->hand_dep[c * kHandStride] = 0;
+    for (int c = 0; c < kHandSlices; ++c) ctl->hand_dep[c * kHandStride] = ctl->hand_ent[c * kHandStride] = 0;
   }
   bstrip_body<NW>(vis, q, qw, qoff, ctl, d);
 }
@@ -1863,7 +1880,8 @@ __device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, cons
                                                  const uint32_t *__restrict__ qoff, axbc::BCtl *ctl,
                                                  uint32_t *sp_e, uint64_t *sp_base, uint32_t sp_cap, uint64_t part_rows,
                                                  unsigned long long *kb, uint32_t kChunk,
-                                                 const uint8_t *__restrict__ lmode, int depth) {
+                                                 const uint8_t *__restrict__ lmode, int depth,
+                                                 unsigned long long *hand) {
   if (depth < ctl->dmin) return;
   const uint64_t root_mask0 = ctl->rootmask[0], root_mask1 = ctl->rootmask[1];
   const uint32_t b = qoff[depth], e = qoff[depth + 1];
@@ -1872,10 +1890,24 @@ __device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, cons
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwv = (gridDim.x * blockDim.x) / kWave;
   if (!lmode[depth]) {
     // dense level: one entry per wave at a time, lanes = roots; the wave's
-    // entries b + gw + t nwv are loaded 64 at a time (lane = entry) so that
-    // their split entries are claimed together
-    for (uint64_t i0 = (uint64_t)b + gw; i0 < e; i0 += (uint64_t)nwv * kWave) {
-      const uint64_t il = i0 + (uint64_t)lane * nwv;
+    // entries are loaded 64 at a time (lane = entry) so that their split
+    // entries are claimed together.  Group (r, w) holds the entries
+    // b + w + (64 r + lane) nwv, so the groups that run at one time hold
+    // neighbouring list entries (listed together, they share neighbours and
+    // cache lines).  Without a hand-out, wave w takes the groups (0, w),
+    // (1, w), ...; with one (hand), a wave takes the groups r nwv + w one at
+    // a time.
+    const uint64_t rounds = ((uint64_t)e - b + (uint64_t)nwv * kWave - 1) / ((uint64_t)nwv * kWave);
+    Hand h = hand_begin(rounds * nwv);
+    uint64_t g = gw;
+    for (;;) {
+      if (hand) {
+        if (!hand_next(h, hand, rounds * nwv, g)) break;
+      } else if (g >= rounds * nwv) {
+        break;
+      }
+      const uint64_t il = (uint64_t)b + g % nwv + ((g / nwv) * kWave + (uint64_t)lane) * nwv;
+      if (!hand) g += nwv;
       const bool in = il < e;
       const uint32_t i = in ? (uint32_t)il : 0u;
       uint32_t v = 0, st = 0, d = 0;
@@ -1954,7 +1986,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd) void k_bdep(const uint32_t *
   const int d = ctl->depth;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->sdepth = d >= ctl->dmin ? d - 1 : d;  // the next step's
   bdep_body<NW>(col, row, vis, sig, dlt, xr, reach, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, part_rows, kb, kChunk,
-                lmode, d);
+                lmode, d, ctl->hand_ent);
 }
 
 // Chunks of the split entries: per item the partial sums of its kChunk
@@ -2211,7 +2243,7 @@ __global__ __launch_bounds__(kBlock) void k_bfwd_persist(
     if (ld_flag(&ctl->done)) return;
     if (ctl->bu)
       bpull_body<NW>(n, rcol, rrow, vis, nx, fr0, fr1, sig, q, qoff, ctl, wslot, sp_e, sp_base, sp_cap, item_cap,
-                     qcap, nullptr, kChunk);
+                     qcap, nullptr, kChunk, nullptr);
     else
       bpush_body<NW>(col, row, vis, nx, mark, fr0, fr1, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, item_cap, qcap,
                      nullptr, kChunk);
@@ -2239,7 +2271,7 @@ __global__ __launch_bounds__(kBlock) void k_bbwd_persist(
     bstrip_body<NW>(vis, q, qw, qoff, ctl, depth);
     if (!grid_sync(bar, nblocks, limit, prog)) return;
     bdep_body<NW>(col, row, vis, sig, dlt, xr, reach, q, qw, qoff, ctl, sp_e, sp_base, sp_cap, part_rows, nullptr,
-                  kChunk, lmode, depth);
+                  kChunk, lmode, depth, nullptr);
     if (!grid_sync(bar, nblocks, limit, prog)) return;
   }
 }
