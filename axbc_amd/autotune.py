@@ -24,7 +24,7 @@ PULLS = (0, 16, 32, 64)
 
 def _trial(e: Engine, roots: np.ndarray, width: int, chunk: int, reps: int, pull: int = 32) -> float:
     L = _ffi.hip()
-    _check(L.axbc_set_batch_width(e.ctx, 128 if width == 128 else 64))
+    _check(L.axbc_set_batch_width(e.ctx, width if width >= 128 else 64))
     _check(L.axbc_set_batch_chunk(e.ctx, chunk))
     _check(L.axbc_set_batch_pull(e.ctx, pull))
     p = np.full(width, 1.0 / e.N)
@@ -58,7 +58,7 @@ def tune(e: Engine, roots: np.ndarray, reps: int = 2, widths=WIDTHS, chunks=CHUN
             table[(w, ch, pl)] = _trial(e, roots, w, ch, reps, pl)
     (w, ch, pl), best = min(table.items(), key=lambda kv: kv[1])
     L = _ffi.hip()
-    _check(L.axbc_set_batch_width(e.ctx, 128 if w == 128 else 64))
+    _check(L.axbc_set_batch_width(e.ctx, w if w >= 128 else 64))
     _check(L.axbc_set_batch_chunk(e.ctx, ch))
     _check(L.axbc_set_batch_pull(e.ctx, pl))
     e.set_bc(saved)

@@ -160,6 +160,7 @@ struct BatchState {
   uint32_t stamp = 0;
   uint32_t epoch = 0;  // level stamps for `mark`, unique across batches
   bool have = false;
+  bool had_prev = false;  // a sweep ran in this state before the current one
   int lev_guess = 12;
   uint32_t chunk = 256;  // edges per work item of split vertices (tunable)
   // sigma rows of the vertices with more than this many pull edges are zero
@@ -2563,6 +2564,40 @@ void batch_free(axbc_ctx *c) {
   c->batch = nullptr;
 }
 
+// The batch functions act on c->batch and c->stream; for the second state
+// of a batch of more than 128 bit-packed roots this swaps in batch2 and
+// stream2 for its lifetime (one host thread drives both states).
+struct SecondSlot {
+  axbc_ctx *c;
+  explicit SecondSlot(axbc_ctx *cc) : c(cc) {
+    std::swap(c->batch, c->batch2);
+    std::swap(c->stream, c->stream2);
+  }
+  ~SecondSlot() {
+    std::swap(c->batch, c->batch2);
+    std::swap(c->stream, c->stream2);
+  }
+  SecondSlot(const SecondSlot &) = delete;
+  SecondSlot &operator=(const SecondSlot &) = delete;
+};
+
+void batch_free_all(axbc_ctx *c) {
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  batch_free(c);
+  if (c->batch2) {
+    SecondSlot two(c);
+    batch_free(c);
+  }
+  for (auto &e : c->pair_ev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  c->stream2 = nullptr;
+  c->pair_last = false;
+}
+
 }  // namespace axbc
 
 namespace {
@@ -3137,25 +3172,64 @@ int wg_batch(axbc_ctx *c, int nb, std::vector<int> &nlev) {
 // The bit-packed sweep of a batch whose roots, probabilities and lanes are
 // on the device: forward levels, per-root counters, the delta rows and the
 // backward depths.  nlev[l] receives lane l's deepest level.
+// A bit-packed sweep in three phases, so that two states can be driven side
+// by side from one host thread: sweep_begin enqueues the set-up and the
+// first chunk of forward levels, sweep_forward waits for the chunk and
+// enqueues the next until the forward sweep is done, sweep_backward takes
+// the per-root depths and enqueues the backward sweep.
+struct SweepRun {
+  int nb = 0;
+  bool split_out = false, split_in = false, graphs = false, persist = false, done = false;
+  int L = 1, chunk = 0, zeroed = 0;
+};
+
+// One chunk of forward levels from level r.L on.
 template <int NW>
-int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs, std::vector<int> &nlev) {
+int forward_chunk(axbc_ctx *c, SweepRun &r) {
   BatchState *b = c->batch;
   hipStream_t s = c->stream;
   const uint32_t n = c->n;
+  const int steps = r.graphs ? (r.chunk + kGraphSteps - 1) / kGraphSteps * kGraphSteps : r.chunk;
+  // per-level root words of the levels about to run (a BFS has at most n
+  // levels: the buffer holds n + 2 rows)
+  if (r.L + steps + 2 > r.zeroed && r.zeroed < (int)n + 2) {
+    const int upto = std::min(r.L + steps + 2, (int)n + 2);
+    HIP_TRY(hipMemsetAsync(b->act + (size_t)r.zeroed * NW, 0, sizeof(uint64_t) * (size_t)(upto - r.zeroed) * NW, s));
+    r.zeroed = upto;
+  }
+  if (r.graphs) {
+    for (int j = 0; j < steps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gfwd, s));
+  } else {
+    Sink k;
+    k.stream = s;
+    for (int j = 0; j < steps; ++j) enqueue_level<NW>(k, c, b, r.split_out, r.split_in);
+    HIP_TRY(k.err);
+  }
+  r.L += steps;
+  return AXBC_OK;
+}
+
+template <int NW>
+int sweep_begin(axbc_ctx *c, SweepRun &r, bool may_persist) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  const int nb = r.nb;
   int rc;
   // rows of vertices above the split threshold start at zero: clear them
   // all when the threshold dropped below the one they were cleared for
   if (b->chunk < b->sig_zero_above)
     hipLaunchKernelGGL(k_bsig_clear<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, c->rcol, b->chunk, b->sig);
   b->sig_zero_above = b->chunk;
-  const bool had_prev = b->have;
+  r.done = false;
+  b->had_prev = b->have;
   b->have = true;
   b->stamp += 1;
   // roots of this batch, per word (the "reached by every root" test)
   const uint64_t all0 = nb >= 64 ? ~0ull : (1ull << nb) - 1ull;
   const uint64_t all1 = NW == 1 || nb <= 64 ? 0ull : (nb >= 128 ? ~0ull : (1ull << (nb - 64)) - 1ull);
-  int zeroed = std::min(b->lev_guess + 2, (int)n + 2);
-  HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)zeroed * NW, s));
+  r.zeroed = std::min(b->lev_guess + 2, (int)n + 2);
+  HIP_TRY(hipMemsetAsync(b->act, 0, sizeof(uint64_t) * (size_t)r.zeroed * NW, s));
   // per-wave counter slots start at zero
   hipLaunchKernelGGL(k_bzero, dim3(1024), dim3(kBlock), 0, s, b->wslot, (size_t)4 * kSlotWaves * NW * 64);
   hipLaunchKernelGGL(k_broots<NW>, dim3((nb + 63) / 64), dim3(64), 0, s, b->src, nb, b->vis, b->fr[0], b->sig, b->q,
@@ -3163,18 +3237,19 @@ int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs
                      all0, all1, c->reach, b->dlt, b->lmode);
   hipLaunchKernelGGL(k_broot_deg, dim3((nb + 63) / 64), dim3(64), 0, s, c->col, b->src, nb, b->cnt);
   HIP_TRY(hipGetLastError());
-  int L = 1, chunk = b->lev_guess;
+  r.L = 1;
+  r.chunk = b->lev_guess;
   // deep graphs without split vertices: the whole sweep in one persistent
   // cooperative launch per direction (no launch per level)
-  bool persist = c->batch_persist != 0 && !split_out && !split_in && !c->ktimer.on &&
+  bool persist = may_persist && c->batch_persist != 0 && !r.split_out && !r.split_in && !c->ktimer.on &&
                  (c->batch_persist == 2 || b->lev_guess > kPersistLevels);
   if (persist && !b->coop_fwd) {
     b->coop_fwd = coop_grid(c, k_bfwd_persist<NW>);
     b->coop_bwd = coop_grid(c, k_bbwd_persist<NW>);
   }
-  persist = persist && b->coop_fwd > 0 && b->coop_bwd > 0;
-  const unsigned long long limit = c->batch_persist_limit;
-  if (persist) {
+  r.persist = persist && b->coop_fwd > 0 && b->coop_bwd > 0;
+  if (r.persist) {
+    const unsigned long long limit = c->batch_persist_limit;
     HIP_TRY(hipMemsetAsync(b->prog, 0, sizeof(uint32_t) * kBGrid, s));
     hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, s, b->bar);
     HIP_TRY(coop_launch(k_bfwd_persist<NW>, dim3(b->coop_fwd), dim3(kBlock), s, n, c->col, c->row, c->rcol, c->rrow,
@@ -3184,44 +3259,48 @@ int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs
     if ((rc = pull_bctl(c)) || (rc = pull_bar(c))) return rc;
     if (b->h_bar[kBarErr]) return persist_failed(c);
     if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
-    L = (int)b->h_ctl->empty_lev + 1;  // levels run (the epoch of the level stamps moves past them)
+    r.L = (int)b->h_ctl->empty_lev + 1;  // levels run (the epoch of the level stamps moves past them)
+    r.done = true;
+    return AXBC_OK;
   }
   // forward levels, speculatively in chunks; kernels past the last level
   // exit on the device done flag
-  for (; !persist;) {
-    const int steps = graphs ? (chunk + kGraphSteps - 1) / kGraphSteps * kGraphSteps : chunk;
-    // per-level root words of the levels about to run (a BFS has at most
-    // n levels: the buffer holds n + 2 rows)
-    if (L + steps + 2 > zeroed && zeroed < (int)n + 2) {
-      const int upto = std::min(L + steps + 2, (int)n + 2);
-      HIP_TRY(hipMemsetAsync(b->act + (size_t)zeroed * NW, 0, sizeof(uint64_t) * (size_t)(upto - zeroed) * NW, s));
-      zeroed = upto;
-    }
-    if (graphs) {
-      for (int j = 0; j < steps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gfwd, s));
-    } else {
-      Sink k;
-      k.stream = s;
-      for (int j = 0; j < steps; ++j) enqueue_level<NW>(k, c, b, split_out, split_in);
-      HIP_TRY(k.err);
-    }
-    L += steps;
-    if ((rc = pull_bctl(c))) return rc;
-    if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
-    if (b->h_ctl->done) break;
-    if ((uint64_t)L > (uint64_t)n + 1) return fail(AXBC_ESTATE, "batch forward did not terminate%s (line %d)", "", __LINE__);
-    // past the previous batch's depth: grow by an eighth of the levels run
-    // (levels launched past the last one still cost a few microseconds per
-    // kernel, so doubling overshot deep graphs by thousands of levels)
-    chunk = std::max(16, L / 8);
+  return forward_chunk<NW>(c, r);
+}
+
+template <int NW>
+int sweep_forward(axbc_ctx *c, SweepRun &r) {
+  BatchState *b = c->batch;
+  int rc;
+  if ((rc = pull_bctl(c))) return rc;
+  if (b->h_ctl->overflow) return fail(AXBC_ENOMEM, "batch list capacity exceeded%s (line %d)", "", __LINE__);
+  if (b->h_ctl->done) {
+    r.done = true;
+    return AXBC_OK;
   }
+  if ((uint64_t)r.L > (uint64_t)c->n + 1)
+    return fail(AXBC_ESTATE, "batch forward did not terminate%s (line %d)", "", __LINE__);
+  // past the previous batch's depth: grow by an eighth of the levels run
+  // (levels launched past the last one still cost a few microseconds per
+  // kernel, so doubling overshot deep graphs by thousands of levels)
+  r.chunk = std::max(16, r.L / 8);
+  return forward_chunk<NW>(c, r);
+}
+
+template <int NW>
+int sweep_backward(axbc_ctx *c, SweepRun &r, std::vector<int> &nlev) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  const int nb = r.nb;
+  int rc;
   // per-root counters from the wave slots of the forward sweep
   hipLaunchKernelGGL(k_bcnt_reduce<NW>, dim3(64, 4), dim3(64 * NW), 0, s, (const unsigned long long *)b->wslot, b->cnt);
   const int empty = (int)b->h_ctl->empty_lev;
   const int dmax = empty - 1;  // deepest non-empty level over the batch
   b->levels = empty;
   b->lev_guess = dmax + 2;
-  b->epoch += (uint32_t)(L + 1);
+  b->epoch += (uint32_t)(r.L + 1);
   // per-root deepest level from the active words: roots whose deepest
   // level is 1 also accumulate their own dependency (bc_func_mono's do-while)
   HIP_TRY(hipMemcpyAsync(b->h_act, b->act, sizeof(uint64_t) * (size_t)(empty + 1) * NW, hipMemcpyDeviceToHost, s));
@@ -3234,7 +3313,7 @@ int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs
   }
   // delta = reach where the batch did not reach (VIS is still whole)
   hipLaunchKernelGGL(k_bfix<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, b->stamp - 1,
-                     had_prev ? 1 : 0, b->vis, c->reach, b->dlt);
+                     b->had_prev ? 1 : 0, b->vis, c->reach, b->dlt);
   // delta and x of the deepest level's entries (the dependency pass sets
   // the others)
   if (dmax >= 1)
@@ -3247,21 +3326,21 @@ int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs
   if (dsteps > 0) {
     hipLaunchKernelGGL(k_bset_depth, dim3(1), dim3(64), 0, s, b->ctl, dmax - 1, dmin, rootmask[0],
                        NW > 1 ? rootmask[1] : 0ull);
-    if (persist) {
+    if (r.persist) {
       HIP_TRY(hipMemsetAsync(b->prog, 0, sizeof(uint32_t) * kBGrid, s));
       hipLaunchKernelGGL(k_bar_reset, dim3(1), dim3(64), 0, s, b->bar);
       HIP_TRY(coop_launch(k_bbwd_persist<NW>, dim3(b->coop_bwd), dim3(kBlock), s, c->col, c->row, b->vis, b->sig,
                           b->dlt, b->xr, c->reach, b->q, b->qw, b->qoff, b->ctl, b->sp_e, b->sp_base, b->sp_cap,
-                          b->part_rows, b->chunk, (const uint8_t *)b->lmode, dmax - 1, dmin, limit,
+                          b->part_rows, b->chunk, (const uint8_t *)b->lmode, dmax - 1, dmin, c->batch_persist_limit,
                           b->bar, b->prog));
       if ((rc = pull_bar(c))) return rc;
       if (b->h_bar[kBarErr]) return persist_failed(c);
-    } else if (graphs) {
+    } else if (r.graphs) {
       for (int j = 0; j < dsteps; j += kGraphSteps) HIP_TRY(hipGraphLaunch(b->gbwd, s));
     } else {
       Sink k;
       k.stream = s;
-      for (int j = 0; j < dsteps; ++j) enqueue_depth<NW>(k, c, b, split_out);
+      for (int j = 0; j < dsteps; ++j) enqueue_depth<NW>(k, c, b, r.split_out);
       HIP_TRY(k.err);
     }
   }
@@ -3269,6 +3348,15 @@ int packed_sweep(axbc_ctx *c, int nb, bool split_out, bool split_in, bool graphs
   b->last_wg = false;
   b->wg_pair = false;
   return AXBC_OK;
+}
+
+template <int NW>
+int packed_sweep(axbc_ctx *c, SweepRun &r, std::vector<int> &nlev) {
+  int rc;
+  if ((rc = sweep_begin<NW>(c, r, true))) return rc;
+  while (!r.done)
+    if ((rc = sweep_forward<NW>(c, r))) return rc;
+  return sweep_backward<NW>(c, r, nlev);
 }
 
 // Lanes of a group of at most B roots: by degree, highest first (ties in
@@ -3313,6 +3401,24 @@ int upload_lanes(axbc_ctx *c, const Lanes &ln, int nb) {
   return AXBC_OK;
 }
 
+// The per-root stats of the last sweep in c->batch (nb entries, by draw
+// order; nlev by lane) from its counters.
+int fill_stats(axbc_ctx *c, int nb, const Lanes &ln, const std::vector<int> &nlev, axbc_pivot_stats *stats) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(b->h_cnt, b->cnt, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int k = 0; k < nb; ++k) {
+    const uint32_t l = ln.lane_of[k];
+    stats[k].nvisited = b->h_cnt[4 * l + 0];
+    stats[k].nlevels = (uint64_t)nlev[l];
+    stats[k].e_r = b->h_cnt[4 * l + 2];
+    stats[k].e_dag = b->h_cnt[4 * l + 3];
+    stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * l + 1] * (c->heuristic == 1 || c->heuristic == 3) + ln.r0[l]);
+  }
+  return AXBC_OK;
+}
+
 // After a sweep of nb roots whose lanes, counters (b->cnt) and delta rows
 // (b->dlt) are in place: the exact stop test per root, the bc update of the
 // applied roots, the context's delta row, and the stats (nb entries, by
@@ -3353,49 +3459,128 @@ int finish_batch(axbc_ctx *c, int nb, uint32_t stop_c, const Lanes &ln, const st
     c->dirty_delta = true;
   }
   if (applied) *applied = nr;
-  if (stats) {
-    HIP_TRY(hipMemcpyAsync(b->h_cnt, b->cnt, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    for (int k = 0; k < nb; ++k) {
-      const uint32_t l = ln.lane_of[k];
-      stats[k].nvisited = b->h_cnt[4 * l + 0];
-      stats[k].nlevels = (uint64_t)nlev[l];
-      stats[k].e_r = b->h_cnt[4 * l + 2];
-      stats[k].e_dag = b->h_cnt[4 * l + 3];
-      stats[k].all = (uint32_t)((uint32_t)b->h_cnt[4 * l + 1] * (c->heuristic == 1 || c->heuristic == 3) + ln.r0[l]);
-    }
-  }
+  return stats ? fill_stats(c, nb, ln, nlev, stats) : AXBC_OK;
+}
+
+// Lanes, graphs and the reset of the state's previous bit-packed sweep,
+// before a sweep of nb roots in c->batch on c->stream.
+template <int NW>
+int batch_prep(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, bool wg, Lanes &ln, SweepRun &r) {
+  BatchState *b = c->batch;
+  hipStream_t s = c->stream;
+  int rc;
+  ln = assign_lanes(c, b->B, v0s, ps, nb);
+  b->h_lane_of = ln.lane_of;
+  r.nb = nb;
+  // split kernels only where some vertex has more edges than a chunk
+  r.split_out = c->max_out > b->chunk;
+  r.split_in = c->max_in > b->chunk;
+  // the level loop runs from prebuilt graphs unless per-kernel timing is on
+  r.graphs = !wg && c->use_graphs && !c->ktimer.on;
+  if (r.graphs && (rc = ensure_graphs<NW>(c, r.split_out, r.split_in))) return rc;
+  if ((rc = upload_lanes(c, ln, nb))) return rc;
+  // undo the previous bit-packed sweep (a workgroup sweep leaves the
+  // bit-packed state as it found it)
+  if (b->have && !b->last_wg)
+    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, c->n, b->seen, b->stamp, c->rcol, b->chunk,
+                       b->vis, b->fr[0], b->fr[1], b->sig);
   return AXBC_OK;
 }
 
 template <int NW>
 int run_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
               axbc_pivot_stats *stats) {
-  BatchState *b = c->batch;
-  hipStream_t s = c->stream;
-  const uint32_t n = c->n;
   int rc;
-  const Lanes ln = assign_lanes(c, b->B, v0s, ps, nb);
-  b->h_lane_of = ln.lane_of;
-  // split kernels only where some vertex has more edges than a chunk
-  const bool split_out = c->max_out > b->chunk, split_in = c->max_in > b->chunk;
   const bool wg = use_wg(c);
-  // the level loop runs from prebuilt graphs unless per-kernel timing is on
-  const bool graphs = !wg && c->use_graphs && !c->ktimer.on;
-  if (graphs && (rc = ensure_graphs<NW>(c, split_out, split_in))) return rc;
-  if ((rc = upload_lanes(c, ln, nb))) return rc;
-  // undo the previous bit-packed sweep (a workgroup sweep leaves the
-  // bit-packed state as it found it)
-  if (b->have && !b->last_wg)
-    hipLaunchKernelGGL(k_breset<NW>, dim3(kBGrid), dim3(kBlock), 0, s, n, b->seen, b->stamp, c->rcol, b->chunk,
-                       b->vis, b->fr[0], b->fr[1], b->sig);
+  Lanes ln;
+  SweepRun r;
+  if ((rc = batch_prep<NW>(c, v0s, ps, nb, wg, ln, r))) return rc;
   std::vector<int> nlev(nb, 0);  // per lane
   if (wg)
     rc = wg_batch<NW>(c, nb, nlev);
   else
-    rc = packed_sweep<NW>(c, nb, split_out, split_in, graphs, nlev);
+    rc = packed_sweep<NW>(c, r, nlev);
   if (rc) return rc;
   return finish_batch<NW>(c, nb, stop_c, ln, nlev, applied, stats);
+}
+
+// A batch of nb > B roots through the bit-packed sweep: roots 0 .. B-1 in
+// the first state on the context stream, the others in the second state on
+// the second stream, the two sweeps enqueued phase by phase so that they
+// run side by side (each state is a complete sweep of its own roots, so
+// its rows are those of a batch of those roots alone).  Then the stop test
+// and the bc update in draw order: the first half, and the second only
+// when the stop test held for none of the first, after the first's update.
+// *applied and the stats count both halves.
+template <int NW>
+int run_packed_pair(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c, int *applied,
+                    axbc_pivot_stats *stats) {
+  int rc;
+  const int B = c->batch->B;
+  const int nh[2] = {B, nb - B};
+  c->pair_last = false;
+  if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  for (auto &e : c->pair_ev)
+    if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!c->batch2) {
+    SecondSlot two(c);
+    if ((rc = batch_create(c, B, c->batch2->qcap))) return rc;
+  }
+  {
+    SecondSlot two(c);
+    // the second state follows the first's settings
+    c->batch->chunk = c->batch2->chunk;
+    c->batch->pull16 = c->batch2->pull16;
+  }
+  // the second stream starts after the work enqueued on the context stream
+  HIP_TRY(hipEventRecord(c->pair_ev[0], c->stream));
+  HIP_TRY(hipStreamWaitEvent(c->stream2, c->pair_ev[0], 0));
+  Lanes ln[2];
+  SweepRun r[2];
+  std::vector<int> nlev[2] = {std::vector<int>(nh[0], 0), std::vector<int>(nh[1], 0)};
+  if ((rc = batch_prep<NW>(c, v0s, ps, nh[0], false, ln[0], r[0]))) return rc;
+  {
+    SecondSlot two(c);
+    if ((rc = batch_prep<NW>(c, v0s + B, ps + B, nh[1], false, ln[1], r[1]))) return rc;
+  }
+  // no persistent loop here: it waits on the host for its whole sweep
+  if ((rc = sweep_begin<NW>(c, r[0], false))) return rc;
+  {
+    SecondSlot two(c);
+    if ((rc = sweep_begin<NW>(c, r[1], false))) return rc;
+  }
+  while (!r[0].done || !r[1].done) {
+    if (!r[0].done && (rc = sweep_forward<NW>(c, r[0]))) return rc;
+    if (!r[1].done) {
+      SecondSlot two(c);
+      if ((rc = sweep_forward<NW>(c, r[1]))) return rc;
+    }
+  }
+  if ((rc = sweep_backward<NW>(c, r[0], nlev[0]))) return rc;
+  {
+    SecondSlot two(c);
+    if ((rc = sweep_backward<NW>(c, r[1], nlev[1]))) return rc;
+  }
+  int ap[2] = {0, 0};
+  if ((rc = finish_batch<NW>(c, nh[0], stop_c, ln[0], nlev[0], &ap[0], stats))) return rc;
+  // the second half's stop test reads bc after the first half's update
+  HIP_TRY(hipEventRecord(c->pair_ev[0], c->stream));
+  {
+    SecondSlot two(c);
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->pair_ev[0], 0));
+    if (ap[0] == nh[0]) {
+      if ((rc = finish_batch<NW>(c, nh[1], stop_c, ln[1], nlev[1], &ap[1], stats ? stats + B : nullptr))) return rc;
+    } else if (stats) {
+      // swept but not applied: the counters all the same
+      if ((rc = fill_stats(c, nh[1], ln[1], nlev[1], stats + B))) return rc;
+    }
+    HIP_TRY(hipEventRecord(c->pair_ev[1], c->stream));
+  }
+  // later work on the context stream follows the second half
+  HIP_TRY(hipStreamWaitEvent(c->stream, c->pair_ev[1], 0));
+  c->pair_last = true;
+  if (applied) *applied = ap[0] + ap[1];
+  return AXBC_OK;
 }
 
 // A batch of B < nb <= 2B roots (width 2B, workgroup sweep only): one sweep
@@ -3503,8 +3688,15 @@ int batch_utility_on(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double
 int batch_utility_any(axbc_ctx *c, uint32_t *dv, double *dw, double *part, double *dout, const uint32_t *vertex,
                       const double *weight, uint32_t count, double *out) {
   BatchState *b = c->batch;
-  if (!b->wg_pair) return batch_utility_on(c, dv, dw, part, dout, vertex, weight, count, out);
   int rc;
+  if (c->pair_last) {
+    // each state's roots from its own rows (every call ends synchronised)
+    const int B = b->B;
+    if ((rc = batch_utility_on(c, dv, dw, part, dout, vertex, weight, count, out))) return rc;
+    SecondSlot two(c);
+    return batch_utility_on(c, dv, dw, part, dout, vertex, weight, count, out + B);
+  }
+  if (!b->wg_pair) return batch_utility_on(c, dv, dw, part, dout, vertex, weight, count, out);
   for (int h = 0; h < 2; ++h) {
     HIP_TRY(wg_out(c->stream, b->nw, c->n, b->pair_nb[h], h * b->B, b->wtag_last, b->wst, b->wdl, c->reach, b->dlt,
                    b->seen, b->stamp));
@@ -3523,6 +3715,15 @@ int batch_delta_row(axbc_ctx *c, int k, const double **row) {
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
   BatchState *b = c->batch;
   int rc;
+  if (c->pair_last && k >= b->B) {
+    // a root of the second state: its row on the second stream, complete
+    // before the caller reads it on the context stream
+    const int B = b->B;
+    SecondSlot two(c);
+    if ((rc = batch_delta_row(c, k - B, row))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return AXBC_OK;
+  }
   HIP_TRY(hipSetDevice(c->device));
   if (!b->row_out) {
     if ((rc = balloc_more(c, (double **)&b->row_out, c->n, "a per-root row copy"))) return rc;
@@ -3552,11 +3753,8 @@ extern "C" {
 int axbc_set_batch_width(axbc_ctx *c, int width) {
   if (!c || (width != 64 && width != 128 && width != 256))
     return fail(AXBC_EINVAL, "batch width must be 64, 128 or 256%s (line %d)", "", __LINE__);
-  if (width == 256 && c->batch_mode != 2)
-    return fail(AXBC_EINVAL, "batch width 256 needs the workgroup sweep (axbc_set_batch_mode 2)%s (line %d)", "",
-                __LINE__);
   // the state's lanes (and the workgroup sweep's) follow the width
-  if (c->batch && c->batch_width != width) batch_free(c);
+  if (c->batch && c->batch_width != width) batch_free_all(c);
   c->batch_width = width;
   return AXBC_OK;
 }
@@ -3588,8 +3786,6 @@ int axbc_set_batch_persistent(axbc_ctx *c, int mode, double limit_seconds) {
 
 int axbc_set_batch_mode(axbc_ctx *c, int mode) {
   if (!c || mode < 0 || mode > 2) return fail(AXBC_EINVAL, "batch mode must be 0, 1 or 2%s (line %d)", "", __LINE__);
-  if (c->batch_width > 128 && mode != 2)
-    return fail(AXBC_EINVAL, "batch width 256 runs only as a workgroup sweep%s (line %d)", "", __LINE__);
   c->batch_mode = mode;
   return AXBC_OK;
 }
@@ -3631,24 +3827,30 @@ int axbc_batch(axbc_ctx *c, const uint32_t *v0s, const double *ps, int nb, uint3
     if (v0s[k] >= c->n) return fail(AXBC_EINVAL, "root out of range%s (line %d)", "", __LINE__);
   HIP_TRY(hipSetDevice(c->device));
   int rc;
-  // the bit-packed state holds at most 128 lanes; width 256 sweeps the
-  // roots of a batch above 128 in one workgroup sweep and applies them in
-  // two halves
+  // a state holds at most 128 lanes; width 256 sweeps the roots of a batch
+  // above 128 in one workgroup sweep, or in a second bit-packed state beside
+  // the first, and applies them in two halves
   const int sw = std::min(c->batch_width, 128);
   const uint64_t bound = list_bound(c->n, sw);
   if (!c->batch) {
     if ((rc = batch_create(c, sw, list_first(c->n, sw)))) return rc;
   }
-  if (nb > c->batch->B) return run_wg_pair<2>(c, v0s, ps, nb, stop_c, applied, stats);
+  const bool pair = nb > c->batch->B;
+  c->pair_last = false;
+  if (pair && use_wg(c)) return run_wg_pair<2>(c, v0s, ps, nb, stop_c, applied, stats);
   for (int attempt = 0; attempt < 8; ++attempt) {
-    rc = c->batch->nw == 1 ? run_batch<1>(c, v0s, ps, nb, stop_c, applied, stats)
-                           : run_batch<2>(c, v0s, ps, nb, stop_c, applied, stats);
-    if (rc != AXBC_ENOMEM || !c->batch || !c->batch->h_ctl->overflow) return rc;
+    if (pair)
+      rc = run_packed_pair<2>(c, v0s, ps, nb, stop_c, applied, stats);
+    else
+      rc = c->batch->nw == 1 ? run_batch<1>(c, v0s, ps, nb, stop_c, applied, stats)
+                             : run_batch<2>(c, v0s, ps, nb, stop_c, applied, stats);
+    const bool over = (c->batch && c->batch->h_ctl->overflow) || (c->batch2 && c->batch2->h_ctl->overflow);
+    if (rc != AXBC_ENOMEM || !c->batch || !over) return rc;
     // grow the level list and retry the batch from scratch (bc untouched:
     // the update runs only after a complete sweep)
     const uint64_t cap = std::min<uint64_t>((uint64_t)c->batch->qcap * 4, bound);
     const int w = c->batch->B;
-    batch_free(c);
+    batch_free_all(c);
     if ((rc = batch_create(c, w, (uint32_t)cap))) return rc;
   }
   return fail(AXBC_ENOMEM, "batch list capacity%s (line %d)", "", __LINE__);
@@ -3699,6 +3901,12 @@ int axbc_batch_delta(axbc_ctx *c, int k, double *delta) {
 int axbc_batch_levels(axbc_ctx *c, int k, int32_t *lvl, uint32_t *sigma) {
   if (!c || !c->batch || !lvl || !sigma || k < 0 || k >= c->batch_width)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  if (c->pair_last && k >= c->batch->B) {
+    // a root of the second state (this call ends synchronised)
+    const int B = c->batch->B;
+    SecondSlot two(c);
+    return axbc_batch_levels(c, k - B, lvl, sigma);
+  }
   BatchState *b = c->batch;
   if (!b->have || b->levels <= 0) return fail(AXBC_ESTATE, "no batch has run%s (line %d)", "", __LINE__);
   if (!(b->last_wg && k < (int)b->h_wlane.size()) && k >= b->B)

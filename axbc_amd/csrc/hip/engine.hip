@@ -969,7 +969,7 @@ int axbc_ctx_destroy(axbc_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   sampler_free(c);
-  batch_free(c);
+  batch_free_all(c);
   mlp_free(c);
   for (auto &pr : c->ktimer.pending) {
     (void)hipEventDestroy(pr.second.first);
@@ -1270,7 +1270,7 @@ int axbc_set_reach(axbc_ctx *c, const uint32_t *reach) {
   hipLaunchKernelGGL(k_init_state, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->reach, c->lvl,
                      c->sigma, c->delta, c->bc);
   c->have_round = false;
-  batch_free(c);
+  batch_free_all(c);
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AXBC_OK;
 }

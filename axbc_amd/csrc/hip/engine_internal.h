@@ -163,6 +163,13 @@ struct axbc_ctx {
   // samplers and batches
   axbc::SamplerBufs *sampler = nullptr;
   axbc::BatchState *batch = nullptr;
+  // batches of more than 128 roots through the bit-packed sweep: the roots
+  // from 128 on are swept in a second state on a second stream, beside the
+  // first 128 (one host thread drives both)
+  axbc::BatchState *batch2 = nullptr;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t pair_ev[2] = {nullptr, nullptr};
+  bool pair_last = false;  // the last batch ran in both states
   void *mlp = nullptr;  // learned proposal buffers (mlp.hip)
   int batch_width = 64;
   uint32_t batch_chunk = 256;
@@ -186,6 +193,8 @@ int batch_delta_row(axbc_ctx *c, int k, const double **row);
 int mirror_symmetric(axbc_ctx *c, bool *sym);
 void sampler_free(axbc_ctx *c);
 void batch_free(axbc_ctx *c);
+// Both batch states, the second stream and its events.
+void batch_free_all(axbc_ctx *c);
 void mlp_free(axbc_ctx *c);
 int grid_for(uint64_t n, int cap = 2048);
 // Kronecker generator on the device (kronecker.hip): nedges packed

@@ -97,7 +97,7 @@ class PivotLoop:
         engine.set_checks(self.checks)
         if batch:
             if batch not in (32, 64, 128, 256):
-                raise ValueError("roots per batch: 32, 64, 128 or 256 (256: workgroup sweep, batch mode 2)")
+                raise ValueError("roots per batch: 32, 64, 128 or 256")
             _check(_ffi.hip().axbc_set_batch_width(engine.ctx, batch if batch >= 128 else 64))
         self.mix = z
         if z == 0:

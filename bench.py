@@ -327,7 +327,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24, help="timed batches")
     ap.add_argument("--warmup", type=int, default=2, help="untimed batches")
-    ap.add_argument("--batch", type=int, default=0, choices=[0, 32, 64, 128],
+    ap.add_argument("--batch", type=int, default=0, choices=[0, 32, 64, 128, 256],
                     help="roots per batch; 0 = chosen by the autotuner (with the split and pull settings)")
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--plain-rounds", type=int, default=48)
