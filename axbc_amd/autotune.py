@@ -1,9 +1,9 @@
 """Autotuner for the batched sweep: picks, per graph, the number of roots
-per batch (the two widths of the bit-packed kernels, 64 and 128; a batch
-of 32 is a partial batch of the 64-lane kernels, not a width of its own,
-and is tuned only when asked for), the split threshold (edges per work item
-for high-degree vertices) and the bottom-up switch ratio by timing real
-batches of the graph's own roots.
+per batch (the two widths of the bit-packed kernels, 64 and 128, and 256:
+two 128-lane sweeps side by side; a batch of 32 is a partial batch of the
+64-lane kernels, not a width of its own, and is tuned only when asked for),
+the split threshold (edges per work item for high-degree vertices) and the
+bottom-up switch ratio by timing real batches of the graph's own roots.
 Width and threshold are searched jointly, the switch ratio afterwards at
 the chosen pair.  bc is saved and restored around the trials, so tuning
 leaves the engine's result untouched.
@@ -17,7 +17,7 @@ import numpy as np
 from . import _ffi
 from .engine import Engine, _check
 
-WIDTHS = (64, 128)
+WIDTHS = (64, 128, 256)
 CHUNKS = (128, 256, 512, 1024)
 PULLS = (0, 16, 32, 64)
 

@@ -10,8 +10,9 @@
  * through libaxbc_hip.so (include/axbc.h).
  *
  * Flags of this engine only:
- *   -B W   roots per batch: 0 one root per round (default), 64 or 128, or 256
- *          (one workgroup per root: the workgroup sweep, for deep graphs)
+ *   -B W   roots per batch: 0 one root per round (default), 64, 128 or 256
+ *          (256: two bit-packed sweeps side by side, or one workgroup per
+ *          root on deep graphs)
  *   -s S   generator seed pair (S, S+1) (default: the reference's 23, 24)
  *   -T T   host threads for graph construction
  *   -q     no per-pivot lines
@@ -69,7 +70,7 @@ static void usage(const char *p) {
           "\t\t -H <0|1>       1-degree reduction\n"
           "\t\t -U             DO NOT make graph Undirected\n"
           "\t\t -R <0|1>       fixed / time-based seed\n"
-          "\t\t -B <0|64|128|256>  roots per GPU batch (256: one workgroup per root)\n"
+          "\t\t -B <0|64|128|256>  roots per GPU batch\n"
           "\t\t -g             graph generation and preprocessing on the GPU\n",
           p);
   exit(EXIT_FAILURE);
@@ -401,7 +402,6 @@ int main(int argc, char **argv) {
   axbc_ctx *ctx = NULL;
   size_t dev_bytes = 0;
   check(axbc_ctx_create(0, N, g.ned, g.col, g.row, g.reach, heuristic, &ctx, &dev_bytes), "axbc_ctx_create");
-  if (width == 256) check(axbc_set_batch_mode(ctx, 2), "axbc_set_batch_mode");
   if (width) check(axbc_set_batch_width(ctx, width), "axbc_set_batch_width");
   printf("Device memory allocated (MB): %zu\n", dev_bytes >> 20);
 

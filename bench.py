@@ -108,6 +108,33 @@ def measured_traffic(kernel, batch, chunk, pull):
     return None
 
 
+def roofline_alone(e, dom, no_graphs):
+    """The dominant kernel's rate on batches of 128 roots (one sweep, one
+    stream) of the workload's distribution: 2 warm-up and 6 timed batches."""
+    from axbc_amd import _ffi
+    from axbc_amd.pivots import PivotLoop
+    L = _ffi.hip()
+    roots = PivotLoop(e, 4096, c=0, x=1024, z=1, seed=4343543 + 2).sample_roots(128 * 8)
+    saved = e.bc()
+    if L.axbc_set_batch_width(e.ctx, 128) != 0:
+        raise RuntimeError(L.axbc_last_error())
+    p = np.full(128, 1.0 / e.N)
+    for i in range(2):
+        e.batch(roots[128 * i:128 * (i + 1)], p)
+    e.sync()
+    set_timing_kinds(e, [dom])
+    for i in range(2, 8):
+        e.batch(roots[128 * i:128 * (i + 1)], p)
+    e.sync()
+    ms, n, b = kernel_stats(e, dom)
+    set_timing(e, False)
+    e.set_bc(saved)
+    achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"batch_width": 128, "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "avg_launch_us": round(ms / max(n, 1) * 1e3, 2), "launches": n,
+            "algorithmic_bytes_per_launch": int(b / max(n, 1))}
+
+
 def copy_calibration(mib):
     """The calibration copy kernel (reads and writes `mib` MiB per launch),
     timed with HIP events; used to scale the hardware byte counters."""
@@ -391,7 +418,7 @@ def main():
         from axbc_amd.autotune import tune
         # roots from the workload's own distribution (high-degree neighbourhood),
         # drawn with another seed so that the tuning does not run the timed roots
-        sample = PivotLoop(e, 4096, c=0, x=1024, z=1, seed=4343543 + 1).sample_roots(128 * 4)
+        sample = PivotLoop(e, 4096, c=0, x=1024, z=1, seed=4343543 + 1).sample_roots(256 * 4)
         e.reset_bc()
         t = time.perf_counter()
         tuned = tune(e, sample, reps=3)  # 3 timed batches per setting: one batch alone is too noisy to rank them
@@ -458,6 +485,12 @@ def main():
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(ms / max(n, 1) * 1e3, 2), "launches": n,
                 "algorithmic_bytes_per_launch": int(b / max(n, 1)),
                 "kernel_ms_share": round(wstats[dom][0] / max(warm_ms, 1e-9), 3)}
+    if batch > 128:
+        # batches of 256 run two 128-lane sweeps side by side, so each launch
+        # shares the GPU with the other sweep's kernels and takes longer than
+        # alone: the same kernel on batches of 128 of the workload's roots
+        # (after the timed region, untimed for the rate) gives its own rate
+        roofline["alone"] = roofline_alone(e, dom, args.no_graphs)
 
     # ---- plain one-root-per-round loop on the same engine --------------------
     # the warm-up rounds carry events on every kernel (the breakdown); the

@@ -201,9 +201,10 @@ int axbc_draw_window(axbc_ctx *ctx, int table, uint64_t state, uint64_t total, u
 
 /*
  * Multi-source rounds: nb (1..width) roots in one sweep, width 64 or 128
- * (axbc_set_batch_width; default 64), or 256 with the workgroup sweep
- * (axbc_set_batch_mode 2 first: one workgroup per root, the roots applied in
- * two halves of draw order).  Results are those of nb
+ * (axbc_set_batch_width; default 64), or 256: as a workgroup sweep of all
+ * the roots, or as two 128-lane bit-packed sweeps side by side (roots
+ * 0..127 on the context stream, the others in a second state on a second
+ * stream), the roots applied in two halves of draw order.  Results are those of nb
  * consecutive axbc_pivot calls in the given order: the bc update applies
  * the roots' contributions vertex by vertex in root order.  With stop_c > 0
  * and a check list (axbc_set_checks), the adaptive stop test
@@ -255,8 +256,7 @@ int axbc_set_batch_persistent(axbc_ctx *ctx, int mode, double limit_seconds);
  * plus 4 per level and lane, allocated on first use); 0 (default) chooses 2
  * on graphs whose previous batch ran more than 256 levels and whose lists
  * hold at most 64 edges (high-diameter graphs, where the roots' frontiers
- * rarely meet), 1 otherwise.  With width 256 only mode 2 is accepted.
- * Results do not depend on it. */
+ * rarely meet), 1 otherwise.  Results do not depend on it. */
 int axbc_set_batch_mode(axbc_ctx *ctx, int mode);
 int axbc_batch_width(axbc_ctx *ctx);
 int axbc_batch(axbc_ctx *ctx, const uint32_t *v0s, const double *ps, int nb, uint32_t stop_c,

@@ -76,7 +76,7 @@ def test_plain_config_scale16(require_gpu, tmp_path, directed):
     ["-z", "1", "-H", "1", "-c", "1", "-x", "8"],
     ["-z", "6", "-H", "0", "-c", "0"],
     ["-z", "0", "-H", "1", "-c", "0", "-B", "64", "-g"],
-    ["-z", "0", "-H", "1", "-c", "0", "-B", "256"],  # one workgroup per root, applied in two halves
+    ["-z", "0", "-H", "1", "-c", "0", "-B", "256"],  # two bit-packed sweeps side by side, applied in two halves
     ["-z", "1", "-H", "1", "-c", "1", "-x", "16", "-B", "256"],
 ])
 def test_driver_on_a_file_matches_oracle(require_gpu, tmp_path, flags):

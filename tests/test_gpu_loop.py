@@ -97,9 +97,9 @@ def test_autotuned_loop_matches_oracle(require_gpu):
         before = e.bc()
         rng = np.random.default_rng(5)
         sample = rng.choice(np.flatnonzero(g.degrees() > 0), 384, replace=False)
-        t = tune(e, sample, reps=1, widths=(32, 64, 128), chunks=(64, 256), pulls=(0, 32, 65536))
+        t = tune(e, sample, reps=1, widths=(32, 64, 128, 256), chunks=(64, 256), pulls=(0, 32, 65536))
         print(f"[autotune] {t['width']} {t['chunk']} {t['pull']}", flush=True)
-        assert t["width"] in (32, 64, 128) and t["chunk"] in (64, 256) and t["pull"] in (0, 32, 65536)
+        assert t["width"] in (32, 64, 128, 256) and t["chunk"] in (64, 256) and t["pull"] in (0, 32, 65536)
         np.testing.assert_array_equal(e.bc(), before)
         e.reset_bc()
         res = PivotLoop(e, nbfs, c=1, x=4, z=1, batch=t["width"]).run()

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from axbc_amd import Engine, Graph, _ffi
+from axbc_amd.pivots import PivotLoop
 from oracle import oracle
 from tests.parity import close
 from tests.test_gpu_batch import delta_of, run_batch
@@ -160,3 +161,18 @@ def test_pair_with_kernel_timing(require_gpu):
                 assert n.value > 0 and ms.value > 0
             out.append(e.bc())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("z", [0, 1])
+def test_pair_pivot_loop_equals_oracle_loop(require_gpu, z):
+    """The adaptive loop in batches of 256 (static distributions): the same
+    pivots, rounds and stop as the oracle's one-root loop."""
+    g = Graph.kronecker(11, 16, seed=(3, 4), heuristic=1)
+    nbfs = 700
+    pv, pr, bc_ref, rs = oracle.run(g, nbfs, c=1, x=4, z=z)
+    with _engine(g, 256, 0) as e:
+        res = PivotLoop(e, nbfs, c=1, x=4, z=z, batch=256).run()
+        assert res.pivots == [int(v) for v in pv]
+        assert res.nrounds == rs["nrounds"] and res.stopped == rs["stopped"]
+        ok, err = close(e.bc(), bc_ref)
+        assert ok, err

@@ -188,17 +188,29 @@ def test_automatic_choice_on_a_deep_lattice(require_gpu):
         assert ok, err
 
 
-def test_width_256_needs_the_workgroup_sweep(require_gpu):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_width_256_in_every_mode(require_gpu, mode):
+    """Width 256 in every mode: a workgroup sweep of all the roots (mode 2),
+    or two bit-packed sweeps side by side (mode 1, and mode 0 on a graph
+    of few levels); bc against the oracle either way."""
     g = Graph.lattice(32, 0.03, seed=3, heuristic=0)
+    deg = g.degrees()
+    roots = np.random.default_rng(9).choice(np.flatnonzero(deg > 0), size=200, replace=False)
     L = _ffi.hip()
+    bc_ref = np.zeros(g.N)
+    for v0 in roots:
+        bc_ref = oracle.pivot(g, int(v0), 1.0 / g.N, bc_ref)[3]
     with Engine(g) as e:
-        assert L.axbc_set_batch_width(e.ctx, 256) != 0 and b"workgroup sweep" in L.axbc_last_error()
-        _mode(e, 2)
+        _mode(e, mode)
         assert L.axbc_set_batch_width(e.ctx, 256) == 0
-        assert L.axbc_set_batch_mode(e.ctx, 1) != 0
-        assert L.axbc_set_batch_mode(e.ctx, 0) != 0
-        assert L.axbc_set_batch_width(e.ctx, 128) == 0
-        _mode(e, 1)
+        assert L.axbc_kernel_timing(e.ctx, 1) == 0
+        applied, _ = run_batch(e, roots, np.full(len(roots), 1.0 / g.N))
+        assert applied == len(roots)
+        ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
+        assert L.axbc_kernel_stats(e.ctx, 16, C.byref(ms), C.byref(n), C.byref(b)) == 0
+        assert n.value == (1 if mode == 2 else 0), f"workgroup sweeps: {n.value}"
+        ok, err = close(e.bc(), bc_ref)
+        assert ok, err
 
 
 @pytest.mark.parametrize("stop_at", [None, 40, 170])
