@@ -3791,17 +3791,19 @@ int axbc_set_batch_mode(axbc_ctx *c, int mode) {
 }
 
 int axbc_batch_state_bytes(uint64_t n, uint64_t ned, const uint32_t *col, int width, size_t *bytes) {
-  if (!col || !bytes || (width != 64 && width != 128) || n == 0 || n > 0xFFFFFFF0ull)
+  if (!col || !bytes || (width != 64 && width != 128 && width != 256) || n == 0 || n > 0xFFFFFFF0ull)
     return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  // width 256: two states of 128 lanes (a batch of more than 128 roots)
+  const int lanes = std::min(width, 128);
   BatchState b;
-  b.nw = width / 64;
-  b.B = width;
+  b.nw = lanes / 64;
+  b.B = lanes;
   b.n = (uint32_t)n;
-  b.qcap = list_first(n, width);
+  b.qcap = list_first(n, lanes);
   size_splits(&b, col, n, ned);
   b.plan_only = true;
   (void)alloc_buffers(&b);
-  *bytes = b.bytes;
+  *bytes = b.bytes * (width > 128 ? 2 : 1);
   return AXBC_OK;
 }
 

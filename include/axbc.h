@@ -85,7 +85,8 @@ int axbc_device_bytes(axbc_ctx *ctx, size_t *bytes);
 int axbc_set_memory_budget(axbc_ctx *ctx, size_t bytes);
 /* Device bytes of the batch state that the first batch of a context on
  * this graph (n vertices, ned CSC entries, column offsets col[n+1]) creates
- * at width 64 or 128, with the level list at its first capacity; a graph
+ * at width 64 or 128 (256: the two states of a batch of more than 128
+ * roots), with the level list at its first capacity; a graph
  * that is not symmetric adds its transpose, 4 (n + 1 + ned) bytes.  For
  * admission before a context exists.  No reference counterpart. */
 int axbc_batch_state_bytes(uint64_t n, uint64_t ned, const uint32_t *col, int width, size_t *bytes);

@@ -118,10 +118,11 @@ def test_budget_below_use_refuses_only_new_allocations(require_gpu):
         e.batch_delta(0)
 
 
-@pytest.mark.parametrize("width", [64, 128])
+@pytest.mark.parametrize("width", [64, 128, 256])
 def test_batch_state_bytes_are_exact(require_gpu, width):
     """The admission estimate of the batch state equals what the first
-    batch allocates."""
+    batch allocates (at width 256, a batch of more than 128 roots: both
+    states)."""
     import ctypes as C
 
     from axbc_amd import _ffi
@@ -131,5 +132,7 @@ def test_batch_state_bytes_are_exact(require_gpu, width):
     with Engine(g) as e:
         e.set_batch_width(width)
         base = e.bytes_in_use()
-        e.batch(np.arange(4, dtype=np.uint32), np.full(4, 1.0 / g.N))
+        nb = 200 if width == 256 else 4
+        e.set_batch_mode(1)
+        e.batch(np.arange(nb, dtype=np.uint32), np.full(nb, 1.0 / g.N))
         assert e.bytes_in_use() - base == size.value
