@@ -454,10 +454,17 @@ __device__ __forceinline__ void claim_split_wave(axbc::BCtl *ctl, bool pred, uin
 // slice (workgroups are dealt to the XCDs round robin) and moves on to the
 // next slice when its own is used up.  Atomics on one address are served
 // one at a time, so a single counter would bound the launch at ~10 ns per
-// item.  Returns false when every slice is used up.
+// item.  Returns false when every slice is used up.  A launch of fewer
+// than kHandMin items per wave deals them out in a fixed order instead
+// (item w + t waves to wave w): each claim is a round trip to the counter's
+// line, and the slices a wave tries once its own is used up cost a round
+// trip each, which on small levels cost more than the balance gains.
+constexpr uint64_t kHandMin = 2;
+
 struct Hand {
-  uint64_t per;
+  uint64_t per, next, step;
   int c, left;
+  bool fixed;
 };
 
 __device__ __forceinline__ Hand hand_begin(uint64_t total) {
@@ -465,16 +472,24 @@ __device__ __forceinline__ Hand hand_begin(uint64_t total) {
   h.per = (total + kHandSlices - 1) / kHandSlices;
   h.c = (int)(blockIdx.x % kHandSlices);
   h.left = kHandSlices;
+  h.step = (uint64_t)gridDim.x * blockDim.x / kWave;
+  h.next = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  h.fixed = total < kHandMin * h.step;
   return h;
 }
 
 __device__ __forceinline__ bool hand_next(Hand &h, unsigned long long *ctr, uint64_t total, uint64_t &item) {
+  if (h.fixed) {
+    item = h.next;
+    h.next += h.step;
+    return item < total;
+  }
   while (h.left > 0) {
     const uint64_t lo = (uint64_t)h.c * h.per, hi = min(total, lo + h.per);
     unsigned long long got = ~0ull;
     if (lo < hi && lane_id() == 0) got = atomicAdd(&ctr[h.c * kHandStride], 1ull);
     got = __shfl(got, 0);
-    if (lo < hi && lo + got < hi) {
+    if (lo < hi && got < hi - lo) {
       item = lo + got;
       return true;
     }
@@ -1899,11 +1914,15 @@ __device__ __forceinline__ void bdep_body(const uint32_t *__restrict__ col, cons
     // (1, w), ...; with one (hand), a wave takes the groups r nwv + w one at
     // a time.
     const uint64_t rounds = ((uint64_t)e - b + (uint64_t)nwv * kWave - 1) / ((uint64_t)nwv * kWave);
-    Hand h = hand_begin(rounds * nwv);
+    // groups holding entries: every group of the rounds before the last, and
+    // in the last round those whose first entry lies below e
+    const uint64_t last = rounds ? std::min<uint64_t>(nwv, (uint64_t)e - b - (rounds - 1) * nwv * kWave) : 0;
+    const uint64_t ngroups = rounds ? (rounds - 1) * nwv + last : 0;
+    Hand h = hand_begin(ngroups);
     uint64_t g = gw;
     for (;;) {
       if (hand) {
-        if (!hand_next(h, hand, rounds * nwv, g)) break;
+        if (!hand_next(h, hand, ngroups, g)) break;
       } else if (g >= rounds * nwv) {
         break;
       }
