@@ -459,7 +459,7 @@ __device__ __forceinline__ void claim_split_wave(axbc::BCtl *ctl, bool pred, uin
 // (item w + t waves to wave w): each claim is a round trip to the counter's
 // line, and the slices a wave tries once its own is used up cost a round
 // trip each, which on small levels cost more than the balance gains.
-constexpr uint64_t kHandMin = 2;
+constexpr uint64_t kHandMin = 4;
 
 struct Hand {
   uint64_t per, next, step;
