@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from . import _ffi
-from .engine import Engine, _check
+from .engine import Engine, EngineError, _check
 
 WIDTHS = (64, 128, 256)
 CHUNKS = (128, 256, 512, 1024)
@@ -50,8 +50,17 @@ def tune(e: Engine, roots: np.ndarray, reps: int = 2, widths=WIDTHS, chunks=CHUN
     saved = e.bc()
     table = {}
     for w in widths:
-        for ch in chunks:
-            table[(w, ch, 32)] = _trial(e, roots, w, ch, reps, 32)
+        try:
+            for ch in chunks:
+                table[(w, ch, 32)] = _trial(e, roots, w, ch, reps, 32)
+        except EngineError:
+            # width 256 needs a second batch state; where the device memory
+            # or the context's budget has no room for it, the width is out
+            if w <= 128:
+                raise
+            for ch in chunks:
+                table.pop((w, ch, 32), None)
+            _check(_ffi.hip().axbc_set_batch_width(e.ctx, 128))
     (w, ch, _), _best = min(table.items(), key=lambda kv: kv[1])
     for pl in pulls:
         if (w, ch, pl) not in table:
