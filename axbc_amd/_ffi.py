@@ -123,6 +123,7 @@ def hip() -> C.CDLL:
         "axbc_copy_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
         "axbc_read_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
         "axbc_gather_kernel": (C.c_int, [ctxp, C.c_uint64, C.c_int]),
+        "axbc_kernel_busy": (C.c_int, [ctxp, C.c_int, C.POINTER(C.c_double)]),
     })
     optional = {}
     for name, (res, args) in sig.items():

@@ -33,6 +33,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../../include/axbc.h"
@@ -94,10 +95,13 @@ static int kt_collect(axbc_ctx *c) {
   if (t.pending.empty()) return AXBC_OK;
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (auto &pr : t.pending) {
-    float ms = 0;
+    float ms = 0, t0 = 0, t1 = 0;
     if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
       t.ms[pr.first] += ms;
       t.launches[pr.first] += 1;
+      if (t.ref && hipEventElapsedTime(&t0, t.ref, pr.second.first) == hipSuccess &&
+          hipEventElapsedTime(&t1, t.ref, pr.second.second) == hipSuccess)
+        t.spans[pr.first].push_back({(double)t0, (double)t1});
     }
     t.pool.push_back(pr.second.first);
     t.pool.push_back(pr.second.second);
@@ -976,6 +980,7 @@ int axbc_ctx_destroy(axbc_ctx *c) {
     (void)hipEventDestroy(pr.second.second);
   }
   for (hipEvent_t e : c->ktimer.pool) (void)hipEventDestroy(e);
+  if (c->ktimer.ref) (void)hipEventDestroy(c->ktimer.ref);
   if (c->ktimer.dbytes) (void)hipFree(c->ktimer.dbytes);
   if (!c->symmetric) {
     if (c->rcol) (void)hipFree(c->rcol);
@@ -1071,7 +1076,10 @@ static int kt_enable(axbc_ctx *c, bool enable, uint32_t mask) {
     for (int k = 0; k < KK_COUNT; ++k) {
       t.ms[k] = 0;
       t.launches[k] = 0;
+      t.spans[k].clear();
     }
+    if (!t.ref) HIP_TRY(hipEventCreate(&t.ref));
+    HIP_TRY(hipEventRecord(t.ref, c->stream));
   } else {
     int rc = kt_collect(c);
     if (rc) return rc;
@@ -1106,6 +1114,26 @@ int axbc_kernel_stats(axbc_ctx *c, int kind, double *ms, uint64_t *launches, uin
   if (ms) *ms = c->ktimer.ms[kind];
   if (launches) *launches = c->ktimer.launches[kind];
   if (bytes) *bytes = b;
+  return AXBC_OK;
+}
+
+int axbc_kernel_busy(axbc_ctx *c, int kind, double *ms) {
+  if (!c || !ms || kind < 0 || kind >= KK_COUNT) return fail(AXBC_EINVAL, "bad argument%s (line %d)", "", __LINE__);
+  int rc = kt_collect(c);
+  if (rc) return rc;
+  std::vector<std::pair<double, double>> s = c->ktimer.spans[kind];
+  std::sort(s.begin(), s.end());
+  double busy = 0, end = -1e300;
+  for (const auto &iv : s) {
+    if (iv.first > end) {
+      busy += iv.second - iv.first;
+      end = iv.second;
+    } else if (iv.second > end) {
+      busy += iv.second - end;
+      end = iv.second;
+    }
+  }
+  *ms = busy;
   return AXBC_OK;
 }
 

@@ -106,6 +106,11 @@ struct KernelTimer {
   std::vector<hipEvent_t> pool;
   double ms[KK_COUNT] = {0};
   uint64_t launches[KK_COUNT] = {0};
+  // each timed launch as [start, end] ms after `ref` (recorded when timing
+  // is enabled): launches on two streams overlap, and the union of a kind's
+  // spans is the time any of its launches ran
+  hipEvent_t ref = nullptr;
+  std::vector<std::pair<double, double>> spans[KK_COUNT];
 };
 
 thread_local extern char g_err[512];

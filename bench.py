@@ -488,8 +488,16 @@ def main():
     if batch > 128:
         # batches of 256 run two 128-lane sweeps side by side, so each launch
         # shares the GPU with the other sweep's kernels and takes longer than
-        # alone: the same kernel on batches of 128 of the workload's roots
-        # (after the timed region, untimed for the rate) gives its own rate
+        # alone: the kernel's launches together over the time any of them ran
+        # (overlapped), and the same kernel on batches of 128 of the
+        # workload's roots after the timed region (alone)
+        from axbc_amd import _ffi
+        busy = C.c_double()
+        if _ffi.hip().axbc_kernel_busy(e.ctx, dom, C.byref(busy)) != 0:
+            raise RuntimeError(_ffi.hip().axbc_last_error())
+        ach = b / (busy.value * 1e-3) / 1e9 if busy.value > 0 else 0.0
+        roofline["overlapped"] = {"busy_ms": round(busy.value, 2), "summed_ms": round(ms, 2),
+                                  "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
         roofline["alone"] = roofline_alone(e, dom, args.no_graphs)
 
     # ---- plain one-root-per-round loop on the same engine --------------------

@@ -317,6 +317,10 @@ int axbc_batch_utility_noalloc(axbc_ctx *ctx, const uint32_t *vertex, const doub
 int axbc_kernel_timing(axbc_ctx *ctx, int enable);
 int axbc_kernel_timing_kinds(axbc_ctx *ctx, uint32_t mask);
 int axbc_kernel_stats(axbc_ctx *ctx, int kind, double *ms, uint64_t *launches, uint64_t *bytes);
+/* The milliseconds during which at least one timed launch of a kind ran
+ * since enabling (launches on the two streams of a width-256 batch overlap,
+ * so this can be less than axbc_kernel_stats' sum). */
+int axbc_kernel_busy(axbc_ctx *ctx, int kind, double *ms);
 int axbc_copy_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 int axbc_read_kernel(axbc_ctx *ctx, uint64_t bytes, int repeats);
 int axbc_gather_kernel(axbc_ctx *ctx, uint64_t window, int repeats);
