@@ -136,3 +136,34 @@ def test_batch_state_bytes_are_exact(require_gpu, width):
         e.set_batch_mode(1)
         e.batch(np.arange(nb, dtype=np.uint32), np.full(nb, 1.0 / g.N))
         assert e.bytes_in_use() - base == size.value
+
+
+def test_second_state_over_budget(require_gpu):
+    """A batch of more than 128 bit-packed roots needs a second batch state:
+    with the budget at the bytes held after a batch of 128, the wider batch
+    fails cleanly (bc untouched, nothing more held), and a batch of 128
+    still runs."""
+    g = Graph.kronecker(12, 16, seed=(7, 9), heuristic=1)
+    deg = g.degrees()
+    roots = np.random.default_rng(4).choice(np.flatnonzero(deg > 0), size=256, replace=False).astype(np.uint32)
+    probs = np.full(256, 1.0 / g.N)
+    with Engine(g) as e:
+        e.set_batch_mode(1)
+        e.set_batch_width(256)
+        e.batch(roots[:128], probs[:128])  # the first state only
+        held = e.bytes_in_use()
+        e.reset_bc()
+        e.set_budget(held)
+        with pytest.raises(BudgetError, match="the batch state"):
+            e.batch(roots, probs)
+        assert not e.bc().any()
+        assert e.bytes_in_use() == held
+        applied, _ = e.batch(roots[:128], probs[:128])
+        assert applied == 128
+        e.set_budget(0)
+        applied, _ = e.batch(roots, probs)
+        assert applied == 256
+        assert e.bytes_in_use() > held
+        bc_ref = bc_of_roots(g, np.concatenate([roots[:128], roots]))
+        ok, err = close(e.bc(), bc_ref)
+        assert ok, err
