@@ -15,6 +15,11 @@
 //               between batches)
 //   dl[l][v]  = delta (FP64), valid for visited v
 //   lst[l][]  = the visited vertices in level order, off[l][L] = start of L
+//               (graphs of at most 2^28 vertices: an entry of a thread's pair
+//               of lists of at most 4 edges each gets, once expanded, the
+//               slots of its list that lead to DAG children in its top 4
+//               bits, so that the backward pass reads the state words of
+//               those children only)
 // Forward, level L -> L+1 (src/cuda_kernels.cu:588-658): every frontier
 // vertex u pushes sigma(u) to each neighbour w; the first visitor claims w
 // with a compare-and-swap of the whole word (tag and sigma at once) and
@@ -41,6 +46,8 @@ namespace {
 
 constexpr int kWgBlock = 1024;  // threads per root (16 waves on one CU)
 constexpr int kEdgeRun = 8;     // edge loads in flight per thread
+constexpr int kIdBits = 28;     // vertex id bits of a list entry that carries a child mask
+constexpr uint32_t kIdMask = (1u << kIdBits) - 1u;
 
 // The state words change under atomics from every wave of the workgroup:
 // they are read and changed with workgroup-scope atomics, and the barrier
@@ -109,14 +116,15 @@ __device__ __forceinline__ void short_list(const uint32_t *__restrict__ row, uin
 // round trips.  A vertex unvisited in this batch is claimed with tag and
 // path count at once (on failure word[t] receives the winner's word); later
 // visitors of the same level add to the word's low half (uint32, wrapping).
-// Returns the run's DAG edges; *all gains the reach of the claimed vertices.
+// Returns the run's DAG edges as a slot mask; *all gains the reach of the
+// claimed vertices.
 __device__ __forceinline__ uint32_t push_run(uint64_t *st, uint32_t *lst, uint32_t e, uint32_t *here,
                                              const uint32_t *__restrict__ reach, uint32_t tag0, uint32_t tnext,
                                              const uint32_t (&w)[kEdgeRun], uint32_t valid,
                                              const uint32_t (&su)[kEdgeRun], unsigned long long *all) {
   uint64_t word[kEdgeRun];
   bool won[kEdgeRun];
-  uint32_t dag = 0;
+  uint32_t dag = 0;  // bit t: slot t is a DAG edge
 #pragma unroll
   for (int t = 0; t < kEdgeRun; ++t) word[t] = (valid >> t) & 1u ? ld_wg(st + w[t]) : ~0ull;
 #pragma unroll
@@ -128,11 +136,11 @@ __device__ __forceinline__ uint32_t push_run(uint64_t *st, uint32_t *lst, uint32
   for (int t = 0; t < kEdgeRun; ++t) {
     if (won[t]) {
       lst[e + atomicAdd(here, 1u)] = w[t];
-      ++dag;
+      dag |= 1u << t;
       if (reach) *all += reach[w[t]];
     } else if ((uint32_t)(word[t] >> 32) == tnext) {
       __hip_atomic_fetch_add((uint32_t *)(st + w[t]), su[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      ++dag;
+      dag |= 1u << t;
     }
   }
   return dag;
@@ -187,6 +195,8 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
   double *dl = pdl + (size_t)l * n;
   uint32_t *lst = plst + (size_t)l * n;
   uint32_t *off = ploff + (size_t)l * lcap;
+  // list entries carry child masks where the ids leave 4 bits free
+  const bool packm = (uint64_t)n <= (uint64_t)kIdMask + 1u;
   const uint32_t v0 = src[l];
   __shared__ uint32_t s_cnt[3];
   __shared__ unsigned long long s_acc[4];
@@ -228,8 +238,15 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
         short_list<4>(row, ned_al, b1, d1, w);
 #pragma unroll
         for (int t = 0; t < kEdgeRun; ++t) su[t] = t < 4 ? s0 : s1;
-        dag += push_run(st, lst, e, here, reach, tag0, tnext, w, ((1u << d0) - 1u) | (((1u << d1) - 1u) << 4), su,
-                        &all);
+        const uint32_t bits = push_run(st, lst, e, here, reach, tag0, tnext, w,
+                                       ((1u << d0) - 1u) | (((1u << d1) - 1u) << 4), su, &all);
+        dag += (uint32_t)__popc(bits);
+        if (packm) {
+          // this thread's entries (read above; no other thread reads them
+          // until the backward pass, which pairs the same entries)
+          lst[i0] = u0 | ((bits & 15u) << kIdBits);
+          if (h1) lst[i1] = u1 | ((bits >> 4) << kIdBits);
+        }
         continue;
       }
       for (uint32_t p0 = 0; p0 < d0 + d1; p0 += kEdgeRun) {
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
           su[t] = q < d0 ? s0 : s1;
           valid |= (q < d0 + d1 ? 1u : 0u) << t;
         }
-        dag += push_run(st, lst, e, here, reach, tag0, tnext, w, valid, su, &all);
+        dag += (uint32_t)__popc(push_run(st, lst, e, here, reach, tag0, tnext, w, valid, su, &all));
       }
     }
     __syncthreads();
@@ -276,7 +293,7 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
   unsigned long long bytes = 0;
   if (L >= 1)
     for (uint32_t i = b + tid; i < e; i += kWgBlock) {
-      const uint32_t v = lst[i];
+      const uint32_t v = packm ? lst[i] & kIdMask : lst[i];
       dl[v] = reach ? (double)reach[v] : 0.0;
     }
   __syncthreads();
@@ -289,7 +306,8 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
     for (uint32_t i0 = db + tid; i0 < de; i0 += 2 * kWgBlock) {
       const uint32_t i1 = i0 + kWgBlock;
       const bool h1 = i1 < de;
-      const uint32_t v0 = lst[i0], v1 = h1 ? lst[i1] : 0u;
+      const uint32_t e0 = lst[i0], e1 = h1 ? lst[i1] : 0u;
+      const uint32_t v0 = packm ? e0 & kIdMask : e0, v1 = packm ? e1 & kIdMask : e1;
       const double sv0 = (double)(uint32_t)st[v0], sv1 = h1 ? (double)(uint32_t)st[v1] : 1.0;
       double acc0 = reach ? (double)reach[v0] : 0.0, acc1 = reach && h1 ? (double)reach[v1] : 0.0;
       const uint2 c0 = ld_pair(col + v0), c1 = h1 ? ld_pair(col + v1) : make_uint2(0u, 0u);
@@ -299,7 +317,11 @@ __global__ __launch_bounds__(kWgBlock) void k_wgsweep(uint32_t n, const uint32_t
       if (d0 <= 4 && d1 <= 4) {
         short_list<0>(row, ned_al, b0, d0, w);
         short_list<4>(row, ned_al, b1, d1, w);
-        pull_run(st, dl, tnext, w, ((1u << d0) - 1u) | (((1u << d1) - 1u) << 4), 4, sv0, sv1, acc0, acc1, terms);
+        // only the slots that lead to DAG children (the others cannot hold a
+        // vertex of level d+1)
+        const uint32_t k0 = packm ? e0 >> kIdBits : 15u, k1 = packm ? e1 >> kIdBits : 15u;
+        pull_run(st, dl, tnext, w, (k0 & ((1u << d0) - 1u)) | ((k1 & ((1u << d1) - 1u)) << 4), 4, sv0, sv1, acc0, acc1,
+                 terms);
       } else {
         for (uint32_t p0 = 0; p0 < d0 + d1; p0 += kEdgeRun) {
           uint32_t valid = 0;
